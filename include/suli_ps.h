@@ -1,0 +1,144 @@
+/*
+ * suli_ps: MI355X (gfx950) pressure-projection solver for SuLi.
+ *
+ * C ABI that replaces the reference's pressure step.  SuLi's time loop calls
+ *     CALL graddin()   (SuLi-IC/1_principal.f90:102, SuLi-IC/9_grad.f90:10-663)
+ *     CALL posdin()    (SuLi-IC/1_principal.f90:103, SuLi-IC/10_pos.f90:1-99)
+ * with every operand in module variables.  suli_amd/fortran/suli_ps_graddin.f90
+ * keeps those argument-less signatures and forwards the module arrays here.
+ *
+ * Array layouts are the reference's (Fortran column-major, x fastest, lower
+ * bounds as declared in SuLi-IC/0_modules.f90:135-137,170-172,188-189):
+ *   u    (0:nx+2, 0:ny+1, 0:nz+1)       v (0:nx+1, 0:ny+2, 0:nz+1)
+ *   w    (0:nx+1, 0:ny+1, 0:nz+2)       prd1, prd (0:nx+1, 0:ny+1, 0:nz+1)
+ *   rho  (1:nx, 1:ny, 1:nz)
+ *   obs_lsx (nx+1,ny,nz)  obs_lsy (nx,ny+1,nz)  obs_lsz (nx,ny,nz+1)
+ *   dz, dzz (0:nz+1)  (SuLi-ICdz/0_modules.f90:72-73; NULL on uniform grids)
+ *
+ * Return codes: 0 success, 1 not converged within max_iter (the reference
+ * prints "pulou pressão" and continues, 9_grad.f90:590), < 0 error (message
+ * from suli_ps_last_error).  A context is used by one host thread at a time.
+ */
+#ifndef SULI_PS_H
+#define SULI_PS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SULI_PS_ABI_VERSION 1
+
+enum suli_ps_precision {
+    SULI_PS_FP64 = 0,   /* FP64 CG, as the reference */
+    SULI_PS_MIXED = 1   /* FP32 SpMV/axpy inside FP64 residual replacement */
+};
+
+enum suli_ps_stop_rule {
+    SULI_PS_STOP_REFERENCE = 0, /* r.r <= 1e-4/(nx*ny*nz), 9_grad.f90:393,398 */
+    SULI_PS_STOP_RELATIVE = 1   /* ||r|| <= tol * ||b|| on the scaled system   */
+};
+
+enum suli_ps_status {
+    SULI_PS_OK = 0,
+    SULI_PS_NOT_CONVERGED = 1,
+    SULI_PS_EINVAL = -1,
+    SULI_PS_EHIP = -2,
+    SULI_PS_ENOMEM = -3
+};
+
+/* Scalars graddin/posdin read from SuLi's modules: disc (nx..dt, t_press,
+ * obst_t, ibm_t; SuLi-IC/0_modules.f90:2-58), cond (ccx0, ccy0; :68-80),
+ * ls_param (rho_f1; :282-295). */
+typedef struct suli_ps_grid {
+    int32_t nx, ny, nz;
+    int32_t ccx0, ccy0;     /* 0 = periodic in x / y */
+    int32_t t_press;        /* 1 = projection; 2 = IPA, faces use rho_f1 (9_grad.f90:109-116) */
+    int32_t obst_t, ibm_t;  /* obst_t != 0 && ibm_t == 2: masked RHS (9_grad.f90:45-88) */
+    double dx, dy, dz, dt;  /* dz unused when a dz(0:nz+1) array is passed */
+    double rho_f1;
+} suli_ps_grid;
+
+typedef struct suli_ps_opts {
+    int32_t precision;     /* enum suli_ps_precision */
+    int32_t stop_rule;     /* enum suli_ps_stop_rule */
+    double tol;            /* relative tolerance for SULI_PS_STOP_RELATIVE */
+    int32_t max_iter;      /* <= 0: 10000, the reference cap (9_grad.f90:398) */
+    int32_t chunk;         /* CG iterations between host polls; <= 0: automatic */
+    int32_t verbose;       /* 1: print the reference's non-convergence line */
+    int32_t timing;        /* 1: time SpMV launches with HIP events (stats) */
+} suli_ps_opts;
+
+typedef struct suli_ps_stats {
+    int32_t status;        /* enum suli_ps_status */
+    int32_t iters;         /* CG iterations (reference `cont`) */
+    int32_t refreshes;     /* FP64 residual replacements (mixed) */
+    int32_t spmv_timed;    /* SpMV launches timed (opts.timing) */
+    double rr0;            /* r.r of the initial scaled residual */
+    double bb;             /* b.b of the scaled right-hand side */
+    double rr;             /* final r.r the stop rule saw */
+    double thresh;         /* the stop rule's threshold on r.r */
+    double rel_true;       /* FP64 ||b - A e|| / ||b|| recomputed at exit */
+    double solve_ms;       /* device time of the solve, setup included */
+    double spmv_ms;        /* mean duration of a timed SpMV launch */
+} suli_ps_stats;
+
+typedef struct suli_ps_ctx suli_ps_ctx;
+
+/* Context on the current HIP device with its own stream (default priority).
+ * opts may be NULL (FP64, reference stop rule).  Returns NULL on failure. */
+suli_ps_ctx *suli_ps_create(const suli_ps_opts *opts);
+void suli_ps_destroy(suli_ps_ctx *ctx);
+int suli_ps_set_opts(suli_ps_ctx *ctx, const suli_ps_opts *opts);
+const char *suli_ps_last_error(const suli_ps_ctx *ctx);
+int suli_ps_abi_version(void);
+
+/* graddin (SuLi-IC/9_grad.f90:10-663; SuLi-ICdz/9_grad.f90 when dz/dzz are
+ * given): solves -div(dt/rho grad p) = -div u* with the reference's
+ * symmetric Jacobi scaling and CG, warm-started from prd1.  prd1 is read
+ * (initial guess, whole halo box) and written (whole halo box).
+ * obs_ls* are read only when obst_t != 0 && ibm_t == 2.  Host pointers. */
+int suli_ps_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u,
+                    const double *v, const double *w, const double *rho,
+                    const double *obs_lsx, const double *obs_lsy,
+                    const double *obs_lsz, const double *dz, const double *dzz,
+                    double *prd1, suli_ps_stats *stats);
+
+/* posdin, t_press == 1 branch (SuLi-IC/10_pos.f90:17-46,95; SuLi-ICdz/
+ * 10_pos.f90:21,41): u, v, w -= dt grad(prd1) / rho_face on the faces the
+ * reference updates; prd = prd1 (prd may be NULL).  Host pointers. */
+int suli_ps_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v,
+                   double *w, const double *rho, const double *prd1,
+                   const double *dz, const double *dzz, double *prd);
+
+/* The same two operations on device-resident arrays (same layouts, device
+ * pointers), enqueued on the context's stream.  graddin_dev returns once
+ * the solve has finished. */
+int suli_ps_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u,
+                        const double *v, const double *w, const double *rho,
+                        const double *obs_lsx, const double *obs_lsy,
+                        const double *obs_lsz, const double *dz, const double *dzz,
+                        double *prd1, suli_ps_stats *stats);
+int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v,
+                       double *w, const double *rho, const double *prd1,
+                       const double *dz, const double *dzz, double *prd);
+
+/* Device-memory helpers for callers without their own HIP code. */
+void *suli_ps_dev_alloc(suli_ps_ctx *ctx, size_t bytes);
+int suli_ps_dev_free(suli_ps_ctx *ctx, void *ptr);
+int suli_ps_memcpy_h2d(suli_ps_ctx *ctx, void *dst, const void *src, size_t bytes);
+int suli_ps_memcpy_d2h(suli_ps_ctx *ctx, void *dst, const void *src, size_t bytes);
+int suli_ps_dev_zero(suli_ps_ctx *ctx, void *ptr, size_t bytes); /* async, ctx stream */
+int suli_ps_synchronize(suli_ps_ctx *ctx);
+
+/* Measurement hook: runs the solver's SpMV kernel `reps` times on the
+ * operator of the last graddin call (precision as given) and returns the
+ * mean launch duration in ms, timed with HIP events on the context stream. */
+int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

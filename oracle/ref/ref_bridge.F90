@@ -1,0 +1,125 @@
+! Test-infrastructure bridge to the reference SuLi pressure step.
+!
+! This file is NOT part of the product. It is compiled (by oracle/Makefile,
+! in the development container only) together with the reference's own
+! hot-path sources
+!     SuLi-IC/0_modules.f90, 8_derivadas.f90, 9_grad.f90, 10_pos.f90
+!   or SuLi-ICdz/ (same files, built with -DSULI_ICDZ)
+! into oracle/_ref/libsuli_ref_ic.so / libsuli_ref_icdz.so.  It exposes a
+! flat C ABI so that the tests can run the reference graddin()/posdin()
+! (SuLi-IC/9_grad.f90:10-663, SuLi-IC/10_pos.f90:1-99) on identical inputs.
+!
+! The reference's subroutines take no arguments: everything flows through
+! module variables (SuLi-IC/0_modules.f90:2-58 disc, :68-80 cond,
+! :108-118 obst, :144-162 velpre, :282-295 ls_param).  The bridge sets those
+! module variables, copies the caller's arrays in, calls the reference, and
+! copies the results out.  All arrays use the reference's Fortran layouts
+! (column-major, x fastest, halos as declared at 0_modules.f90:170-172,188-189).
+module suli_ref_bridge
+   use iso_c_binding
+   implicit none
+contains
+
+   subroutine ref_setup(nx_, ny_, nz_, dx_, dy_, dz_, dt_, ccx0_, ccy0_, t_press_, &
+                        obst_t_, ibm_t_, omp_t_, rho_f1_, dzarr, dzzarr) &
+      bind(C, name="suli_ref_setup")
+      use disc
+      use cond
+      use velpre
+      use obst
+      use ls_param
+      integer(c_int), value :: nx_, ny_, nz_, ccx0_, ccy0_, t_press_, obst_t_, ibm_t_, omp_t_
+      real(c_double), value :: dx_, dy_, dz_, dt_, rho_f1_
+      ! dz(0:nz+1) and dzz(0:nz+1); read only in the ICdz build
+      real(c_double), intent(in) :: dzarr(0:nz_+1), dzzarr(0:nz_+1)
+
+      nx = nx_; ny = ny_; nz = nz_
+      nx1 = nx + 1; ny1 = ny + 1; nz1 = nz + 1
+      dx = dx_; dy = dy_
+      dt = dt_
+      ccx0 = ccx0_; ccxf = ccx0_
+      ccy0 = ccy0_; ccyf = ccy0_
+      t_press = t_press_
+      obst_t = obst_t_; ibm_t = ibm_t_
+      omp_t = omp_t_
+      rho_f1 = rho_f1_
+
+#ifdef SULI_ICDZ
+      if (allocated(dz)) deallocate(dz)
+      if (allocated(dzz)) deallocate(dzz)
+      allocate(dz(0:nz+1), dzz(0:nz+1))
+      dz = dzarr
+      dzz = dzzarr
+#else
+      dz = dz_
+      if (.false.) dz = dzarr(0) + dzzarr(0)
+#endif
+
+      if (allocated(u)) deallocate(u, v, w, prd1, prd, rho)
+      allocate(u(0:nx1+1, 0:ny+1, 0:nz+1))
+      allocate(v(0:nx+1, 0:ny1+1, 0:nz+1))
+      allocate(w(0:nx+1, 0:ny+1, 0:nz1+1))
+      allocate(prd1(0:nx+1, 0:ny+1, 0:nz+1), prd(0:nx+1, 0:ny+1, 0:nz+1))
+      allocate(rho(nx, ny, nz))
+      if (allocated(obs_lsx)) deallocate(obs_lsx, obs_lsy, obs_lsz)
+      allocate(obs_lsx(nx1, ny, nz), obs_lsy(nx, ny1, nz), obs_lsz(nx, ny, nz1))
+      obs_lsx = -1.0d0
+      obs_lsy = -1.0d0
+      obs_lsz = -1.0d0
+   end subroutine ref_setup
+
+   ! Runs the reference graddin() once.  u/v/w/rho/prd1 are read in the
+   ! reference layouts; prd1 (the full halo box) is written back.  When
+   ! have_obs /= 0 the three obs_ls* face arrays are copied in as well.
+   subroutine ref_graddin(uin, vin, win, rhoin, have_obs, obsx, obsy, obsz, prd1io) &
+      bind(C, name="suli_ref_graddin")
+      use disc
+      use velpre
+      use obst
+      real(c_double), intent(in) :: uin(0:nx+2, 0:ny+1, 0:nz+1)
+      real(c_double), intent(in) :: vin(0:nx+1, 0:ny+2, 0:nz+1)
+      real(c_double), intent(in) :: win(0:nx+1, 0:ny+1, 0:nz+2)
+      real(c_double), intent(in) :: rhoin(nx, ny, nz)
+      integer(c_int), value :: have_obs
+      real(c_double), intent(in) :: obsx(nx+1, ny, nz), obsy(nx, ny+1, nz), obsz(nx, ny, nz+1)
+      real(c_double), intent(inout) :: prd1io(0:nx+1, 0:ny+1, 0:nz+1)
+
+      u = uin
+      v = vin
+      w = win
+      rho = rhoin
+      if (have_obs /= 0) then
+         obs_lsx = obsx
+         obs_lsy = obsy
+         obs_lsz = obsz
+      end if
+      prd1 = prd1io
+      call graddin()
+      prd1io = prd1
+   end subroutine ref_graddin
+
+   ! Runs the reference posdin() once: u/v/w are updated in place and prd
+   ! is written (posdin copies prd = prd1).
+   subroutine ref_posdin(uio, vio, wio, rhoin, prd1in, prdout) bind(C, name="suli_ref_posdin")
+      use disc
+      use velpre
+      real(c_double), intent(inout) :: uio(0:nx+2, 0:ny+1, 0:nz+1)
+      real(c_double), intent(inout) :: vio(0:nx+1, 0:ny+2, 0:nz+1)
+      real(c_double), intent(inout) :: wio(0:nx+1, 0:ny+1, 0:nz+2)
+      real(c_double), intent(in) :: rhoin(nx, ny, nz)
+      real(c_double), intent(in) :: prd1in(0:nx+1, 0:ny+1, 0:nz+1)
+      real(c_double), intent(out) :: prdout(0:nx+1, 0:ny+1, 0:nz+1)
+
+      u = uio
+      v = vio
+      w = wio
+      rho = rhoin
+      prd1 = prd1in
+      call posdin()
+      uio = u
+      vio = v
+      wio = w
+      prdout = prd
+   end subroutine ref_posdin
+
+end module suli_ref_bridge

@@ -1,0 +1,476 @@
+/*
+ * CPU restatement of SuLi's pressure step: graddin() and posdin().
+ *
+ * TEST INFRASTRUCTURE ONLY (see suli_oracle.h).  It restates the reference
+ * algorithm sweep by sweep, in the reference's operation order, so that on
+ * x86-64 without FMA contraction it reproduces the compiled Fortran bit for
+ * bit when omp_t = 0 (serial dot products).  Each block cites the reference
+ * lines it follows.  Edge and corner cells of the halo box follow the
+ * reference's sequential (one-thread) order: x ghosts, then y, then z.
+ */
+#include "suli_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Fortran layouts -> 0-based linear offsets (see suli_oracle.h). */
+#define IP(i, j, k) ((size_t)(i) + (size_t)(nx + 2) * ((size_t)(j) + (size_t)(ny + 2) * (size_t)(k)))
+#define IU(i, j, k) ((size_t)(i) + (size_t)(nx + 3) * ((size_t)(j) + (size_t)(ny + 2) * (size_t)(k)))
+#define IV(i, j, k) ((size_t)(i) + (size_t)(nx + 2) * ((size_t)(j) + (size_t)(ny + 3) * (size_t)(k)))
+#define IW(i, j, k) ((size_t)(i) + (size_t)(nx + 2) * ((size_t)(j) + (size_t)(ny + 2) * (size_t)(k)))
+#define IC(i, j, k) ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)ny * (size_t)((k) - 1)))
+#define IFX(i, j, k) ((size_t)((i) - 1) + (size_t)(nx + 1) * ((size_t)((j) - 1) + (size_t)ny * (size_t)((k) - 1)))
+#define IFY(i, j, k) ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)(ny + 1) * (size_t)((k) - 1)))
+#define IFZ(i, j, k) ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)ny * (size_t)((k) - 1)))
+
+static int nthreads(const oracle_grid *g)
+{
+#ifdef _OPENMP
+    return g->threads > 0 ? g->threads : omp_get_max_threads();
+#else
+    (void)g;
+    return 1;
+#endif
+}
+
+/* SuLi-IC/8_derivadas.f90:305-357 (x), :381-418 (y), :443-481 (z);
+ * SuLi-ICdz/8_derivadas.f90:450-489 (dz-weighted z). */
+void oracle_interp_faces(const oracle_grid *g, const double *rho, double *rhox,
+                         double *rhoy, double *rhoz)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int nt = nthreads(g);
+    int i, j, k;
+
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++) {
+            for (i = 2; i <= nx; i++)
+                rhox[IFX(i, j, k)] = (rho[IC(i, j, k)] + rho[IC(i - 1, j, k)]) * 0.5;
+            if (g->ccx0 != 0) {
+                rhox[IFX(1, j, k)] = rhox[IFX(2, j, k)];
+                rhox[IFX(nx + 1, j, k)] = rhox[IFX(nx, j, k)];
+            } else {
+                rhox[IFX(1, j, k)] = (rho[IC(1, j, k)] + rho[IC(nx, j, k)]) * 0.5;
+                rhox[IFX(nx + 1, j, k)] = (rho[IC(1, j, k)] + rho[IC(nx, j, k)]) * 0.5;
+            }
+        }
+
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++) {
+        for (j = 2; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                rhoy[IFY(i, j, k)] = (rho[IC(i, j, k)] + rho[IC(i, j - 1, k)]) * 0.5;
+        /* no periodic branch in y (8_derivadas.f90:403-414) */
+        for (i = 1; i <= nx; i++) {
+            rhoy[IFY(i, 1, k)] = rhoy[IFY(i, 2, k)];
+            rhoy[IFY(i, ny + 1, k)] = rhoy[IFY(i, ny, k)];
+        }
+    }
+
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 2; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                if (g->dzv)
+                    rhoz[IFZ(i, j, k)] = (rho[IC(i, j, k)] * g->dzv[k] + rho[IC(i, j, k - 1)] * g->dzv[k - 1]) /
+                                         (g->dzv[k] + g->dzv[k - 1]);
+                else
+                    rhoz[IFZ(i, j, k)] = (rho[IC(i, j, k)] + rho[IC(i, j, k - 1)]) * 0.5;
+            }
+    for (j = 1; j <= ny; j++)
+        for (i = 1; i <= nx; i++) {
+            rhoz[IFZ(i, j, 1)] = rhoz[IFZ(i, j, 2)];
+            rhoz[IFZ(i, j, nz + 1)] = rhoz[IFZ(i, j, nz)];
+        }
+}
+
+/* Ghost fill of a halo-box array in the reference's order: x faces (mirror
+ * or periodic), then y, then z (always mirror).  9_grad.f90:188-239 (d),
+ * :322-370 (r), :531-579 (p), :594-643 (e). */
+static void fill_ghosts(const oracle_grid *g, double *a)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    int i, j, k;
+    for (k = 0; k <= nz + 1; k++)
+        for (j = 0; j <= ny + 1; j++) {
+            if (g->ccx0 == 0) {
+                a[IP(0, j, k)] = a[IP(nx, j, k)];
+                a[IP(nx + 1, j, k)] = a[IP(1, j, k)];
+            } else {
+                a[IP(0, j, k)] = a[IP(1, j, k)];
+                a[IP(nx + 1, j, k)] = a[IP(nx, j, k)];
+            }
+        }
+    for (k = 0; k <= nz + 1; k++)
+        for (i = 0; i <= nx + 1; i++) {
+            if (g->ccy0 == 0) {
+                a[IP(i, 0, k)] = a[IP(i, ny, k)];
+                a[IP(i, ny + 1, k)] = a[IP(i, 1, k)];
+            } else {
+                a[IP(i, 0, k)] = a[IP(i, 1, k)];
+                a[IP(i, ny + 1, k)] = a[IP(i, ny, k)];
+            }
+        }
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) {
+            a[IP(i, j, 0)] = a[IP(i, j, 1)];
+            a[IP(i, j, nz + 1)] = a[IP(i, j, nz)];
+        }
+}
+
+/* Sum of a[c]*b[c] over interior cells.  Serial k-j-i order (omp_t = 0,
+ * 9_grad.f90:422-433, :482-493) or an OpenMP reduction (omp_t = 1,
+ * :436-447, :496-507). */
+static double dot_interior(const oracle_grid *g, const double *a, const double *b, int serial)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int nt = nthreads(g);
+    double s = 0.0;
+    int i, j, k;
+    if (serial || nt <= 1) {
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    s = s + a[IP(i, j, k)] * b[IP(i, j, k)];
+    } else {
+#pragma omp parallel for num_threads(nt) private(i, j) reduction(+ : s)
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    s = s + a[IP(i, j, k)] * b[IP(i, j, k)];
+    }
+    return s;
+}
+
+/* Everything graddin computes before the CG loop that tests want to see:
+ * face coefficients, diagonal with ghosts, q = -div u (9_grad.f90:45-239). */
+static void build_unscaled(const oracle_grid *g, const double *u, const double *v, const double *w,
+                           const double *rho, double *sx, double *sy, double *sz, double *d, double *q)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int nt = nthreads(g);
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz,
+                 nfz = (size_t)nx * ny * (nz + 1);
+    int i, j, k;
+
+    /* IBM: zero face velocities inside obstacles (9_grad.f90:45-88) */
+    double *u1 = NULL, *v1 = NULL, *w1 = NULL;
+    if (g->ibm_mask) {
+        u1 = malloc(nfx * sizeof(double));
+        v1 = malloc(nfy * sizeof(double));
+        w1 = malloc(nfz * sizeof(double));
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx + 1; i++)
+                    u1[IFX(i, j, k)] = g->obsx[IFX(i, j, k)] >= 0.0 ? 0.0 : u[IU(i, j, k)];
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny + 1; j++)
+                for (i = 1; i <= nx; i++)
+                    v1[IFY(i, j, k)] = g->obsy[IFY(i, j, k)] >= 0.0 ? 0.0 : v[IV(i, j, k)];
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+        for (k = 1; k <= nz + 1; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    w1[IFZ(i, j, k)] = g->obsz[IFZ(i, j, k)] >= 0.0 ? 0.0 : w[IW(i, j, k)];
+    }
+
+    /* aux = dt / delta^2 (9_grad.f90:93-100) */
+    const double aux1 = g->dt / (g->dx * g->dx);
+    const double aux2 = g->dt / (g->dy * g->dy);
+    const double aux3 = g->dt / (g->dz * g->dz);
+
+    /* face densities (9_grad.f90:103-116) */
+    double *rhox = malloc(nfx * sizeof(double));
+    double *rhoy = malloc(nfy * sizeof(double));
+    double *rhoz = malloc(nfz * sizeof(double));
+    if (g->t_press == 1) {
+        oracle_interp_faces(g, rho, rhox, rhoy, rhoz);
+    } else {
+        for (size_t c = 0; c < nfx; c++) rhox[c] = g->rho_f1;
+        for (size_t c = 0; c < nfy; c++) rhoy[c] = g->rho_f1;
+        for (size_t c = 0; c < nfz; c++) rhoz[c] = g->rho_f1;
+    }
+
+    /* face coefficients (9_grad.f90:118-148; SuLi-ICdz/9_grad.f90:141) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                sx[IFX(i, j, k)] = aux1 / rhox[IFX(i, j, k)];
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                sy[IFY(i, j, k)] = aux2 / rhoy[IFY(i, j, k)];
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                if (g->dzv)
+                    sz[IFZ(i, j, k)] = g->dt / (g->dzv[k] * g->dzv[k] * rhoz[IFZ(i, j, k)]);
+                else
+                    sz[IFZ(i, j, k)] = aux3 / rhoz[IFZ(i, j, k)];
+            }
+    free(rhox);
+    free(rhoy);
+    free(rhoz);
+
+    /* diagonal (9_grad.f90:152-160) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                d[IP(i, j, k)] = sx[IFX(i + 1, j, k)] + sx[IFX(i, j, k)] + sy[IFY(i, j + 1, k)] +
+                                 sy[IFY(i, j, k)] + sz[IFZ(i, j, k + 1)] + sz[IFZ(i, j, k)];
+
+    /* q = -div u (9_grad.f90:163-183; SuLi-ICdz/9_grad.f90:166,176) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                double du, dv, dw;
+                if (g->ibm_mask) {
+                    du = u1[IFX(i, j, k)] - u1[IFX(i + 1, j, k)];
+                    dv = v1[IFY(i, j, k)] - v1[IFY(i, j + 1, k)];
+                    dw = w1[IFZ(i, j, k)] - w1[IFZ(i, j, k + 1)];
+                } else {
+                    du = u[IU(i, j, k)] - u[IU(i + 1, j, k)];
+                    dv = v[IV(i, j, k)] - v[IV(i, j + 1, k)];
+                    dw = w[IW(i, j, k)] - w[IW(i, j, k + 1)];
+                }
+                const double dzk = g->dzzv ? g->dzzv[k] : g->dz;
+                q[IC(i, j, k)] = du / g->dx + dv / g->dy + dw / dzk;
+            }
+    free(u1);
+    free(v1);
+    free(w1);
+
+    /* ghosts of d (9_grad.f90:188-239) */
+    fill_ghosts(g, d);
+}
+
+/* Symmetric Jacobi scaling of the face coefficients (9_grad.f90:256-288). */
+static void scale_faces(const oracle_grid *g, const double *d, double *sx, double *sy, double *sz)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int nt = nthreads(g);
+    int i, j, k;
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                sx[IFX(i, j, k)] = sx[IFX(i, j, k)] / sqrt(d[IP(i, j, k)] * d[IP(i - 1, j, k)]);
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                sy[IFY(i, j, k)] = sy[IFY(i, j, k)] / sqrt(d[IP(i, j, k)] * d[IP(i, j - 1, k)]);
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                sz[IFZ(i, j, k)] = sz[IFZ(i, j, k)] / sqrt(d[IP(i, j, k)] * d[IP(i, j, k - 1)]);
+}
+
+void oracle_scaled_operator(const oracle_grid *g, const double *u, const double *v, const double *w,
+                            const double *rho, double *shx, double *shy, double *shz, double *d,
+                            double *q, double *b)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    memset(d, 0, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
+    build_unscaled(g, u, v, w, rho, shx, shy, shz, d, q);
+    scale_faces(g, d, shx, shy, shz);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                b[IC(i, j, k)] = q[IC(i, j, k)] / sqrt(d[IP(i, j, k)]);
+}
+
+int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const double *w,
+                   const double *rho, double *prd1, oracle_stats *st)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    if (nx < 2 || ny < 2 || nz < 2) return -1;
+    const int nt = nthreads(g);
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz,
+                 nfz = (size_t)nx * ny * (nz + 1), ncell = (size_t)nx * ny * nz;
+    int i, j, k;
+
+    double *sx = malloc(nfx * sizeof(double));
+    double *sy = malloc(nfy * sizeof(double));
+    double *sz = malloc(nfz * sizeof(double));
+    double *q = malloc(ncell * sizeof(double));
+    double *d = calloc(nbox, sizeof(double));
+    double *e = calloc(nbox, sizeof(double));
+    double *r = calloc(nbox, sizeof(double));
+    double *p = calloc(nbox, sizeof(double));
+    double *mp = calloc(nbox, sizeof(double));
+    if (!sx || !sy || !sz || !q || !d || !e || !r || !p || !mp) {
+        free(sx); free(sy); free(sz); free(q); free(d); free(e); free(r); free(p); free(mp);
+        return -2;
+    }
+
+    build_unscaled(g, u, v, w, rho, sx, sy, sz, d, q);
+
+    /* e = prd1 * sqrt(d) on the whole box (9_grad.f90:242-252) */
+#pragma omp parallel for num_threads(nt) if (nt > 1)
+    for (size_t c = 0; c < nbox; c++) e[c] = prd1[c] * sqrt(d[c]);
+
+    scale_faces(g, d, sx, sy, sz);
+
+    /* initial residual r = A e - q/sqrt(d) (9_grad.f90:290-301) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                r[IP(i, j, k)] = e[IP(i, j, k)] - sx[IFX(i + 1, j, k)] * e[IP(i + 1, j, k)] -
+                                 sx[IFX(i, j, k)] * e[IP(i - 1, j, k)] - sy[IFY(i, j + 1, k)] * e[IP(i, j + 1, k)] -
+                                 sy[IFY(i, j, k)] * e[IP(i, j - 1, k)] - sz[IFZ(i, j, k + 1)] * e[IP(i, j, k + 1)] -
+                                 sz[IFZ(i, j, k)] * e[IP(i, j, k - 1)] - q[IC(i, j, k)] / sqrt(d[IP(i, j, k)]);
+
+    /* r.r, always serial (9_grad.f90:303-319) */
+    double rr = dot_interior(g, r, r, 1);
+
+    /* b.b for the relative stop rule (not in the reference) */
+    double bb = 0.0;
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double bi = q[IC(i, j, k)] / sqrt(d[IP(i, j, k)]);
+                bb = bb + bi * bi;
+            }
+
+    /* ghosts of r, p = r (9_grad.f90:322-380) */
+    fill_ghosts(g, r);
+    memcpy(p, r, nbox * sizeof(double));
+
+    /* stop rule (9_grad.f90:384-398) */
+    int cont = 0;
+    double aux1 = fabs(rr);
+    double thresh;
+    if (g->stop_rule == ORACLE_STOP_RELATIVE)
+        thresh = g->tol * g->tol * bb;
+    else
+        thresh = 0.0001 / (double)(nx * ny * nz);
+    const int max_iter = g->max_iter > 0 ? g->max_iter : 10000;
+    const int serial = g->omp_t == 0;
+    if (st) st->rr0 = rr;
+
+    /* CG loop (9_grad.f90:398-587) */
+    while (aux1 > thresh && cont < max_iter) {
+        cont = cont + 1;
+
+        /* mp = A p (:408-420) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    mp[IP(i, j, k)] = p[IP(i, j, k)] - p[IP(i + 1, j, k)] * sx[IFX(i + 1, j, k)] -
+                                      p[IP(i - 1, j, k)] * sx[IFX(i, j, k)] - p[IP(i, j + 1, k)] * sy[IFY(i, j + 1, k)] -
+                                      p[IP(i, j - 1, k)] * sy[IFY(i, j, k)] - p[IP(i, j, k + 1)] * sz[IFZ(i, j, k + 1)] -
+                                      p[IP(i, j, k - 1)] * sz[IFZ(i, j, k)];
+
+        /* alpha (:422-455) */
+        const double pap = dot_interior(g, p, mp, serial);
+        const double alpha = rr / pap;
+
+        /* e -= alpha p, r -= alpha A p (:458-480) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++) {
+                    e[IP(i, j, k)] = e[IP(i, j, k)] - alpha * p[IP(i, j, k)];
+                    r[IP(i, j, k)] = r[IP(i, j, k)] - alpha * mp[IP(i, j, k)];
+                }
+
+        /* beta (:482-515) */
+        const double rrn = dot_interior(g, r, r, serial);
+        const double beta = rrn / rr;
+        rr = rrn;
+
+        /* p = r + beta p (:518-527), ghosts (:531-579) */
+#pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    p[IP(i, j, k)] = r[IP(i, j, k)] + beta * p[IP(i, j, k)];
+        fill_ghosts(g, p);
+
+        aux1 = fabs(rr);
+    }
+
+    /* ghosts of e and un-scaling on the whole box (9_grad.f90:594-659) */
+    fill_ghosts(g, e);
+#pragma omp parallel for num_threads(nt) if (nt > 1)
+    for (size_t c = 0; c < nbox; c++) prd1[c] = e[c] / sqrt(d[c]);
+
+    if (st) {
+        st->iters = cont;
+        st->converged = aux1 <= thresh;
+        st->bb = bb;
+        st->rr = rr;
+        st->thresh = thresh;
+    }
+    free(sx); free(sy); free(sz); free(q); free(d); free(e); free(r); free(p); free(mp);
+    return aux1 <= thresh ? 0 : 1;
+}
+
+/* SuLi-IC/10_pos.f90:17-46,95; SuLi-ICdz/10_pos.f90:21,41. */
+void oracle_posdin(const oracle_grid *g, double *u, double *v, double *w, const double *rho,
+                   const double *prd1, double *prd)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz,
+                 nfz = (size_t)nx * ny * (nz + 1);
+    int i, j, k;
+    double *rhox = malloc(nfx * sizeof(double));
+    double *rhoy = malloc(nfy * sizeof(double));
+    double *rhoz = malloc(nfz * sizeof(double));
+    oracle_interp_faces(g, rho, rhox, rhoy, rhoz);
+    const double aux1 = g->dt / g->dx;
+    const double aux2 = g->dt / g->dy;
+    for (k = 1; k <= nz; k++) {
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                u[IU(i, j, k)] = u[IU(i, j, k)] - aux1 * (prd1[IP(i, j, k)] - prd1[IP(i - 1, j, k)]) / rhox[IFX(i, j, k)];
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                v[IV(i, j, k)] = v[IV(i, j, k)] - aux2 * (prd1[IP(i, j, k)] - prd1[IP(i, j - 1, k)]) / rhoy[IFY(i, j, k)];
+    }
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double dzk = g->dzzv ? g->dzzv[k] : g->dz;
+                w[IW(i, j, k)] = w[IW(i, j, k)] - g->dt / dzk * (prd1[IP(i, j, k)] - prd1[IP(i, j, k - 1)]) / rhoz[IFZ(i, j, k)];
+            }
+    memcpy(prd, prd1, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
+    free(rhox);
+    free(rhoy);
+    free(rhoz);
+}
+
+/* SuLi-IC/14_level_set.f90:555-587 (with dx1 = max(dx,dy,dz) from :41 and
+ * eps = alpha1 * dx1 supplied by the caller). */
+void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1, double rho2, double *rho)
+{
+    const double pi = acos(-1.0);
+    for (int c = 0; c < n; c++) {
+        double hs;
+        if (ls[c] < -eps)
+            hs = 0.0;
+        else if (ls[c] > eps)
+            hs = 1.0;
+        else
+            hs = 0.5 * (1.0 + ls[c] / eps + 1.0 / pi * sin(pi * ls[c] / eps));
+        rho[c] = rho1 * (1.0 - hs) + rho2 * hs;
+    }
+}

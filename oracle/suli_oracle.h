@@ -1,0 +1,90 @@
+/*
+ * CPU restatement of SuLi's pressure step (graddin + posdin).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product in suli_amd/ and the host-core CPU baseline in bench.py.  Nothing
+ * in the product links, loads or calls it; only tests/, the smoke check
+ * and bench.py's cpu_baseline leg do.
+ *
+ * All arrays use the reference's Fortran layouts (column-major, x fastest):
+ *   u    (0:nx+2, 0:ny+1, 0:nz+1)      SuLi-IC/0_modules.f90:170
+ *   v    (0:nx+1, 0:ny+2, 0:nz+1)      :171
+ *   w    (0:nx+1, 0:ny+1, 0:nz+2)      :172
+ *   prd1 (0:nx+1, 0:ny+1, 0:nz+1)      :188
+ *   rho  (1:nx, 1:ny, 1:nz)            :189
+ *   obs_lsx (nx+1,ny,nz), obs_lsy (nx,ny+1,nz), obs_lsz (nx,ny,nz+1)  :135-137
+ *   dz, dzz (0:nz+1)                   SuLi-ICdz/0_modules.f90:72-73
+ */
+#ifndef SULI_ORACLE_H
+#define SULI_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_STOP_REFERENCE = 0, ORACLE_STOP_RELATIVE = 1 };
+
+typedef struct oracle_grid {
+    int nx, ny, nz;
+    double dx, dy, dz;      /* dz is used when dzv == NULL (SuLi-IC) */
+    const double *dzv;      /* dz(0:nz+1), SuLi-ICdz; NULL for uniform */
+    const double *dzzv;     /* dzz(0:nz+1), SuLi-ICdz; NULL for uniform */
+    double dt;
+    int ccx0, ccy0;         /* 0 = periodic in that direction */
+    int t_press;            /* 1 = projection, 2 = IPA (faces use rho_f1) */
+    double rho_f1;
+    int ibm_mask;           /* obst_t != 0 && ibm_t == 2 */
+    const double *obsx, *obsy, *obsz;
+    int omp_t;              /* 0 = serial k-j-i dot products, 1 = OpenMP reductions */
+    int stop_rule;          /* ORACLE_STOP_* */
+    double tol;             /* relative tolerance for ORACLE_STOP_RELATIVE */
+    int max_iter;           /* reference: 10000 */
+    int threads;            /* OpenMP threads for the sweeps; <= 0 = runtime default */
+} oracle_grid;
+
+typedef struct oracle_stats {
+    int iters;          /* CG iterations performed (reference `cont`) */
+    int converged;      /* 1 if the stop rule was met before max_iter */
+    double rr0;         /* r.r of the initial scaled residual */
+    double bb;          /* b.b of the scaled right-hand side */
+    double rr;          /* final iterated r.r (reference `alfamupr`) */
+    double thresh;      /* r.r threshold that was used */
+} oracle_stats;
+
+/* SuLi-IC/9_grad.f90:10-663 (SuLi-ICdz/9_grad.f90 when dzv != NULL).
+ * prd1 is read (initial guess, full halo box) and written (full box).
+ * Returns 0 when converged, 1 when max_iter was hit, <0 on bad input. */
+int oracle_graddin(const oracle_grid *g, const double *u, const double *v,
+                   const double *w, const double *rho, double *prd1,
+                   oracle_stats *st);
+
+/* SuLi-IC/10_pos.f90:17-46,95 (t_press == 1 branch): u, v, w updated in
+ * place on every face the reference loops over; prd = prd1 (full box). */
+void oracle_posdin(const oracle_grid *g, double *u, double *v, double *w,
+                   const double *rho, const double *prd1, double *prd);
+
+/* SuLi-IC/8_derivadas.f90:305-357, :381-418, :443-481 (SuLi-ICdz z variant
+ * :450-489 when dzv != NULL).  Outputs are 1-based face arrays laid out as
+ * (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1). */
+void oracle_interp_faces(const oracle_grid *g, const double *rho,
+                         double *rhox, double *rhoy, double *rhoz);
+
+/* Scaled operator pieces, exposed so that tests can assemble the matrix:
+ * face coefficients after symmetric Jacobi scaling (9_grad.f90:256-288),
+ * diagonal d with mirror/periodic ghosts (:152-160, :188-239) and the
+ * scaled right-hand side b = q / sqrt(d) (:297).  Face arrays as in
+ * oracle_interp_faces; d is (0:nx+1,0:ny+1,0:nz+1); b and q are (nx,ny,nz). */
+void oracle_scaled_operator(const oracle_grid *g, const double *u,
+                            const double *v, const double *w,
+                            const double *rho, double *shx, double *shy,
+                            double *shz, double *d, double *q, double *b);
+
+/* Smoothed Heaviside density, SuLi-IC/14_level_set.f90:41,555-587:
+ * rho = rho1 (1-H) + rho2 H with H(ls) over half-width eps. */
+void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1,
+                          double rho2, double *rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""ctypes binding of libsuli_ps.so (the C ABI in include/suli_ps.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsuli_ps.so")
+
+SULI_PS_FP64 = 0
+SULI_PS_MIXED = 1
+SULI_PS_STOP_REFERENCE = 0
+SULI_PS_STOP_RELATIVE = 1
+SULI_PS_OK = 0
+SULI_PS_NOT_CONVERGED = 1
+
+_dp = C.POINTER(C.c_double)
+
+# Every symbol include/suli_ps.h declares.
+EXPORTS = (
+    "suli_ps_abi_version", "suli_ps_create", "suli_ps_destroy", "suli_ps_set_opts",
+    "suli_ps_last_error", "suli_ps_graddin", "suli_ps_posdin", "suli_ps_graddin_dev",
+    "suli_ps_posdin_dev", "suli_ps_dev_alloc", "suli_ps_dev_free", "suli_ps_memcpy_h2d",
+    "suli_ps_memcpy_d2h", "suli_ps_dev_zero", "suli_ps_synchronize", "suli_ps_time_spmv",
+)
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("ccx0", C.c_int32), ("ccy0", C.c_int32),
+        ("t_press", C.c_int32), ("obst_t", C.c_int32), ("ibm_t", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("rho_f1", C.c_double),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
+        ("max_iter", C.c_int32), ("chunk", C.c_int32), ("verbose", C.c_int32),
+        ("timing", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("iters", C.c_int32), ("refreshes", C.c_int32),
+        ("spmv_timed", C.c_int32),
+        ("rr0", C.c_double), ("bb", C.c_double), ("rr", C.c_double), ("thresh", C.c_double),
+        ("rel_true", C.c_double), ("solve_ms", C.c_double), ("spmv_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -m suli_amd.build` "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.suli_ps_abi_version.restype = C.c_int
+    L.suli_ps_create.restype = vp
+    L.suli_ps_create.argtypes = [C.POINTER(Opts)]
+    L.suli_ps_destroy.argtypes = [vp]
+    L.suli_ps_set_opts.argtypes = [vp, C.POINTER(Opts)]
+    L.suli_ps_last_error.restype = C.c_char_p
+    L.suli_ps_last_error.argtypes = [vp]
+    gargs = [vp, C.POINTER(Grid)] + [vp] * 10 + [C.POINTER(Stats)]
+    L.suli_ps_graddin.argtypes = gargs
+    L.suli_ps_graddin_dev.argtypes = gargs
+    pargs = [vp, C.POINTER(Grid)] + [vp] * 8
+    L.suli_ps_posdin.argtypes = pargs
+    L.suli_ps_posdin_dev.argtypes = pargs
+    L.suli_ps_dev_alloc.restype = vp
+    L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]
+    L.suli_ps_dev_free.argtypes = [vp, vp]
+    L.suli_ps_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+    L.suli_ps_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+    L.suli_ps_dev_zero.argtypes = [vp, vp, C.c_size_t]
+    L.suli_ps_synchronize.argtypes = [vp]
+    L.suli_ps_time_spmv.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    _lib = L
+    return L

@@ -1,0 +1,409 @@
+// Once-per-solve kernels: face densities and coefficients, diagonal and
+// right-hand side, symmetric Jacobi scaling, initial residual, un-scaling of
+// the solution back to prd1, and the posdin velocity correction.
+//
+// This file is compiled with -ffp-contract=off: every expression below is
+// evaluated in the reference's operation order with no fused multiply-add,
+// so the operator and the correction are bit-identical to SuLi's FP64
+// arithmetic (SuLi-IC/9_grad.f90:93-301, SuLi-IC/10_pos.f90:17-46,
+// SuLi-IC/8_derivadas.f90:305-481).
+#include "ps_device.h"
+
+namespace sps {
+namespace {
+
+__device__ __forceinline__ long long cidx(const Geo &g, int i, int j, int k)
+{
+    return ((long long)k * g.ny + j) * g.nx + i;
+}
+
+// Face densities, interpx_cf / interpy_cf / interpz_cf
+// (SuLi-IC/8_derivadas.f90:305-357, :381-418, :443-481; dz-weighted z
+// faces: SuLi-ICdz/8_derivadas.f90:450-489).  Face index f is 0-based:
+// face f of a direction lies between cells f-1 and f.  Boundary faces copy
+// the adjacent interior face, except x-periodic where both take the mean of
+// the first and last cell.  y and z have no periodic branch.
+__device__ __forceinline__ double rho_face_x(const Geo &g, const SetupIn &in, int f, int j, int k)
+{
+    const double *r = in.rho + ((long long)k * g.ny + j) * g.nx;
+    const int nx = g.nx;
+    if (f >= 1 && f <= nx - 1) return (r[f] + r[f - 1]) * 0.5;
+    if (!g.px) {
+        const int a = f == 0 ? 1 : nx - 1;
+        return (r[a] + r[a - 1]) * 0.5;
+    }
+    return (r[0] + r[nx - 1]) * 0.5;
+}
+
+__device__ __forceinline__ double rho_face_y(const Geo &g, const SetupIn &in, int i, int f, int k)
+{
+    const double *r = in.rho + (long long)k * g.ny * g.nx + i;
+    const int ny = g.ny;
+    const int a = f == 0 ? 1 : (f == ny ? ny - 1 : f);
+    return (r[(long long)a * g.nx] + r[(long long)(a - 1) * g.nx]) * 0.5;
+}
+
+__device__ __forceinline__ double rho_face_z(const Geo &g, const SetupIn &in, int i, int j, int f)
+{
+    const long long plane = (long long)g.nx * g.ny;
+    const double *r = in.rho + (long long)j * g.nx + i;
+    const int nz = g.nz;
+    const int a = f == 0 ? 1 : (f == nz ? nz - 1 : f);
+    if (in.dz) {
+        // Fortran face a+1 between cells a and a+1 (1-based): weights dz(a+1), dz(a)
+        const double d1 = in.dz[a + 1], d0 = in.dz[a];
+        return (r[a * plane] * d1 + r[(a - 1) * plane] * d0) / (d1 + d0);
+    }
+    return (r[a * plane] + r[(a - 1) * plane]) * 0.5;
+}
+
+// s = dt/(delta^2 rho_face) (9_grad.f90:118-148; SuLi-ICdz/9_grad.f90:141)
+__global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, double *sx, double *sy, double *sz)
+{
+    const long long nfx = (long long)(g.nx + 1) * g.ny * g.nz;
+    const long long nfy = (long long)g.nx * (g.ny + 1) * g.nz;
+    const long long nfz = (long long)g.nx * g.ny * (g.nz + 1);
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
+        if (t < nfx) {
+            const int f = (int)(t % (g.nx + 1));
+            const long long jk = t / (g.nx + 1);
+            const int j = (int)(jk % g.ny), k = (int)(jk / g.ny);
+            const double rf = in.ipa ? in.rho_f1 : rho_face_x(g, in, f, j, k);
+            sx[t] = in.aux1 / rf;
+        } else if (t < nfx + nfy) {
+            const long long u = t - nfx;
+            const int i = (int)(u % g.nx);
+            const long long jk = u / g.nx;
+            const int f = (int)(jk % (g.ny + 1)), k = (int)(jk / (g.ny + 1));
+            const double rf = in.ipa ? in.rho_f1 : rho_face_y(g, in, i, f, k);
+            sy[u] = in.aux2 / rf;
+        } else {
+            const long long u = t - nfx - nfy;
+            const int i = (int)(u % g.nx);
+            const long long jk = u / g.nx;
+            const int j = (int)(jk % g.ny), f = (int)(jk / g.ny);
+            const double rf = in.ipa ? in.rho_f1 : rho_face_z(g, in, i, j, f);
+            if (in.dz) {
+                const double dzk = in.dz[f + 1];
+                sz[u] = in.dt / (dzk * dzk * rf);
+            } else {
+                sz[u] = in.aux3 / rf;
+            }
+        }
+    }
+}
+
+// d = sum of the six face coefficients (9_grad.f90:152-160) and the scaled
+// right-hand side b = q/sqrt(d) with q = -div u (9_grad.f90:163-183,297;
+// SuLi-ICdz/9_grad.f90:166,176; IBM face mask 9_grad.f90:45-88).
+__global__ __launch_bounds__(kThreads) void k_diag_rhs(Geo g, SetupIn in, const double *sx, const double *sy,
+                                                       const double *sz, double *d, double *b)
+{
+    const int nx = g.nx, ny = g.ny;
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
+        const int i = (int)(c % nx);
+        const long long jk = c / nx;
+        const int j = (int)(jk % ny), k = (int)(jk / ny);
+        const long long fx = ((long long)k * ny + j) * (nx + 1) + i;
+        const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
+        const long long fz = ((long long)k * ny + j) * nx + i;
+        const long long pl = (long long)nx * ny;
+        const double dd = sx[fx + 1] + sx[fx] + sy[fy + nx] + sy[fy] + sz[fz + pl] + sz[fz];
+        d[c] = dd;
+
+        const int I = i + 1, J = j + 1, K = k + 1;
+        const long long iu = I + (long long)(nx + 3) * (J + (long long)(ny + 2) * K);
+        const long long iv = I + (long long)(nx + 2) * (J + (long long)(ny + 3) * K);
+        const long long iw = I + (long long)(nx + 2) * (J + (long long)(ny + 2) * K);
+        double u0 = in.u[iu], u1 = in.u[iu + 1];
+        double v0 = in.v[iv], v1 = in.v[iv + (nx + 2)];
+        double w0 = in.w[iw], w1 = in.w[iw + (long long)(nx + 2) * (ny + 2)];
+        if (in.obsx) {
+            if (in.obsx[fx] >= 0.0) u0 = 0.0;
+            if (in.obsx[fx + 1] >= 0.0) u1 = 0.0;
+            if (in.obsy[fy] >= 0.0) v0 = 0.0;
+            if (in.obsy[fy + nx] >= 0.0) v1 = 0.0;
+            if (in.obsz[fz] >= 0.0) w0 = 0.0;
+            if (in.obsz[fz + pl] >= 0.0) w1 = 0.0;
+        }
+        const double dzk = in.dzz ? in.dzz[K] : in.dzu;
+        const double q = (u0 - u1) / in.dx + (v0 - v1) / in.dy + (w0 - w1) / dzk;
+        b[c] = q / sqrt(dd);
+    }
+}
+
+// s_hat = s / sqrt(d_a d_b) with the ghost diagonal mirrored or periodic
+// (9_grad.f90:188-239 ghosts, :256-288 scaling).  Also writes FP32 copies
+// for the mixed-precision inner solve when requested.
+__global__ __launch_bounds__(kThreads) void k_scale_faces(Geo g, const double *d, double *sx, double *sy,
+                                                          double *sz, float *sx32, float *sy32, float *sz32)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long nfx = (long long)(nx + 1) * ny * nz;
+    const long long nfy = (long long)nx * (ny + 1) * nz;
+    const long long nfz = (long long)nx * ny * (nz + 1);
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
+        if (t < nfx) {
+            const int f = (int)(t % (nx + 1));
+            const long long jk = t / (nx + 1);
+            const int j = (int)(jk % ny), k = (int)(jk / ny);
+            const double da = d[cidx(g, box_to_cell(f + 1, nx, g.px), j, k)];
+            const double db = d[cidx(g, box_to_cell(f, nx, g.px), j, k)];
+            const double s = sx[t] / sqrt(da * db);
+            sx[t] = s;
+            if (sx32) sx32[t] = (float)s;
+        } else if (t < nfx + nfy) {
+            const long long u = t - nfx;
+            const int i = (int)(u % nx);
+            const long long jk = u / nx;
+            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
+            const double da = d[cidx(g, i, box_to_cell(f + 1, ny, g.py), k)];
+            const double db = d[cidx(g, i, box_to_cell(f, ny, g.py), k)];
+            const double s = sy[u] / sqrt(da * db);
+            sy[u] = s;
+            if (sy32) sy32[u] = (float)s;
+        } else {
+            const long long u = t - nfx - nfy;
+            const int i = (int)(u % nx);
+            const long long jk = u / nx;
+            const int j = (int)(jk % ny), f = (int)(jk / ny);
+            const double da = d[cidx(g, i, j, box_to_cell(f + 1, nz, 0))];
+            const double db = d[cidx(g, i, j, box_to_cell(f, nz, 0))];
+            const double s = sz[u] / sqrt(da * db);
+            sz[u] = s;
+            if (sz32) sz32[u] = (float)s;
+        }
+    }
+}
+
+// Scaled unknown e = prd1 sqrt(d) at Fortran box index (I, J, K), ghosts of
+// d taken from the interior by the mirror/periodic rule (9_grad.f90:242-252).
+__device__ __forceinline__ double e_box(const Geo &g, const double *prd1, const double *d, int I, int J, int K)
+{
+    const double p = prd1[I + (long long)(g.nx + 2) * (J + (long long)(g.ny + 2) * K)];
+    const double dd = d[cidx(g, box_to_cell(I, g.nx, g.px), box_to_cell(J, g.ny, g.py), box_to_cell(K, g.nz, 0))];
+    return p * sqrt(dd);
+}
+
+// Initial residual r = A e - b with the caller's prd1 ghost values
+// (9_grad.f90:290-301) and the first CG direction p = r (:372-380).
+// b is replaced by b_eff = b + (A_mirror e - A_ghosts e): the right-hand
+// side under which later FP64 residual recomputations (which use mirrored
+// ghosts, as every CG iteration does) reproduce this residual.  With
+// mirror-consistent prd1 ghosts, b_eff == b bit for bit.
+__global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, const double *sx, const double *sy,
+                                                            const double *sz, const double *d, double *b, double *e,
+                                                            double *r64, double *p64, float *r32, float *p32,
+                                                            float *c32, double *partial, CgScalars *S, int rel_rule,
+                                                            double tol, double thresh_ref)
+{
+    const int nx = g.nx, ny = g.ny;
+    const long long stride = (long long)gridDim.x * kThreads;
+    double acc_rr = 0.0, acc_bb = 0.0, acc_rs = 0.0;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
+        const int i = (int)(c % nx);
+        const long long jk = c / nx;
+        const int j = (int)(jk % ny), k = (int)(jk / ny);
+        const int I = i + 1, J = j + 1, K = k + 1;
+        const long long fx = ((long long)k * ny + j) * (nx + 1) + i;
+        const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
+        const long long fz = ((long long)k * ny + j) * nx + i;
+        const long long pl = (long long)nx * ny;
+        const double ec = e_box(g, in.prd1, d, I, J, K);
+        // neighbours with the caller's ghosts
+        const double eE = e_box(g, in.prd1, d, I + 1, J, K), eW = e_box(g, in.prd1, d, I - 1, J, K);
+        const double eN = e_box(g, in.prd1, d, I, J + 1, K), eS = e_box(g, in.prd1, d, I, J - 1, K);
+        const double eT = e_box(g, in.prd1, d, I, J, K + 1), eB = e_box(g, in.prd1, d, I, J, K - 1);
+        const double a_in = ec - sx[fx + 1] * eE - sx[fx] * eW - sy[fy + nx] * eN - sy[fy] * eS -
+                            sz[fz + pl] * eT - sz[fz] * eB;
+        // neighbours with mirrored/periodic ghosts
+        const int iE = box_to_cell(I + 1, nx, g.px) + 1, iW = box_to_cell(I - 1, nx, g.px) + 1;
+        const int jN = box_to_cell(J + 1, ny, g.py) + 1, jS = box_to_cell(J - 1, ny, g.py) + 1;
+        const int kT = box_to_cell(K + 1, g.nz, 0) + 1, kB = box_to_cell(K - 1, g.nz, 0) + 1;
+        const double mE = e_box(g, in.prd1, d, iE, J, K), mW = e_box(g, in.prd1, d, iW, J, K);
+        const double mN = e_box(g, in.prd1, d, I, jN, K), mS = e_box(g, in.prd1, d, I, jS, K);
+        const double mT = e_box(g, in.prd1, d, I, J, kT), mB = e_box(g, in.prd1, d, I, J, kB);
+        const double a_mir = ec - sx[fx + 1] * mE - sx[fx] * mW - sy[fy + nx] * mN - sy[fy] * mS -
+                             sz[fz + pl] * mT - sz[fz] * mB;
+        const double bc = b[c];
+        const double r = a_in - bc;
+        b[c] = bc + (a_mir - a_in);
+        e[c] = ec;
+        acc_rr += r * r;
+        acc_bb += bc * bc;
+        if (r64) {
+            r64[c] = r;
+            p64[c] = r;
+        } else {
+            const float rf = (float)r;
+            r32[c] = rf;
+            p32[c] = rf;
+            c32[c] = 0.0f;
+            acc_rs += (double)rf * (double)rf;
+        }
+    }
+    const double s0 = block_sum(acc_rr);
+    const double s1 = block_sum(acc_bb);
+    const double s2 = block_sum(acc_rs);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        __hip_atomic_store(partial + blockIdx.x, s0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(partial + nb + blockIdx.x, s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(partial + 2 * nb + blockIdx.x, s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        last = atomicAdd(&S->ticket[kTkInit], 1u) == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int nb = gridDim.x;
+    const double rr = sum_partials(partial, nb);
+    const double bb = sum_partials(partial + nb, nb);
+    const double rs = sum_partials(partial + 2 * nb, nb);
+    if (threadIdx.x == 0) {
+        const double thresh = rel_rule ? tol * tol * bb : thresh_ref;
+        S->rr0 = rr;
+        S->rr_true = rr;
+        S->rr = S->mixed ? rs : rr;
+        S->bb = bb;
+        S->thresh = thresh;
+        S->iters = 0;
+        S->k2n = 0;
+        S->k3n = 0;
+        S->refreshes = 0;
+        S->refresh_mark = 0;
+        S->status = 0;
+        // reference loop test: while (|r.r| > thresh .and. cont < max) (9_grad.f90:398)
+        const bool done = !(fabs(rr) > thresh) || S->max_iter <= 0;
+        S->stop = done ? 1 : 0;
+        S->final_ = done ? 1 : 0;
+        S->status = (!(fabs(rr) > thresh)) ? 0 : 1;
+        S->ticket[kTkInit] = 0u;
+    }
+}
+
+// prd1 = e / sqrt(d) on the whole halo box; ghosts copy the interior by the
+// mirror/periodic rule in x, then y, then z order (9_grad.f90:594-659).
+__global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, const double *d, double *prd1)
+{
+    const int bx = g.nx + 2, by = g.ny + 2;
+    const long long nbox = (long long)bx * by * (g.nz + 2);
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nbox; t += stride) {
+        const int I = (int)(t % bx);
+        const long long jk = t / bx;
+        const int J = (int)(jk % by), K = (int)(jk / by);
+        const long long c = cidx(g, box_to_cell(I, g.nx, g.px), box_to_cell(J, g.ny, g.py), box_to_cell(K, g.nz, 0));
+        prd1[t] = e[c] / sqrt(d[c]);
+    }
+}
+
+// posdin, projection branch (SuLi-IC/10_pos.f90:17-46; SuLi-ICdz/10_pos.f90:41):
+//   u(i,j,k) -= dt/dx (p(i)-p(i-1)) / rhox,   i = 1..nx+1, j = 1..ny, k = 1..nz
+//   v(i,j,k) -= dt/dy (p(j)-p(j-1)) / rhoy,   j = 1..ny+1
+//   w(i,j,k) -= dt/dz (p(k)-p(k-1)) / rhoz,   k = 1..nz+1 (dz -> dzz(k) on stretched grids)
+__global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long nfx = (long long)(nx + 1) * ny * nz;
+    const long long nfy = (long long)nx * (ny + 1) * nz;
+    const long long nfz = (long long)nx * ny * (nz + 1);
+    const long long stride = (long long)gridDim.x * kThreads;
+    const double aux1 = in.dt / in.dx, aux2 = in.dt / in.dy;
+    const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
+        if (t < nfx) {
+            const int f = (int)(t % (nx + 1));
+            const long long jk = t / (nx + 1);
+            const int j = (int)(jk % ny), k = (int)(jk / ny);
+            const int I = f + 1, J = j + 1, K = k + 1;
+            const long long ip = I + pb * J + pbb * K;
+            const long long iu = I + (long long)(nx + 3) * (J + (long long)(ny + 2) * K);
+            const double rf = rho_face_x(g, in, f, j, k);
+            u[iu] = u[iu] - aux1 * (in.prd1[ip] - in.prd1[ip - 1]) / rf;
+        } else if (t < nfx + nfy) {
+            const long long s = t - nfx;
+            const int i = (int)(s % nx);
+            const long long jk = s / nx;
+            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
+            const int I = i + 1, J = f + 1, K = k + 1;
+            const long long ip = I + pb * J + pbb * K;
+            const long long iv = I + (long long)(nx + 2) * (J + (long long)(ny + 3) * K);
+            const double rf = rho_face_y(g, in, i, f, k);
+            v[iv] = v[iv] - aux2 * (in.prd1[ip] - in.prd1[ip - pb]) / rf;
+        } else {
+            const long long s = t - nfx - nfy;
+            const int i = (int)(s % nx);
+            const long long jk = s / nx;
+            const int j = (int)(jk % ny), f = (int)(jk / ny);
+            const int I = i + 1, J = j + 1, K = f + 1;
+            const long long ip = I + pb * J + pbb * K;
+            const long long iw = I + (long long)(nx + 2) * (J + (long long)(ny + 2) * K);
+            const double rf = rho_face_z(g, in, i, j, f);
+            const double dzk = in.dzz ? in.dzz[K] : in.dzu;
+            w[iw] = w[iw] - in.dt / dzk * (in.prd1[ip] - in.prd1[ip - pbb]) / rf;
+        }
+    }
+}
+
+int grid_for(long long n)
+{
+    long long b = (n + kThreads - 1) / kThreads;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+}  // namespace
+
+void launch_face_coefs(const Geo &g, const SetupIn &in, double *sx, double *sy, double *sz, hipStream_t s)
+{
+    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
+                         (long long)g.nx * g.ny * (g.nz + 1);
+    hipLaunchKernelGGL(k_face_coefs, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, sx, sy, sz);
+}
+
+void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
+                     double *d, double *b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_diag_rhs, dim3(grid_for(g.n)), dim3(kThreads), 0, s, g, in, sx, sy, sz, d, b);
+}
+
+void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
+                        float *sy32, float *sz32, hipStream_t s)
+{
+    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
+                         (long long)g.nx * g.ny * (g.nz + 1);
+    hipLaunchKernelGGL(k_scale_faces, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, d, sx, sy, sz, sx32, sy32,
+                       sz32);
+}
+
+void launch_init_residual(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
+                          const double *d, double *b, double *e, double *r64, double *p64, float *r32, float *p32,
+                          float *c32, double *partial, double * /*partial2*/, int nblk, CgScalars *S, int rel_rule,
+                          double tol, double thresh_ref, hipStream_t s)
+{
+    int nb = grid_for(g.n);
+    if (nb > nblk) nb = nblk;
+    hipLaunchKernelGGL(k_init_residual, dim3(nb), dim3(kThreads), 0, s, g, in, sx, sy, sz, d, b, e, r64, p64,
+                       r32, p32, c32, partial, S, rel_rule, tol, thresh_ref);
+}
+
+void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, hipStream_t s)
+{
+    const long long nbox = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1);
+}
+
+void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s)
+{
+    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
+                         (long long)g.nx * g.ny * (g.nz + 1);
+    hipLaunchKernelGGL(k_posdin, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, u, v, w);
+}
+
+}  // namespace sps

@@ -1,0 +1,187 @@
+"""Python mirror of SuLi's pressure step on the MI355X library.
+
+``PressureSolver.graddin(grid, fields)`` and ``PressureSolver.posdin(grid,
+fields)`` do what SuLi's argument-less ``graddin()`` and ``posdin()`` do
+(SuLi-IC/9_grad.f90:10-663, SuLi-IC/10_pos.f90:1-99) to the module arrays,
+here passed explicitly as a :class:`suli_amd.cases.Grid` and
+:class:`suli_amd.cases.Fields` in the reference layouts.  Every call goes
+through the C ABI of libsuli_ps.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .cases import Fields, Grid
+
+
+class SuliPsError(RuntimeError):
+    pass
+
+
+def _c_grid(g: Grid) -> _lib.Grid:
+    cg = _lib.Grid()
+    cg.nx, cg.ny, cg.nz = g.nx, g.ny, g.nz
+    cg.ccx0, cg.ccy0 = g.ccx0, g.ccy0
+    cg.t_press, cg.obst_t, cg.ibm_t = g.t_press, g.obst_t, g.ibm_t
+    cg.dx, cg.dy, cg.dz, cg.dt = g.dx, g.dy, g.dz, g.dt
+    cg.rho_f1 = g.rho_f1
+    return cg
+
+
+def _host_ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim == 1):
+        raise ValueError("arrays must be float64 in Fortran order")
+    return C.c_void_p(a.ctypes.data)
+
+
+PRECISIONS = {"fp64": _lib.SULI_PS_FP64, "mixed": _lib.SULI_PS_MIXED}
+STOP_RULES = {"reference": _lib.SULI_PS_STOP_REFERENCE, "relative": _lib.SULI_PS_STOP_RELATIVE}
+
+
+class DeviceFields:
+    """Device copies of the pressure-step arrays (reference layouts)."""
+
+    NAMES = ("u", "v", "w", "rho", "prd1", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz")
+
+    def __init__(self, solver: "PressureSolver", grid: Grid, fields: Fields):
+        self.solver = solver
+        self.shapes = {}
+        self.ptrs = {}
+        host = dict(u=fields.u, v=fields.v, w=fields.w, rho=fields.rho, prd1=fields.prd1,
+                    obs_lsx=fields.obs_lsx, obs_lsy=fields.obs_lsy, obs_lsz=fields.obs_lsz,
+                    dz=grid.dzv, dzz=grid.dzzv)
+        for name, arr in host.items():
+            if arr is None:
+                self.ptrs[name] = None
+                continue
+            arr = np.asfortranarray(arr, dtype=np.float64)
+            p = solver.lib.suli_ps_dev_alloc(solver.ctx, arr.nbytes)
+            if not p:
+                self.free()
+                raise SuliPsError(f"device allocation of {name} ({arr.nbytes} B) failed")
+            self.ptrs[name] = C.c_void_p(p)
+            self.shapes[name] = arr.shape
+            solver._check(solver.lib.suli_ps_memcpy_h2d(solver.ctx, self.ptrs[name],
+                                                         C.c_void_p(arr.ctypes.data), arr.nbytes))
+
+    def download(self, name: str) -> np.ndarray:
+        out = np.zeros(self.shapes[name], dtype=np.float64, order="F")
+        self.solver._check(self.solver.lib.suli_ps_memcpy_d2h(
+            self.solver.ctx, C.c_void_p(out.ctypes.data), self.ptrs[name], out.nbytes))
+        return out
+
+    def upload(self, name: str, arr: np.ndarray) -> None:
+        arr = np.asfortranarray(arr, dtype=np.float64)
+        assert arr.shape == self.shapes[name]
+        self.solver._check(self.solver.lib.suli_ps_memcpy_h2d(
+            self.solver.ctx, self.ptrs[name], C.c_void_p(arr.ctypes.data), arr.nbytes))
+
+    def zero(self, name: str) -> None:
+        """Asynchronously zero a device array (e.g. prd1 for a cold start)."""
+        nbytes = int(np.prod(self.shapes[name])) * 8
+        self.solver._check(self.solver.lib.suli_ps_dev_zero(self.solver.ctx, self.ptrs[name], nbytes))
+
+    def free(self) -> None:
+        for name, p in list(self.ptrs.items()):
+            if p is not None:
+                self.solver.lib.suli_ps_dev_free(self.solver.ctx, p)
+            self.ptrs[name] = None
+
+
+class PressureSolver:
+    def __init__(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
+                 max_iter: int = 10000, chunk: int = 0, verbose: bool = True, timing: bool = False):
+        self.lib = _lib.load()
+        self.opts = _lib.Opts()
+        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing)
+        self.ctx = C.c_void_p(self.lib.suli_ps_create(C.byref(self.opts)))
+        if not self.ctx:
+            raise SuliPsError("suli_ps_create failed (no HIP device?)")
+
+    def _set(self, precision, stop_rule, tol, max_iter, chunk, verbose, timing):
+        self.opts.precision = PRECISIONS[precision]
+        self.opts.stop_rule = STOP_RULES[stop_rule]
+        self.opts.tol = tol
+        self.opts.max_iter = max_iter
+        self.opts.chunk = chunk
+        self.opts.verbose = 1 if verbose else 0
+        self.opts.timing = 1 if timing else 0
+
+    def configure(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
+                  max_iter: int = 10000, chunk: int = 0, verbose: bool = True,
+                  timing: bool = False) -> None:
+        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing)
+        self._check(self.lib.suli_ps_set_opts(self.ctx, C.byref(self.opts)))
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            msg = self.lib.suli_ps_last_error(self.ctx)
+            raise SuliPsError(f"suli_ps error {rc}: {msg.decode() if msg else ''}")
+        return rc
+
+    # -- host arrays --------------------------------------------------------
+    def graddin(self, grid: Grid, fields: Fields) -> _lib.Stats:
+        """Solve for fields.prd1 in place (whole halo box)."""
+        st = _lib.Stats()
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_graddin(
+            self.ctx, C.byref(cg), _host_ptr(fields.u), _host_ptr(fields.v), _host_ptr(fields.w),
+            _host_ptr(fields.rho), _host_ptr(fields.obs_lsx), _host_ptr(fields.obs_lsy),
+            _host_ptr(fields.obs_lsz), _host_ptr(grid.dzv), _host_ptr(grid.dzzv),
+            _host_ptr(fields.prd1), C.byref(st)))
+        return st
+
+    def posdin(self, grid: Grid, fields: Fields) -> np.ndarray:
+        """Correct fields.u/v/w in place; returns prd (= prd1)."""
+        prd = np.zeros_like(fields.prd1, order="F")
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_posdin(
+            self.ctx, C.byref(cg), _host_ptr(fields.u), _host_ptr(fields.v), _host_ptr(fields.w),
+            _host_ptr(fields.rho), _host_ptr(fields.prd1), _host_ptr(grid.dzv),
+            _host_ptr(grid.dzzv), _host_ptr(prd)))
+        return prd
+
+    # -- device-resident arrays ---------------------------------------------
+    def upload(self, grid: Grid, fields: Fields) -> DeviceFields:
+        return DeviceFields(self, grid, fields)
+
+    def graddin_dev(self, grid: Grid, df: DeviceFields) -> _lib.Stats:
+        st = _lib.Stats()
+        cg = _c_grid(grid)
+        p = df.ptrs
+        self._check(self.lib.suli_ps_graddin_dev(
+            self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["obs_lsx"], p["obs_lsy"],
+            p["obs_lsz"], p["dz"], p["dzz"], p["prd1"], C.byref(st)))
+        return st
+
+    def posdin_dev(self, grid: Grid, df: DeviceFields) -> None:
+        cg = _c_grid(grid)
+        p = df.ptrs
+        self._check(self.lib.suli_ps_posdin_dev(
+            self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["prd1"], p["dz"], p["dzz"],
+            None))
+
+    def time_spmv(self, precision: str = "fp64", reps: int = 50) -> float:
+        ms = C.c_double()
+        self._check(self.lib.suli_ps_time_spmv(self.ctx, PRECISIONS[precision], reps, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self) -> None:
+        self._check(self.lib.suli_ps_synchronize(self.ctx))
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.suli_ps_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,34 @@
+"""The Fortran drop-in (suli_amd/fortran/suli_ps_graddin.F90) inside a
+SuLi-style program: graddin()/posdin() through iso_c_binding on the GPU
+versus the reference's own graddin/posdin on the same module arrays.
+The check programs are built by `make -C oracle dropin` where the
+reference sources are present."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("variant,args", [
+    ("ic", ["24", "16", "12", "0"]),
+    ("ic", ["33", "20", "14", "1"]),
+    ("icdz", ["24", "16", "12", "0"]),
+])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fortran_dropin_matches_reference(variant, args, precision):
+    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{variant}")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in check program not built (needs the reference sources)")
+    env = dict(os.environ, SULI_PS_PRECISION=precision)
+    if precision == "mixed":
+        # the reference rule on these inputs is ~1e-11 relative; mixed
+        # precision is asked for 1e-11 relative explicitly
+        env["SULI_PS_TOL"] = "1e-11"
+    print(f"dropin {variant} {args} {precision}", flush=True)
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok =  T" in r.stdout

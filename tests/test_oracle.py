@@ -1,0 +1,318 @@
+"""Pins the CPU oracle (oracle/suli_oracle.c) before it is trusted.
+
+* G: golden vectors from the reference's own compiled Fortran (tests/golden,
+  written by make_golden.py) -- bit for bit.
+* R: the compiled reference itself (oracle/_ref), when present, on fresh
+  random inputs -- bit for bit.
+* C: closed forms (separable cosine modes of the constant-density operator)
+  and brute force (dense assembly from first principles + scipy).
+* I: invariants (linearity, conserved weighted mean, divergence identity,
+  wall faces untouched by posdin, symmetry).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle
+from suli_amd import cases
+from tests import helpers
+
+
+# ---------------------------------------------------------------- G ------
+def test_golden_manifest():
+    with open(os.path.join(helpers.GOLDEN, "MANIFEST.json")) as fh:
+        manifest = json.load(fh)
+    assert sorted(manifest) == [n + ".npz" for n in helpers.golden_names()]
+    for name, digest in manifest.items():
+        with open(os.path.join(helpers.GOLDEN, name), "rb") as fh:
+            assert hashlib.sha256(fh.read()).hexdigest() == digest, name
+
+
+@pytest.mark.parametrize("name", helpers.golden_names())
+def test_oracle_matches_reference_golden(name):
+    g, f, exp = helpers.load_golden(name)
+    prd1, st, rc = oracle.graddin(g, f)
+    assert rc == 0 and st.converged
+    mask = helpers.face_mask(prd1.shape)
+    np.testing.assert_array_equal(prd1[mask], exp["prd1"][mask])
+    if "u_out" in exp:
+        u, v, w, prd = oracle.posdin(g, f, exp["prd1"])
+        np.testing.assert_array_equal(u, exp["u_out"])
+        np.testing.assert_array_equal(v, exp["v_out"])
+        np.testing.assert_array_equal(w, exp["w_out"])
+        np.testing.assert_array_equal(prd, exp["prd1"])
+
+
+# ---------------------------------------------------------------- R ------
+needs_ref = pytest.mark.skipif(not oracle.ref_available("ic"), reason="oracle/_ref not built")
+
+
+@needs_ref
+@pytest.mark.parametrize("kw", [dict(), dict(periodic_x=True), dict(periodic_y=True),
+                                dict(ibm=True), dict(ratio=1.0), dict(t_press=2)])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oracle_matches_compiled_reference(kw, seed):
+    c = cases.random_case(10 + 2 * seed, 9, 7 + seed, seed=seed, **kw)
+    lam = 3.0e4
+    f = cases.scale_velocity(c.fields, lam)
+    p_or, st, rc = oracle.graddin(c.grid, f)
+    p_ref = oracle.ref_graddin(c.grid, f)
+    mask = helpers.face_mask(p_or.shape)
+    np.testing.assert_array_equal(p_or[mask], p_ref[mask])
+    if c.grid.t_press == 1:
+        a = oracle.posdin(c.grid, f, p_ref)
+        b = oracle.ref_posdin(c.grid, f, p_ref)
+        for x, y in zip(a[:3], b[:3]):
+            np.testing.assert_array_equal(x, y)
+
+
+@needs_ref
+def test_oracle_matches_reference_at_iteration_cap():
+    """Iteration cap path (the reference prints "pulou pressão" and carries
+    on).  On a stretched grid the reference's right-hand side (w differences
+    over dzz) is inconsistent with its dz^2 operator unless w is chosen to
+    match; with random w, CG diverges and both stop at 10000 iterations."""
+    c = cases.random_case(12, 10, 8, seed=3, stretched=True)
+    f = c.fields
+    p_or, st, rc = oracle.graddin(c.grid, f, max_iter=10000)
+    assert rc == 1 and st.iters == 10000
+    p_ref = oracle.ref_graddin(c.grid, f)
+    mask = helpers.face_mask(p_or.shape)
+    np.testing.assert_array_equal(p_or[mask], p_ref[mask])
+
+
+@needs_ref
+def test_oracle_interp_matches_reference_posdin_zero_pressure_gradient():
+    # posdin with a linear pressure in x isolates 1/rho_x on x faces
+    c = cases.random_case(8, 6, 5, seed=4, stretched=True)
+    g, f = c.grid, c.fields
+    p = np.zeros_like(f.prd1)
+    p[:] = np.arange(g.nx + 2)[:, None, None] * 1.0
+    a = oracle.posdin(g, f, p)
+    b = oracle.ref_posdin(g, f, p)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+
+
+# ---------------------------------------------------------------- C ------
+def cosine_mode_case(nx, ny, nz, kx, ky, kz, rho=1000.0, d=0.01, dt=1e-3):
+    """Velocity with -div u = f, f = cos(kx pi (i-1/2)/nx) cos(...) cos(...),
+    built with u only: u(i+1) = u(i) - dx f(i), u(1) = 0 (u(nx+1) = 0 since
+    the mode sums to zero over i)."""
+    g = cases.Grid(nx, ny, nz, d, d, d, dt=dt, rho_f1=rho)
+    f = cases.zeros_fields(g)
+    f.rho[...] = rho
+    ci = np.cos(kx * np.pi * (np.arange(nx) + 0.5) / nx)
+    cj = np.cos(ky * np.pi * (np.arange(ny) + 0.5) / ny)
+    ck = np.cos(kz * np.pi * (np.arange(nz) + 0.5) / nz)
+    fm = ci[:, None, None] * cj[None, :, None] * ck[None, None, :]
+    u = np.zeros((nx + 1, ny, nz))
+    u[1:] = -d * np.cumsum(fm, axis=0)
+    u[-1] = 0.0
+    f.u[1:nx + 2, 1:ny + 1, 1:nz + 1] = u
+    lam = (dt / rho) * sum((2 - 2 * np.cos(k * np.pi / n)) / (d * d)
+                          for k, n in ((kx, nx), (ky, ny), (kz, nz)))
+    return g, f, fm / lam
+
+
+@pytest.mark.parametrize("modes", [(1, 0, 0), (2, 1, 0), (1, 2, 3), (3, 3, 1)])
+def test_closed_form_cosine_modes(modes):
+    g, f, p_exact = cosine_mode_case(12, 10, 8, *modes)
+    prd1, st, rc = oracle.graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-13)
+    assert rc == 0
+    p = prd1[1:-1, 1:-1, 1:-1]
+    # x0 = 0 and constant d: CG keeps the mean at zero, as the exact mode has
+    assert np.abs(p - p_exact).max() <= 1e-10 * np.abs(p_exact).max()
+
+
+def dense_operator(g, rho, with_dref=False):
+    """Unscaled A = -div(dt/rho_face grad) with zero-flux walls, assembled
+    from first principles: face density = mean of the two adjacent cells,
+    periodic wraps.  With with_dref, also returns the reference's diagonal
+    d_ref, which adds the wall-face coefficients (a wall face takes the
+    density of the adjacent interior face) although they cancel against the
+    mirror ghosts."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    n = nx * ny * nz
+    A = np.zeros((n, n))
+    dref = np.zeros(n)
+    idx = lambda i, j, k: i + nx * (j + ny * k)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = idx(i, j, k)
+                for (di, dj, dk), h, per in (((1, 0, 0), g.dx, g.ccx0 == 0), ((0, 1, 0), g.dy, g.ccy0 == 0),
+                                             ((0, 0, 1), g.dz, False)):
+                    for sgn in (1, -1):
+                        ii, jj, kk = i + sgn * di, j + sgn * dj, k + sgn * dk
+                        n_i, n_j, n_k = (nx, ny, nz)
+                        if per:
+                            ii %= n_i
+                            jj %= n_j
+                        if not (0 <= ii < n_i and 0 <= jj < n_j and 0 <= kk < n_k):
+                            # wall: zero flux; the reference's d counts the face
+                            # with the density of the opposite interior face
+                            oi, oj, ok = i - sgn * di, j - sgn * dj, k - sgn * dk
+                            rf = 0.5 * (rho[i, j, k] + rho[oi, oj, ok])
+                            dref[c] += g.dt / (h * h * rf)
+                            continue
+                        rf = 0.5 * (rho[i, j, k] + rho[ii, jj, kk])
+                        s = g.dt / (h * h * rf)
+                        A[c, c] += s
+                        dref[c] += s
+                        A[c, idx(ii, jj, kk)] -= s
+    return (A, dref) if with_dref else A
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(periodic_x=True)])
+def test_brute_force_dense_solve(kw):
+    c = cases.random_case(8, 6, 5, seed=21, ratio=1000.0, **kw)
+    g, f = c.grid, c.fields
+    A, dvec = dense_operator(g, f.rho, with_dref=True)
+    u, v, w = f.u, f.v, f.w
+    q = ((u[1:g.nx + 1, 1:-1, 1:-1] - u[2:g.nx + 2, 1:-1, 1:-1]) / g.dx +
+         (v[1:-1, 1:g.ny + 1, 1:-1] - v[1:-1, 2:g.ny + 2, 1:-1]) / g.dy +
+         (w[1:-1, 1:-1, 1:g.nz + 1] - w[1:-1, 1:-1, 2:g.nz + 2]) / g.dz)
+    qv = q.ravel(order="F")
+    # fix the null space: CG from x0 = 0 keeps sum(d_ref p) = 0, because the
+    # scaled operator's null vector is sqrt(d_ref)
+    n = qv.size
+    M = np.zeros((n + 1, n + 1))
+    M[:n, :n] = A
+    M[:n, n] = dvec
+    M[n, :n] = dvec
+    sol = scipy.linalg.solve(M, np.append(qv, 0.0))
+    p_dense = sol[:n].reshape((g.nx, g.ny, g.nz), order="F")
+    prd1, st, rc = oracle.graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-13)
+    assert rc == 0
+    p = prd1[1:-1, 1:-1, 1:-1]
+    assert np.abs(p - p_dense).max() <= 1e-8 * np.abs(p_dense).max()
+
+
+# ---------------------------------------------------------------- I ------
+def test_linearity_power_of_two():
+    c = cases.random_case(10, 8, 6, seed=31)
+    p1, _, _ = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-10)
+    f2 = cases.scale_velocity(c.fields, 8.0)
+    p2, _, _ = oracle.graddin(c.grid, f2, stop_rule=oracle.STOP_RELATIVE, tol=1e-10)
+    np.testing.assert_array_equal(p2, 8.0 * p1)
+
+
+def test_weighted_mean_conserved():
+    """All-Neumann: A 1 = 0 (unscaled), so CG never changes sum_i d_i p_i."""
+    c = cases.random_case(10, 8, 6, seed=32)
+    _, _, _, d, _, _ = oracle.scaled_operator(c.grid, c.fields)
+    dint = d[1:-1, 1:-1, 1:-1]
+    f = c.fields.copy()
+    rng = np.random.default_rng(3)
+    f.prd1[...] = rng.uniform(-1, 1, f.prd1.shape)
+    m0 = (dint * f.prd1[1:-1, 1:-1, 1:-1]).sum()
+    # mirror-consistent ghosts so the operator is the pure Neumann one
+    p = f.prd1
+    p[0], p[-1] = p[1], p[-2]
+    p[:, 0], p[:, -1] = p[:, 1], p[:, -2]
+    p[:, :, 0], p[:, :, -1] = p[:, :, 1], p[:, :, -2]
+    m0 = (dint * p[1:-1, 1:-1, 1:-1]).sum()
+    prd1, st, rc = oracle.graddin(c.grid, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-11)
+    m1 = (dint * prd1[1:-1, 1:-1, 1:-1]).sum()
+    scale = (dint * np.abs(prd1[1:-1, 1:-1, 1:-1])).sum()
+    assert abs(m1 - m0) <= 1e-10 * scale
+
+
+def discrete_div(g, u, v, w):
+    nx, ny, nz = g.nx, g.ny, g.nz
+    dzc = g.dzzv[1:nz + 1] if g.dzzv is not None else g.dz
+    return ((u[2:nx + 2, 1:-1, 1:-1] - u[1:nx + 1, 1:-1, 1:-1]) / g.dx +
+            (v[1:-1, 2:ny + 2, 1:-1] - v[1:-1, 1:ny + 1, 1:-1]) / g.dy +
+            (w[1:-1, 1:-1, 2:nz + 2] - w[1:-1, 1:-1, 1:nz + 1]) / dzc)
+
+
+def test_divergence_identity_and_walls():
+    """After posdin, div u = -sqrt(d) * (A e - b) ... i.e. the discrete
+    divergence (4_plot.f90:389) equals the solver's own scaled residual
+    times sqrt(d) (sign: r = A e - b, div = sqrt(d) r); wall-normal faces
+    are unchanged."""
+    c = cases.random_case(12, 10, 8, seed=33)
+    g, f = c.grid, c.fields
+    prd1, st, rc = oracle.graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-6)
+    u, v, w, _ = oracle.posdin(g, f, prd1)
+    div = discrete_div(g, u, v, w)
+    shx, shy, shz, d, q, b = oracle.scaled_operator(g, f)
+    dint = d[1:-1, 1:-1, 1:-1]
+    e = prd1 * np.sqrt(np.where(d > 0, d, 1.0))
+    ei = e[1:-1, 1:-1, 1:-1]
+    Ae = (ei - shx[1:, :, :] * e[2:, 1:-1, 1:-1] - shx[:-1] * e[:-2, 1:-1, 1:-1]
+          - shy[:, 1:, :] * e[1:-1, 2:, 1:-1] - shy[:, :-1] * e[1:-1, :-2, 1:-1]
+          - shz[:, :, 1:] * e[1:-1, 1:-1, 2:] - shz[:, :, :-1] * e[1:-1, 1:-1, :-2])
+    r = Ae - b
+    np.testing.assert_allclose(div, np.sqrt(dint) * r, rtol=0, atol=1e-9 * np.abs(div).max() + 1e-15)
+    # the divergence norm is the residual norm the solver stopped at
+    assert np.linalg.norm(r) <= 1e-6 * np.sqrt(st.bb) * 1.01
+    # wall-normal faces unchanged (zero gradient through the mirror ghosts)
+    np.testing.assert_array_equal(u[1], f.u[1])
+    np.testing.assert_array_equal(u[g.nx + 1], f.u[g.nx + 1])
+    np.testing.assert_array_equal(v[:, 1], f.v[:, 1])
+    np.testing.assert_array_equal(w[:, :, 1], f.w[:, :, 1])
+    np.testing.assert_array_equal(w[:, :, g.nz + 1], f.w[:, :, g.nz + 1])
+
+
+def scaled_from_faces(g, shx, shy, shz):
+    """Dense scaled operator from the oracle's face coefficients: unit
+    diagonal minus the wall coefficients (mirror ghosts fold back), minus
+    s_hat towards each neighbour (periodic wraps)."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    n = nx * ny * nz
+    M = np.eye(n)
+    idx = lambda i, j, k: i + nx * (j + ny * k)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = idx(i, j, k)
+                nbrs = [((i + 1, j, k), shx[i + 1, j, k], 0), ((i - 1, j, k), shx[i, j, k], 0),
+                        ((i, j + 1, k), shy[i, j + 1, k], 1), ((i, j - 1, k), shy[i, j, k], 1),
+                        ((i, j, k + 1), shz[i, j, k + 1], 2), ((i, j, k - 1), shz[i, j, k], 2)]
+                for (ii, jj, kk), s, ax in nbrs:
+                    if ax == 0 and g.ccx0 == 0:
+                        ii %= nx
+                    if ax == 1 and g.ccy0 == 0:
+                        jj %= ny
+                    if 0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz:
+                        M[c, idx(ii, jj, kk)] -= s
+                    else:
+                        M[c, c] -= s
+    return M
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(periodic_x=True)])
+def test_scaled_operator_against_first_principles(kw):
+    """I1/I2: the scaled operator equals D^-1/2 A D^-1/2 with A assembled
+    from first principles and D the reference's diagonal (which counts wall
+    faces), so it is symmetric with diagonal 1 - sum of wall s_hat."""
+    c = cases.random_case(6, 5, 4, seed=34, **kw)
+    g = c.grid
+    shx, shy, shz, d, q, b = oracle.scaled_operator(g, c.fields)
+    dv = d[1:-1, 1:-1, 1:-1].ravel(order="F")
+    A, dref = dense_operator(g, c.fields.rho, with_dref=True)
+    np.testing.assert_allclose(dv, dref, rtol=1e-14)
+    Ahat = A / np.sqrt(np.outer(dv, dv))
+    M = scaled_from_faces(g, shx, shy, shz)
+    np.testing.assert_allclose(M, Ahat, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(M, M.T, rtol=0, atol=1e-15)
+    # b = q / sqrt(d)
+    np.testing.assert_allclose(b, q / np.sqrt(d[1:-1, 1:-1, 1:-1]), rtol=1e-15)
+
+
+def test_heaviside_matches_numpy_generator():
+    ls = np.linspace(-0.03, 0.03, 401)
+    eps = 1.5 * 5e-3
+    a = oracle.heaviside_rho(ls, eps, 1.2, 1000.0)
+    b = cases.heaviside_rho(ls, eps, 1.2, 1000.0)
+    # same operation order; numpy's sin may differ from libm's by an ulp
+    np.testing.assert_allclose(a, b, rtol=1e-14, atol=0)
+    assert a[0] == 1.2 and a[-1] == 1000.0
+    # H(0) = 1/2: the mean density sits on the interface
+    assert np.isclose(oracle.heaviside_rho(np.zeros(1), eps, 1.2, 1000.0)[0], 500.6, rtol=1e-15)
