@@ -69,6 +69,10 @@ typedef struct suli_ps_opts {
     int32_t chunk;         /* CG iterations between host polls; <= 0: automatic */
     int32_t verbose;       /* 1: print the reference's non-convergence line */
     int32_t timing;        /* 1: time SpMV launches with HIP events (stats) */
+    int32_t keep_null;     /* 1: keep the initial residual's component along the
+                              operator's null space (constant pressure), as the
+                              reference does; 0 (default): remove it, which
+                              changes the solution by a constant only */
 } suli_ps_opts;
 
 typedef struct suli_ps_stats {

@@ -36,7 +36,7 @@ class OracleGrid(C.Structure):
         ("ibm_mask", C.c_int),
         ("obsx", _dp), ("obsy", _dp), ("obsz", _dp),
         ("omp_t", C.c_int), ("stop_rule", C.c_int), ("tol", C.c_double),
-        ("max_iter", C.c_int), ("threads", C.c_int),
+        ("max_iter", C.c_int), ("threads", C.c_int), ("project_null", C.c_int),
     ]
 
 
@@ -75,7 +75,7 @@ def lib():
 
 
 def make_grid(g, fields=None, *, stop_rule=STOP_REFERENCE, tol=1e-8, max_iter=10000,
-              omp_t=0, threads=1) -> OracleGrid:
+              omp_t=0, threads=1, project_null=False) -> OracleGrid:
     og = OracleGrid()
     og.nx, og.ny, og.nz = g.nx, g.ny, g.nz
     og.dx, og.dy, og.dz = g.dx, g.dy, g.dz
@@ -89,6 +89,7 @@ def make_grid(g, fields=None, *, stop_rule=STOP_REFERENCE, tol=1e-8, max_iter=10
         og.obsx, og.obsy, og.obsz = _ptr(fields.obs_lsx), _ptr(fields.obs_lsy), _ptr(fields.obs_lsz)
     og.omp_t, og.stop_rule, og.tol = omp_t, stop_rule, tol
     og.max_iter, og.threads = max_iter, threads
+    og.project_null = 1 if project_null else 0
     return og
 
 

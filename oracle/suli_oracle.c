@@ -337,6 +337,24 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
                                  sy[IFY(i, j, k)] * e[IP(i, j - 1, k)] - sz[IFZ(i, j, k + 1)] * e[IP(i, j, k + 1)] -
                                  sz[IFZ(i, j, k)] * e[IP(i, j, k - 1)] - q[IC(i, j, k)] / sqrt(d[IP(i, j, k)]);
 
+    /* Extension (project_null, not in the reference): every reference BC
+     * leaves A singular with null vector 1, so the scaled operator has null
+     * vector sqrt(d); remove the residual's component along it, making the
+     * floating-point right-hand side exactly consistent.  Serial sums. */
+    if (g->project_null) {
+        double rn = 0.0, dd = 0.0;
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++) {
+                    rn = rn + r[IP(i, j, k)] * sqrt(d[IP(i, j, k)]);
+                    dd = dd + d[IP(i, j, k)];
+                }
+        const double gamma = rn / dd;
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++) r[IP(i, j, k)] = r[IP(i, j, k)] - gamma * sqrt(d[IP(i, j, k)]);
+    }
+
     /* r.r, always serial (9_grad.f90:303-319) */
     double rr = dot_interior(g, r, r, 1);
 

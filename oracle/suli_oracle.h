@@ -40,6 +40,8 @@ typedef struct oracle_grid {
     double tol;             /* relative tolerance for ORACLE_STOP_RELATIVE */
     int max_iter;           /* reference: 10000 */
     int threads;            /* OpenMP threads for the sweeps; <= 0 = runtime default */
+    int project_null;       /* not in the reference: remove the initial residual's
+                               component along the null vector sqrt(d) */
 } oracle_grid;
 
 typedef struct oracle_stats {

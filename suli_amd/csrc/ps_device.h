@@ -1,5 +1,5 @@
 // Device helpers shared by the setup and CG kernels: wave64/workgroup sums,
-// the last-block ticket, XCD-aware tile order and ghost index rules.
+// partial-sum totals, XCD-aware tile order and ghost index rules.
 #pragma once
 
 #include "ps_internal.h"
@@ -32,41 +32,21 @@ __device__ __forceinline__ double block_sum(double v)
     return s;
 }
 
-// Thread 0 publishes this block's partial sum(s); returns true in every
-// thread of the block that finished last.  The ticket is reset by that
-// block (finish_ticket).  __threadfence() is an agent-scope release/acquire,
-// which on gfx950 writes back / invalidates the per-XCD L2 so that the last
-// block sees partials written on other XCDs.
-__device__ __forceinline__ bool publish_and_ticket(double *partial, double v, double *partial2, double v2,
-                                                   unsigned *ticket)
+// Total of n per-workgroup partial sums written by an earlier kernel,
+// returned to every thread.  The summation order depends only on n, so every
+// workgroup (and every kernel) that sums the same partials gets the same
+// bits, and a problem's result does not depend on what else runs.
+__device__ __forceinline__ double total_of(const double *partial, int n)
 {
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        if (partial) __hip_atomic_store(partial + blockIdx.x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (partial2) __hip_atomic_store(partial2 + blockIdx.x, v2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
-__device__ __forceinline__ void finish_ticket(unsigned *ticket)
-{
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of n partials in a fixed order (thread-strided, then block_sum).
-// The order depends only on n, so a problem's result does not depend on
-// what else runs on the device.
-__device__ __forceinline__ double sum_partials(const double *partial, int n)
-{
+    __shared__ double tot;
     double s = 0.0;
-    for (int t = threadIdx.x; t < n; t += kThreads)
-        s += __hip_atomic_load(partial + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return block_sum(s);
+    for (int t = threadIdx.x; t < n; t += kThreads) s += partial[t];
+    s = block_sum(s);
+    if (threadIdx.x == 0) tot = s;
+    __syncthreads();
+    const double r = tot;
+    __syncthreads();
+    return r;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on

@@ -1,5 +1,6 @@
-// Internal declarations shared by the kernels (ps_kernels.hip) and the host
-// solver (ps_solver.hip).  Not part of the public ABI (include/suli_ps.h).
+// Internal declarations shared by the kernels (ps_setup.hip, ps_cg.hip)
+// and the host solver (ps_solver.hip).  Not part of the public ABI
+// (include/suli_ps.h).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,6 +12,9 @@ constexpr int kThreads = 256;        // every kernel: 4 wave64s
 constexpr int kTileX = 64;           // SpMV tile: one wave per x row
 constexpr int kTileY = 4;            // four rows per workgroup
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, round-robin dispatch
+constexpr int kVecBlocks = 512;      // workgroups of the 1-D vector kernels
+constexpr int kSpmvBlocks = 1024;    // target workgroups of the SpMV
+constexpr int kNotDone = 0x7fffffff;
 
 // Grid geometry as the kernels see it.  Cells are 0-based (i, j, k) with x
 // fastest; a cell vector holds nx*ny*nz values with no halo.  Ghost values
@@ -23,31 +27,48 @@ struct Geo {
     int tiles_x, tiles_y, tiles_z, kz;   // SpMV tiling: kz planes per workgroup
 };
 
-// Device-resident CG state.  Written by exactly one block at a time (the
-// last block of a reduction, found with a ticket), read by all blocks at
-// kernel entry.
+// Device-resident CG state.
+//
+// No kernel communicates between its own workgroups: every reduction
+// leaves one partial sum per workgroup, and the kernel that needs the total
+// sums those partials itself (each workgroup in the same fixed order, so all
+// get the same bits) after the kernel boundary has made them visible on
+// every XCD.  Scalars derived from a total are stored by workgroup 0 for
+// later kernels; values a kernel reads while its own workgroup 0 may be
+// writing are double-buffered by iteration parity (rr) or are benign
+// (done_at only ever moves from "running" to the current iteration).
 struct CgScalars {
-    double rr;        // r.r the recurrence uses (FP32 residual in mixed mode)
-    double pap;       // p.Ap
-    double alpha, beta;
+    double rr[2];     // r.r after iteration n in slot n & 1 (slot 0 also holds r0.r0)
+    double alpha;     // alpha of the current iteration (update_r -> update_ep)
+    double beta;
+    double pap;
     double bb;        // b.b (scaled right-hand side)
     double thresh;    // stop threshold on r.r
     double rr0;       // initial r.r
     double rr_true;   // FP64 r.r of the last recomputed true residual
-    int stop;         // 1: the CG kernels skip (converged, capped, or refresh due)
-    int final_;       // 1: solve finished; every kernel skips
-    int k2n, k3n;     // completed update_r / update_ep launches
-    int iters;        // CG iterations (reference `cont`)
+    double gamma;     // null-space component removed from the initial residual
+    int done_at;      // iteration at which CG stopped; kNotDone while running
+    int final_;       // 1: the solve is over (mixed: decided at a replacement)
+    int iters;        // CG iterations performed (reference `cont`)
     int max_iter;
     int refreshes;    // FP64 residual replacements (mixed)
-    int refresh_mark; // k3n at the last replacement
+    int since_refresh;// iterations since the last replacement
     int status;       // 0 converged, 1 capped
     int mixed;        // 1: convergence is judged on rr_true at replacements
-    int pad[2];
-    unsigned ticket[8];
+    int project;      // 1: remove the null-space component of the residual
+    int rel_rule;     // 1: thresh = tol^2 b.b
+    double tol;
+    double thresh_ref;
 };
 
-enum TicketSlot { kTkSpmv = 0, kTkUpdR = 1, kTkUpdEP = 2, kTkTrue = 3, kTkInit = 4, kTkRefresh = 5 };
+// Offsets (in doubles) of the per-workgroup partial sums in one buffer.
+struct Partials {
+    double *spmv;     // [B1]      p.Ap of the SpMV
+    double *upd;      // [B2]      r.r of update_r
+    double *tres;     // [2*B1]    true residual r.r (FP64, and of its FP32 copy)
+    double *init;     // [4*B0]    setup: r0.r0, b.b, r0.sqrt(d), d
+    double *proj;     // [2*B0]    setup after projection: r0.r0, FP32 r0.r0
+};
 
 // Setup inputs in the reference's Fortran layouts (device pointers).
 struct SetupIn {
@@ -61,33 +82,43 @@ struct SetupIn {
 };
 
 // Operator storage (scaled face coefficients, x-face array (nx+1)*ny*nz,
-// y-face nx*(ny+1)*nz, z-face nx*ny*(nz+1)) plus the diagonal d.
+// y-face nx*(ny+1)*nz, z-face nx*ny*(nz+1)).
 template <typename T>
 struct Op {
     const T *sx, *sy, *sz;
 };
 
-// ---- launchers (ps_kernels.hip) ----
+// ---- setup (ps_setup.hip) ----
 void launch_face_coefs(const Geo &g, const SetupIn &in, double *sx, double *sy, double *sz, hipStream_t s);
 void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
                      double *d, double *b, hipStream_t s);
 void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
                         float *sy32, float *sz32, hipStream_t s);
-void launch_init_residual(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
-                          const double *d, double *b, double *e, double *r64, double *p64, float *r32, float *p32,
-                          float *c32, double *partial, double *partial2, int nblk, CgScalars *S, int rel_rule,
-                          double tol, double thresh_ref, hipStream_t s);
-template <typename T>
-void launch_spmv_pap(const Geo &g, const T *p, Op<T> op, T *q, double *partial, CgScalars *S, hipStream_t s);
-template <typename T>
-void launch_update_r(const Geo &g, T *r, const T *q, double *partial, int nblk, CgScalars *S, hipStream_t s);
-template <typename T, typename TE>
-void launch_update_ep(const Geo &g, TE *e, T *p, const T *r, int nblk, CgScalars *S, hipStream_t s);
-void launch_refresh_e(const Geo &g, double *e, float *c32, int nblk, CgScalars *S, hipStream_t s);
-void launch_true_residual(const Geo &g, const double *e, Op<double> op, const double *b, float *r32,
-                          double *partial, double *partial2, int final_check, CgScalars *S, hipStream_t s);
+// Initial residual, null-space projection, first direction; leaves S ready
+// for iteration 1.  r64/p64 (FP64 solve) or r32/p32/c32 (mixed) are written.
+void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
+                 double *r0, double *r64, double *p64, float *r32, float *p32, float *c32, const Partials &P,
+                 CgScalars *S, hipStream_t s);
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, hipStream_t s);
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s);
+
+// ---- CG iteration n (ps_cg.hip), n = 1, 2, ... ----
+// ran (nullable): set to 1 by the launch if it did the work (timing samples)
+template <typename T>
+void launch_spmv_pap(const Geo &g, int n, const T *p, Op<T> op, T *q, const Partials &P, CgScalars *S, int *ran,
+                     hipStream_t s);
+template <typename T>
+void launch_update_r(const Geo &g, int n, T *r, const T *q, const Partials &P, CgScalars *S, hipStream_t s);
+template <typename T>
+void launch_update_ep(const Geo &g, int n, T *e, T *p, const T *r, const Partials &P, CgScalars *S,
+                      hipStream_t s);
+// Mixed precision residual replacement: e += c; r = b - A e in FP64 handed
+// to the FP32 recurrence; decide converged / capped / continue.
+void launch_refresh(const Geo &g, double *e, float *c32, Op<double> op, const double *b, float *r32,
+                    const Partials &P, CgScalars *S, hipStream_t s);
+// FP64 true residual r.r into S->rr_true (reporting; runs after the solve).
+void launch_record_true(const Geo &g, const double *e, Op<double> op, const double *b, const Partials &P,
+                        CgScalars *S, hipStream_t s);
 
 int spmv_blocks(const Geo &g);
 Geo make_geo(int nx, int ny, int nz, int px, int py);

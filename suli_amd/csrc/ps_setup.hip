@@ -7,6 +7,8 @@
 // so the operator and the correction are bit-identical to SuLi's FP64
 // arithmetic (SuLi-IC/9_grad.f90:93-301, SuLi-IC/10_pos.f90:17-46,
 // SuLi-IC/8_derivadas.f90:305-481).
+#include <algorithm>
+
 #include "ps_device.h"
 
 namespace sps {
@@ -189,20 +191,18 @@ __device__ __forceinline__ double e_box(const Geo &g, const double *prd1, const 
 }
 
 // Initial residual r = A e - b with the caller's prd1 ghost values
-// (9_grad.f90:290-301) and the first CG direction p = r (:372-380).
-// b is replaced by b_eff = b + (A_mirror e - A_ghosts e): the right-hand
-// side under which later FP64 residual recomputations (which use mirrored
-// ghosts, as every CG iteration does) reproduce this residual.  With
-// mirror-consistent prd1 ghosts, b_eff == b bit for bit.
-__global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, const double *sx, const double *sy,
-                                                            const double *sz, const double *d, double *b, double *e,
-                                                            double *r64, double *p64, float *r32, float *p32,
-                                                            float *c32, double *partial, CgScalars *S, int rel_rule,
-                                                            double tol, double thresh_ref)
+// (9_grad.f90:242-301).  b is replaced by b_eff = b + (A_mirror e -
+// A_ghosts e): the right-hand side under which later FP64 residual
+// recomputations (which use mirrored ghosts, as every CG iteration does)
+// reproduce this residual.  With mirror-consistent prd1 ghosts (SuLi's
+// contorno(1) leaves them so), b_eff == b bit for bit.
+// Partials: r.r, b.b, r.sqrt(d), sum of d.
+__global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, Op<double> op, const double *d,
+                                                            double *b, double *e, double *r0, Partials P)
 {
     const int nx = g.nx, ny = g.ny;
     const long long stride = (long long)gridDim.x * kThreads;
-    double acc_rr = 0.0, acc_bb = 0.0, acc_rs = 0.0;
+    double acc_rr = 0.0, acc_bb = 0.0, acc_rn = 0.0, acc_dd = 0.0;
     for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
         const int i = (int)(c % nx);
         const long long jk = c / nx;
@@ -217,8 +217,8 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, c
         const double eE = e_box(g, in.prd1, d, I + 1, J, K), eW = e_box(g, in.prd1, d, I - 1, J, K);
         const double eN = e_box(g, in.prd1, d, I, J + 1, K), eS = e_box(g, in.prd1, d, I, J - 1, K);
         const double eT = e_box(g, in.prd1, d, I, J, K + 1), eB = e_box(g, in.prd1, d, I, J, K - 1);
-        const double a_in = ec - sx[fx + 1] * eE - sx[fx] * eW - sy[fy + nx] * eN - sy[fy] * eS -
-                            sz[fz + pl] * eT - sz[fz] * eB;
+        const double a_in = ec - op.sx[fx + 1] * eE - op.sx[fx] * eW - op.sy[fy + nx] * eN - op.sy[fy] * eS -
+                            op.sz[fz + pl] * eT - op.sz[fz] * eB;
         // neighbours with mirrored/periodic ghosts
         const int iE = box_to_cell(I + 1, nx, g.px) + 1, iW = box_to_cell(I - 1, nx, g.px) + 1;
         const int jN = box_to_cell(J + 1, ny, g.py) + 1, jS = box_to_cell(J - 1, ny, g.py) + 1;
@@ -226,14 +226,53 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, c
         const double mE = e_box(g, in.prd1, d, iE, J, K), mW = e_box(g, in.prd1, d, iW, J, K);
         const double mN = e_box(g, in.prd1, d, I, jN, K), mS = e_box(g, in.prd1, d, I, jS, K);
         const double mT = e_box(g, in.prd1, d, I, J, kT), mB = e_box(g, in.prd1, d, I, J, kB);
-        const double a_mir = ec - sx[fx + 1] * mE - sx[fx] * mW - sy[fy + nx] * mN - sy[fy] * mS -
-                             sz[fz + pl] * mT - sz[fz] * mB;
+        const double a_mir = ec - op.sx[fx + 1] * mE - op.sx[fx] * mW - op.sy[fy + nx] * mN - op.sy[fy] * mS -
+                             op.sz[fz + pl] * mT - op.sz[fz] * mB;
         const double bc = b[c];
         const double r = a_in - bc;
         b[c] = bc + (a_mir - a_in);
         e[c] = ec;
+        r0[c] = r;
+        const double dc = d[c];
         acc_rr += r * r;
         acc_bb += bc * bc;
+        acc_rn += r * sqrt(dc);
+        acc_dd += dc;
+    }
+    const double s0 = block_sum(acc_rr), s1 = block_sum(acc_bb), s2 = block_sum(acc_rn), s3 = block_sum(acc_dd);
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        P.init[blockIdx.x] = s0;
+        P.init[nb + blockIdx.x] = s1;
+        P.init[2 * nb + blockIdx.x] = s2;
+        P.init[3 * nb + blockIdx.x] = s3;
+    }
+}
+
+// Every reference boundary condition leaves the operator singular: its rows
+// sum to zero (zero-flux walls, periodic sides), so the scaled operator
+// annihilates n = sqrt(d).  A right-hand side computed in floating point is
+// consistent only to rounding, and at tight tolerances CG then drifts along
+// n.  When S->project is set the n-component gamma n of the initial
+// residual is removed (gamma = r.n / n.n) and b_eff adjusted to match; this
+// changes the solution only by a constant (the pressure gauge).
+// Then the first search direction p = r (9_grad.f90:372-380).
+__global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const double *d, double *b, const double *r0,
+                                                           double *r64, double *p64, float *r32, float *p32,
+                                                           float *c32, Partials P, int nb0, CgScalars *S)
+{
+    const double rr = total_of(P.init, nb0);
+    const double bb = total_of(P.init + nb0, nb0);
+    const double rn = total_of(P.init + 2 * nb0, nb0);
+    const double dd = total_of(P.init + 3 * nb0, nb0);
+    const double gamma = S->project ? rn / dd : 0.0;
+    const long long stride = (long long)gridDim.x * kThreads;
+    double acc = 0.0, acc32 = 0.0;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
+        const double sq = sqrt(d[c]);
+        const double r = gamma != 0.0 ? r0[c] - gamma * sq : r0[c];
+        if (gamma != 0.0) b[c] = b[c] + gamma * sq;
+        acc += r * r;
         if (r64) {
             r64[c] = r;
             p64[c] = r;
@@ -242,47 +281,43 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, c
             r32[c] = rf;
             p32[c] = rf;
             c32[c] = 0.0f;
-            acc_rs += (double)rf * (double)rf;
+            acc32 += (double)rf * (double)rf;
         }
     }
-    const double s0 = block_sum(acc_rr);
-    const double s1 = block_sum(acc_bb);
-    const double s2 = block_sum(acc_rs);
-    __shared__ int last;
+    const double s0 = block_sum(acc), s1 = block_sum(acc32);
     if (threadIdx.x == 0) {
-        const int nb = gridDim.x;
-        __hip_atomic_store(partial + blockIdx.x, s0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(partial + nb + blockIdx.x, s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(partial + 2 * nb + blockIdx.x, s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        last = atomicAdd(&S->ticket[kTkInit], 1u) == (unsigned)(nb - 1);
+        P.proj[blockIdx.x] = s0;
+        P.proj[gridDim.x + blockIdx.x] = s1;
+        if (blockIdx.x == 0) {
+            S->gamma = gamma;
+            S->bb = bb;
+            S->rr0 = rr;
+            S->thresh = S->rel_rule ? S->tol * S->tol * bb : S->thresh_ref;
+        }
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const int nb = gridDim.x;
-    const double rr = sum_partials(partial, nb);
-    const double bb = sum_partials(partial + nb, nb);
-    const double rs = sum_partials(partial + 2 * nb, nb);
+}
+
+// One workgroup: totals after the projection; S ready for iteration 1.
+// The reference tests the stop rule before the first iteration
+// (do while (|r.r| > thresh .and. cont < 10000), 9_grad.f90:398).
+__global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0, CgScalars *S)
+{
+    const double rr = total_of(P.proj, nb0);
+    const double rs = total_of(P.proj + nb0, nb0);
     if (threadIdx.x == 0) {
-        const double thresh = rel_rule ? tol * tol * bb : thresh_ref;
+        const double rrc = S->mixed ? rs : rr;
+        S->rr[0] = rrc;
+        S->rr[1] = rrc;
         S->rr0 = rr;
         S->rr_true = rr;
-        S->rr = S->mixed ? rs : rr;
-        S->bb = bb;
-        S->thresh = thresh;
         S->iters = 0;
-        S->k2n = 0;
-        S->k3n = 0;
         S->refreshes = 0;
-        S->refresh_mark = 0;
-        S->status = 0;
-        // reference loop test: while (|r.r| > thresh .and. cont < max) (9_grad.f90:398)
-        const bool done = !(fabs(rr) > thresh) || S->max_iter <= 0;
-        S->stop = done ? 1 : 0;
+        S->since_refresh = 0;
+        const bool conv = !(fabs(rr) > S->thresh);
+        const bool done = conv || S->max_iter <= 0;
+        S->done_at = done ? 0 : kNotDone;
         S->final_ = done ? 1 : 0;
-        S->status = (!(fabs(rr) > thresh)) ? 0 : 1;
-        S->ticket[kTkInit] = 0u;
+        S->status = conv ? 0 : 1;
     }
 }
 
@@ -382,15 +417,15 @@ void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, d
                        sz32);
 }
 
-void launch_init_residual(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
-                          const double *d, double *b, double *e, double *r64, double *p64, float *r32, float *p32,
-                          float *c32, double *partial, double * /*partial2*/, int nblk, CgScalars *S, int rel_rule,
-                          double tol, double thresh_ref, hipStream_t s)
+void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
+                 double *r0, double *r64, double *p64, float *r32, float *p32, float *c32, const Partials &P,
+                 CgScalars *S, hipStream_t s)
 {
-    int nb = grid_for(g.n);
-    if (nb > nblk) nb = nblk;
-    hipLaunchKernelGGL(k_init_residual, dim3(nb), dim3(kThreads), 0, s, g, in, sx, sy, sz, d, b, e, r64, p64,
-                       r32, p32, c32, partial, S, rel_rule, tol, thresh_ref);
+    const int nb0 = std::min(grid_for(g.n), kVecBlocks);
+    hipLaunchKernelGGL(k_init_residual, dim3(nb0), dim3(kThreads), 0, s, g, in, op, d, b, e, r0, P);
+    hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, d, b, r0, r64, p64, r32, p32, c32, P,
+                       nb0, S);
+    hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kThreads), 0, s, P, nb0, S);
 }
 
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, hipStream_t s)

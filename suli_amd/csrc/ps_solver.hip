@@ -18,6 +18,8 @@ using namespace sps;
 
 namespace {
 
+constexpr int kMaxTimed = 256;  // SpMV launches timed per solve
+
 struct Workspace {
     int nx = 0, ny = 0, nz = 0;
     size_t bytes = 0;
@@ -29,11 +31,11 @@ struct Workspace {
     // FP32 inner-solve operator and vectors (mixed precision)
     float *sx32 = nullptr, *sy32 = nullptr, *sz32 = nullptr;
     float *c32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr;
-    double *partial = nullptr;
-    int nblk = 0;          // workgroups of the 1-D vector kernels
-    int npartial = 0;      // capacity of partial (per slot)
+    double *partial = nullptr;   // one buffer, carved into Partials
+    Partials P{};
     CgScalars *S = nullptr;
     CgScalars *Stime = nullptr;  // scratch state for suli_ps_time_spmv
+    int *ran = nullptr;          // timed SpMV launches that did the work
     // staging for the host-pointer API (reference layouts)
     double *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *prd1 = nullptr;
     double *obsx = nullptr, *obsy = nullptr, *obsz = nullptr, *dz = nullptr, *dzz = nullptr;
@@ -127,13 +129,13 @@ int ensure_ws(suli_ps_ctx *ctx, int nx, int ny, int nz, bool mixed)
         ws.ny = ny;
         ws.nz = nz;
         ws.geo = make_geo(nx, ny, nz, 0, 0);
-        ws.nblk = (int)std::min<long long>(2048, std::max<long long>(1, (long long)(n / 4 + kThreads - 1) / kThreads));
-        ws.npartial = std::max(ws.nblk, spmv_blocks(ws.geo));
-        ws.npartial = std::max(ws.npartial, 8192);
+        const size_t b1 = (size_t)spmv_blocks(ws.geo), b2 = (size_t)kVecBlocks;
+        const size_t npart = b1 + b2 + 2 * b1 + 4 * b2 + 2 * b2;
         if ((rc = dalloc(ctx, &ws.sx, nfx)) || (rc = dalloc(ctx, &ws.sy, nfy)) || (rc = dalloc(ctx, &ws.sz, nfz)) ||
             (rc = dalloc(ctx, &ws.d, n)) || (rc = dalloc(ctx, &ws.b, n)) || (rc = dalloc(ctx, &ws.e, n)) ||
             (rc = dalloc(ctx, &ws.r, n)) || (rc = dalloc(ctx, &ws.p, n)) || (rc = dalloc(ctx, &ws.q, n)) ||
-            (rc = dalloc(ctx, &ws.partial, 3 * (size_t)ws.npartial)) || (rc = dalloc(ctx, &ws.S, 1)) ||
+            (rc = dalloc(ctx, &ws.partial, npart)) || (rc = dalloc(ctx, &ws.S, 1)) ||
+            (rc = dalloc(ctx, &ws.ran, (size_t)kMaxTimed)) ||
             (rc = dalloc(ctx, &ws.Stime, 1)) || (rc = dalloc(ctx, &ws.u, (size_t)(nx + 3) * (ny + 2) * (nz + 2))) ||
             (rc = dalloc(ctx, &ws.v, (size_t)(nx + 2) * (ny + 3) * (nz + 2))) ||
             (rc = dalloc(ctx, &ws.w, (size_t)(nx + 2) * (ny + 2) * (nz + 3))) || (rc = dalloc(ctx, &ws.rho, n)) ||
@@ -143,6 +145,11 @@ int ensure_ws(suli_ps_ctx *ctx, int nx, int ny, int nz, bool mixed)
             free_ws(ws);
             return rc;
         }
+        ws.P.spmv = ws.partial;
+        ws.P.upd = ws.P.spmv + b1;
+        ws.P.tres = ws.P.upd + b2;
+        ws.P.init = ws.P.tres + 2 * b1;
+        ws.P.proj = ws.P.init + 4 * b2;
         HIPCHK(ctx, hipMemsetAsync(ws.S, 0, sizeof(CgScalars), ctx->stream));
         HIPCHK(ctx, hipMemsetAsync(ws.Stime, 0, sizeof(CgScalars), ctx->stream));
     }
@@ -235,56 +242,60 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
     CgScalars s0{};
     s0.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
     s0.mixed = mixed ? 1 : 0;
+    s0.project = o.keep_null ? 0 : 1;
+    s0.rel_rule = o.stop_rule == SULI_PS_STOP_RELATIVE;
+    s0.tol = o.tol;
+    // threshold of the reference stop rule: 0.0001/(nx*ny*nz) (9_grad.f90:393)
+    s0.thresh_ref = 0.0001 / (double)(gr->nx * gr->ny * gr->nz);
+    s0.done_at = kNotDone;
     HIPCHK(ctx, hipMemcpyAsync(ws.S, &s0, sizeof s0, hipMemcpyHostToDevice, s));
+    if (o.timing) HIPCHK(ctx, hipMemsetAsync(ws.ran, 0, kMaxTimed * sizeof(int), s));
 
     launch_face_coefs(g, in, ws.sx, ws.sy, ws.sz, s);
     launch_diag_rhs(g, in, ws.sx, ws.sy, ws.sz, ws.d, ws.b, s);
     launch_scale_faces(g, ws.d, ws.sx, ws.sy, ws.sz, mixed ? ws.sx32 : nullptr, mixed ? ws.sy32 : nullptr,
                        mixed ? ws.sz32 : nullptr, s);
-    // threshold of the reference stop rule: 0.0001/(nx*ny*nz) (9_grad.f90:393)
-    const double thresh_ref = 0.0001 / (double)(gr->nx * gr->ny * gr->nz);
-    launch_init_residual(g, in, ws.sx, ws.sy, ws.sz, ws.d, ws.b, ws.e, mixed ? nullptr : ws.r,
-                         mixed ? nullptr : ws.p, ws.r32, ws.p32, ws.c32, ws.partial, nullptr, ws.npartial, ws.S,
-                         o.stop_rule == SULI_PS_STOP_RELATIVE, o.tol, thresh_ref, s);
+    const Op<double> op64{ws.sx, ws.sy, ws.sz};
+    const Op<float> op32{ws.sx32, ws.sy32, ws.sz32};
+    launch_init(g, in, op64, ws.d, ws.b, ws.e, ws.r, mixed ? nullptr : ws.r, mixed ? nullptr : ws.p, ws.r32,
+                ws.p32, ws.c32, ws.P, ws.S, s);
     HIPCHK(ctx, hipGetLastError());
     ws.have_operator = true;
 
     const int K = o.chunk > 0 ? o.chunk : auto_chunk(g);
-    const Op<double> op64{ws.sx, ws.sy, ws.sz};
-    const Op<float> op32{ws.sx32, ws.sy32, ws.sz32};
     const int max_iter = s0.max_iter;
-    const int max_chunks = max_iter / K + 4;
-    // SpMV timing: events around every 16th SpMV launch
+    // a mixed solve can stop early on its FP32 estimate and resume once per
+    // replacement; bound the chunks generously and let the device decide
+    const int max_chunks = 2 * (max_iter / K) + 8;
+    // SpMV timing: events around every 16th SpMV launch; ws.ran[slot] says
+    // whether that launch did the work (it skips after the stop rule)
     const int tstride = 16;
-    std::vector<std::pair<int, int>> tpairs;  // (event index, global iteration)
-    int nev = 0;
-    int it_global = 0;
+    int ntimed = 0;
+    int n = 0;  // CG iteration number of the launch sequence (1-based)
 
     auto enqueue_chunk = [&](int chunk_idx) -> int {
-        for (int it = 0; it < K; ++it, ++it_global) {
-            const bool timed = o.timing && (it_global % tstride == 0) && nev + 2 <= (int)ctx->ev_spmv.size();
-            if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[nev], s));
+        for (int it = 0; it < K; ++it) {
+            ++n;
+            const bool timed = o.timing && (n % tstride == 1) && ntimed < kMaxTimed;
+            int *ran = timed ? ws.ran + ntimed : nullptr;
+            if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed], s));
             if (mixed)
-                launch_spmv_pap<float>(g, ws.p32, op32, ws.q32, ws.partial, ws.S, s);
+                launch_spmv_pap<float>(g, n, ws.p32, op32, ws.q32, ws.P, ws.S, ran, s);
             else
-                launch_spmv_pap<double>(g, ws.p, op64, ws.q, ws.partial, ws.S, s);
+                launch_spmv_pap<double>(g, n, ws.p, op64, ws.q, ws.P, ws.S, ran, s);
             if (timed) {
-                HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[nev + 1], s));
-                tpairs.push_back({nev, it_global});
-                nev += 2;
+                HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed + 1], s));
+                ++ntimed;
             }
             if (mixed) {
-                launch_update_r<float>(g, ws.r32, ws.q32, ws.partial, ws.nblk, ws.S, s);
-                launch_update_ep<float, float>(g, ws.c32, ws.p32, ws.r32, ws.nblk, ws.S, s);
+                launch_update_r<float>(g, n, ws.r32, ws.q32, ws.P, ws.S, s);
+                launch_update_ep<float>(g, n, ws.c32, ws.p32, ws.r32, ws.P, ws.S, s);
             } else {
-                launch_update_r<double>(g, ws.r, ws.q, ws.partial, ws.nblk, ws.S, s);
-                launch_update_ep<double, double>(g, ws.e, ws.p, ws.r, ws.nblk, ws.S, s);
+                launch_update_r<double>(g, n, ws.r, ws.q, ws.P, ws.S, s);
+                launch_update_ep<double>(g, n, ws.e, ws.p, ws.r, ws.P, ws.S, s);
             }
         }
-        if (mixed) {
-            launch_refresh_e(g, ws.e, ws.c32, ws.nblk, ws.S, s);
-            launch_true_residual(g, ws.e, op64, ws.b, ws.r32, ws.partial, nullptr, 0, ws.S, s);
-        }
+        if (mixed) launch_refresh(g, ws.e, ws.c32, op64, ws.b, ws.r32, ws.P, ws.S, s);
         HIPCHK(ctx, hipGetLastError());
         const int slot = chunk_idx & 1;
         HIPCHK(ctx, hipMemcpyAsync(&ctx->host_status[slot], ws.S, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
@@ -292,27 +303,24 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
         return 0;
     };
 
-    // Is the solve already over after setup (initial residual below threshold)?
-    HIPCHK(ctx, hipMemcpyAsync(&ctx->host_status[1], ws.S, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    bool finished = ctx->host_status[1].final_ != 0;
-    if (!finished) {
-        if ((rc = enqueue_chunk(0))) return rc;
-        for (int c = 1; c <= max_chunks; ++c) {
-            if ((rc = enqueue_chunk(c))) return rc;
-            const int prev = (c - 1) & 1;
-            HIPCHK(ctx, hipEventSynchronize(ctx->ev_status[prev]));
-            if (ctx->host_status[prev].final_) {
-                finished = true;
-                break;
-            }
-        }
-        if (!finished) {
-            HIPCHK(ctx, hipStreamSynchronize(s));
-            return fail(ctx, SULI_PS_EHIP, "solver did not finish within %d chunks", max_chunks);
+    // Two chunks in flight: the host reads chunk c-1's state while chunk c
+    // runs.  Launches after the stop rule return at once.
+    bool finished = false;
+    if ((rc = enqueue_chunk(0))) return rc;
+    for (int c = 1; c <= max_chunks; ++c) {
+        if ((rc = enqueue_chunk(c))) return rc;
+        const int prev = (c - 1) & 1;
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev_status[prev]));
+        if (ctx->host_status[prev].final_) {
+            finished = true;
+            break;
         }
     }
-    if (!mixed) launch_true_residual(g, ws.e, op64, ws.b, nullptr, ws.partial, nullptr, 1, ws.S, s);
+    if (!finished) {
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        return fail(ctx, SULI_PS_EHIP, "solver did not finish within %d chunks", max_chunks);
+    }
+    if (!mixed) launch_record_true(g, ws.e, op64, ws.b, ws.P, ws.S, s);
     launch_unscale(g, ws.e, ws.d, prd1, s);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev_solve[1], s));
@@ -321,6 +329,8 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
     HIPCHK(ctx, hipStreamSynchronize(s));
 
     const int status = fin.status == 0 ? SULI_PS_OK : SULI_PS_NOT_CONVERGED;
+    // r.r of the recurrence after the last iteration (slot of iteration done_at)
+    const double rr_last = fin.rr[(fin.done_at == kNotDone ? 0 : fin.done_at) & 1];
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->status = status;
@@ -328,7 +338,7 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
         stats->refreshes = fin.refreshes;
         stats->rr0 = fin.rr0;
         stats->bb = fin.bb;
-        stats->rr = mixed ? fin.rr_true : fin.rr;
+        stats->rr = mixed ? fin.rr_true : rr_last;
         stats->thresh = fin.thresh;
         stats->rel_true = fin.bb > 0 ? std::sqrt(fin.rr_true / fin.bb) : 0.0;
         float ms = 0.f;
@@ -336,11 +346,13 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
         stats->solve_ms = ms;
         double tsum = 0.0;
         int tcount = 0;
-        for (auto &tp : tpairs) {
-            if (tp.second >= fin.iters) break;
-            float t = 0.f;
-            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev_spmv[tp.first], ctx->ev_spmv[tp.first + 1]));
-            tsum += t;
+        int ran[kMaxTimed] = {0};
+        if (ntimed > 0) HIPCHK(ctx, hipMemcpy(ran, ws.ran, ntimed * sizeof(int), hipMemcpyDeviceToHost));
+        for (int t = 0; t < ntimed; ++t) {
+            if (!ran[t]) continue;
+            float dt = 0.f;
+            HIPCHK(ctx, hipEventElapsedTime(&dt, ctx->ev_spmv[2 * t], ctx->ev_spmv[2 * t + 1]));
+            tsum += dt;
             ++tcount;
         }
         stats->spmv_timed = tcount;
@@ -348,7 +360,7 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
     }
     if (status == SULI_PS_NOT_CONVERGED && o.verbose) {
         // the reference's warning (9_grad.f90:590)
-        std::printf(" pulou pressão; erro = %.15E\n", mixed ? fin.rr_true : fin.rr);
+        std::printf(" pulou pressão; erro = %.15E\n", mixed ? fin.rr_true : rr_last);
         std::fflush(stdout);
     }
     return status;
@@ -577,15 +589,17 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
     if (mixed && !ws.have_mixed) return fail(ctx, SULI_PS_EINVAL, "no FP32 operator: solve in mixed mode first");
     hipStream_t s = ctx->stream;
     CgScalars s0{};
-    s0.rr = 1.0;
+    s0.rr[0] = s0.rr[1] = 1.0;
+    s0.done_at = kNotDone;
     HIPCHK(ctx, hipMemcpyAsync(ws.Stime, &s0, sizeof s0, hipMemcpyHostToDevice, s));
     const Geo &g = ws.geo;
     // one untimed launch, then `reps` launches between two events
     auto launch = [&]() {
         if (mixed)
-            launch_spmv_pap<float>(g, ws.p32, Op<float>{ws.sx32, ws.sy32, ws.sz32}, ws.q32, ws.partial, ws.Stime, s);
+            launch_spmv_pap<float>(g, 1, ws.p32, Op<float>{ws.sx32, ws.sy32, ws.sz32}, ws.q32, ws.P, ws.Stime,
+                                   nullptr, s);
         else
-            launch_spmv_pap<double>(g, ws.p, Op<double>{ws.sx, ws.sy, ws.sz}, ws.q, ws.partial, ws.Stime, s);
+            launch_spmv_pap<double>(g, 1, ws.p, Op<double>{ws.sx, ws.sy, ws.sz}, ws.q, ws.P, ws.Stime, nullptr, s);
     };
     launch();
     HIPCHK(ctx, hipEventRecord(ctx->ev_solve[0], s));

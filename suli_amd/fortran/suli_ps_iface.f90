@@ -29,6 +29,7 @@ module suli_ps_iface
       integer(c_int32_t) :: chunk = 0
       integer(c_int32_t) :: verbose = 0
       integer(c_int32_t) :: timing = 0
+      integer(c_int32_t) :: keep_null = 0
    end type suli_ps_opts
 
    ! struct suli_ps_stats

@@ -316,3 +316,43 @@ def test_heaviside_matches_numpy_generator():
     assert a[0] == 1.2 and a[-1] == 1000.0
     # H(0) = 1/2: the mean density sits on the interface
     assert np.isclose(oracle.heaviside_rho(np.zeros(1), eps, 1.2, 1000.0)[0], 500.6, rtol=1e-15)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(ratio=1.0), dict(periodic_x=True)])
+def test_project_null_changes_only_the_gauge(kw):
+    """The null-space projection (an extension) leaves a consistent solve
+    unchanged up to rounding, and agrees with the dense solve."""
+    c = cases.random_case(8, 6, 5, seed=41, **kw)
+    g, f = c.grid, c.fields
+    p0, _, _ = oracle.graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-12)
+    p1, st, rc = oracle.graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-12, project_null=True)
+    assert rc == 0
+    i = (slice(1, -1),) * 3
+    assert np.abs(p1[i] - p0[i]).max() <= 1e-9 * np.abs(p0[i]).max()
+
+
+def test_project_null_repairs_inconsistent_rhs():
+    """An inflow through a wall makes the all-Neumann system inconsistent:
+    without the projection CG stalls at r.r ~ (b.n)^2; with it CG converges,
+    and the pressure gradients (what posdin uses) are those of the dense
+    least-squares solution."""
+    c = cases.random_case(8, 6, 5, seed=42, ratio=1.0)
+    g, f = c.grid, c.fields
+    f.u[1, 1:-1, 1:-1] = 0.01
+    p1, st, rc = oracle.graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-10, project_null=True)
+    assert rc == 0
+    A, dref = dense_operator(g, f.rho, with_dref=True)
+    u, v, w = f.u, f.v, f.w
+    q = ((u[1:g.nx + 1, 1:-1, 1:-1] - u[2:g.nx + 2, 1:-1, 1:-1]) / g.dx +
+         (v[1:-1, 1:g.ny + 1, 1:-1] - v[1:-1, 2:g.ny + 2, 1:-1]) / g.dy +
+         (w[1:-1, 1:-1, 1:g.nz + 1] - w[1:-1, 1:-1, 2:g.nz + 2]) / g.dz).ravel(order="F")
+    # scaled system: project b = q/sqrt(d) onto range(A_hat) = n-perp, n = sqrt(d)
+    sd = np.sqrt(dref)
+    Ah = A / np.outer(sd, sd)
+    b = q / sd
+    b = b - (b @ sd) / (sd @ sd) * sd
+    e = np.linalg.lstsq(Ah, b, rcond=None)[0]
+    p_dense = (e / sd).reshape((g.nx, g.ny, g.nz), order="F")
+    p = p1[1:-1, 1:-1, 1:-1]
+    gx = lambda a: np.diff(a, axis=0)
+    assert np.abs(gx(p) - gx(p_dense)).max() <= 1e-7 * np.abs(gx(p_dense)).max()
