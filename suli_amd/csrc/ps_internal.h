@@ -38,21 +38,23 @@ struct Geo {
 // writing are double-buffered by iteration parity (rr) or are benign
 // (done_at only ever moves from "running" to the current iteration).
 struct CgScalars {
-    double rr[2];     // r.r after iteration n in slot n & 1 (slot 0 also holds r0.r0)
-    double alpha;     // alpha of the current iteration (update_r -> update_ep)
-    double beta;
+    double rr[2];     // r.r entering iteration n in slot n & 1 (r_{n-1}.r_{n-1})
+    double alpha;     // alpha of iteration n (cg_b -> next cg_a, refresh)
     double pap;
     double bb;        // b.b (scaled right-hand side)
     double thresh;    // stop threshold on r.r
     double rr0;       // initial r.r
     double rr_true;   // FP64 r.r of the last recomputed true residual
     double gamma;     // null-space component removed from the initial residual
-    int done_at;      // iteration at which CG stopped; kNotDone while running
+    double en0;       // e0 . sqrt(d): the initial gauge (null-space coordinate)
+    double dd;        // sum of d = sqrt(d) . sqrt(d)
+    double gshift;    // constant removed from prd1 to restore the initial gauge
+    int done_at;      // iteration whose cg_a met the stop rule; kNotDone while running
     int final_;       // 1: the solve is over (mixed: decided at a replacement)
-    int iters;        // CG iterations performed (reference `cont`)
+    int iters;        // CG iterations completed (reference `cont`)
     int max_iter;
     int refreshes;    // FP64 residual replacements (mixed)
-    int since_refresh;// iterations since the last replacement
+    int skip_e_at;    // cg_a of this iteration skips the deferred e update
     int status;       // 0 converged, 1 capped
     int mixed;        // 1: convergence is judged on rr_true at replacements
     int project;      // 1: remove the null-space component of the residual
@@ -63,10 +65,10 @@ struct CgScalars {
 
 // Offsets (in doubles) of the per-workgroup partial sums in one buffer.
 struct Partials {
-    double *spmv;     // [B1]      p.Ap of the SpMV
-    double *upd;      // [B2]      r.r of update_r
-    double *tres;     // [2*B1]    true residual r.r (FP64, and of its FP32 copy)
-    double *init;     // [4*B0]    setup: r0.r0, b.b, r0.sqrt(d), d
+    double *spmv;     // [B1]      p.Ap of cg_a
+    double *upd;      // [B2]      r.r of cg_b (or one total, rest zero, after a replacement)
+    double *tres;     // [3*B1]    true residual r.r (FP64, FP32 copy) and e.sqrt(d)
+    double *init;     // [5*B0]    setup: r0.r0, b.b, r0.sqrt(d), d, e0.sqrt(d)
     double *proj;     // [2*B0]    setup after projection: r0.r0, FP32 r0.r0
 };
 
@@ -94,31 +96,37 @@ void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const do
                      double *d, double *b, hipStream_t s);
 void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
                         float *sy32, float *sz32, hipStream_t s);
-// Initial residual, null-space projection, first direction; leaves S ready
-// for iteration 1.  r64/p64 (FP64 solve) or r32/p32/c32 (mixed) are written.
+// Initial residual, null-space projection; leaves r (r64, or r32 and c32 = 0
+// for mixed precision) and S ready for iteration 1.
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
-                 double *r0, double *r64, double *p64, float *r32, float *p32, float *c32, const Partials &P,
-                 CgScalars *S, hipStream_t s);
-void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, hipStream_t s);
+                 double *r0, double *r64, float *r32, float *c32, const Partials &P, CgScalars *S, hipStream_t s);
+// prd1 = e / sqrt(d) - S->gshift on the whole halo box.
+void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
+                    hipStream_t s);
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s);
 
 // ---- CG iteration n (ps_cg.hip), n = 1, 2, ... ----
-// ran (nullable): set to 1 by the launch if it did the work (timing samples)
+// cg_a(n): deferred e -= alpha_{n-1} p_{n-1}; stop rule on r.r; p_n = r +
+// beta p_{n-1} (p_old -> p_new, two buffers); q = A p_n; p.q partials.
+// ran (nullable): set to 1 by the launch if it did the SpMV (timing samples).
 template <typename T>
-void launch_spmv_pap(const Geo &g, int n, const T *p, Op<T> op, T *q, const Partials &P, CgScalars *S, int *ran,
-                     hipStream_t s);
+void launch_cg_a(const Geo &g, int n, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
+                 const Partials &P, CgScalars *S, int *ran, hipStream_t s);
+// cg_b(n): alpha_n = r.r / p.q; r -= alpha_n q; r.r partials.
 template <typename T>
-void launch_update_r(const Geo &g, int n, T *r, const T *q, const Partials &P, CgScalars *S, hipStream_t s);
+void launch_cg_b(const Geo &g, int n, T *r, const T *q, const Partials &P, CgScalars *S, hipStream_t s);
+// Mixed precision residual replacement after iteration n_last: e += c (with
+// the deferred alpha p term), r = b - A e in FP64 handed to the FP32
+// recurrence; decide converged / capped / continue.
+void launch_refresh(const Geo &g, int n_last, double *e, float *c32, const float *p_last, Op<double> op,
+                    const double *b, float *r32, const Partials &P, CgScalars *S, hipStream_t s);
+// After the solve: FP64 true residual r.r into S->rr_true and, when the
+// null space is projected, the gauge shift that restores e0's coordinate.
+void launch_finish(const Geo &g, const double *e, const double *d, Op<double> op, const double *b,
+                   const Partials &P, CgScalars *S, hipStream_t s);
+// Stand-alone SpMV q = A p (no vector updates), for timing.
 template <typename T>
-void launch_update_ep(const Geo &g, int n, T *e, T *p, const T *r, const Partials &P, CgScalars *S,
-                      hipStream_t s);
-// Mixed precision residual replacement: e += c; r = b - A e in FP64 handed
-// to the FP32 recurrence; decide converged / capped / continue.
-void launch_refresh(const Geo &g, double *e, float *c32, Op<double> op, const double *b, float *r32,
-                    const Partials &P, CgScalars *S, hipStream_t s);
-// FP64 true residual r.r into S->rr_true (reporting; runs after the solve).
-void launch_record_true(const Geo &g, const double *e, Op<double> op, const double *b, const Partials &P,
-                        CgScalars *S, hipStream_t s);
+void launch_spmv_only(const Geo &g, const T *p, Op<T> op, T *q, const Partials &P, hipStream_t s);
 
 int spmv_blocks(const Geo &g);
 Geo make_geo(int nx, int ny, int nz, int px, int py);

@@ -196,13 +196,13 @@ __device__ __forceinline__ double e_box(const Geo &g, const double *prd1, const 
 // recomputations (which use mirrored ghosts, as every CG iteration does)
 // reproduce this residual.  With mirror-consistent prd1 ghosts (SuLi's
 // contorno(1) leaves them so), b_eff == b bit for bit.
-// Partials: r.r, b.b, r.sqrt(d), sum of d.
+// Partials: r.r, b.b, r.sqrt(d), sum of d, e.sqrt(d).
 __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, Op<double> op, const double *d,
                                                             double *b, double *e, double *r0, Partials P)
 {
     const int nx = g.nx, ny = g.ny;
     const long long stride = (long long)gridDim.x * kThreads;
-    double acc_rr = 0.0, acc_bb = 0.0, acc_rn = 0.0, acc_dd = 0.0;
+    double acc_rr = 0.0, acc_bb = 0.0, acc_rn = 0.0, acc_dd = 0.0, acc_en = 0.0;
     for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
         const int i = (int)(c % nx);
         const long long jk = c / nx;
@@ -238,14 +238,17 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
         acc_bb += bc * bc;
         acc_rn += r * sqrt(dc);
         acc_dd += dc;
+        acc_en += ec * sqrt(dc);
     }
     const double s0 = block_sum(acc_rr), s1 = block_sum(acc_bb), s2 = block_sum(acc_rn), s3 = block_sum(acc_dd);
+    const double s4 = block_sum(acc_en);
     if (threadIdx.x == 0) {
         const int nb = gridDim.x;
         P.init[blockIdx.x] = s0;
         P.init[nb + blockIdx.x] = s1;
         P.init[2 * nb + blockIdx.x] = s2;
         P.init[3 * nb + blockIdx.x] = s3;
+        P.init[4 * nb + blockIdx.x] = s4;
     }
 }
 
@@ -255,16 +258,17 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
 // consistent only to rounding, and at tight tolerances CG then drifts along
 // n.  When S->project is set the n-component gamma n of the initial
 // residual is removed (gamma = r.n / n.n) and b_eff adjusted to match; this
-// changes the solution only by a constant (the pressure gauge).
-// Then the first search direction p = r (9_grad.f90:372-380).
+// changes the solution only by a constant (the pressure gauge).  The first
+// search direction p = r (9_grad.f90:372-380) is formed by the first cg_a.
 __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const double *d, double *b, const double *r0,
-                                                           double *r64, double *p64, float *r32, float *p32,
-                                                           float *c32, Partials P, int nb0, CgScalars *S)
+                                                           double *r64, float *r32, float *c32, Partials P, int nb0,
+                                                           CgScalars *S)
 {
     const double rr = total_of(P.init, nb0);
     const double bb = total_of(P.init + nb0, nb0);
     const double rn = total_of(P.init + 2 * nb0, nb0);
     const double dd = total_of(P.init + 3 * nb0, nb0);
+    const double en = total_of(P.init + 4 * nb0, nb0);
     const double gamma = S->project ? rn / dd : 0.0;
     const long long stride = (long long)gridDim.x * kThreads;
     double acc = 0.0, acc32 = 0.0;
@@ -275,11 +279,9 @@ __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const double *
         acc += r * r;
         if (r64) {
             r64[c] = r;
-            p64[c] = r;
         } else {
             const float rf = (float)r;
             r32[c] = rf;
-            p32[c] = rf;
             c32[c] = 0.0f;
             acc32 += (double)rf * (double)rf;
         }
@@ -292,6 +294,8 @@ __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const double *
             S->gamma = gamma;
             S->bb = bb;
             S->rr0 = rr;
+            S->en0 = en;
+            S->dd = dd;
             S->thresh = S->rel_rule ? S->tol * S->tol * bb : S->thresh_ref;
         }
     }
@@ -312,7 +316,8 @@ __global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0,
         S->rr_true = rr;
         S->iters = 0;
         S->refreshes = 0;
-        S->since_refresh = 0;
+        S->skip_e_at = 0;
+        S->gshift = 0.0;
         const bool conv = !(fabs(rr) > S->thresh);
         const bool done = conv || S->max_iter <= 0;
         S->done_at = done ? 0 : kNotDone;
@@ -323,8 +328,11 @@ __global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0,
 
 // prd1 = e / sqrt(d) on the whole halo box; ghosts copy the interior by the
 // mirror/periodic rule in x, then y, then z order (9_grad.f90:594-659).
-__global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, const double *d, double *prd1)
+// gshift (0 unless the null space is projected) restores the initial gauge.
+__global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, const double *d, double *prd1,
+                                                      const CgScalars *S)
 {
+    const double gshift = S->gshift;
     const int bx = g.nx + 2, by = g.ny + 2;
     const long long nbox = (long long)bx * by * (g.nz + 2);
     const long long stride = (long long)gridDim.x * kThreads;
@@ -333,7 +341,8 @@ __global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, co
         const long long jk = t / bx;
         const int J = (int)(jk % by), K = (int)(jk / by);
         const long long c = cidx(g, box_to_cell(I, g.nx, g.px), box_to_cell(J, g.ny, g.py), box_to_cell(K, g.nz, 0));
-        prd1[t] = e[c] / sqrt(d[c]);
+        const double p = e[c] / sqrt(d[c]);
+        prd1[t] = gshift != 0.0 ? p - gshift : p;
     }
 }
 
@@ -418,20 +427,19 @@ void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, d
 }
 
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
-                 double *r0, double *r64, double *p64, float *r32, float *p32, float *c32, const Partials &P,
-                 CgScalars *S, hipStream_t s)
+                 double *r0, double *r64, float *r32, float *c32, const Partials &P, CgScalars *S, hipStream_t s)
 {
     const int nb0 = std::min(grid_for(g.n), kVecBlocks);
     hipLaunchKernelGGL(k_init_residual, dim3(nb0), dim3(kThreads), 0, s, g, in, op, d, b, e, r0, P);
-    hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, d, b, r0, r64, p64, r32, p32, c32, P,
-                       nb0, S);
+    hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, d, b, r0, r64, r32, c32, P, nb0, S);
     hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kThreads), 0, s, P, nb0, S);
 }
 
-void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, hipStream_t s)
+void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
+                    hipStream_t s)
 {
     const long long nbox = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
-    hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1);
+    hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1, S);
 }
 
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s)

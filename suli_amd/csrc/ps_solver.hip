@@ -26,11 +26,11 @@ struct Workspace {
     std::vector<void *> allocs;
     // operator (FP64 scaled coefficients, diagonal, effective rhs)
     double *sx = nullptr, *sy = nullptr, *sz = nullptr, *d = nullptr, *b = nullptr;
-    // FP64 CG vectors
-    double *e = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    // FP64 CG vectors (p and p2: the search direction, alternating by iteration)
+    double *e = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
     // FP32 inner-solve operator and vectors (mixed precision)
     float *sx32 = nullptr, *sy32 = nullptr, *sz32 = nullptr;
-    float *c32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr;
+    float *c32 = nullptr, *r32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr;
     double *partial = nullptr;   // one buffer, carved into Partials
     Partials P{};
     CgScalars *S = nullptr;
@@ -130,10 +130,12 @@ int ensure_ws(suli_ps_ctx *ctx, int nx, int ny, int nz, bool mixed)
         ws.nz = nz;
         ws.geo = make_geo(nx, ny, nz, 0, 0);
         const size_t b1 = (size_t)spmv_blocks(ws.geo), b2 = (size_t)kVecBlocks;
-        const size_t npart = b1 + b2 + 2 * b1 + 4 * b2 + 2 * b2;
+        // spmv[b1] upd[b2] tres[3 b1] init[5 b2] proj[2 b2]
+        const size_t npart = b1 + b2 + 3 * b1 + 5 * b2 + 2 * b2;
         if ((rc = dalloc(ctx, &ws.sx, nfx)) || (rc = dalloc(ctx, &ws.sy, nfy)) || (rc = dalloc(ctx, &ws.sz, nfz)) ||
             (rc = dalloc(ctx, &ws.d, n)) || (rc = dalloc(ctx, &ws.b, n)) || (rc = dalloc(ctx, &ws.e, n)) ||
-            (rc = dalloc(ctx, &ws.r, n)) || (rc = dalloc(ctx, &ws.p, n)) || (rc = dalloc(ctx, &ws.q, n)) ||
+            (rc = dalloc(ctx, &ws.r, n)) || (rc = dalloc(ctx, &ws.p, n)) || (rc = dalloc(ctx, &ws.p2, n)) ||
+            (rc = dalloc(ctx, &ws.q, n)) ||
             (rc = dalloc(ctx, &ws.partial, npart)) || (rc = dalloc(ctx, &ws.S, 1)) ||
             (rc = dalloc(ctx, &ws.ran, (size_t)kMaxTimed)) ||
             (rc = dalloc(ctx, &ws.Stime, 1)) || (rc = dalloc(ctx, &ws.u, (size_t)(nx + 3) * (ny + 2) * (nz + 2))) ||
@@ -148,15 +150,15 @@ int ensure_ws(suli_ps_ctx *ctx, int nx, int ny, int nz, bool mixed)
         ws.P.spmv = ws.partial;
         ws.P.upd = ws.P.spmv + b1;
         ws.P.tres = ws.P.upd + b2;
-        ws.P.init = ws.P.tres + 2 * b1;
-        ws.P.proj = ws.P.init + 4 * b2;
+        ws.P.init = ws.P.tres + 3 * b1;
+        ws.P.proj = ws.P.init + 5 * b2;
         HIPCHK(ctx, hipMemsetAsync(ws.S, 0, sizeof(CgScalars), ctx->stream));
         HIPCHK(ctx, hipMemsetAsync(ws.Stime, 0, sizeof(CgScalars), ctx->stream));
     }
     if (mixed && !ws.have_mixed) {
         if ((rc = dalloc(ctx, &ws.sx32, nfx)) || (rc = dalloc(ctx, &ws.sy32, nfy)) ||
             (rc = dalloc(ctx, &ws.sz32, nfz)) || (rc = dalloc(ctx, &ws.c32, n)) || (rc = dalloc(ctx, &ws.r32, n)) ||
-            (rc = dalloc(ctx, &ws.p32, n)) || (rc = dalloc(ctx, &ws.q32, n))) {
+            (rc = dalloc(ctx, &ws.p32, n)) || (rc = dalloc(ctx, &ws.p32b, n)) || (rc = dalloc(ctx, &ws.q32, n))) {
             free_ws(ws);
             return rc;
         }
@@ -257,45 +259,40 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
                        mixed ? ws.sz32 : nullptr, s);
     const Op<double> op64{ws.sx, ws.sy, ws.sz};
     const Op<float> op32{ws.sx32, ws.sy32, ws.sz32};
-    launch_init(g, in, op64, ws.d, ws.b, ws.e, ws.r, mixed ? nullptr : ws.r, mixed ? nullptr : ws.p, ws.r32,
-                ws.p32, ws.c32, ws.P, ws.S, s);
+    launch_init(g, in, op64, ws.d, ws.b, ws.e, ws.r, mixed ? nullptr : ws.r, ws.r32, ws.c32, ws.P, ws.S, s);
     HIPCHK(ctx, hipGetLastError());
     ws.have_operator = true;
 
-    const int K = o.chunk > 0 ? o.chunk : auto_chunk(g);
     const int max_iter = s0.max_iter;
-    // a mixed solve can stop early on its FP32 estimate and resume once per
-    // replacement; bound the chunks generously and let the device decide
-    const int max_chunks = 2 * (max_iter / K) + 8;
-    // SpMV timing: events around every 16th SpMV launch; ws.ran[slot] says
-    // whether that launch did the work (it skips after the stop rule)
+    // SpMV timing: events around every 16th cg_a launch; ws.ran[slot] says
+    // whether that launch did the SpMV (it skips after the stop rule)
     const int tstride = 16;
     int ntimed = 0;
     int n = 0;  // CG iteration number of the launch sequence (1-based)
+    double *p64[2] = {ws.p, ws.p2};
+    float *p32[2] = {ws.p32, ws.p32b};
 
-    auto enqueue_chunk = [&](int chunk_idx) -> int {
+    auto enqueue_chunk = [&](int chunk_idx, int K) -> int {
         for (int it = 0; it < K; ++it) {
             ++n;
-            const bool timed = o.timing && (n % tstride == 1) && ntimed < kMaxTimed;
+            const bool timed = o.timing && (n % tstride == 2) && ntimed < kMaxTimed;
             int *ran = timed ? ws.ran + ntimed : nullptr;
             if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed], s));
             if (mixed)
-                launch_spmv_pap<float>(g, n, ws.p32, op32, ws.q32, ws.P, ws.S, ran, s);
+                launch_cg_a<float>(g, n, ws.c32, p32[(n - 1) & 1], p32[n & 1], ws.r32, op32, ws.q32, ws.P, ws.S, ran,
+                                   s);
             else
-                launch_spmv_pap<double>(g, n, ws.p, op64, ws.q, ws.P, ws.S, ran, s);
+                launch_cg_a<double>(g, n, ws.e, p64[(n - 1) & 1], p64[n & 1], ws.r, op64, ws.q, ws.P, ws.S, ran, s);
             if (timed) {
                 HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed + 1], s));
                 ++ntimed;
             }
-            if (mixed) {
-                launch_update_r<float>(g, n, ws.r32, ws.q32, ws.P, ws.S, s);
-                launch_update_ep<float>(g, n, ws.c32, ws.p32, ws.r32, ws.P, ws.S, s);
-            } else {
-                launch_update_r<double>(g, n, ws.r, ws.q, ws.P, ws.S, s);
-                launch_update_ep<double>(g, n, ws.e, ws.p, ws.r, ws.P, ws.S, s);
-            }
+            if (mixed)
+                launch_cg_b<float>(g, n, ws.r32, ws.q32, ws.P, ws.S, s);
+            else
+                launch_cg_b<double>(g, n, ws.r, ws.q, ws.P, ws.S, s);
         }
-        if (mixed) launch_refresh(g, ws.e, ws.c32, op64, ws.b, ws.r32, ws.P, ws.S, s);
+        if (mixed) launch_refresh(g, n, ws.e, ws.c32, p32[n & 1], op64, ws.b, ws.r32, ws.P, ws.S, s);
         HIPCHK(ctx, hipGetLastError());
         const int slot = chunk_idx & 1;
         HIPCHK(ctx, hipMemcpyAsync(&ctx->host_status[slot], ws.S, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
@@ -303,25 +300,60 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
         return 0;
     };
 
+    // Chunk sizes.  FP64: fixed; the stop rule is tested on the device every
+    // iteration and later launches return at once.  Mixed: every chunk ends
+    // with an FP64 residual replacement, which is also where convergence is
+    // decided, so chunk lengths follow the observed convergence rate: long
+    // enough to amortise the replacement, short enough not to overshoot the
+    // tolerance and to keep the FP32 recurrence within ~3 decades of the
+    // residual it started from.
+    const int K_fixed = o.chunk > 0 ? o.chunk : auto_chunk(g);
+    int known_iters = 0;          // from the latest status read
+    double known_rr = 0.0, known_thresh = 0.0, rate = 0.0;
+    std::vector<int> chunk_len;
+    auto next_chunk = [&](int c) -> int {
+        if (!mixed || o.chunk > 0) return K_fixed;
+        if (!(rate < 0.0) || !(known_rr > 0.0)) return 32;
+        int in_flight = 0;  // iterations enqueued beyond the known status
+        for (int i = 0; i < c; ++i) in_flight += chunk_len[i];
+        in_flight -= known_iters;
+        const double remaining =
+            known_thresh > 0 ? std::log(known_thresh / known_rr) / rate - in_flight : 32.0;
+        const double fp32_span = std::log(1e-6) / rate;  // rr drop of 1e6 at most
+        double k = std::min(remaining + 2.0, fp32_span);
+        return (int)std::max(8.0, std::min(128.0, std::ceil(k)));
+    };
+
     // Two chunks in flight: the host reads chunk c-1's state while chunk c
-    // runs.  Launches after the stop rule return at once.
+    // runs.
     bool finished = false;
-    if ((rc = enqueue_chunk(0))) return rc;
+    const int max_chunks = max_iter / 8 + 16;
+    chunk_len.push_back(next_chunk(0));
+    if ((rc = enqueue_chunk(0, chunk_len[0]))) return rc;
     for (int c = 1; c <= max_chunks; ++c) {
-        if ((rc = enqueue_chunk(c))) return rc;
+        chunk_len.push_back(next_chunk(c));
+        if ((rc = enqueue_chunk(c, chunk_len[c]))) return rc;
         const int prev = (c - 1) & 1;
         HIPCHK(ctx, hipEventSynchronize(ctx->ev_status[prev]));
-        if (ctx->host_status[prev].final_) {
+        const CgScalars &st = ctx->host_status[prev];
+        if (st.final_) {
             finished = true;
             break;
         }
+        if (mixed && st.iters > known_iters && st.rr_true > 0.0) {
+            if (known_rr > 0.0 && st.rr_true < known_rr)
+                rate = std::log(st.rr_true / known_rr) / (st.iters - known_iters);
+            known_iters = st.iters;
+            known_rr = st.rr_true;
+        }
+        known_thresh = st.thresh;
     }
     if (!finished) {
         HIPCHK(ctx, hipStreamSynchronize(s));
         return fail(ctx, SULI_PS_EHIP, "solver did not finish within %d chunks", max_chunks);
     }
-    if (!mixed) launch_record_true(g, ws.e, op64, ws.b, ws.P, ws.S, s);
-    launch_unscale(g, ws.e, ws.d, prd1, s);
+    launch_finish(g, ws.e, ws.d, op64, ws.b, ws.P, ws.S, s);
+    launch_unscale(g, ws.e, ws.d, prd1, ws.S, s);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev_solve[1], s));
     CgScalars fin{};
@@ -588,18 +620,14 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
     const bool mixed = precision == SULI_PS_MIXED;
     if (mixed && !ws.have_mixed) return fail(ctx, SULI_PS_EINVAL, "no FP32 operator: solve in mixed mode first");
     hipStream_t s = ctx->stream;
-    CgScalars s0{};
-    s0.rr[0] = s0.rr[1] = 1.0;
-    s0.done_at = kNotDone;
-    HIPCHK(ctx, hipMemcpyAsync(ws.Stime, &s0, sizeof s0, hipMemcpyHostToDevice, s));
     const Geo &g = ws.geo;
+    // the plain SpMV q = A p (5 words per cell) on the last solve's operator:
     // one untimed launch, then `reps` launches between two events
     auto launch = [&]() {
         if (mixed)
-            launch_spmv_pap<float>(g, 1, ws.p32, Op<float>{ws.sx32, ws.sy32, ws.sz32}, ws.q32, ws.P, ws.Stime,
-                                   nullptr, s);
+            launch_spmv_only<float>(g, ws.p32, Op<float>{ws.sx32, ws.sy32, ws.sz32}, ws.q32, ws.P, s);
         else
-            launch_spmv_pap<double>(g, 1, ws.p, Op<double>{ws.sx, ws.sy, ws.sz}, ws.q, ws.P, ws.Stime, nullptr, s);
+            launch_spmv_only<double>(g, ws.p, Op<double>{ws.sx, ws.sy, ws.sz}, ws.q, ws.P, s);
     };
     launch();
     HIPCHK(ctx, hipEventRecord(ctx->ev_solve[0], s));
