@@ -56,7 +56,12 @@ __device__ __forceinline__ Tile tile_of(const Geo &g)
 // hits).  Tiles are ordered x, y, z and given to XCDs in contiguous ranges.
 //   (A p)_c = p_c - sx_E p_E - sx_W p_W - sy_N p_N - sy_S p_S - sz_T p_T - sz_B p_B
 // (9_grad.f90:408-420; unit diagonal after the symmetric Jacobi scaling).
-template <typename T>
+// FIRST: iteration 1 (p_1 = r_0, nothing to finish).  DO_E: apply the
+// deferred e -= alpha p of iteration n-1 (false right after a residual
+// replacement, which folded it into e already).  Both are template
+// parameters so that the plane loop has no branches: every load of a plane
+// is issued before any of them is waited for.
+template <typename T, bool FIRST, bool DO_E>
 __global__ __launch_bounds__(kThreads) void k_cg_a(Geo g, int n, T *__restrict__ e, const T *__restrict__ p_old,
                                                    T *__restrict__ p_new, const T *__restrict__ r,
                                                    const T *__restrict__ sx, const T *__restrict__ sy,
@@ -65,11 +70,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(Geo g, int n, T *__restrict__
                                                    double *__restrict__ pap_part, CgScalars *S, int *ran)
 {
     if (n > S->done_at) return;
-    const bool first = n == 1;
+    constexpr bool first = FIRST;
+    constexpr bool do_e = DO_E && !FIRST;
     // r.r entering this iteration: r_0.r_0 from setup, else cg_b(n-1)'s partials
     const double rr_in = first ? S->rr[1] : total_of(rr_part, n_rr);
     const double beta = first ? 0.0 : rr_in / S->rr[(n - 1) & 1];   // 9_grad.f90:510-515
-    const bool do_e = !first && n != S->skip_e_at;
     const T a_prev = (T)S->alpha;
     // stop rule (9_grad.f90:398,582): FP64 tests the recurrence's r.r; the
     // mixed solver decides at its FP64 residual replacements
@@ -113,11 +118,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(Geo g, int n, T *__restrict__
             T szb = sz[c];
             for (int k = t.k0; k < t.k1; ++k) {
                 const long long r0 = c - t.i;
-                T pot = T(0), pt = pc;
-                if (k < nz - 1) {
-                    pot = first ? T(0) : p_old[c + plane];
-                    pt = first ? r[c + plane] : r[c + plane] + b * pot;
-                }
+                // top neighbour; at the lid the mirror ghost is the cell itself
+                const long long up = (k < nz - 1) ? plane : 0;
+                const T pot = first ? T(0) : p_old[c + up];
+                const T pt = first ? r[c + up] : r[c + up] + b * pot;
                 const T szt = sz[c + plane];
                 const T pw = pn(r0 + iw), pe = pn(r0 + ie);
                 const T ps = pn(c + dS), pnn = pn(c + dN);
@@ -303,10 +307,9 @@ __global__ __launch_bounds__(kThreads) void k_refresh_decide(int n_last, const d
             S->status = conv ? 0 : 1;
             // launches after n_last skip (cg_a tests done_at only)
             if (pending) S->done_at = n_last;
-        } else {
-            // the deferred alpha p of iteration n_last is already in e
-            S->skip_e_at = n_last + 1;
         }
+        // otherwise continue: the deferred alpha p of iteration n_last is
+        // already in e, so the host launches cg_a(n_last + 1) without it
     }
 }
 
@@ -395,11 +398,19 @@ Geo make_geo(int nx, int ny, int nz, int px, int py)
 int spmv_blocks(const Geo &g) { return g.tiles_x * g.tiles_y * g.tiles_z; }
 
 template <typename T>
-void launch_cg_a(const Geo &g, int n, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
+void launch_cg_a(const Geo &g, int n, bool do_e, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
                  const Partials &P, CgScalars *S, int *ran, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg_a<T>, dim3(spmv_blocks(g)), dim3(kThreads), 0, s, g, n, e, p_old, p_new, r, op.sx,
-                       op.sy, op.sz, q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
+    const dim3 grid(spmv_blocks(g)), block(kThreads);
+    if (n == 1)
+        hipLaunchKernelGGL((k_cg_a<T, true, false>), grid, block, 0, s, g, n, e, p_old, p_new, r, op.sx, op.sy, op.sz,
+                           q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
+    else if (do_e)
+        hipLaunchKernelGGL((k_cg_a<T, false, true>), grid, block, 0, s, g, n, e, p_old, p_new, r, op.sx, op.sy,
+                           op.sz, q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
+    else
+        hipLaunchKernelGGL((k_cg_a<T, false, false>), grid, block, 0, s, g, n, e, p_old, p_new, r, op.sx, op.sy,
+                           op.sz, q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
 }
 
 template <typename T>
@@ -433,9 +444,9 @@ void launch_spmv_only(const Geo &g, const T *p, Op<T> op, T *q, const Partials &
     hipLaunchKernelGGL(k_spmv<T>, dim3(spmv_blocks(g)), dim3(kThreads), 0, s, g, p, op.sx, op.sy, op.sz, q, P.spmv);
 }
 
-template void launch_cg_a<double>(const Geo &, int, double *, const double *, double *, const double *, Op<double>,
+template void launch_cg_a<double>(const Geo &, int, bool, double *, const double *, double *, const double *, Op<double>,
                                   double *, const Partials &, CgScalars *, int *, hipStream_t);
-template void launch_cg_a<float>(const Geo &, int, float *, const float *, float *, const float *, Op<float>,
+template void launch_cg_a<float>(const Geo &, int, bool, float *, const float *, float *, const float *, Op<float>,
                                  float *, const Partials &, CgScalars *, int *, hipStream_t);
 template void launch_cg_b<double>(const Geo &, int, double *, const double *, const Partials &, CgScalars *,
                                   hipStream_t);

@@ -54,7 +54,6 @@ struct CgScalars {
     int iters;        // CG iterations completed (reference `cont`)
     int max_iter;
     int refreshes;    // FP64 residual replacements (mixed)
-    int skip_e_at;    // cg_a of this iteration skips the deferred e update
     int status;       // 0 converged, 1 capped
     int mixed;        // 1: convergence is judged on rr_true at replacements
     int project;      // 1: remove the null-space component of the residual
@@ -109,8 +108,10 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
 // cg_a(n): deferred e -= alpha_{n-1} p_{n-1}; stop rule on r.r; p_n = r +
 // beta p_{n-1} (p_old -> p_new, two buffers); q = A p_n; p.q partials.
 // ran (nullable): set to 1 by the launch if it did the SpMV (timing samples).
+// do_e = false right after a residual replacement (the deferred update is
+// already in e).
 template <typename T>
-void launch_cg_a(const Geo &g, int n, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
+void launch_cg_a(const Geo &g, int n, bool do_e, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
                  const Partials &P, CgScalars *S, int *ran, hipStream_t s);
 // cg_b(n): alpha_n = r.r / p.q; r -= alpha_n q; r.r partials.
 template <typename T>

@@ -316,7 +316,6 @@ __global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0,
         S->rr_true = rr;
         S->iters = 0;
         S->refreshes = 0;
-        S->skip_e_at = 0;
         S->gshift = 0.0;
         const bool conv = !(fabs(rr) > S->thresh);
         const bool done = conv || S->max_iter <= 0;

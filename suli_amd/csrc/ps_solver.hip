@@ -277,12 +277,16 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
             ++n;
             const bool timed = o.timing && (n % tstride == 2) && ntimed < kMaxTimed;
             int *ran = timed ? ws.ran + ntimed : nullptr;
+            // after a residual replacement (every mixed chunk but the first
+            // starts after one) the deferred e update is already in e
+            const bool do_e = !(mixed && it == 0 && chunk_idx > 0);
             if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed], s));
             if (mixed)
-                launch_cg_a<float>(g, n, ws.c32, p32[(n - 1) & 1], p32[n & 1], ws.r32, op32, ws.q32, ws.P, ws.S, ran,
-                                   s);
+                launch_cg_a<float>(g, n, do_e, ws.c32, p32[(n - 1) & 1], p32[n & 1], ws.r32, op32, ws.q32, ws.P,
+                                   ws.S, ran, s);
             else
-                launch_cg_a<double>(g, n, ws.e, p64[(n - 1) & 1], p64[n & 1], ws.r, op64, ws.q, ws.P, ws.S, ran, s);
+                launch_cg_a<double>(g, n, do_e, ws.e, p64[(n - 1) & 1], p64[n & 1], ws.r, op64, ws.q, ws.P, ws.S,
+                                    ran, s);
             if (timed) {
                 HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed + 1], s));
                 ++ntimed;
