@@ -10,10 +10,12 @@ the same steps for the comparison.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-solves C]
 
-Prints one JSON line.  `roofline` is for the FP32 SpMV kernel (k_spmv_pap):
-algorithmic bytes 5 words/cell (p, three face coefficients, Ap) divided by
-its mean launch duration, timed with HIP events on the solver's stream
-during the timed solves.  `cpu_baseline` runs the reference's own Fortran
+Prints one JSON line.  `roofline` is for the dominant kernel, the fused
+FP32 CG phase k_cg_a<float> (SpMV q = A p fused with the deferred update of
+the iterate and the direction update): algorithmic bytes 9 words/cell (reads
+c, p_{n-1}, r, sx, sy, sz; writes c, p_n, q) divided by its mean launch
+duration, timed with HIP events on the solver's stream during the timed
+solves (every 16th iteration).  `cpu_baseline` runs the reference's own Fortran
 graddin (oracle/_ref, compiled from the SuLi sources) with OpenMP on the
 host cores, on one solve of the same case.
 """
@@ -138,17 +140,17 @@ def main():
     df.free()
     solver.close()
 
-    spmv_bytes32 = 5 * 4 * n
+    bytes32 = 9 * 4 * n
     spmv_ms = mixed["spmv_ms"]
-    achieved = spmv_bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None
+    achieved = bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                "traffic": load_traffic("k_spmv_pap<float> 256x128x96"),
-                "kernel": "k_spmv_pap<float>", "bytes_per_launch": spmv_bytes32,
+                "traffic": load_traffic(f"k_cg_a<float> {g.nx}x{g.ny}x{g.nz}"),
+                "kernel": "k_cg_a<float>", "bytes_per_launch": bytes32,
                 "launch_ms": spmv_ms,
-                "fp64_spmv": {"launch_ms": fp64["spmv_ms"],
-                              "achieved": (2 * spmv_bytes32 / (fp64["spmv_ms"] * 1e-3) / 1e9)
-                              if fp64["spmv_ms"] else None}}
+                "fp64_kernel": {"kernel": "k_cg_a<double>", "launch_ms": fp64["spmv_ms"],
+                                "achieved": (2 * bytes32 / (fp64["spmv_ms"] * 1e-3) / 1e9)
+                                if fp64["spmv_ms"] else None}}
 
     cpu = None
     if not args.no_cpu:

@@ -129,6 +129,26 @@ int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, doubl
                        double *w, const double *rho, const double *prd1,
                        const double *dz, const double *dzz, double *prd);
 
+/* One problem of a batch: its grid, device arrays (layouts as above) and,
+ * on return, its statistics. */
+typedef struct suli_ps_problem {
+    suli_ps_grid grid;
+    const double *u, *v, *w, *rho;
+    const double *obs_lsx, *obs_lsy, *obs_lsz;
+    const double *dz, *dzz;
+    double *prd1;
+    suli_ps_stats stats;
+} suli_ps_problem;
+
+/* graddin on `count` independent problems (device pointers, any mix of
+ * sizes), solved together: each CG phase is one launch over the tiles of
+ * every problem still iterating.  Each problem stops by its own rule and
+ * its result is bit-identical to solving it alone with suli_ps_graddin_dev.
+ * The reference solves one grid per call (9_grad.f90:10); this is the same
+ * operation applied to many.  Returns the largest per-problem status (0 or
+ * 1) or an error; stats.solve_ms is the time of the whole batch. */
+int suli_ps_graddin_batch_dev(suli_ps_ctx *ctx, suli_ps_problem *probs, int32_t count);
+
 /* Device-memory helpers for callers without their own HIP code. */
 void *suli_ps_dev_alloc(suli_ps_ctx *ctx, size_t bytes);
 int suli_ps_dev_free(suli_ps_ctx *ctx, void *ptr);

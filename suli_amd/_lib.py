@@ -25,6 +25,7 @@ EXPORTS = (
     "suli_ps_last_error", "suli_ps_graddin", "suli_ps_posdin", "suli_ps_graddin_dev",
     "suli_ps_posdin_dev", "suli_ps_dev_alloc", "suli_ps_dev_free", "suli_ps_memcpy_h2d",
     "suli_ps_memcpy_d2h", "suli_ps_dev_zero", "suli_ps_synchronize", "suli_ps_time_spmv",
+    "suli_ps_graddin_batch_dev",
 )
 
 
@@ -56,6 +57,12 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Problem(C.Structure):
+    _fields_ = [("grid", Grid)] + [(n, C.c_void_p) for n in (
+        "u", "v", "w", "rho", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "prd1")] + [
+        ("stats", Stats)]
 
 
 _lib = None
@@ -92,5 +99,6 @@ def load(path: str = LIB_PATH):
     L.suli_ps_dev_zero.argtypes = [vp, vp, C.c_size_t]
     L.suli_ps_synchronize.argtypes = [vp]
     L.suli_ps_time_spmv.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    L.suli_ps_graddin_batch_dev.argtypes = [vp, C.POINTER(Problem), C.c_int32]
     _lib = L
     return L

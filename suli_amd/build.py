@@ -29,6 +29,7 @@ COMMON = [
 UNITS = {
     "ps_setup.hip": ["-ffp-contract=off"],
     "ps_cg.hip": [],
+    "ps_engine.hip": [],
     "ps_solver.hip": [],
 }
 

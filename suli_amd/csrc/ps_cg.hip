@@ -1,12 +1,13 @@
-// CG iteration kernels (the hot path).
+// CG iteration kernels (the hot path), batched over problems.
 //
 // SuLi's CG iteration (SuLi-IC/9_grad.f90:398-587) is restated for a
-// bandwidth-bound device as two launches per iteration n:
-//   cg_a(n)  finishes iteration n-1 (e -= alpha p, deferred), tests the
-//            stop rule on r.r, forms p_n = r + beta p_{n-1} (on the fly at
-//            every stencil point, into the other p buffer), q = A p_n, and
-//            one p.q partial per workgroup;
-//   cg_b(n)  alpha = r.r / p.q; r -= alpha q; one r.r partial per workgroup.
+// bandwidth-bound device as two launches per global step; a step runs one
+// iteration n of every problem in the launch table:
+//   cg_a  finishes iteration n-1 (e -= alpha p, deferred), tests the stop
+//         rule on r.r, forms p_n = r + beta p_{n-1} (on the fly at every
+//         stencil point, into the other p buffer), q = A p_n, and one p.q
+//         partial per tile;
+//   cg_b  alpha = r.r / p.q; r -= alpha q; one r.r partial per tile.
 // That moves 9 + 3 = 12 words per cell instead of the reference's 17
 // (cg_a reads e, p_old, r and the three face-coefficient arrays and writes
 // e, p_new and q; cg_b reads r and q and writes r).  The reference's ghost
@@ -14,8 +15,15 @@
 // indices.  Totals of partial sums are formed by the consuming kernel (see
 // CgScalars), so no kernel needs a device-wide fence.
 //
-// T is double for the FP64 solver and float for the inner solve of the
-// mixed-precision one (e is then the FP32 correction c); dot products and
+// Kernels are persistent: a fixed number of workgroups walks the launch
+// table of (problem, tile) entries.  Each workgroup takes a contiguous run
+// of entries and workgroups on one XCD take neighbouring runs, so tiles
+// that share halo planes share an L2.  A problem's arithmetic depends only
+// on its own tiling, never on which workgroup handles a tile or what else
+// is in the batch: results are identical alone and in a mix.
+//
+// Precision: double for the FP64 solver; float for the inner solve of the
+// mixed-precision one (e is then the FP32 correction c).  Dot products and
 // the CG scalars are always FP64.
 #include <algorithm>
 
@@ -30,14 +38,28 @@ struct alignas(16) Vec {
     T v[W];
 };
 
-// Tile of the 3-D kernels: 64 x 4 columns, kz planes marched in z.
+// Contiguous run [begin, end) of the launch table for this workgroup.
+struct Run {
+    int begin, end;
+};
+
+__device__ __forceinline__ Run my_run(int ntab)
+{
+    const int G = gridDim.x;
+    const long long L = xcd_tile(blockIdx.x, G);
+    Run r;
+    r.begin = (int)(L * ntab / G);
+    r.end = (int)((L + 1) * ntab / G);
+    return r;
+}
+
+// 3-D tile `tile` of a problem: 64 x 4 columns, planes [k0, k1).
 struct Tile {
     int i, j, k0, k1;
 };
 
-__device__ __forceinline__ Tile tile_of(const Geo &g)
+__device__ __forceinline__ Tile tile3(const Geo &g, int tile)
 {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % g.tiles_x;
     const int rest = tile / g.tiles_x;
     const int ty = rest % g.tiles_y;
@@ -50,254 +72,363 @@ __device__ __forceinline__ Tile tile_of(const Geo &g)
     return t;
 }
 
-// cg_a(n).  Each lane marches its (i, j) column over kz planes keeping p_n
-// at k-1, k, k+1 (and p_{n-1} at k) in registers; x/y neighbours of p_n are
-// formed from r and p_{n-1} loaded from the same or adjacent rows (L1/L2
-// hits).  Tiles are ordered x, y, z and given to XCDs in contiguous ranges.
+// Per-workgroup cache of a problem's partial-sum total.
+struct TotalCache {
+    int prob = -1;
+    double value = 0.0;
+    __device__ __forceinline__ double get(int p, const double *partial, int n)
+    {
+        if (p != prob) {
+            value = total_of(partial, n);
+            prob = p;
+        }
+        return value;
+    }
+};
+
+// One 3-D tile of cg_a.  Each lane marches its (i, j) column over the
+// tile's planes keeping p_n at k-1, k, k+1 (and p_{n-1} at k) in registers;
+// x/y neighbours of p_n are formed from r and p_{n-1} loaded from the same
+// or adjacent rows (L1/L2 hits).
 //   (A p)_c = p_c - sx_E p_E - sx_W p_W - sy_N p_N - sy_S p_S - sz_T p_T - sz_B p_B
 // (9_grad.f90:408-420; unit diagonal after the symmetric Jacobi scaling).
-// FIRST: iteration 1 (p_1 = r_0, nothing to finish).  DO_E: apply the
-// deferred e -= alpha p of iteration n-1 (false right after a residual
-// replacement, which folded it into e already).  Both are template
-// parameters so that the plane loop has no branches: every load of a plane
-// is issued before any of them is waited for.
+// FIRST (iteration 1: p_1 = r_0, nothing to finish) and DO_E (apply the
+// deferred e -= alpha p) are template parameters so that the plane loop has
+// no branches: every load of a plane is issued before any is waited for.
 template <typename T, bool FIRST, bool DO_E>
-__global__ __launch_bounds__(kThreads) void k_cg_a(Geo g, int n, T *__restrict__ e, const T *__restrict__ p_old,
-                                                   T *__restrict__ p_new, const T *__restrict__ r,
-                                                   const T *__restrict__ sx, const T *__restrict__ sy,
-                                                   const T *__restrict__ sz, T *__restrict__ q,
-                                                   const double *__restrict__ rr_part, int n_rr,
-                                                   double *__restrict__ pap_part, CgScalars *S, int *ran)
+__device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, T *__restrict__ e,
+                                            const T *__restrict__ p_old, T *__restrict__ p_new,
+                                            const T *__restrict__ r, const T *__restrict__ sx,
+                                            const T *__restrict__ sy, const T *__restrict__ sz, T *__restrict__ q,
+                                            T b, T a_prev)
 {
-    if (n > S->done_at) return;
-    constexpr bool first = FIRST;
-    constexpr bool do_e = DO_E && !FIRST;
-    // r.r entering this iteration: r_0.r_0 from setup, else cg_b(n-1)'s partials
-    const double rr_in = first ? S->rr[1] : total_of(rr_part, n_rr);
-    const double beta = first ? 0.0 : rr_in / S->rr[(n - 1) & 1];   // 9_grad.f90:510-515
-    const T a_prev = (T)S->alpha;
-    // stop rule (9_grad.f90:398,582): FP64 tests the recurrence's r.r; the
-    // mixed solver decides at its FP64 residual replacements
-    const bool conv = !(fabs(rr_in) > S->thresh);
-    const bool cap = n - 1 >= S->max_iter;
-    const bool stop = cap || (conv && !S->mixed);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->rr[n & 1] = rr_in;
-        S->iters = n - 1;
-        if (stop) {
-            S->done_at = n;
-            if (!S->mixed) {
-                S->final_ = 1;
-                S->status = conv ? 0 : 1;
-            }
-        }
-        if (ran && !stop) *ran = 1;
-    }
-    const Tile t = tile_of(g);
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
     double acc = 0.0;
-    if (t.i < nx && t.j < ny) {
-        long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-        if (stop) {
-            // finish iteration n-1 only (e -= alpha p, 9_grad.f90:458-468)
-            if (do_e)
-                for (int k = t.k0; k < t.k1; ++k, c += plane) e[c] = e[c] - a_prev * p_old[c];
-        } else {
-            const T b = (T)beta;
-            const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
-            const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
-            const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
-            long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
-            long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-            // p_n = r + beta p_{n-1} (9_grad.f90:518-527), formed where needed
-            auto pn = [&](long long x) -> T { return first ? r[x] : r[x] + b * p_old[x]; };
-            T poc = first ? T(0) : p_old[c];
-            T pc = first ? r[c] : r[c] + b * poc;
-            T pb = t.k0 > 0 ? pn(c - plane) : pc;
-            T szb = sz[c];
-            for (int k = t.k0; k < t.k1; ++k) {
-                const long long r0 = c - t.i;
-                // top neighbour; at the lid the mirror ghost is the cell itself
-                const long long up = (k < nz - 1) ? plane : 0;
-                const T pot = first ? T(0) : p_old[c + up];
-                const T pt = first ? r[c + up] : r[c + up] + b * pot;
-                const T szt = sz[c + plane];
-                const T pw = pn(r0 + iw), pe = pn(r0 + ie);
-                const T ps = pn(c + dS), pnn = pn(c + dN);
-                const T sxw = sx[fx], sxe = sx[fx + 1];
-                const T sys = sy[fy], syn = sy[fy + nx];
-                const T y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * pt - szb * pb;
-                q[c] = y;
-                p_new[c] = pc;
-                if (do_e) e[c] = e[c] - a_prev * poc;
-                acc += (double)pc * (double)y;
-                pb = pc;
-                pc = pt;
-                poc = pot;
-                szb = szt;
-                c += plane;
-                fx += (long long)ny * (nx + 1);
-                fy += (long long)(ny + 1) * nx;
-            }
-        }
+    if (t.i >= nx || t.j >= ny) return acc;
+    long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
+    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
+    const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
+    const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
+    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
+    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+    // p_n = r + beta p_{n-1} (9_grad.f90:518-527), formed where needed
+    auto pn = [&](long long x) -> T { return FIRST ? r[x] : r[x] + b * p_old[x]; };
+    T poc = FIRST ? T(0) : p_old[c];
+    T pc = FIRST ? r[c] : r[c] + b * poc;
+    T pb = t.k0 > 0 ? pn(c - plane) : pc;
+    T szb = sz[c];
+    for (int k = t.k0; k < t.k1; ++k) {
+        const long long r0 = c - t.i;
+        // top neighbour; at the lid the mirror ghost is the cell itself
+        const long long up = (k < nz - 1) ? plane : 0;
+        const T pot = FIRST ? T(0) : p_old[c + up];
+        const T pt = FIRST ? r[c + up] : r[c + up] + b * pot;
+        const T szt = sz[c + plane];
+        const T pw = pn(r0 + iw), pe = pn(r0 + ie);
+        const T ps = pn(c + dS), pnn = pn(c + dN);
+        const T sxw = sx[fx], sxe = sx[fx + 1];
+        const T sys = sy[fy], syn = sy[fy + nx];
+        const T y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * pt - szb * pb;
+        q[c] = y;
+        p_new[c] = pc;
+        if (DO_E) e[c] = e[c] - a_prev * poc;   // 9_grad.f90:458-468
+        acc += (double)pc * (double)y;
+        pb = pc;
+        pc = pt;
+        poc = pot;
+        szb = szt;
+        c += plane;
+        fx += (long long)ny * (nx + 1);
+        fy += (long long)(ny + 1) * nx;
     }
-    if (stop) return;
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) pap_part[blockIdx.x] = s;
+    return acc;
 }
 
-// cg_b(n): alpha = r.r / p.Ap (9_grad.f90:450-455); r -= alpha q
-// (:470-480); r.r partials (:482-508).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_cg_b(long long len, int n, T *__restrict__ r, const T *__restrict__ q,
-                                                   const double *__restrict__ pap_part, int n_pap,
-                                                   double *__restrict__ partial, CgScalars *S)
+struct Vecs {
+    T *e, *r, *q, *p0, *p1;
+    const T *sx, *sy, *sz;
+};
+
+__device__ __forceinline__ Vecs<double> vecs(const ProbDev &pd, double)
 {
-    if (n >= S->done_at) return;
-    const double pap = total_of(pap_part, n_pap);
-    const double alpha = S->rr[n & 1] / pap;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->pap = pap;
-        S->alpha = alpha;
+    return {pd.e, pd.r, pd.q, pd.p0, pd.p1, pd.sx, pd.sy, pd.sz};
+}
+__device__ __forceinline__ Vecs<float> vecs(const ProbDev &pd, float)
+{
+    return {pd.c32, pd.r32, pd.q32, pd.p32_0, pd.p32_1, pd.sx32, pd.sy32, pd.sz32};
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
+                                                   int ntab, int step, int *ran)
+{
+    const Run run = my_run(ntab);
+    TotalCache rr_tot;
+    for (int t = run.begin; t < run.end; ++t) {
+        const TileRef tr = tab[t];
+        const ProbDev &pd = P[tr.prob];
+        CgScalars *S = pd.S;
+        const int n = step - pd.n0 + 1;
+        if (n < 1 || n > S->done_at) continue;
+        const bool first = n == 1;
+        const bool do_e = !first && step != S->skip_e_at;
+        // r.r entering this iteration: r_0.r_0 from setup, else cg_b's partials
+        const double rr_in = first ? S->rr[1] : rr_tot.get(tr.prob, pd.P.upd, pd.nt1);
+        const double beta = first ? 0.0 : rr_in / S->rr[(n - 1) & 1];   // 9_grad.f90:510-515
+        // stop rule (9_grad.f90:398,582): FP64 tests the recurrence's r.r;
+        // the mixed solver decides at its FP64 residual replacements
+        const bool conv = !(fabs(rr_in) > S->thresh);
+        const bool cap = n - 1 >= S->max_iter;
+        const bool stop = cap || (conv && !S->mixed);
+        if (tr.tile == 0 && threadIdx.x == 0) {
+            S->rr[n & 1] = rr_in;
+            S->iters = n - 1;
+            if (stop) {
+                S->done_at = n;
+                if (!S->mixed) {
+                    S->final_ = 1;
+                    S->status = conv ? 0 : 1;
+                }
+            }
+            if (ran && tr.prob == 0 && !stop) *ran = 1;
+        }
+        const Vecs<T> V = vecs(pd, T());
+        T *p_old = (n & 1) ? V.p0 : V.p1;
+        T *p_new = (n & 1) ? V.p1 : V.p0;
+        const T a_prev = (T)S->alpha;
+        const Tile tl = tile3(pd.g, tr.tile);
+        if (stop) {
+            // finish iteration n-1 only (e -= alpha p, 9_grad.f90:458-468)
+            if (do_e && tl.i < pd.g.nx && tl.j < pd.g.ny) {
+                const long long plane = (long long)pd.g.nx * pd.g.ny;
+                long long c = (long long)tl.k0 * plane + (long long)tl.j * pd.g.nx + tl.i;
+                for (int k = tl.k0; k < tl.k1; ++k, c += plane) V.e[c] = V.e[c] - a_prev * p_old[c];
+            }
+            continue;
+        }
+        const T b = (T)beta;
+        double acc;
+        if (first)
+            acc = cg_a_tile<T, true, false>(pd.g, tl, V.e, p_old, p_new, V.r, V.sx, V.sy, V.sz, V.q, b, a_prev);
+        else if (do_e)
+            acc = cg_a_tile<T, false, true>(pd.g, tl, V.e, p_old, p_new, V.r, V.sx, V.sy, V.sz, V.q, b, a_prev);
+        else
+            acc = cg_a_tile<T, false, false>(pd.g, tl, V.e, p_old, p_new, V.r, V.sx, V.sy, V.sz, V.q, b, a_prev);
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) pd.P.spmv[tr.tile] = s;
     }
-    const T a = (T)alpha;
-    constexpr int W = Vec<T>::W;
-    const long long nv = len / W;
-    const long long stride = (long long)gridDim.x * kThreads;
-    double acc = 0.0;
-    for (long long v = (long long)blockIdx.x * kThreads + threadIdx.x; v < nv; v += stride) {
-        Vec<T> rv = reinterpret_cast<Vec<T> *>(r)[v];
-        const Vec<T> qv = reinterpret_cast<const Vec<T> *>(q)[v];
+}
+
+// cg_b: alpha = r.r / p.Ap (9_grad.f90:450-455); r -= alpha q (:470-480);
+// r.r partials (:482-508).  1-D tile k of a problem with nt1 tiles handles
+// the 16-byte vectors k*256 + lane + m*nt1*256.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
+                                                   int ntab, int step)
+{
+    const Run run = my_run(ntab);
+    TotalCache pap_tot;
+    for (int t = run.begin; t < run.end; ++t) {
+        const TileRef tr = tab[t];
+        const ProbDev &pd = P[tr.prob];
+        CgScalars *S = pd.S;
+        const int n = step - pd.n0 + 1;
+        if (n < 1 || n >= S->done_at) continue;
+        const double pap = pap_tot.get(tr.prob, pd.P.spmv, pd.nt3);
+        const double alpha = S->rr[n & 1] / pap;
+        if (tr.tile == 0 && threadIdx.x == 0) {
+            S->pap = pap;
+            S->alpha = alpha;
+        }
+        const Vecs<T> V = vecs(pd, T());
+        T *r = V.r;
+        const T *q = V.q;
+        const T a = (T)alpha;
+        constexpr int W = Vec<T>::W;
+        const long long len = pd.g.n;
+        const long long nv = len / W;
+        const long long stride = (long long)pd.nt1 * kThreads;
+        double acc = 0.0;
+        for (long long v = (long long)tr.tile * kThreads + threadIdx.x; v < nv; v += stride) {
+            Vec<T> rv = reinterpret_cast<Vec<T> *>(r)[v];
+            const Vec<T> qv = reinterpret_cast<const Vec<T> *>(q)[v];
 #pragma unroll
-        for (int l = 0; l < W; ++l) {
-            rv.v[l] = rv.v[l] - a * qv.v[l];
-            acc += (double)rv.v[l] * (double)rv.v[l];
+            for (int l = 0; l < W; ++l) {
+                rv.v[l] = rv.v[l] - a * qv.v[l];
+                acc += (double)rv.v[l] * (double)rv.v[l];
+            }
+            reinterpret_cast<Vec<T> *>(r)[v] = rv;
         }
-        reinterpret_cast<Vec<T> *>(r)[v] = rv;
-    }
-    if (blockIdx.x == 0) {
-        const long long c = nv * W + threadIdx.x;
-        if (c < len) {
-            const T rv = r[c] - a * q[c];
-            r[c] = rv;
-            acc += (double)rv * (double)rv;
+        if (tr.tile == 0) {
+            const long long c = nv * W + threadIdx.x;
+            if (c < len) {
+                const T rv = r[c] - a * q[c];
+                r[c] = rv;
+                acc += (double)rv * (double)rv;
+            }
         }
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) pd.P.upd[tr.tile] = s;
     }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Mixed precision: does the residual replacement after global step `step`
+// concern this problem?  Its scheduled iteration, or a cap met since the
+// last one (the stop path of cg_a left the correction to fold).
+__device__ __forceinline__ bool refresh_due(const ProbDev &pd, int step)
+{
+    const CgScalars *S = pd.S;
+    const int n = step - pd.n0 + 1;
+    return n >= 1 && !S->final_ && (n == S->next_refresh || S->done_at != kNotDone);
 }
 
 // Mixed precision, residual replacement step 1: fold the FP32 correction
 // into the FP64 iterate, e += c - alpha p (the deferred update of the last
 // iteration, when cg_b ran last), c = 0.
-__global__ __launch_bounds__(kThreads) void k_refresh_e(long long len, double *__restrict__ e,
-                                                        float *__restrict__ c, const float *__restrict__ p_last,
-                                                        const CgScalars *S)
+__global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restrict__ P,
+                                                        const TileRef *__restrict__ tab, int ntab, int step)
 {
-    if (S->final_) return;
-    const bool pending = S->done_at == kNotDone;
-    const float a = (float)S->alpha;
-    const long long stride = (long long)gridDim.x * kThreads;
-    const long long nv = len / 4;
-    for (long long v = (long long)blockIdx.x * kThreads + threadIdx.x; v < nv; v += stride) {
-        Vec<float> cv = reinterpret_cast<const Vec<float> *>(c)[v];
-        if (pending) {
-            const Vec<float> pv = reinterpret_cast<const Vec<float> *>(p_last)[v];
+    const Run run = my_run(ntab);
+    for (int t = run.begin; t < run.end; ++t) {
+        const TileRef tr = tab[t];
+        const ProbDev &pd = P[tr.prob];
+        const CgScalars *S = pd.S;
+        const int n = step - pd.n0 + 1;
+        if (!refresh_due(pd, step)) continue;
+        const bool pending = S->done_at == kNotDone;
+        const float a = (float)S->alpha;
+        const float *p_last = (n & 1) ? pd.p32_1 : pd.p32_0;
+        double *e = pd.e;
+        float *c = pd.c32;
+        const long long len = pd.g.n;
+        const long long nv = len / 4;
+        const long long stride = (long long)pd.nt1 * kThreads;
+        for (long long v = (long long)tr.tile * kThreads + threadIdx.x; v < nv; v += stride) {
+            Vec<float> cv = reinterpret_cast<const Vec<float> *>(c)[v];
+            if (pending) {
+                const Vec<float> pv = reinterpret_cast<const Vec<float> *>(p_last)[v];
 #pragma unroll
-            for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - a * pv.v[l];
-        }
-        Vec<double> e0 = reinterpret_cast<Vec<double> *>(e)[2 * v];
-        Vec<double> e1 = reinterpret_cast<Vec<double> *>(e)[2 * v + 1];
-        e0.v[0] += (double)cv.v[0];
-        e0.v[1] += (double)cv.v[1];
-        e1.v[0] += (double)cv.v[2];
-        e1.v[1] += (double)cv.v[3];
-        reinterpret_cast<Vec<double> *>(e)[2 * v] = e0;
-        reinterpret_cast<Vec<double> *>(e)[2 * v + 1] = e1;
-        reinterpret_cast<Vec<float> *>(c)[v] = Vec<float>{{0.f, 0.f, 0.f, 0.f}};
-    }
-    if (blockIdx.x == 0) {
-        const long long x = nv * 4 + threadIdx.x;
-        if (x < len) {
-            float cv = c[x];
-            if (pending) cv = cv - a * p_last[x];
-            e[x] += (double)cv;
-            c[x] = 0.0f;
-        }
-    }
-}
-
-// FP64 true residual r = A e - b (mirrored ghosts), same tiling as cg_a.
-// Partials (3 slots of gridDim.x): r.r; when r32 is given, r.r of its FP32
-// copy handed to the inner recurrence; when d is given, e.sqrt(d).
-// check_final: skip once the solve is over.
-__global__ __launch_bounds__(kThreads) void k_true_residual(Geo g, const double *__restrict__ e,
-                                                            const double *__restrict__ d,
-                                                            const double *__restrict__ sx,
-                                                            const double *__restrict__ sy,
-                                                            const double *__restrict__ sz,
-                                                            const double *__restrict__ b, float *__restrict__ r32,
-                                                            double *__restrict__ partial, int check_final,
-                                                            const CgScalars *S)
-{
-    if (check_final && S->final_) return;
-    const Tile t = tile_of(g);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
-    if (t.i < nx && t.j < ny) {
-        const long long plane = (long long)nx * ny;
-        const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
-        const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
-        const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
-        long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-        long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
-        long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-        for (int k = t.k0; k < t.k1; ++k) {
-            const long long r0 = c - t.i;
-            const double ec = e[c];
-            const double et = (k < nz - 1) ? e[c + plane] : ec;
-            const double eb = (k > 0) ? e[c - plane] : ec;
-            const double r = ec - sx[fx + 1] * e[r0 + ie] - sx[fx] * e[r0 + iw] - sy[fy + nx] * e[c + dN] -
-                             sy[fy] * e[c + dS] - sz[c + plane] * et - sz[c] * eb - b[c];
-            acc += r * r;
-            if (r32) {
-                const float rf = (float)r;
-                r32[c] = rf;
-                acc32 += (double)rf * (double)rf;
+                for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - a * pv.v[l];
             }
-            if (d) acc_en += ec * sqrt(d[c]);
-            c += plane;
-            fx += (long long)ny * (nx + 1);
-            fy += (long long)(ny + 1) * nx;
+            Vec<double> e0 = reinterpret_cast<Vec<double> *>(e)[2 * v];
+            Vec<double> e1 = reinterpret_cast<Vec<double> *>(e)[2 * v + 1];
+            e0.v[0] += (double)cv.v[0];
+            e0.v[1] += (double)cv.v[1];
+            e1.v[0] += (double)cv.v[2];
+            e1.v[1] += (double)cv.v[3];
+            reinterpret_cast<Vec<double> *>(e)[2 * v] = e0;
+            reinterpret_cast<Vec<double> *>(e)[2 * v + 1] = e1;
+            reinterpret_cast<Vec<float> *>(c)[v] = Vec<float>{{0.f, 0.f, 0.f, 0.f}};
         }
-    }
-    const double s = block_sum(acc);
-    const double s32 = block_sum(acc32);
-    const double sen = block_sum(acc_en);
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = s;
-        partial[gridDim.x + blockIdx.x] = s32;
-        partial[2 * gridDim.x + blockIdx.x] = sen;
+        if (tr.tile == 0) {
+            const long long x = nv * 4 + threadIdx.x;
+            if (x < len) {
+                float cv = c[x];
+                if (pending) cv = cv - a * p_last[x];
+                e[x] += (double)cv;
+                c[x] = 0.0f;
+            }
+        }
     }
 }
 
-// One workgroup after the residual replacement following iteration n_last:
-// converged (on the FP64 true residual), capped, or continue with the
-// replaced residual r (its r.r becomes the "entering" r.r of the next cg_a,
-// through cg_b's partial slots).
-__global__ __launch_bounds__(kThreads) void k_refresh_decide(int n_last, const double *__restrict__ partial, int nb,
-                                                             double *__restrict__ upd_part, int n_upd, CgScalars *S)
+// FP64 true residual r = A e - b (mirrored ghosts) over 3-D tiles.
+// Partials (3 slots of nt3): r.r; with WRITE32, r.r of its FP32 copy handed
+// to the inner recurrence; with GAUGE, e.sqrt(d).
+template <bool WRITE32, bool GAUGE>
+__global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__restrict__ P,
+                                                            const TileRef *__restrict__ tab, int ntab, int step)
 {
-    if (S->final_) return;
-    const double rrt = total_of(partial, nb);
-    const double rrs = total_of(partial + nb, nb);
+    const Run run = my_run(ntab);
+    for (int tt = run.begin; tt < run.end; ++tt) {
+        const TileRef tr = tab[tt];
+        const ProbDev &pd = P[tr.prob];
+        if (WRITE32 && !refresh_due(pd, step)) continue;
+        const Geo &g = pd.g;
+        const Tile t = tile3(g, tr.tile);
+        const int nx = g.nx, ny = g.ny, nz = g.nz;
+        const double *__restrict__ e = pd.e;
+        const double *__restrict__ sx = pd.sx;
+        const double *__restrict__ sy = pd.sy;
+        const double *__restrict__ sz = pd.sz;
+        const double *__restrict__ b = pd.b;
+        double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
+        if (t.i < nx && t.j < ny) {
+            const long long plane = (long long)nx * ny;
+            const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
+            const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
+            const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
+            long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
+            long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
+            long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+            for (int k = t.k0; k < t.k1; ++k) {
+                const long long r0 = c - t.i;
+                const long long up = (k < nz - 1) ? plane : 0, dn = (k > 0) ? plane : 0;
+                const double ec = e[c];
+                const double r = ec - sx[fx + 1] * e[r0 + ie] - sx[fx] * e[r0 + iw] - sy[fy + nx] * e[c + dN] -
+                                 sy[fy] * e[c + dS] - sz[c + plane] * e[c + up] - sz[c] * e[c - dn] - b[c];
+                acc += r * r;
+                if (WRITE32) {
+                    const float rf = (float)r;
+                    pd.r32[c] = rf;
+                    acc32 += (double)rf * (double)rf;
+                }
+                if (GAUGE) acc_en += ec * sqrt(pd.d[c]);
+                c += plane;
+                fx += (long long)ny * (nx + 1);
+                fy += (long long)(ny + 1) * nx;
+            }
+        }
+        const double s = block_sum(acc);
+        const double s32 = block_sum(acc32);
+        const double sen = block_sum(acc_en);
+        if (threadIdx.x == 0) {
+            pd.P.tres[tr.tile] = s;
+            pd.P.tres[pd.nt3 + tr.tile] = s32;
+            pd.P.tres[2 * pd.nt3 + tr.tile] = sen;
+        }
+    }
+}
+
+// Iterations until the next residual replacement, from the problem's own
+// history: enough to reach the threshold at the observed rate, but no more
+// than lets r.r of the FP32 recurrence fall six decades (beyond that its
+// rounding error dominates); a multiple of kRefreshGrain.
+__device__ __forceinline__ int refresh_gap(double slope, double rr, double thresh)
+{
+    double k = (double)kFirstRefresh;
+    if (slope < 0.0 && rr > 0.0 && thresh > 0.0) {
+        const double remaining = log(thresh / rr) / slope;
+        const double span = log(1e-6) / slope;
+        k = fmin(remaining + 2.0, span);
+    }
+    k = fmax((double)kRefreshGrain, fmin((double)kMaxRefreshGap, ceil(k)));
+    const int ki = (int)k;
+    return (ki + kRefreshGrain - 1) / kRefreshGrain * kRefreshGrain;
+}
+
+// One workgroup per problem after the residual replacement that follows
+// global step `step`: converged (on the FP64 true residual), capped, or
+// continue with the replaced residual r (its r.r becomes the "entering" r.r
+// of the next cg_a, through cg_b's partial slots; that cg_a skips the
+// deferred e update, which the replacement already folded into e).
+__global__ __launch_bounds__(kThreads) void k_refresh_decide(const ProbDev *__restrict__ P,
+                                                             const int *__restrict__ probs, int step)
+{
+    const ProbDev &pd = P[probs[blockIdx.x]];
+    CgScalars *S = pd.S;
+    if (!refresh_due(pd, step)) return;
+    const int n_last = step - pd.n0 + 1;
+    const double rrt = total_of(pd.P.tres, pd.nt3);
+    const double rrs = total_of(pd.P.tres + pd.nt3, pd.nt3);
     const bool pending = S->done_at == kNotDone;
     const int iters = pending ? n_last : S->iters;
     const bool conv = !(fabs(rrt) > S->thresh);
     const bool cap = iters >= S->max_iter;
     if (!conv && !cap)
-        for (int t = threadIdx.x; t < n_upd; t += kThreads) upd_part[t] = t == 0 ? rrs : 0.0;
+        for (int t = threadIdx.x; t < pd.nt1; t += kThreads) pd.P.upd[t] = t == 0 ? rrs : 0.0;
     if (threadIdx.x == 0) {
         S->rr_true = rrt;
         S->refreshes += 1;
@@ -305,19 +436,27 @@ __global__ __launch_bounds__(kThreads) void k_refresh_decide(int n_last, const d
         if (conv || cap) {
             S->final_ = 1;
             S->status = conv ? 0 : 1;
-            // launches after n_last skip (cg_a tests done_at only)
+            // later launches skip (cg_a tests done_at)
             if (pending) S->done_at = n_last;
+        } else {
+            S->skip_e_at = step + 1;
+            if (S->prev_iters > 0 && S->prev_rr > 0.0 && rrt < S->prev_rr)
+                S->slope = log(rrt / S->prev_rr) / (double)(n_last - S->prev_iters);
+            S->prev_iters = n_last;
+            S->prev_rr = rrt;
+            S->next_refresh = n_last + refresh_gap(S->slope, rrt, S->thresh);
         }
-        // otherwise continue: the deferred alpha p of iteration n_last is
-        // already in e, so the host launches cg_a(n_last + 1) without it
     }
 }
 
-// After the solve: record the true r.r and the gauge shift.
-__global__ __launch_bounds__(kThreads) void k_finish(const double *__restrict__ partial, int nb, CgScalars *S)
+// After the solve: record the true r.r and the gauge shift (one workgroup
+// per problem).
+__global__ __launch_bounds__(kThreads) void k_finish(const ProbDev *__restrict__ P, const int *__restrict__ probs)
 {
-    const double rrt = total_of(partial, nb);
-    const double en = total_of(partial + 2 * nb, nb);
+    const ProbDev &pd = P[probs[blockIdx.x]];
+    CgScalars *S = pd.S;
+    const double rrt = total_of(pd.P.tres, pd.nt3);
+    const double en = total_of(pd.P.tres + 2 * pd.nt3, pd.nt3);
     if (threadIdx.x == 0) {
         S->rr_true = rrt;
         // e's coordinate along sqrt(d) moved by rounding (and by FP32
@@ -326,55 +465,58 @@ __global__ __launch_bounds__(kThreads) void k_finish(const double *__restrict__ 
     }
 }
 
-// Plain SpMV q = A p (timing only).
+// Plain SpMV q = A p of the FP64 or FP32 operator (timing only).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_spmv(Geo g, const T *__restrict__ p, const T *__restrict__ sx,
-                                                   const T *__restrict__ sy, const T *__restrict__ sz,
-                                                   T *__restrict__ q, double *__restrict__ partial)
+__global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
+                                                   int ntab)
 {
-    const Tile t = tile_of(g);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    double acc = 0.0;
-    if (t.i < nx && t.j < ny) {
-        const long long plane = (long long)nx * ny;
-        const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
-        const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
-        const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
-        long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-        long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
-        long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-        T pc = p[c];
-        T pb = t.k0 > 0 ? p[c - plane] : pc;
-        T szb = sz[c];
-        for (int k = t.k0; k < t.k1; ++k) {
-            const long long r0 = c - t.i;
-            const T pt = (k < nz - 1) ? p[c + plane] : pc;
-            const T szt = sz[c + plane];
-            const T y = pc - sx[fx + 1] * p[r0 + ie] - sx[fx] * p[r0 + iw] - sy[fy + nx] * p[c + dN] -
-                        sy[fy] * p[c + dS] - szt * pt - szb * pb;
-            q[c] = y;
-            acc += (double)pc * (double)y;
-            pb = pc;
-            pc = pt;
-            szb = szt;
-            c += plane;
-            fx += (long long)ny * (nx + 1);
-            fy += (long long)(ny + 1) * nx;
+    const Run run = my_run(ntab);
+    for (int tt = run.begin; tt < run.end; ++tt) {
+        const TileRef tr = tab[tt];
+        const ProbDev &pd = P[tr.prob];
+        const Geo &g = pd.g;
+        const Vecs<T> V = vecs(pd, T());
+        const T *__restrict__ p = V.p0;
+        T *__restrict__ q = V.q;
+        const Tile t = tile3(g, tr.tile);
+        const int nx = g.nx, ny = g.ny, nz = g.nz;
+        double acc = 0.0;
+        if (t.i < nx && t.j < ny) {
+            const long long plane = (long long)nx * ny;
+            const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
+            const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
+            const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
+            long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
+            long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
+            long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+            T pc = p[c];
+            T pb = t.k0 > 0 ? p[c - plane] : pc;
+            T szb = V.sz[c];
+            for (int k = t.k0; k < t.k1; ++k) {
+                const long long r0 = c - t.i;
+                const long long up = (k < nz - 1) ? plane : 0;
+                const T pt = p[c + up];
+                const T szt = V.sz[c + plane];
+                const T y = pc - V.sx[fx + 1] * p[r0 + ie] - V.sx[fx] * p[r0 + iw] - V.sy[fy + nx] * p[c + dN] -
+                            V.sy[fy] * p[c + dS] - szt * pt - szb * pb;
+                q[c] = y;
+                acc += (double)pc * (double)y;
+                pb = pc;
+                pc = pt;
+                szb = szt;
+                c += plane;
+                fx += (long long)ny * (nx + 1);
+                fy += (long long)(ny + 1) * nx;
+            }
         }
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) pd.P.spmv[tr.tile] = s;
     }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-int vec_blocks(long long n)
-{
-    long long b = (n / 4 + kThreads - 1) / kThreads;
-    return (int)std::max<long long>(1, std::min<long long>(kVecBlocks, b));
 }
 
 }  // namespace
 
-Geo make_geo(int nx, int ny, int nz, int px, int py)
+Geo make_geo(int nx, int ny, int nz, int px, int py, int kz)
 {
     Geo g{};
     g.nx = nx;
@@ -385,76 +527,67 @@ Geo make_geo(int nx, int ny, int nz, int px, int py)
     g.n = (long long)nx * ny * nz;
     g.tiles_x = (nx + kTileX - 1) / kTileX;
     g.tiles_y = (ny + kTileY - 1) / kTileY;
-    // about kSpmvBlocks workgroups (4 per CU), at least 4 planes each
-    const long long cols = (long long)g.tiles_x * g.tiles_y;
-    int kz = (int)((cols * nz + kSpmvBlocks - 1) / kSpmvBlocks);
-    if (kz < 4) kz = 4;
+    if (kz <= 0) {
+        // about kSpmvBlocks tiles (4 per CU) for a large grid, at least 4
+        // planes each
+        const long long cols = (long long)g.tiles_x * g.tiles_y;
+        kz = (int)((cols * nz + kSpmvBlocks - 1) / kSpmvBlocks);
+        if (kz < 4) kz = 4;
+    }
     if (kz > nz) kz = nz;
     g.kz = kz;
     g.tiles_z = (nz + kz - 1) / kz;
     return g;
 }
 
-int spmv_blocks(const Geo &g) { return g.tiles_x * g.tiles_y * g.tiles_z; }
+int spmv_tiles(const Geo &g) { return g.tiles_x * g.tiles_y * g.tiles_z; }
 
-template <typename T>
-void launch_cg_a(const Geo &g, int n, bool do_e, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
-                 const Partials &P, CgScalars *S, int *ran, hipStream_t s)
+int vec_tiles(long long n)
 {
-    const dim3 grid(spmv_blocks(g)), block(kThreads);
-    if (n == 1)
-        hipLaunchKernelGGL((k_cg_a<T, true, false>), grid, block, 0, s, g, n, e, p_old, p_new, r, op.sx, op.sy, op.sz,
-                           q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
-    else if (do_e)
-        hipLaunchKernelGGL((k_cg_a<T, false, true>), grid, block, 0, s, g, n, e, p_old, p_new, r, op.sx, op.sy,
-                           op.sz, q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
+    long long b = (n / 4 + kThreads - 1) / kThreads;
+    return (int)std::max<long long>(1, std::min<long long>(kVecBlocks, b));
+}
+
+void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, int step, int *ran,
+                 hipStream_t s)
+{
+    if (fp32)
+        hipLaunchKernelGGL(k_cg_a<float>, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
     else
-        hipLaunchKernelGGL((k_cg_a<T, false, false>), grid, block, 0, s, g, n, e, p_old, p_new, r, op.sx, op.sy,
-                           op.sz, q, P.upd, vec_blocks(g.n), P.spmv, S, ran);
+        hipLaunchKernelGGL(k_cg_a<double>, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
 }
 
-template <typename T>
-void launch_cg_b(const Geo &g, int n, T *r, const T *q, const Partials &P, CgScalars *S, hipStream_t s)
+void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, int step, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg_b<T>, dim3(vec_blocks(g.n)), dim3(kThreads), 0, s, g.n, n, r, q, P.spmv,
-                       spmv_blocks(g), P.upd, S);
+    if (fp32)
+        hipLaunchKernelGGL(k_cg_b<float>, dim3(grid), dim3(kThreads), 0, s, P, t1, nt1, step);
+    else
+        hipLaunchKernelGGL(k_cg_b<double>, dim3(grid), dim3(kThreads), 0, s, P, t1, nt1, step);
 }
 
-void launch_refresh(const Geo &g, int n_last, double *e, float *c32, const float *p_last, Op<double> op,
-                    const double *b, float *r32, const Partials &P, CgScalars *S, hipStream_t s)
+void launch_refresh(const ProbDev *P, const TileRef *t1, int nt1, const TileRef *t3, int nt3, const int *probs,
+                    int nprobs, int grid, int step, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_refresh_e, dim3(vec_blocks(g.n)), dim3(kThreads), 0, s, g.n, e, c32, p_last, S);
-    hipLaunchKernelGGL(k_true_residual, dim3(spmv_blocks(g)), dim3(kThreads), 0, s, g, e, (const double *)nullptr,
-                       op.sx, op.sy, op.sz, b, r32, P.tres, 1, S);
-    hipLaunchKernelGGL(k_refresh_decide, dim3(1), dim3(kThreads), 0, s, n_last, P.tres, spmv_blocks(g), P.upd,
-                       vec_blocks(g.n), S);
+    hipLaunchKernelGGL(k_refresh_e, dim3(std::min(grid, nt1)), dim3(kThreads), 0, s, P, t1, nt1, step);
+    hipLaunchKernelGGL((k_true_residual<true, false>), dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3,
+                       step);
+    hipLaunchKernelGGL(k_refresh_decide, dim3(nprobs), dim3(kThreads), 0, s, P, probs, step);
 }
 
-void launch_finish(const Geo &g, const double *e, const double *d, Op<double> op, const double *b,
-                   const Partials &P, CgScalars *S, hipStream_t s)
+void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *probs, int nprobs, int grid,
+                   hipStream_t s)
 {
-    hipLaunchKernelGGL(k_true_residual, dim3(spmv_blocks(g)), dim3(kThreads), 0, s, g, e, d, op.sx, op.sy, op.sz,
-                       b, (float *)nullptr, P.tres, 0, S);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, s, P.tres, spmv_blocks(g), S);
+    hipLaunchKernelGGL((k_true_residual<false, true>), dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3,
+                       0);
+    hipLaunchKernelGGL(k_finish, dim3(nprobs), dim3(kThreads), 0, s, P, probs);
 }
 
-template <typename T>
-void launch_spmv_only(const Geo &g, const T *p, Op<T> op, T *q, const Partials &P, hipStream_t s)
+void launch_spmv_only(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spmv<T>, dim3(spmv_blocks(g)), dim3(kThreads), 0, s, g, p, op.sx, op.sy, op.sz, q, P.spmv);
+    if (fp32)
+        hipLaunchKernelGGL(k_spmv<float>, dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3);
+    else
+        hipLaunchKernelGGL(k_spmv<double>, dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3);
 }
-
-template void launch_cg_a<double>(const Geo &, int, bool, double *, const double *, double *, const double *, Op<double>,
-                                  double *, const Partials &, CgScalars *, int *, hipStream_t);
-template void launch_cg_a<float>(const Geo &, int, bool, float *, const float *, float *, const float *, Op<float>,
-                                 float *, const Partials &, CgScalars *, int *, hipStream_t);
-template void launch_cg_b<double>(const Geo &, int, double *, const double *, const Partials &, CgScalars *,
-                                  hipStream_t);
-template void launch_cg_b<float>(const Geo &, int, float *, const float *, const Partials &, CgScalars *,
-                                 hipStream_t);
-template void launch_spmv_only<double>(const Geo &, const double *, Op<double>, double *, const Partials &,
-                                       hipStream_t);
-template void launch_spmv_only<float>(const Geo &, const float *, Op<float>, float *, const Partials &,
-                                      hipStream_t);
 
 }  // namespace sps

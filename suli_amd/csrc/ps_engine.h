@@ -1,0 +1,143 @@
+// Host-side solve engine: problem slots resident in HBM, launch tables and
+// the chunked CG loop shared by single solves, batches and the server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/suli_ps.h"
+#include "ps_internal.h"
+
+namespace sps {
+
+// Device memory handed out in whole blocks and kept for reuse by size.
+class Pool {
+public:
+    ~Pool();
+    void *get(size_t bytes, std::string &err);
+    void put(void *p, size_t bytes);
+    void release_all();
+    size_t held() const { return held_; }
+
+private:
+    std::multimap<size_t, void *> free_;
+    size_t held_ = 0;   // bytes allocated from the device (free + in use)
+};
+
+// What a problem needs besides its arrays.
+struct SolveOpts {
+    bool mixed = false;
+    int stop_rule = SULI_PS_STOP_REFERENCE;
+    double tol = 1e-8;
+    int max_iter = 10000;
+    bool keep_null = false;
+    int kz = 0;          // planes per 3-D tile (0: default for the size)
+};
+
+// One problem resident on the device.
+struct Slot {
+    bool used = false;
+    bool active = false;       // in the launch tables
+    bool finished = false;     // final_ seen, finish/unscale enqueued
+    Geo g{};
+    SolveOpts o;
+    ProbDev pd{};
+    void *block = nullptr;     // the problem's device memory
+    size_t bytes = 0;
+    double *prd1_out = nullptr;  // device prd1 (halo box) written at the end
+    int n0 = 0;
+    int user = -1;             // caller's index for this problem
+    CgScalars status{};        // last status read on the host
+};
+
+// Bytes of device memory a problem of this size and precision needs.
+size_t problem_bytes(int nx, int ny, int nz, bool mixed, int kz);
+
+class Engine {
+public:
+    Engine() = default;
+    ~Engine();
+    int init(hipStream_t stream, int max_slots);
+    const std::string &error() const { return err_; }
+
+    // Admits a problem: allocates its slot, enqueues the setup kernels on the
+    // stream and schedules its iteration 1 at the next global step.
+    // prd1_out receives the solution (device, reference layout) when done.
+    int admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, const SolveOpts &o, int user,
+              int *slot_out);
+    // Runs chunks until every admitted problem has finished; each finished
+    // problem's prd1_out holds its solution and status its final state.
+    int run_all(bool timing);
+    // One scheduling round: keeps two chunks of the active problems in
+    // flight and reads the older one's state.  Appends the slots that
+    // finished; their finish and un-scaling kernels are queued on the stream
+    // (prd1_out is final once the stream has passed them).
+    int pump(bool timing, std::vector<int> &finished);
+    // Copies the final scalars of these slots to their status (synchronises).
+    int read_final(const std::vector<int> &slots);
+    int pending_chunks() const { return npend_; }
+    hipStream_t stream() const { return s_; }
+    // Frees a finished slot (its memory returns to the pool).
+    void release(int slot);
+    int active_count() const;
+    const Slot &slot(int i) const { return slots_[i]; }
+    size_t pool_bytes() const { return pool_.held(); }
+
+    // SpMV timing samples of the last run_all (problem slot 0).
+    double spmv_ms_mean() const { return spmv_ms_; }
+    int spmv_samples() const { return spmv_n_; }
+    // Times the plain SpMV of slot 0's operator (FP64 or FP32).
+    int time_spmv(bool fp32, int reps, double *ms);
+
+private:
+    int fail(int code, const std::string &msg);
+    int rebuild_tables();
+    int enqueue_chunk(int K, bool timing);
+    int finish(const std::vector<int> &done);
+    int next_chunk_len() const;
+    bool refresh_may_be_due(int step) const;
+
+    hipStream_t s_ = nullptr;
+    std::string err_;
+    Pool pool_;
+    std::vector<Slot> slots_;
+    ProbDev *d_probs_ = nullptr;       // device descriptors [max_slots]
+    CgScalars *d_status_ = nullptr;    // device scalars [max_slots]
+    CgScalars *h_status_ = nullptr;    // pinned [2][max_slots]
+    CgScalars *h_final_ = nullptr;     // pinned [max_slots]
+    hipEvent_t ev_status_[2] = {nullptr, nullptr};
+    int status_step_[2] = {0, 0};      // global step when each status copy was queued
+    int pend_[2] = {0, 0};             // status slots of chunks not yet read (FIFO)
+    int pend_head_ = 0, npend_ = 0;
+    int max_slots_ = 0;
+    int step_ = 0;                     // global step of the last enqueued iteration
+    int chunk_idx_ = 0;
+    bool tables_dirty_ = true;
+    void *d_fin_ = nullptr;            // tables of the problems being finished
+    size_t fin_cap_ = 0;
+    // launch tables (device, with pinned staging ring)
+    struct Tables {
+        std::vector<TileRef> t3_64, t1_64, t3_32, t1_32;
+        std::vector<int> mixed_probs;
+    } tab_;
+    TileRef *d_tab_ = nullptr;
+    int *d_probs_list_ = nullptr;
+    size_t tab_cap_ = 0, probs_cap_ = 0;
+    void *h_stage_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_stage_[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t stage_cap_ = 0;
+    int stage_next_ = 0;
+    // offsets of the four tables and the mixed-problem list in d_tab_/d_probs_list_
+    int off_t3_64 = 0, off_t1_64 = 0, off_t3_32 = 0, off_t1_32 = 0;
+    // timing
+    std::vector<hipEvent_t> ev_spmv_;
+    int *d_ran_ = nullptr;
+    int ntimed_ = 0;
+    double spmv_ms_ = 0.0;
+    int spmv_n_ = 0;
+};
+
+}  // namespace sps

@@ -1,0 +1,556 @@
+// Host-side solve engine (see ps_engine.h).
+#include "ps_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace sps {
+
+namespace {
+
+constexpr int kMaxGrid = 2048;     // persistent workgroups (8 per CU)
+constexpr int kMaxTimed = 256;     // SpMV launches timed per run_all
+constexpr int kTimeStride = 16;    // time every 16th step
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Carves one device block into a problem's arrays (256-byte aligned).
+struct Carver {
+    char *base;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t count)
+    {
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off = align_up(off + std::max<size_t>(count, 1) * sizeof(T), 256);
+        return p;
+    }
+};
+
+void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
+{
+    const size_t n = (size_t)g.n;
+    const size_t nfx = (size_t)(g.nx + 1) * g.ny * g.nz, nfy = (size_t)g.nx * (g.ny + 1) * g.nz,
+                 nfz = (size_t)g.nx * g.ny * (g.nz + 1);
+    pd.sx = cv.take<double>(nfx);
+    pd.sy = cv.take<double>(nfy);
+    pd.sz = cv.take<double>(nfz);
+    pd.d = cv.take<double>(n);
+    pd.b = cv.take<double>(n);
+    pd.e = cv.take<double>(n);
+    pd.r = cv.take<double>(n);   // FP64 residual; the mixed solver's setup scratch
+    if (!mixed) {
+        pd.q = cv.take<double>(n);
+        pd.p0 = cv.take<double>(n);
+        pd.p1 = cv.take<double>(n);
+    } else {
+        pd.sx32 = cv.take<float>(nfx);
+        pd.sy32 = cv.take<float>(nfy);
+        pd.sz32 = cv.take<float>(nfz);
+        pd.c32 = cv.take<float>(n);
+        pd.r32 = cv.take<float>(n);
+        pd.q32 = cv.take<float>(n);
+        pd.p32_0 = cv.take<float>(n);
+        pd.p32_1 = cv.take<float>(n);
+    }
+    pd.P.spmv = cv.take<double>(nt3);
+    pd.P.upd = cv.take<double>(nt1);
+    pd.P.tres = cv.take<double>(3 * (size_t)nt3);
+    pd.P.init = cv.take<double>(5 * (size_t)nt1);
+    pd.P.proj = cv.take<double>(2 * (size_t)nt1);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- Pool ----
+Pool::~Pool() { release_all(); }
+
+void *Pool::get(size_t bytes, std::string &err)
+{
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+        void *p = it->second;
+        free_.erase(it);
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        // give back cached blocks of other sizes and retry once
+        for (auto &kv : free_) (void)hipFree(kv.second), held_ -= kv.first;
+        free_.clear();
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+    }
+    held_ += bytes;
+    return p;
+}
+
+void Pool::put(void *p, size_t bytes)
+{
+    if (p) free_.emplace(bytes, p);
+}
+
+void Pool::release_all()
+{
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    held_ = 0;
+}
+
+size_t problem_bytes(int nx, int ny, int nz, bool mixed, int kz)
+{
+    const Geo g = make_geo(nx, ny, nz, 0, 0, kz);
+    Carver cv{nullptr};
+    ProbDev pd{};
+    carve(cv, g, mixed, spmv_tiles(g), vec_tiles(g.n), pd);
+    return align_up(cv.off, 2 << 20);
+}
+
+// -------------------------------------------------------------- Engine ----
+#define ECHK(call)                                                                                     \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(SULI_PS_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int Engine::fail(int code, const std::string &msg)
+{
+    err_ = msg;
+    return code;
+}
+
+Engine::~Engine()
+{
+    if (s_) (void)hipStreamSynchronize(s_);
+    for (auto &sl : slots_)
+        if (sl.block) (void)hipFree(sl.block);
+    pool_.release_all();
+    if (d_probs_) (void)hipFree(d_probs_);
+    if (d_status_) (void)hipFree(d_status_);
+    if (h_status_) (void)hipHostFree(h_status_);
+    if (h_final_) (void)hipHostFree(h_final_);
+    if (d_fin_) (void)hipFree(d_fin_);
+    for (auto &e : ev_status_)
+        if (e) (void)hipEventDestroy(e);
+    if (d_tab_) (void)hipFree(d_tab_);
+    if (d_probs_list_) (void)hipFree(d_probs_list_);
+    for (int i = 0; i < 4; ++i) {
+        if (h_stage_[i]) (void)hipHostFree(h_stage_[i]);
+        if (ev_stage_[i]) (void)hipEventDestroy(ev_stage_[i]);
+    }
+    for (auto &e : ev_spmv_)
+        if (e) (void)hipEventDestroy(e);
+    if (d_ran_) (void)hipFree(d_ran_);
+}
+
+int Engine::init(hipStream_t stream, int max_slots)
+{
+    s_ = stream;
+    max_slots_ = max_slots;
+    slots_.assign(max_slots, Slot{});
+    ECHK(hipMalloc(&d_probs_, sizeof(ProbDev) * max_slots));
+    ECHK(hipMalloc(&d_status_, sizeof(CgScalars) * max_slots));
+    ECHK(hipMemset(d_status_, 0, sizeof(CgScalars) * max_slots));
+    ECHK(hipHostMalloc((void **)&h_status_, sizeof(CgScalars) * 2 * max_slots, hipHostMallocDefault));
+    ECHK(hipHostMalloc((void **)&h_final_, sizeof(CgScalars) * max_slots, hipHostMallocDefault));
+    for (auto &e : ev_status_) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : ev_stage_) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev_spmv_.resize(2 * kMaxTimed);
+    for (auto &e : ev_spmv_) ECHK(hipEventCreate(&e));
+    ECHK(hipMalloc(&d_ran_, sizeof(int) * kMaxTimed));
+    return 0;
+}
+
+int Engine::active_count() const
+{
+    int n = 0;
+    for (const auto &s : slots_) n += s.active && !s.finished;
+    return n;
+}
+
+int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, const SolveOpts &o, int user,
+                  int *slot_out)
+{
+    int si = -1;
+    for (int i = 0; i < max_slots_; ++i)
+        if (!slots_[i].used) {
+            si = i;
+            break;
+        }
+    if (si < 0) return fail(SULI_PS_ENOMEM, "no free problem slot");
+    if (step_ % kRefreshGrain != 0) return fail(SULI_PS_EINVAL, "engine: admission off the refresh grain");
+    Slot &sl = slots_[si];
+    sl = Slot{};
+    sl.g = make_geo(gr.nx, gr.ny, gr.nz, gr.ccx0 == 0, gr.ccy0 == 0, o.kz);
+    sl.o = o;
+    const int nt3 = spmv_tiles(sl.g), nt1 = vec_tiles(sl.g.n);
+    sl.bytes = problem_bytes(gr.nx, gr.ny, gr.nz, o.mixed, o.kz);
+    sl.block = pool_.get(sl.bytes, err_);
+    if (!sl.block) return SULI_PS_ENOMEM;
+    Carver cv{static_cast<char *>(sl.block)};
+    ProbDev &pd = sl.pd;
+    pd = ProbDev{};
+    pd.g = sl.g;
+    carve(cv, sl.g, o.mixed, nt3, nt1, pd);
+    pd.nt3 = nt3;
+    pd.nt1 = nt1;
+    pd.mixed = o.mixed ? 1 : 0;
+    pd.S = d_status_ + si;
+    sl.n0 = step_ + 1;   // iteration 1 runs at the next global step
+    pd.n0 = sl.n0;
+    sl.prd1_out = prd1_out;
+    sl.user = user;
+    sl.used = true;
+    sl.active = true;
+
+    // initial scalars and the descriptor, staged through pinned memory
+    CgScalars s0{};
+    s0.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
+    s0.mixed = pd.mixed;
+    s0.project = o.keep_null ? 0 : 1;
+    s0.rel_rule = o.stop_rule == SULI_PS_STOP_RELATIVE;
+    s0.tol = o.tol;
+    // threshold of the reference stop rule: 0.0001/(nx*ny*nz) (9_grad.f90:393)
+    s0.thresh_ref = 0.0001 / (double)(gr.nx * gr.ny * gr.nz);
+    s0.done_at = kNotDone;
+    s0.skip_e_at = -1;
+    s0.next_refresh = kFirstRefresh;
+    sl.status = s0;
+    const size_t need = sizeof(CgScalars) + sizeof(ProbDev);
+    int stg;
+    int rc = 0;
+    {
+        // a staging buffer whose previous copy has completed
+        if (stage_cap_ < need) {
+            ECHK(hipStreamSynchronize(s_));
+            for (int i = 0; i < 4; ++i) {
+                if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
+                ECHK(hipHostMalloc(&h_stage_[i], std::max(need, (size_t)65536), hipHostMallocDefault));
+            }
+            stage_cap_ = std::max(need, (size_t)65536);
+        }
+        stg = stage_next_;
+        stage_next_ = (stage_next_ + 1) & 3;
+        ECHK(hipEventSynchronize(ev_stage_[stg]));
+        char *h = static_cast<char *>(h_stage_[stg]);
+        std::memcpy(h, &s0, sizeof s0);
+        std::memcpy(h + sizeof s0, &pd, sizeof pd);
+        ECHK(hipMemcpyAsync(pd.S, h, sizeof s0, hipMemcpyHostToDevice, s_));
+        ECHK(hipMemcpyAsync(d_probs_ + si, h + sizeof s0, sizeof pd, hipMemcpyHostToDevice, s_));
+        ECHK(hipEventRecord(ev_stage_[stg], s_));
+    }
+
+    // setup kernels (ps_setup.hip)
+    double *sx = const_cast<double *>(pd.sx), *sy = const_cast<double *>(pd.sy), *sz = const_cast<double *>(pd.sz);
+    launch_face_coefs(sl.g, in, sx, sy, sz, s_);
+    launch_diag_rhs(sl.g, in, sx, sy, sz, const_cast<double *>(pd.d), pd.b, s_);
+    launch_scale_faces(sl.g, pd.d, sx, sy, sz, const_cast<float *>(pd.sx32), const_cast<float *>(pd.sy32),
+                       const_cast<float *>(pd.sz32), s_);
+    launch_init(sl.g, in, Op<double>{pd.sx, pd.sy, pd.sz}, pd.d, pd.b, pd.e, pd.r, o.mixed ? nullptr : pd.r,
+                pd.r32, pd.c32, pd.P, nt1, pd.S, s_);
+    ECHK(hipGetLastError());
+    tables_dirty_ = true;
+    if (slot_out) *slot_out = si;
+    return rc;
+}
+
+void Engine::release(int si)
+{
+    Slot &sl = slots_[si];
+    if (!sl.used) return;
+    pool_.put(sl.block, sl.bytes);
+    sl = Slot{};
+    tables_dirty_ = true;
+}
+
+int Engine::rebuild_tables()
+{
+    tab_.t3_64.clear();
+    tab_.t1_64.clear();
+    tab_.t3_32.clear();
+    tab_.t1_32.clear();
+    tab_.mixed_probs.clear();
+    for (int i = 0; i < max_slots_; ++i) {
+        const Slot &sl = slots_[i];
+        if (!sl.active || sl.finished) continue;
+        auto &t3 = sl.o.mixed ? tab_.t3_32 : tab_.t3_64;
+        auto &t1 = sl.o.mixed ? tab_.t1_32 : tab_.t1_64;
+        for (int t = 0; t < sl.pd.nt3; ++t) t3.push_back({i, t});
+        for (int t = 0; t < sl.pd.nt1; ++t) t1.push_back({i, t});
+        if (sl.o.mixed) tab_.mixed_probs.push_back(i);
+    }
+    off_t3_64 = 0;
+    off_t1_64 = off_t3_64 + (int)tab_.t3_64.size();
+    off_t3_32 = off_t1_64 + (int)tab_.t1_64.size();
+    off_t1_32 = off_t3_32 + (int)tab_.t3_32.size();
+    const size_t ntab = off_t1_32 + tab_.t1_32.size();
+    const size_t nprobs = tab_.mixed_probs.size();
+    if (ntab > tab_cap_ || nprobs > probs_cap_ || !d_tab_) {
+        // grow: buffers still read by queued kernels are freed after a sync
+        ECHK(hipStreamSynchronize(s_));
+        if (d_tab_) ECHK(hipFree(d_tab_));
+        if (d_probs_list_) ECHK(hipFree(d_probs_list_));
+        tab_cap_ = std::max<size_t>(ntab, 1) * 2;
+        probs_cap_ = std::max<size_t>(nprobs, (size_t)max_slots_);
+        ECHK(hipMalloc(&d_tab_, sizeof(TileRef) * tab_cap_));
+        ECHK(hipMalloc(&d_probs_list_, sizeof(int) * probs_cap_));
+    }
+    const size_t need = sizeof(TileRef) * ntab + sizeof(int) * nprobs + 64;
+    if (need > stage_cap_) {
+        ECHK(hipStreamSynchronize(s_));
+        for (int i = 0; i < 4; ++i) {
+            if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
+            ECHK(hipHostMalloc(&h_stage_[i], need * 2, hipHostMallocDefault));
+        }
+        stage_cap_ = need * 2;
+    }
+    const int stg = stage_next_;
+    stage_next_ = (stage_next_ + 1) & 3;
+    ECHK(hipEventSynchronize(ev_stage_[stg]));
+    char *h = static_cast<char *>(h_stage_[stg]);
+    TileRef *ht = reinterpret_cast<TileRef *>(h);
+    std::copy(tab_.t3_64.begin(), tab_.t3_64.end(), ht + off_t3_64);
+    std::copy(tab_.t1_64.begin(), tab_.t1_64.end(), ht + off_t1_64);
+    std::copy(tab_.t3_32.begin(), tab_.t3_32.end(), ht + off_t3_32);
+    std::copy(tab_.t1_32.begin(), tab_.t1_32.end(), ht + off_t1_32);
+    int *hp = reinterpret_cast<int *>(h + align_up(sizeof(TileRef) * ntab, 64));
+    std::copy(tab_.mixed_probs.begin(), tab_.mixed_probs.end(), hp);
+    if (ntab) ECHK(hipMemcpyAsync(d_tab_, ht, sizeof(TileRef) * ntab, hipMemcpyHostToDevice, s_));
+    if (nprobs) ECHK(hipMemcpyAsync(d_probs_list_, hp, sizeof(int) * nprobs, hipMemcpyHostToDevice, s_));
+    ECHK(hipEventRecord(ev_stage_[stg], s_));
+    tables_dirty_ = false;
+    return 0;
+}
+
+// Steps of the next chunk (a multiple of kRefreshGrain, so that problems
+// admitted between chunks start on the refresh grain).  Mixed problems:
+// up to the earliest replacement known to be scheduled, so the host reads
+// the state soon after it; FP64 problems decide their stop on the device
+// every iteration and only need the host to notice it.
+int Engine::next_chunk_len() const
+{
+    bool any_mixed = false;
+    long long biggest = 0;
+    int best = kMaxRefreshGap;
+    for (int i = 0; i < max_slots_; ++i) {
+        const Slot &sl = slots_[i];
+        if (!sl.active || sl.finished) continue;
+        biggest = std::max(biggest, sl.g.n);
+        if (!sl.o.mixed) continue;
+        any_mixed = true;
+        const int queued = step_ - sl.n0 + 1;   // iterations already enqueued
+        const int ahead = sl.status.next_refresh - queued;
+        best = std::min(best, ahead > 0 ? ahead : 2 * kRefreshGrain);
+    }
+    if (!any_mixed) return biggest <= 200000 ? 64 : 32;
+    best = std::max(best, kRefreshGrain);
+    return (best + kRefreshGrain - 1) / kRefreshGrain * kRefreshGrain;
+}
+
+// Could a mixed problem's residual replacement be due after global step s?
+// Its scheduled iteration as last read, any later iteration (the schedule
+// read may be stale) or an iteration cap.  Launching when none is due only
+// costs time: the kernels test refresh_due on the device.
+bool Engine::refresh_may_be_due(int s) const
+{
+    for (int i = 0; i < max_slots_; ++i) {
+        const Slot &sl = slots_[i];
+        if (!sl.active || sl.finished || !sl.o.mixed) continue;
+        const int n = s - sl.n0 + 1;
+        if (n >= sl.status.next_refresh || n > sl.o.max_iter || sl.status.done_at != kNotDone) return true;
+    }
+    return false;
+}
+
+int Engine::enqueue_chunk(int K, bool timing)
+{
+    const int n3_64 = (int)tab_.t3_64.size(), n1_64 = (int)tab_.t1_64.size();
+    const int n3_32 = (int)tab_.t3_32.size(), n1_32 = (int)tab_.t1_32.size();
+    const int nmix = (int)tab_.mixed_probs.size();
+    for (int it = 0; it < K; ++it) {
+        ++step_;
+        const bool timed = timing && (step_ % kTimeStride == 2) && ntimed_ < kMaxTimed;
+        int *ran = timed ? d_ran_ + ntimed_ : nullptr;
+        if (timed) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
+        if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), step_, ran, s_);
+        if (n3_32) launch_cg_a(true, d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), step_, ran, s_);
+        if (timed) {
+            ECHK(hipEventRecord(ev_spmv_[2 * ntimed_ + 1], s_));
+            ++ntimed_;
+        }
+        if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), step_, s_);
+        if (n1_32) launch_cg_b(true, d_probs_, d_tab_ + off_t1_32, n1_32, std::min(n1_32, kMaxGrid), step_, s_);
+        if (nmix && step_ % kRefreshGrain == 0 && refresh_may_be_due(step_))
+            launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,
+                           kMaxGrid, step_, s_);
+    }
+    ECHK(hipGetLastError());
+    const int slot = chunk_idx_ & 1;
+    int hi = 0;
+    for (int i = 0; i < max_slots_; ++i)
+        if (slots_[i].used) hi = i + 1;
+    ECHK(hipMemcpyAsync(h_status_ + (size_t)slot * max_slots_, d_status_, sizeof(CgScalars) * std::max(hi, 1),
+                        hipMemcpyDeviceToHost, s_));
+    ECHK(hipEventRecord(ev_status_[slot], s_));
+    status_step_[slot] = step_;
+    ++chunk_idx_;
+    return 0;
+}
+
+int Engine::finish(const std::vector<int> &done)
+{
+    if (done.empty()) return 0;
+    // tables for the finished problems, staged like the launch tables
+    std::vector<TileRef> t3;
+    for (int si : done)
+        for (int t = 0; t < slots_[si].pd.nt3; ++t) t3.push_back({si, t});
+    const size_t poff = align_up(sizeof(TileRef) * t3.size(), 256);
+    const size_t need = poff + sizeof(int) * done.size();
+    if (need > fin_cap_) {
+        // grow: the old buffer may still be read by queued kernels
+        ECHK(hipStreamSynchronize(s_));
+        if (d_fin_) ECHK(hipFree(d_fin_));
+        fin_cap_ = need * 2;
+        ECHK(hipMalloc(&d_fin_, fin_cap_));
+    }
+    if (need > stage_cap_) {
+        ECHK(hipStreamSynchronize(s_));
+        for (int i = 0; i < 4; ++i) {
+            if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
+            ECHK(hipHostMalloc(&h_stage_[i], need * 2, hipHostMallocDefault));
+        }
+        stage_cap_ = need * 2;
+    }
+    const int stg = stage_next_;
+    stage_next_ = (stage_next_ + 1) & 3;
+    ECHK(hipEventSynchronize(ev_stage_[stg]));
+    char *h = static_cast<char *>(h_stage_[stg]);
+    std::memcpy(h, t3.data(), sizeof(TileRef) * t3.size());
+    std::memcpy(h + poff, done.data(), sizeof(int) * done.size());
+    ECHK(hipMemcpyAsync(d_fin_, h, need, hipMemcpyHostToDevice, s_));
+    ECHK(hipEventRecord(ev_stage_[stg], s_));
+    const TileRef *dt = static_cast<const TileRef *>(d_fin_);
+    const int *dp = reinterpret_cast<const int *>(static_cast<char *>(d_fin_) + poff);
+    launch_finish(d_probs_, dt, (int)t3.size(), dp, (int)done.size(), kMaxGrid, s_);
+    for (int si : done) {
+        Slot &sl = slots_[si];
+        launch_unscale(sl.g, sl.pd.e, sl.pd.d, sl.prd1_out, sl.pd.S, s_);
+        sl.finished = true;
+    }
+    ECHK(hipGetLastError());
+    tables_dirty_ = true;
+    return 0;
+}
+
+int Engine::pump(bool timing, std::vector<int> &finished)
+{
+    int rc;
+    if (tables_dirty_ && (rc = rebuild_tables())) return rc;
+    // keep two chunks in flight: the host reads one chunk's state while the
+    // next runs
+    while (active_count() > 0 && npend_ < 2) {
+        if ((rc = enqueue_chunk(next_chunk_len(), timing))) return rc;
+        pend_[(pend_head_ + npend_) & 1] = (chunk_idx_ - 1) & 1;
+        ++npend_;
+    }
+    if (npend_ == 0) return 0;
+    const int which = pend_[pend_head_];
+    pend_head_ = (pend_head_ + 1) & 1;
+    --npend_;
+    ECHK(hipEventSynchronize(ev_status_[which]));
+    const CgScalars *h = h_status_ + (size_t)which * max_slots_;
+    const int last_step = status_step_[which];
+    std::vector<int> done;
+    for (int i = 0; i < max_slots_; ++i) {
+        Slot &sl = slots_[i];
+        // problems admitted after this chunk was queued took no part in it
+        if (!sl.used || !sl.active || sl.finished || sl.n0 > last_step) continue;
+        sl.status = h[i];
+        if (h[i].final_) done.push_back(i);
+    }
+    if ((rc = finish(done))) return rc;
+    for (int i : done) finished.push_back(i);
+    return 0;
+}
+
+int Engine::read_final(const std::vector<int> &slots)
+{
+    int hi = 0;
+    for (int i : slots) hi = std::max(hi, i + 1);
+    if (hi == 0) return 0;
+    ECHK(hipMemcpyAsync(h_final_, d_status_, sizeof(CgScalars) * hi, hipMemcpyDeviceToHost, s_));
+    ECHK(hipStreamSynchronize(s_));
+    for (int i : slots) slots_[i].status = h_final_[i];
+    return 0;
+}
+
+int Engine::run_all(bool timing)
+{
+    int rc;
+    ntimed_ = 0;
+    if (timing) ECHK(hipMemsetAsync(d_ran_, 0, sizeof(int) * kMaxTimed, s_));
+    std::vector<int> all;
+    for (int i = 0; i < max_slots_; ++i)
+        if (slots_[i].used && slots_[i].active && !slots_[i].finished) all.push_back(i);
+    std::vector<int> finished;
+    const int max_rounds = 1 << 20;
+    int r = 0;
+    for (; r < max_rounds && (active_count() > 0 || npend_ > 0); ++r)
+        if ((rc = pump(timing, finished))) return rc;
+    if (active_count() > 0) return fail(SULI_PS_EHIP, "engine: problems did not finish");
+    // final scalars (rr_true, gshift) after the finish kernels
+    if ((rc = read_final(all))) return rc;
+    // SpMV timing samples that did the work
+    spmv_ms_ = 0.0;
+    spmv_n_ = 0;
+    if (timing && ntimed_ > 0) {
+        std::vector<int> ran(ntimed_);
+        ECHK(hipMemcpy(ran.data(), d_ran_, sizeof(int) * ntimed_, hipMemcpyDeviceToHost));
+        double sum = 0.0;
+        for (int t = 0; t < ntimed_; ++t) {
+            if (!ran[t]) continue;
+            float ms = 0.f;
+            ECHK(hipEventElapsedTime(&ms, ev_spmv_[2 * t], ev_spmv_[2 * t + 1]));
+            sum += ms;
+            ++spmv_n_;
+        }
+        spmv_ms_ = spmv_n_ ? sum / spmv_n_ : 0.0;
+    }
+    return 0;
+}
+
+int Engine::time_spmv(bool fp32, int reps, double *ms)
+{
+    const Slot &sl = slots_[0];
+    if (!sl.used) return fail(SULI_PS_EINVAL, "no operator: call graddin first");
+    if (fp32 != (bool)sl.o.mixed)
+        return fail(SULI_PS_EINVAL, "the last solve's operator is in the other precision");
+    std::vector<TileRef> t3;
+    for (int t = 0; t < sl.pd.nt3; ++t) t3.push_back({0, t});
+    TileRef *dt = nullptr;
+    ECHK(hipMalloc(&dt, sizeof(TileRef) * t3.size()));
+    ECHK(hipMemcpy(dt, t3.data(), sizeof(TileRef) * t3.size(), hipMemcpyHostToDevice));
+    const int n3 = (int)t3.size();
+    hipEvent_t a, b;
+    ECHK(hipEventCreate(&a));
+    ECHK(hipEventCreate(&b));
+    launch_spmv_only(fp32, d_probs_, dt, n3, std::min(n3, kMaxGrid), s_);
+    ECHK(hipEventRecord(a, s_));
+    for (int i = 0; i < reps; ++i) launch_spmv_only(fp32, d_probs_, dt, n3, std::min(n3, kMaxGrid), s_);
+    ECHK(hipEventRecord(b, s_));
+    ECHK(hipEventSynchronize(b));
+    float t = 0.f;
+    ECHK(hipEventElapsedTime(&t, a, b));
+    *ms = (double)t / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ECHK(hipFree(dt));
+    return 0;
+}
+
+}  // namespace sps

@@ -1,6 +1,6 @@
 // Internal declarations shared by the kernels (ps_setup.hip, ps_cg.hip)
-// and the host solver (ps_solver.hip).  Not part of the public ABI
-// (include/suli_ps.h).
+// and the host side (ps_engine.hip, ps_solver.hip).  Not part of the public
+// ABI (include/suli_ps.h).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,12 +9,19 @@
 namespace sps {
 
 constexpr int kThreads = 256;        // every kernel: 4 wave64s
-constexpr int kTileX = 64;           // SpMV tile: one wave per x row
+constexpr int kTileX = 64;           // 3-D tile: one wave per x row
 constexpr int kTileY = 4;            // four rows per workgroup
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, round-robin dispatch
-constexpr int kVecBlocks = 512;      // workgroups of the 1-D vector kernels
-constexpr int kSpmvBlocks = 1024;    // target workgroups of the SpMV
+constexpr int kVecBlocks = 512;      // 1-D tiles of a large problem
+constexpr int kSpmvBlocks = 1024;    // target 3-D tiles of a large problem
 constexpr int kNotDone = 0x7fffffff;
+// Mixed precision: residual replacements happen at iterations that are
+// multiples of kRefreshGrain, the first at kFirstRefresh; each problem
+// schedules its next one from its own convergence history, so its results
+// never depend on the other problems of a batch.
+constexpr int kRefreshGrain = 8;
+constexpr int kFirstRefresh = 32;
+constexpr int kMaxRefreshGap = 128;
 
 // Grid geometry as the kernels see it.  Cells are 0-based (i, j, k) with x
 // fastest; a cell vector holds nx*ny*nz values with no halo.  Ghost values
@@ -24,19 +31,20 @@ struct Geo {
     int nx, ny, nz;
     int px, py;               // periodic in x / y (z is always mirrored)
     long long n;              // nx*ny*nz
-    int tiles_x, tiles_y, tiles_z, kz;   // SpMV tiling: kz planes per workgroup
+    int tiles_x, tiles_y, tiles_z, kz;   // 3-D tiling: 64 x 4 columns, kz planes
 };
 
-// Device-resident CG state.
+// Device-resident CG state of one problem.
 //
 // No kernel communicates between its own workgroups: every reduction
-// leaves one partial sum per workgroup, and the kernel that needs the total
-// sums those partials itself (each workgroup in the same fixed order, so all
-// get the same bits) after the kernel boundary has made them visible on
-// every XCD.  Scalars derived from a total are stored by workgroup 0 for
-// later kernels; values a kernel reads while its own workgroup 0 may be
-// writing are double-buffered by iteration parity (rr) or are benign
-// (done_at only ever moves from "running" to the current iteration).
+// leaves one partial sum per tile, and the kernel that needs the total sums
+// those partials itself (each workgroup in the same fixed order, so all get
+// the same bits) after the kernel boundary has made them visible on every
+// XCD.  Scalars derived from a total are stored by the workgroup that
+// handles tile 0 for later kernels; values a kernel reads while that
+// workgroup may be writing are double-buffered by iteration parity (rr) or
+// are benign (done_at only ever moves from "running" to the current
+// iteration).
 struct CgScalars {
     double rr[2];     // r.r entering iteration n in slot n & 1 (r_{n-1}.r_{n-1})
     double alpha;     // alpha of iteration n (cg_b -> next cg_a, refresh)
@@ -54,21 +62,53 @@ struct CgScalars {
     int iters;        // CG iterations completed (reference `cont`)
     int max_iter;
     int refreshes;    // FP64 residual replacements (mixed)
+    int skip_e_at;    // global step whose cg_a skips the deferred e update
     int status;       // 0 converged, 1 capped
     int mixed;        // 1: convergence is judged on rr_true at replacements
     int project;      // 1: remove the null-space component of the residual
     int rel_rule;     // 1: thresh = tol^2 b.b
+    int next_refresh; // iteration of the next residual replacement (mixed)
+    int prev_iters;   // iteration of the previous replacement (0: none yet)
     double tol;
     double thresh_ref;
+    double prev_rr;   // true r.r at the previous replacement
+    double slope;     // d log(r.r) / d iteration between replacements (< 0)
 };
 
-// Offsets (in doubles) of the per-workgroup partial sums in one buffer.
+// Per-tile partial sums of one problem.
 struct Partials {
-    double *spmv;     // [B1]      p.Ap of cg_a
-    double *upd;      // [B2]      r.r of cg_b (or one total, rest zero, after a replacement)
-    double *tres;     // [3*B1]    true residual r.r (FP64, FP32 copy) and e.sqrt(d)
-    double *init;     // [5*B0]    setup: r0.r0, b.b, r0.sqrt(d), d, e0.sqrt(d)
-    double *proj;     // [2*B0]    setup after projection: r0.r0, FP32 r0.r0
+    double *spmv;     // [nt3]      p.Ap of cg_a
+    double *upd;      // [nt1]      r.r of cg_b (or one total, rest zero, after a replacement)
+    double *tres;     // [3*nt3]    true residual r.r (FP64, FP32 copy) and e.sqrt(d)
+    double *init;     // [5*nt1]    setup: r0.r0, b.b, r0.sqrt(d), d, e0.sqrt(d)
+    double *proj;     // [2*nt1]    setup after projection: r0.r0, FP32 r0.r0
+};
+
+// Device descriptor of one problem of a batch: geometry, tiling, where its
+// arrays live, and the global step at which its iteration 1 runs.  Written
+// once when the problem is admitted.
+struct ProbDev {
+    Geo g;
+    int n0;           // global step of iteration 1 (iteration n runs at step n0 + n - 1)
+    int nt3;          // 3-D tiles (cg_a, true residual)
+    int nt1;          // 1-D tiles (cg_b, refresh)
+    int mixed;
+    // operator (Jacobi-scaled face coefficients), diagonal, effective rhs
+    const double *sx, *sy, *sz, *d;
+    double *b;
+    // FP64 iterate and, for FP64 solves, the CG vectors (p alternates)
+    double *e, *r, *q, *p0, *p1;
+    // FP32 inner solve (mixed): operator copy, correction c, vectors
+    const float *sx32, *sy32, *sz32;
+    float *c32, *r32, *q32, *p32_0, *p32_1;
+    Partials P;
+    CgScalars *S;
+};
+
+// Entry of a launch table: tile `tile` of problem `prob`.
+struct TileRef {
+    int prob;
+    int tile;
 };
 
 // Setup inputs in the reference's Fortran layouts (device pointers).
@@ -89,47 +129,47 @@ struct Op {
     const T *sx, *sy, *sz;
 };
 
-// ---- setup (ps_setup.hip) ----
+// ---- setup, per problem (ps_setup.hip) ----
 void launch_face_coefs(const Geo &g, const SetupIn &in, double *sx, double *sy, double *sz, hipStream_t s);
 void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
                      double *d, double *b, hipStream_t s);
 void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
                         float *sy32, float *sz32, hipStream_t s);
-// Initial residual, null-space projection; leaves r (r64, or r32 and c32 = 0
-// for mixed precision) and S ready for iteration 1.
+// Initial residual and null-space projection on nb0 workgroups; leaves r
+// (r64, or r32 with c32 = 0 for mixed precision) and S ready for iteration 1.
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
-                 double *r0, double *r64, float *r32, float *c32, const Partials &P, CgScalars *S, hipStream_t s);
+                 double *r0, double *r64, float *r32, float *c32, const Partials &P, int nb0, CgScalars *S,
+                 hipStream_t s);
 // prd1 = e / sqrt(d) - S->gshift on the whole halo box.
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
                     hipStream_t s);
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s);
 
-// ---- CG iteration n (ps_cg.hip), n = 1, 2, ... ----
-// cg_a(n): deferred e -= alpha_{n-1} p_{n-1}; stop rule on r.r; p_n = r +
-// beta p_{n-1} (p_old -> p_new, two buffers); q = A p_n; p.q partials.
-// ran (nullable): set to 1 by the launch if it did the SpMV (timing samples).
-// do_e = false right after a residual replacement (the deferred update is
-// already in e).
-template <typename T>
-void launch_cg_a(const Geo &g, int n, bool do_e, T *e, const T *p_old, T *p_new, const T *r, Op<T> op, T *q,
-                 const Partials &P, CgScalars *S, int *ran, hipStream_t s);
-// cg_b(n): alpha_n = r.r / p.q; r -= alpha_n q; r.r partials.
-template <typename T>
-void launch_cg_b(const Geo &g, int n, T *r, const T *q, const Partials &P, CgScalars *S, hipStream_t s);
-// Mixed precision residual replacement after iteration n_last: e += c (with
-// the deferred alpha p term), r = b - A e in FP64 handed to the FP32
+// ---- batched CG (ps_cg.hip) ----
+// Global step `step` runs iteration step - n0 + 1 of every problem in the
+// table.  Kernels are persistent: `grid` workgroups walk the tile table.
+// cg_a: deferred e -= alpha p; stop rule on r.r; p_n = r + beta p_{n-1}
+// (p buffers alternate); q = A p_n; p.q partials.  ran (nullable): set to 1
+// if problem 0's tile 0 did the SpMV (timing samples).
+void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, int step, int *ran,
+                 hipStream_t s);
+// cg_b: alpha = r.r / p.q; r -= alpha q; r.r partials.
+void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, int step, hipStream_t s);
+// Mixed precision residual replacement after global step `step` for the
+// problems of the tables (`probs` lists them, one entry per problem): e += c
+// (with the deferred alpha p term), r = b - A e in FP64 handed to the FP32
 // recurrence; decide converged / capped / continue.
-void launch_refresh(const Geo &g, int n_last, double *e, float *c32, const float *p_last, Op<double> op,
-                    const double *b, float *r32, const Partials &P, CgScalars *S, hipStream_t s);
-// After the solve: FP64 true residual r.r into S->rr_true and, when the
-// null space is projected, the gauge shift that restores e0's coordinate.
-void launch_finish(const Geo &g, const double *e, const double *d, Op<double> op, const double *b,
-                   const Partials &P, CgScalars *S, hipStream_t s);
-// Stand-alone SpMV q = A p (no vector updates), for timing.
-template <typename T>
-void launch_spmv_only(const Geo &g, const T *p, Op<T> op, T *q, const Partials &P, hipStream_t s);
+void launch_refresh(const ProbDev *P, const TileRef *t1, int nt1, const TileRef *t3, int nt3, const int *probs,
+                    int nprobs, int grid, int step, hipStream_t s);
+// After a problem's solve: FP64 true residual r.r into S->rr_true and the
+// gauge shift that restores e0's null-space coordinate.
+void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *probs, int nprobs, int grid,
+                   hipStream_t s);
+// Stand-alone SpMV q = A p of one problem (no vector updates), for timing.
+void launch_spmv_only(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, hipStream_t s);
 
-int spmv_blocks(const Geo &g);
-Geo make_geo(int nx, int ny, int nz, int px, int py);
+int spmv_tiles(const Geo &g);
+int vec_tiles(long long n);
+Geo make_geo(int nx, int ny, int nz, int px, int py, int kz = 0);
 
 }  // namespace sps

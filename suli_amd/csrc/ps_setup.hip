@@ -317,6 +317,11 @@ __global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0,
         S->iters = 0;
         S->refreshes = 0;
         S->gshift = 0.0;
+        S->skip_e_at = -1;
+        S->next_refresh = kFirstRefresh;
+        S->prev_iters = 0;
+        S->prev_rr = 0.0;
+        S->slope = 0.0;
         const bool conv = !(fabs(rr) > S->thresh);
         const bool done = conv || S->max_iter <= 0;
         S->done_at = done ? 0 : kNotDone;
@@ -426,9 +431,9 @@ void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, d
 }
 
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
-                 double *r0, double *r64, float *r32, float *c32, const Partials &P, CgScalars *S, hipStream_t s)
+                 double *r0, double *r64, float *r32, float *c32, const Partials &P, int nb0, CgScalars *S,
+                 hipStream_t s)
 {
-    const int nb0 = std::min(grid_for(g.n), kVecBlocks);
     hipLaunchKernelGGL(k_init_residual, dim3(nb0), dim3(kThreads), 0, s, g, in, op, d, b, e, r0, P);
     hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, d, b, r0, r64, r32, c32, P, nb0, S);
     hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kThreads), 0, s, P, nb0, S);

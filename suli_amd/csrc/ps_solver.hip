@@ -1,6 +1,5 @@
-// Host side of the pressure solver: device workspace, the CG drivers
-// (FP64 and mixed-precision residual replacement) and the C ABI declared
-// in include/suli_ps.h.
+// The C ABI declared in include/suli_ps.h: single solves, batches and the
+// velocity correction, on top of the solve engine (ps_engine.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,36 +11,21 @@
 #include <vector>
 
 #include "../../include/suli_ps.h"
+#include "ps_engine.h"
 #include "ps_internal.h"
 
 using namespace sps;
 
 namespace {
 
-constexpr int kMaxTimed = 256;  // SpMV launches timed per solve
+constexpr int kMaxSlots = 256;   // problems resident at once per context
 
-struct Workspace {
+// Device copies of the reference-layout arrays for the host-pointer API.
+struct Staging {
     int nx = 0, ny = 0, nz = 0;
-    size_t bytes = 0;
-    std::vector<void *> allocs;
-    // operator (FP64 scaled coefficients, diagonal, effective rhs)
-    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *d = nullptr, *b = nullptr;
-    // FP64 CG vectors (p and p2: the search direction, alternating by iteration)
-    double *e = nullptr, *r = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
-    // FP32 inner-solve operator and vectors (mixed precision)
-    float *sx32 = nullptr, *sy32 = nullptr, *sz32 = nullptr;
-    float *c32 = nullptr, *r32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr;
-    double *partial = nullptr;   // one buffer, carved into Partials
-    Partials P{};
-    CgScalars *S = nullptr;
-    CgScalars *Stime = nullptr;  // scratch state for suli_ps_time_spmv
-    int *ran = nullptr;          // timed SpMV launches that did the work
-    // staging for the host-pointer API (reference layouts)
+    void *block = nullptr;
     double *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *prd1 = nullptr;
     double *obsx = nullptr, *obsy = nullptr, *obsz = nullptr, *dz = nullptr, *dzz = nullptr;
-    bool have_mixed = false;
-    bool have_operator = false;
-    Geo geo{};
 };
 
 }  // namespace
@@ -51,11 +35,10 @@ struct suli_ps_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    Workspace ws;
-    CgScalars *host_status = nullptr;  // pinned, 2 slots
-    hipEvent_t ev_status[2] = {nullptr, nullptr};
-    hipEvent_t ev_solve[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> ev_spmv;   // timing pairs
+    Engine eng;
+    Staging st;
+    std::vector<int> last_slots;   // slots of the last solve (kept for time_spmv)
+    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -86,85 +69,8 @@ suli_ps_opts default_opts()
     o.stop_rule = SULI_PS_STOP_REFERENCE;
     o.tol = 1e-8;
     o.max_iter = 10000;
-    o.chunk = 0;
     o.verbose = 1;
-    o.timing = 0;
     return o;
-}
-
-void free_ws(Workspace &ws)
-{
-    for (void *p : ws.allocs) (void)hipFree(p);
-    ws = Workspace{};
-}
-
-template <typename T>
-int dalloc(suli_ps_ctx *ctx, T **p, size_t count)
-{
-    void *q = nullptr;
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    ctx->ws.allocs.push_back(q);
-    ctx->ws.bytes += bytes;
-    *p = static_cast<T *>(q);
-    return 0;
-}
-
-// Allocates the device workspace for a grid (kept while the shape is unchanged).
-int ensure_ws(suli_ps_ctx *ctx, int nx, int ny, int nz, bool mixed)
-{
-    Workspace &ws = ctx->ws;
-    if (ws.nx == nx && ws.ny == ny && ws.nz == nz && (!mixed || ws.have_mixed)) return 0;
-    if (ws.nx != nx || ws.ny != ny || ws.nz != nz) {
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        free_ws(ws);
-    }
-    const size_t n = (size_t)nx * ny * nz;
-    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
-    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
-    int rc = 0;
-    if (!ws.sx) {
-        ws.nx = nx;
-        ws.ny = ny;
-        ws.nz = nz;
-        ws.geo = make_geo(nx, ny, nz, 0, 0);
-        const size_t b1 = (size_t)spmv_blocks(ws.geo), b2 = (size_t)kVecBlocks;
-        // spmv[b1] upd[b2] tres[3 b1] init[5 b2] proj[2 b2]
-        const size_t npart = b1 + b2 + 3 * b1 + 5 * b2 + 2 * b2;
-        if ((rc = dalloc(ctx, &ws.sx, nfx)) || (rc = dalloc(ctx, &ws.sy, nfy)) || (rc = dalloc(ctx, &ws.sz, nfz)) ||
-            (rc = dalloc(ctx, &ws.d, n)) || (rc = dalloc(ctx, &ws.b, n)) || (rc = dalloc(ctx, &ws.e, n)) ||
-            (rc = dalloc(ctx, &ws.r, n)) || (rc = dalloc(ctx, &ws.p, n)) || (rc = dalloc(ctx, &ws.p2, n)) ||
-            (rc = dalloc(ctx, &ws.q, n)) ||
-            (rc = dalloc(ctx, &ws.partial, npart)) || (rc = dalloc(ctx, &ws.S, 1)) ||
-            (rc = dalloc(ctx, &ws.ran, (size_t)kMaxTimed)) ||
-            (rc = dalloc(ctx, &ws.Stime, 1)) || (rc = dalloc(ctx, &ws.u, (size_t)(nx + 3) * (ny + 2) * (nz + 2))) ||
-            (rc = dalloc(ctx, &ws.v, (size_t)(nx + 2) * (ny + 3) * (nz + 2))) ||
-            (rc = dalloc(ctx, &ws.w, (size_t)(nx + 2) * (ny + 2) * (nz + 3))) || (rc = dalloc(ctx, &ws.rho, n)) ||
-            (rc = dalloc(ctx, &ws.prd1, nbox)) || (rc = dalloc(ctx, &ws.obsx, nfx)) ||
-            (rc = dalloc(ctx, &ws.obsy, nfy)) || (rc = dalloc(ctx, &ws.obsz, nfz)) ||
-            (rc = dalloc(ctx, &ws.dz, (size_t)nz + 2)) || (rc = dalloc(ctx, &ws.dzz, (size_t)nz + 2))) {
-            free_ws(ws);
-            return rc;
-        }
-        ws.P.spmv = ws.partial;
-        ws.P.upd = ws.P.spmv + b1;
-        ws.P.tres = ws.P.upd + b2;
-        ws.P.init = ws.P.tres + 3 * b1;
-        ws.P.proj = ws.P.init + 5 * b2;
-        HIPCHK(ctx, hipMemsetAsync(ws.S, 0, sizeof(CgScalars), ctx->stream));
-        HIPCHK(ctx, hipMemsetAsync(ws.Stime, 0, sizeof(CgScalars), ctx->stream));
-    }
-    if (mixed && !ws.have_mixed) {
-        if ((rc = dalloc(ctx, &ws.sx32, nfx)) || (rc = dalloc(ctx, &ws.sy32, nfy)) ||
-            (rc = dalloc(ctx, &ws.sz32, nfz)) || (rc = dalloc(ctx, &ws.c32, n)) || (rc = dalloc(ctx, &ws.r32, n)) ||
-            (rc = dalloc(ctx, &ws.p32, n)) || (rc = dalloc(ctx, &ws.p32b, n)) || (rc = dalloc(ctx, &ws.q32, n))) {
-            free_ws(ws);
-            return rc;
-        }
-        ws.have_mixed = true;
-    }
-    return 0;
 }
 
 int check_grid(suli_ps_ctx *ctx, const suli_ps_grid *g)
@@ -180,6 +86,21 @@ int check_grid(suli_ps_ctx *ctx, const suli_ps_grid *g)
         return fail(ctx, SULI_PS_EINVAL, "t_press = %d: only 1 (projection) and 2 (IPA) solve for pressure",
                     g->t_press);
     if (g->t_press == 2 && !(g->rho_f1 > 0)) return fail(ctx, SULI_PS_EINVAL, "IPA needs rho_f1 > 0");
+    return 0;
+}
+
+int check_problem(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, const double *v, const double *w,
+                  const double *rho, const double *ox, const double *oy, const double *oz, const double *dz,
+                  const double *dzz, const double *prd1)
+{
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!u || !v || !w || !prd1 || (g->t_press == 1 && !rho))
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
+    if ((dz == nullptr) != (dzz == nullptr)) return fail(ctx, SULI_PS_EINVAL, "dz and dzz go together");
+    const bool mask = g->obst_t != 0 && g->ibm_t == 2;
+    if (mask && (!ox || !oy || !oz)) return fail(ctx, SULI_PS_EINVAL, "IBM mask on (obst_t, ibm_t=2) needs obs_ls*");
+    if (!dz && !(g->dz > 0)) return fail(ctx, SULI_PS_EINVAL, "dz must be positive");
     return 0;
 }
 
@@ -212,194 +133,75 @@ SetupIn make_setup(const suli_ps_grid *g, const double *u, const double *v, cons
     return in;
 }
 
-int auto_chunk(const Geo &g)
+SolveOpts solve_opts(const suli_ps_opts &o)
 {
-    // small grids iterate fast: poll less often relative to the work
-    if (g.n <= 200000) return 64;
-    return 32;
+    SolveOpts so;
+    so.mixed = o.precision == SULI_PS_MIXED;
+    so.stop_rule = o.stop_rule;
+    so.tol = o.tol;
+    so.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
+    so.keep_null = o.keep_null != 0;
+    return so;
+}
+
+void release_last(suli_ps_ctx *ctx)
+{
+    for (int s : ctx->last_slots) ctx->eng.release(s);
+    ctx->last_slots.clear();
+}
+
+void fill_stats(const Slot &sl, suli_ps_stats *st)
+{
+    const CgScalars &f = sl.status;
+    std::memset(st, 0, sizeof *st);
+    st->status = f.status == 0 ? SULI_PS_OK : SULI_PS_NOT_CONVERGED;
+    st->iters = f.iters;
+    st->refreshes = f.refreshes;
+    st->rr0 = f.rr0;
+    st->bb = f.bb;
+    // r.r the stop rule saw: FP64, the recurrence's (slot of iteration done_at);
+    // mixed, the FP64 true residual of the last replacement
+    st->rr = sl.o.mixed ? f.rr_true : f.rr[(f.done_at == kNotDone ? 0 : f.done_at) & 1];
+    st->thresh = f.thresh;
+    st->rel_true = f.bb > 0 ? std::sqrt(f.rr_true / f.bb) : 0.0;
+}
+
+void warn_capped(const suli_ps_ctx *ctx, const suli_ps_stats &st)
+{
+    if (st.status == SULI_PS_NOT_CONVERGED && ctx->opts.verbose) {
+        // the reference's warning (9_grad.f90:590)
+        std::printf(" pulou pressão; erro = %.15E\n", st.rr);
+        std::fflush(stdout);
+    }
 }
 
 int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, const double *v, const double *w,
                     const double *rho, const double *ox, const double *oy, const double *oz, const double *dz,
                     const double *dzz, double *prd1, suli_ps_stats *stats)
 {
-    int rc = check_grid(ctx, gr);
+    int rc = check_problem(ctx, gr, u, v, w, rho, ox, oy, oz, dz, dzz, prd1);
     if (rc) return rc;
-    if (!u || !v || !w || !prd1 || (gr->t_press == 1 && !rho))
-        return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
-    if ((dz == nullptr) != (dzz == nullptr)) return fail(ctx, SULI_PS_EINVAL, "dz and dzz go together");
-    const bool mask = gr->obst_t != 0 && gr->ibm_t == 2;
-    if (mask && (!ox || !oy || !oz)) return fail(ctx, SULI_PS_EINVAL, "IBM mask on (obst_t, ibm_t=2) needs obs_ls*");
-    if (!dz && !(gr->dz > 0)) return fail(ctx, SULI_PS_EINVAL, "dz must be positive");
-    const suli_ps_opts &o = ctx->opts;
-    const bool mixed = o.precision == SULI_PS_MIXED;
-    if ((rc = ensure_ws(ctx, gr->nx, gr->ny, gr->nz, mixed))) return rc;
-    Workspace &ws = ctx->ws;
+    release_last(ctx);
     hipStream_t s = ctx->stream;
-    Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
-    ws.geo = g;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
     const SetupIn in = make_setup(gr, u, v, w, rho, prd1, ox, oy, oz, dz, dzz);
-
-    HIPCHK(ctx, hipEventRecord(ctx->ev_solve[0], s));
-    CgScalars s0{};
-    s0.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
-    s0.mixed = mixed ? 1 : 0;
-    s0.project = o.keep_null ? 0 : 1;
-    s0.rel_rule = o.stop_rule == SULI_PS_STOP_RELATIVE;
-    s0.tol = o.tol;
-    // threshold of the reference stop rule: 0.0001/(nx*ny*nz) (9_grad.f90:393)
-    s0.thresh_ref = 0.0001 / (double)(gr->nx * gr->ny * gr->nz);
-    s0.done_at = kNotDone;
-    HIPCHK(ctx, hipMemcpyAsync(ws.S, &s0, sizeof s0, hipMemcpyHostToDevice, s));
-    if (o.timing) HIPCHK(ctx, hipMemsetAsync(ws.ran, 0, kMaxTimed * sizeof(int), s));
-
-    launch_face_coefs(g, in, ws.sx, ws.sy, ws.sz, s);
-    launch_diag_rhs(g, in, ws.sx, ws.sy, ws.sz, ws.d, ws.b, s);
-    launch_scale_faces(g, ws.d, ws.sx, ws.sy, ws.sz, mixed ? ws.sx32 : nullptr, mixed ? ws.sy32 : nullptr,
-                       mixed ? ws.sz32 : nullptr, s);
-    const Op<double> op64{ws.sx, ws.sy, ws.sz};
-    const Op<float> op32{ws.sx32, ws.sy32, ws.sz32};
-    launch_init(g, in, op64, ws.d, ws.b, ws.e, ws.r, mixed ? nullptr : ws.r, ws.r32, ws.c32, ws.P, ws.S, s);
-    HIPCHK(ctx, hipGetLastError());
-    ws.have_operator = true;
-
-    const int max_iter = s0.max_iter;
-    // SpMV timing: events around every 16th cg_a launch; ws.ran[slot] says
-    // whether that launch did the SpMV (it skips after the stop rule)
-    const int tstride = 16;
-    int ntimed = 0;
-    int n = 0;  // CG iteration number of the launch sequence (1-based)
-    double *p64[2] = {ws.p, ws.p2};
-    float *p32[2] = {ws.p32, ws.p32b};
-
-    auto enqueue_chunk = [&](int chunk_idx, int K) -> int {
-        for (int it = 0; it < K; ++it) {
-            ++n;
-            const bool timed = o.timing && (n % tstride == 2) && ntimed < kMaxTimed;
-            int *ran = timed ? ws.ran + ntimed : nullptr;
-            // after a residual replacement (every mixed chunk but the first
-            // starts after one) the deferred e update is already in e
-            const bool do_e = !(mixed && it == 0 && chunk_idx > 0);
-            if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed], s));
-            if (mixed)
-                launch_cg_a<float>(g, n, do_e, ws.c32, p32[(n - 1) & 1], p32[n & 1], ws.r32, op32, ws.q32, ws.P,
-                                   ws.S, ran, s);
-            else
-                launch_cg_a<double>(g, n, do_e, ws.e, p64[(n - 1) & 1], p64[n & 1], ws.r, op64, ws.q, ws.P, ws.S,
-                                    ran, s);
-            if (timed) {
-                HIPCHK(ctx, hipEventRecord(ctx->ev_spmv[2 * ntimed + 1], s));
-                ++ntimed;
-            }
-            if (mixed)
-                launch_cg_b<float>(g, n, ws.r32, ws.q32, ws.P, ws.S, s);
-            else
-                launch_cg_b<double>(g, n, ws.r, ws.q, ws.P, ws.S, s);
-        }
-        if (mixed) launch_refresh(g, n, ws.e, ws.c32, p32[n & 1], op64, ws.b, ws.r32, ws.P, ws.S, s);
-        HIPCHK(ctx, hipGetLastError());
-        const int slot = chunk_idx & 1;
-        HIPCHK(ctx, hipMemcpyAsync(&ctx->host_status[slot], ws.S, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
-        HIPCHK(ctx, hipEventRecord(ctx->ev_status[slot], s));
-        return 0;
-    };
-
-    // Chunk sizes.  FP64: fixed; the stop rule is tested on the device every
-    // iteration and later launches return at once.  Mixed: every chunk ends
-    // with an FP64 residual replacement, which is also where convergence is
-    // decided, so chunk lengths follow the observed convergence rate: long
-    // enough to amortise the replacement, short enough not to overshoot the
-    // tolerance and to keep the FP32 recurrence within ~3 decades of the
-    // residual it started from.
-    const int K_fixed = o.chunk > 0 ? o.chunk : auto_chunk(g);
-    int known_iters = 0;          // from the latest status read
-    double known_rr = 0.0, known_thresh = 0.0, rate = 0.0;
-    std::vector<int> chunk_len;
-    auto next_chunk = [&](int c) -> int {
-        if (!mixed || o.chunk > 0) return K_fixed;
-        if (!(rate < 0.0) || !(known_rr > 0.0)) return 32;
-        int in_flight = 0;  // iterations enqueued beyond the known status
-        for (int i = 0; i < c; ++i) in_flight += chunk_len[i];
-        in_flight -= known_iters;
-        const double remaining =
-            known_thresh > 0 ? std::log(known_thresh / known_rr) / rate - in_flight : 32.0;
-        const double fp32_span = std::log(1e-6) / rate;  // rr drop of 1e6 at most
-        double k = std::min(remaining + 2.0, fp32_span);
-        return (int)std::max(8.0, std::min(128.0, std::ceil(k)));
-    };
-
-    // Two chunks in flight: the host reads chunk c-1's state while chunk c
-    // runs.
-    bool finished = false;
-    const int max_chunks = max_iter / 8 + 16;
-    chunk_len.push_back(next_chunk(0));
-    if ((rc = enqueue_chunk(0, chunk_len[0]))) return rc;
-    for (int c = 1; c <= max_chunks; ++c) {
-        chunk_len.push_back(next_chunk(c));
-        if ((rc = enqueue_chunk(c, chunk_len[c]))) return rc;
-        const int prev = (c - 1) & 1;
-        HIPCHK(ctx, hipEventSynchronize(ctx->ev_status[prev]));
-        const CgScalars &st = ctx->host_status[prev];
-        if (st.final_) {
-            finished = true;
-            break;
-        }
-        if (mixed && st.iters > known_iters && st.rr_true > 0.0) {
-            if (known_rr > 0.0 && st.rr_true < known_rr)
-                rate = std::log(st.rr_true / known_rr) / (st.iters - known_iters);
-            known_iters = st.iters;
-            known_rr = st.rr_true;
-        }
-        known_thresh = st.thresh;
-    }
-    if (!finished) {
-        HIPCHK(ctx, hipStreamSynchronize(s));
-        return fail(ctx, SULI_PS_EHIP, "solver did not finish within %d chunks", max_chunks);
-    }
-    launch_finish(g, ws.e, ws.d, op64, ws.b, ws.P, ws.S, s);
-    launch_unscale(g, ws.e, ws.d, prd1, ws.S, s);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipEventRecord(ctx->ev_solve[1], s));
-    CgScalars fin{};
-    HIPCHK(ctx, hipMemcpyAsync(&fin, ws.S, sizeof fin, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-
-    const int status = fin.status == 0 ? SULI_PS_OK : SULI_PS_NOT_CONVERGED;
-    // r.r of the recurrence after the last iteration (slot of iteration done_at)
-    const double rr_last = fin.rr[(fin.done_at == kNotDone ? 0 : fin.done_at) & 1];
-    if (stats) {
-        std::memset(stats, 0, sizeof *stats);
-        stats->status = status;
-        stats->iters = fin.iters;
-        stats->refreshes = fin.refreshes;
-        stats->rr0 = fin.rr0;
-        stats->bb = fin.bb;
-        stats->rr = mixed ? fin.rr_true : rr_last;
-        stats->thresh = fin.thresh;
-        stats->rel_true = fin.bb > 0 ? std::sqrt(fin.rr_true / fin.bb) : 0.0;
-        float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev_solve[0], ctx->ev_solve[1]));
-        stats->solve_ms = ms;
-        double tsum = 0.0;
-        int tcount = 0;
-        int ran[kMaxTimed] = {0};
-        if (ntimed > 0) HIPCHK(ctx, hipMemcpy(ran, ws.ran, ntimed * sizeof(int), hipMemcpyDeviceToHost));
-        for (int t = 0; t < ntimed; ++t) {
-            if (!ran[t]) continue;
-            float dt = 0.f;
-            HIPCHK(ctx, hipEventElapsedTime(&dt, ctx->ev_spmv[2 * t], ctx->ev_spmv[2 * t + 1]));
-            tsum += dt;
-            ++tcount;
-        }
-        stats->spmv_timed = tcount;
-        stats->spmv_ms = tcount ? tsum / tcount : 0.0;
-    }
-    if (status == SULI_PS_NOT_CONVERGED && o.verbose) {
-        // the reference's warning (9_grad.f90:590)
-        std::printf(" pulou pressão; erro = %.15E\n", mixed ? fin.rr_true : rr_last);
-        std::fflush(stdout);
-    }
-    return status;
+    int slot = -1;
+    if ((rc = ctx->eng.admit(*gr, in, prd1, solve_opts(ctx->opts), 0, &slot)))
+        return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    ctx->last_slots.push_back(slot);
+    if ((rc = ctx->eng.run_all(ctx->opts.timing != 0))) return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    suli_ps_stats st;
+    fill_stats(ctx->eng.slot(slot), &st);
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    st.solve_ms = ms;
+    st.spmv_timed = ctx->eng.spmv_samples();
+    st.spmv_ms = ctx->eng.spmv_ms_mean();
+    if (stats) *stats = st;
+    warn_capped(ctx, st);
+    return st.status;
 }
 
 int run_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, double *v, double *w, const double *rho,
@@ -420,6 +222,36 @@ int run_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, double *
     return 0;
 }
 
+// Staging for the host-pointer API, kept while the grid shape is unchanged.
+int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
+{
+    Staging &st = ctx->st;
+    if (st.block && st.nx == nx && st.ny == ny && st.nz == nz) return 0;
+    if (st.block) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipFree(st.block));
+        st = Staging{};
+    }
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t sizes[10] = {(size_t)(nx + 3) * (ny + 2) * (nz + 2), (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                              (size_t)(nx + 2) * (ny + 2) * (nz + 3), n, (size_t)(nx + 2) * (ny + 2) * (nz + 2),
+                              (size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
+                              (size_t)nz + 2, (size_t)nz + 2};
+    size_t off[10], total = 0;
+    for (int i = 0; i < 10; ++i) {
+        off[i] = total;
+        total += (sizes[i] * sizeof(double) + 255) / 256 * 256;
+    }
+    hipError_t e = hipMalloc(&st.block, total);
+    if (e != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
+    double **ptrs[10] = {&st.u, &st.v, &st.w, &st.rho, &st.prd1, &st.obsx, &st.obsy, &st.obsz, &st.dz, &st.dzz};
+    for (int i = 0; i < 10; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
+    st.nx = nx;
+    st.ny = ny;
+    st.nz = nz;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -433,14 +265,8 @@ suli_ps_ctx *suli_ps_create(const suli_ps_opts *opts)
     if (ctx->opts.max_iter <= 0) ctx->opts.max_iter = 10000;
     bool ok = hipGetDevice(&ctx->device) == hipSuccess &&
               hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipHostMalloc((void **)&ctx->host_status, 2 * sizeof(CgScalars), hipHostMallocDefault) == hipSuccess;
-    for (int i = 0; ok && i < 2; ++i)
-        ok = hipEventCreateWithFlags(&ctx->ev_status[i], hipEventDisableTiming) == hipSuccess &&
-             hipEventCreate(&ctx->ev_solve[i]) == hipSuccess;
-    if (ok) {
-        ctx->ev_spmv.resize(512);
-        for (auto &e : ctx->ev_spmv) ok = ok && hipEventCreate(&e) == hipSuccess;
-    }
+              hipEventCreate(&ctx->ev[0]) == hipSuccess && hipEventCreate(&ctx->ev[1]) == hipSuccess;
+    ok = ok && ctx->eng.init(ctx->stream, kMaxSlots) == 0;
     if (!ok) {
         suli_ps_destroy(ctx);
         return nullptr;
@@ -452,14 +278,12 @@ void suli_ps_destroy(suli_ps_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    free_ws(ctx->ws);
-    for (auto &e : ctx->ev_spmv)
+    release_last(ctx);
+    if (ctx->st.block) (void)hipFree(ctx->st.block);
+    for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
-    for (int i = 0; i < 2; ++i) {
-        if (ctx->ev_status[i]) (void)hipEventDestroy(ctx->ev_status[i]);
-        if (ctx->ev_solve[i]) (void)hipEventDestroy(ctx->ev_solve[i]);
-    }
-    if (ctx->host_status) (void)hipHostFree(ctx->host_status);
+    ctx->eng.~Engine();
+    new (&ctx->eng) Engine();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -500,39 +324,34 @@ int suli_ps_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, co
                     const double *dz, const double *dzz, double *prd1, suli_ps_stats *stats)
 {
     if (!ctx) return SULI_PS_EINVAL;
-    int rc = check_grid(ctx, g);
+    int rc = check_problem(ctx, g, u, v, w, rho, obs_lsx, obs_lsy, obs_lsz, dz, dzz, prd1);
     if (rc) return rc;
-    if (!u || !v || !w || !prd1 || (g->t_press == 1 && !rho))
-        return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
-    const bool mask = g->obst_t != 0 && g->ibm_t == 2;
-    if (mask && (!obs_lsx || !obs_lsy || !obs_lsz))
-        return fail(ctx, SULI_PS_EINVAL, "IBM mask on (obst_t, ibm_t=2) needs obs_ls*");
-    if ((dz == nullptr) != (dzz == nullptr)) return fail(ctx, SULI_PS_EINVAL, "dz and dzz go together");
-    if ((rc = ensure_ws(ctx, g->nx, g->ny, g->nz, ctx->opts.precision == SULI_PS_MIXED))) return rc;
-    Workspace &ws = ctx->ws;
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t n = (size_t)nx * ny * nz;
     const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const bool mask = g->obst_t != 0 && g->ibm_t == 2;
     hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(ws.u, u, sizeof(double) * (nx + 3) * (ny + 2) * (nz + 2), hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.v, v, sizeof(double) * (nx + 2) * (ny + 3) * (nz + 2), hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.w, w, sizeof(double) * (nx + 2) * (ny + 2) * (nz + 3), hipMemcpyHostToDevice, s));
-    if (rho) HIPCHK(ctx, hipMemcpyAsync(ws.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * (nx + 3) * (ny + 2) * (nz + 2), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * (nx + 2) * (ny + 3) * (nz + 2), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * (nx + 2) * (ny + 2) * (nz + 3), hipMemcpyHostToDevice, s));
+    if (rho) HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
     if (mask) {
-        HIPCHK(ctx, hipMemcpyAsync(ws.obsx, obs_lsx, sizeof(double) * nfx, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(ws.obsy, obs_lsy, sizeof(double) * nfy, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(ws.obsz, obs_lsz, sizeof(double) * nfz, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.obsx, obs_lsx, sizeof(double) * nfx, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.obsy, obs_lsy, sizeof(double) * nfy, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.obsz, obs_lsz, sizeof(double) * nfz, hipMemcpyHostToDevice, s));
     }
     if (dz) {
-        HIPCHK(ctx, hipMemcpyAsync(ws.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(ws.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
     }
-    rc = run_graddin_dev(ctx, g, ws.u, ws.v, ws.w, rho ? ws.rho : nullptr, ws.obsx, ws.obsy, ws.obsz,
-                         dz ? ws.dz : nullptr, dz ? ws.dzz : nullptr, ws.prd1, stats);
+    rc = run_graddin_dev(ctx, g, st.u, st.v, st.w, rho ? st.rho : nullptr, st.obsx, st.obsy, st.obsz,
+                         dz ? st.dz : nullptr, dz ? st.dzz : nullptr, st.prd1, stats);
     if (rc < 0) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(prd1, ws.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     return rc;
 }
@@ -545,32 +364,74 @@ int suli_ps_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v
     if (rc) return rc;
     if (!u || !v || !w || !rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
     if ((dz == nullptr) != (dzz == nullptr)) return fail(ctx, SULI_PS_EINVAL, "dz and dzz go together");
-    if ((rc = ensure_ws(ctx, g->nx, g->ny, g->nz, false))) return rc;
-    Workspace &ws = ctx->ws;
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t n = (size_t)nx * ny * nz;
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
     hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(ws.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(ws.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
     if (dz) {
-        HIPCHK(ctx, hipMemcpyAsync(ws.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(ws.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
     }
-    rc = run_posdin_dev(ctx, g, ws.u, ws.v, ws.w, ws.rho, ws.prd1, dz ? ws.dz : nullptr, dz ? ws.dzz : nullptr,
+    rc = run_posdin_dev(ctx, g, st.u, st.v, st.w, st.rho, st.prd1, dz ? st.dz : nullptr, dz ? st.dzz : nullptr,
                         nullptr);
     if (rc) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(u, ws.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(v, ws.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(w, ws.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);  // prd = prd1 (10_pos.f90:95)
     return 0;
+}
+
+int suli_ps_graddin_batch_dev(suli_ps_ctx *ctx, suli_ps_problem *probs, int32_t count)
+{
+    if (!ctx || (!probs && count > 0) || count < 0) return SULI_PS_EINVAL;
+    if (count > kMaxSlots) return fail(ctx, SULI_PS_EINVAL, "batch of %d exceeds %d problems", count, kMaxSlots);
+    int rc;
+    for (int i = 0; i < count; ++i) {
+        const suli_ps_problem &p = probs[i];
+        if ((rc = check_problem(ctx, &p.grid, p.u, p.v, p.w, p.rho, p.obs_lsx, p.obs_lsy, p.obs_lsz, p.dz, p.dzz,
+                                p.prd1)))
+            return fail(ctx, rc, "problem %d: %s", i, ctx->err.c_str());
+    }
+    release_last(ctx);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    const SolveOpts so = solve_opts(ctx->opts);
+    std::vector<int> slots(count, -1);
+    for (int i = 0; i < count; ++i) {
+        const suli_ps_problem &p = probs[i];
+        const SetupIn in = make_setup(&p.grid, p.u, p.v, p.w, p.rho, p.prd1, p.obs_lsx, p.obs_lsy, p.obs_lsz, p.dz,
+                                      p.dzz);
+        if ((rc = ctx->eng.admit(p.grid, in, p.prd1, so, i, &slots[i]))) {
+            release_last(ctx);
+            for (int s2 : slots)
+                if (s2 >= 0) ctx->eng.release(s2);
+            return fail(ctx, rc, "problem %d: %s", i, ctx->eng.error().c_str());
+        }
+        ctx->last_slots.push_back(slots[i]);
+    }
+    if ((rc = ctx->eng.run_all(false))) return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    int worst = SULI_PS_OK;
+    for (int i = 0; i < count; ++i) {
+        fill_stats(ctx->eng.slot(slots[i]), &probs[i].stats);
+        probs[i].stats.solve_ms = ms;
+        worst = std::max(worst, probs[i].stats.status);
+    }
+    return worst;
 }
 
 void *suli_ps_dev_alloc(suli_ps_ctx *ctx, size_t bytes)
@@ -619,29 +480,8 @@ int suli_ps_synchronize(suli_ps_ctx *ctx)
 int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double *ms)
 {
     if (!ctx || !ms || reps <= 0) return SULI_PS_EINVAL;
-    Workspace &ws = ctx->ws;
-    if (!ws.have_operator) return fail(ctx, SULI_PS_EINVAL, "no operator: call graddin first");
-    const bool mixed = precision == SULI_PS_MIXED;
-    if (mixed && !ws.have_mixed) return fail(ctx, SULI_PS_EINVAL, "no FP32 operator: solve in mixed mode first");
-    hipStream_t s = ctx->stream;
-    const Geo &g = ws.geo;
-    // the plain SpMV q = A p (5 words per cell) on the last solve's operator:
-    // one untimed launch, then `reps` launches between two events
-    auto launch = [&]() {
-        if (mixed)
-            launch_spmv_only<float>(g, ws.p32, Op<float>{ws.sx32, ws.sy32, ws.sz32}, ws.q32, ws.P, s);
-        else
-            launch_spmv_only<double>(g, ws.p, Op<double>{ws.sx, ws.sy, ws.sz}, ws.q, ws.P, s);
-    };
-    launch();
-    HIPCHK(ctx, hipEventRecord(ctx->ev_solve[0], s));
-    for (int i = 0; i < reps; ++i) launch();
-    HIPCHK(ctx, hipEventRecord(ctx->ev_solve[1], s));
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipEventSynchronize(ctx->ev_solve[1]));
-    float t = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev_solve[0], ctx->ev_solve[1]));
-    *ms = (double)t / reps;
+    int rc = ctx->eng.time_spmv(precision == SULI_PS_MIXED, reps, ms);
+    if (rc) return fail(ctx, rc, "%s", ctx->eng.error().c_str());
     return 0;
 }
 

@@ -163,6 +163,23 @@ class PressureSolver:
             p["obs_lsz"], p["dz"], p["dzz"], p["prd1"], C.byref(st)))
         return st
 
+    def graddin_batch_dev(self, items) -> list:
+        """Solve many problems together; ``items`` is a sequence of
+        ``(grid, DeviceFields)``.  Returns one Stats per problem."""
+        n = len(items)
+        arr = (_lib.Problem * n)()
+        for i, (grid, df) in enumerate(items):
+            arr[i].grid = _c_grid(grid)
+            for name in ("u", "v", "w", "rho", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "prd1"):
+                setattr(arr[i], name, df.ptrs[name])
+        self._check(self.lib.suli_ps_graddin_batch_dev(self.ctx, arr, n))
+        out = []
+        for i in range(n):
+            st = _lib.Stats()
+            C.pointer(st)[0] = arr[i].stats
+            out.append(st)
+        return out
+
     def posdin_dev(self, grid: Grid, df: DeviceFields) -> None:
         cg = _c_grid(grid)
         p = df.ptrs

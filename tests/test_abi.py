@@ -36,6 +36,8 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.Grid) == 8 * 4 + 5 * 8
     assert ctypes.sizeof(_lib.Opts) == 4 + 4 + 8 + 5 * 4 + 4
     assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 7 * 8
+    assert ctypes.sizeof(_lib.Problem) == ctypes.sizeof(_lib.Grid) + 10 * 8 + ctypes.sizeof(_lib.Stats)
+    assert _lib.Problem.stats.offset == ctypes.sizeof(_lib.Grid) + 10 * 8
 
 
 def test_fortran_shim_declares_same_entry_points():
