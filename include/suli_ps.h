@@ -73,6 +73,8 @@ typedef struct suli_ps_opts {
                               operator's null space (constant pressure), as the
                               reference does; 0 (default): remove it, which
                               changes the solution by a constant only */
+    int32_t no_graphs;     /* 1: launch every kernel from the host instead of
+                              replaying captured HIP graphs (same results) */
 } suli_ps_opts;
 
 typedef struct suli_ps_stats {

@@ -43,7 +43,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
         ("max_iter", C.c_int32), ("chunk", C.c_int32), ("verbose", C.c_int32),
-        ("timing", C.c_int32), ("keep_null", C.c_int32),
+        ("timing", C.c_int32), ("keep_null", C.c_int32), ("no_graphs", C.c_int32),
     ]
 
 

@@ -38,6 +38,8 @@ struct alignas(16) Vec {
     T v[W];
 };
 
+__device__ __forceinline__ int step_of(StepRef s) { return s.base ? *s.base + s.off : s.off; }
+
 // Contiguous run [begin, end) of the launch table for this workgroup.
 struct Run {
     int begin, end;
@@ -162,9 +164,10 @@ __device__ __forceinline__ Vecs<float> vecs(const ProbDev &pd, float)
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
-                                                   int ntab, int step, int *ran)
+                                                   int ntab, StepRef sref, int *ran)
 {
     const Run run = my_run(ntab);
+    const int step = step_of(sref);
     TotalCache rr_tot;
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
@@ -226,9 +229,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
 // the 16-byte vectors k*256 + lane + m*nt1*256.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
-                                                   int ntab, int step)
+                                                   int ntab, StepRef sref)
 {
     const Run run = my_run(ntab);
+    const int step = step_of(sref);
     TotalCache pap_tot;
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
@@ -288,9 +292,10 @@ __device__ __forceinline__ bool refresh_due(const ProbDev &pd, int step)
 // into the FP64 iterate, e += c - alpha p (the deferred update of the last
 // iteration, when cg_b ran last), c = 0.
 __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restrict__ P,
-                                                        const TileRef *__restrict__ tab, int ntab, int step)
+                                                        const TileRef *__restrict__ tab, int ntab, StepRef sref)
 {
     const Run run = my_run(ntab);
+    const int step = step_of(sref);
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
         const ProbDev &pd = P[tr.prob];
@@ -339,9 +344,10 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
 // to the inner recurrence; with GAUGE, e.sqrt(d).
 template <bool WRITE32, bool GAUGE>
 __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__restrict__ P,
-                                                            const TileRef *__restrict__ tab, int ntab, int step)
+                                                            const TileRef *__restrict__ tab, int ntab, StepRef sref)
 {
     const Run run = my_run(ntab);
+    const int step = step_of(sref);
     for (int tt = run.begin; tt < run.end; ++tt) {
         const TileRef tr = tab[tt];
         const ProbDev &pd = P[tr.prob];
@@ -415,8 +421,9 @@ __device__ __forceinline__ int refresh_gap(double slope, double rr, double thres
 // of the next cg_a, through cg_b's partial slots; that cg_a skips the
 // deferred e update, which the replacement already folded into e).
 __global__ __launch_bounds__(kThreads) void k_refresh_decide(const ProbDev *__restrict__ P,
-                                                             const int *__restrict__ probs, int step)
+                                                             const int *__restrict__ probs, StepRef sref)
 {
+    const int step = step_of(sref);
     const ProbDev &pd = P[probs[blockIdx.x]];
     CgScalars *S = pd.S;
     if (!refresh_due(pd, step)) return;
@@ -463,6 +470,16 @@ __global__ __launch_bounds__(kThreads) void k_finish(const ProbDev *__restrict__
         // corrections): shift prd1 by the matching constant
         S->gshift = S->project ? (en - S->en0) / S->dd : 0.0;
     }
+}
+
+__global__ void k_advance(int *base, int by)
+{
+    if (threadIdx.x == 0) *base += by;
+}
+
+__global__ void k_set_step(int *base, int value)
+{
+    if (threadIdx.x == 0) *base = value;
 }
 
 // Plain SpMV q = A p of the FP64 or FP32 operator (timing only).
@@ -548,7 +565,7 @@ int vec_tiles(long long n)
     return (int)std::max<long long>(1, std::min<long long>(kVecBlocks, b));
 }
 
-void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, int step, int *ran,
+void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran,
                  hipStream_t s)
 {
     if (fp32)
@@ -557,7 +574,7 @@ void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int gr
         hipLaunchKernelGGL(k_cg_a<double>, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
 }
 
-void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, int step, hipStream_t s)
+void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, StepRef step, hipStream_t s)
 {
     if (fp32)
         hipLaunchKernelGGL(k_cg_b<float>, dim3(grid), dim3(kThreads), 0, s, P, t1, nt1, step);
@@ -566,7 +583,7 @@ void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int gr
 }
 
 void launch_refresh(const ProbDev *P, const TileRef *t1, int nt1, const TileRef *t3, int nt3, const int *probs,
-                    int nprobs, int grid, int step, hipStream_t s)
+                    int nprobs, int grid, StepRef step, hipStream_t s)
 {
     hipLaunchKernelGGL(k_refresh_e, dim3(std::min(grid, nt1)), dim3(kThreads), 0, s, P, t1, nt1, step);
     hipLaunchKernelGGL((k_true_residual<true, false>), dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3,
@@ -578,8 +595,18 @@ void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *prob
                    hipStream_t s)
 {
     hipLaunchKernelGGL((k_true_residual<false, true>), dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3,
-                       0);
+                       StepRef{nullptr, 0});
     hipLaunchKernelGGL(k_finish, dim3(nprobs), dim3(kThreads), 0, s, P, probs);
+}
+
+void launch_advance(int *base, int by, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, base, by);
+}
+
+void launch_set_step(int *base, int value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, base, value);
 }
 
 void launch_spmv_only(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, hipStream_t s)

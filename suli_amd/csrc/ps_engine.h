@@ -91,11 +91,24 @@ public:
     int spmv_samples() const { return spmv_n_; }
     // Times the plain SpMV of slot 0's operator (FP64 or FP32).
     int time_spmv(bool fp32, int reps, double *ms);
+    // Replay captured HIP graphs of kRefreshGrain steps (default) or launch
+    // every kernel from the host.  Timed runs always launch directly.
+    void set_graphs(bool on) { graphs_ = on; }
+    // Fixed chunk length (steps between host polls; rounded up to the
+    // refresh grain), or automatic when <= 0.
+    void set_chunk(int steps) { chunk_ = steps; }
+    bool graphs() const { return graphs_; }
+    int graph_captures() const { return captures_; }
 
 private:
     int fail(int code, const std::string &msg);
     int rebuild_tables();
     int enqueue_chunk(int K, bool timing);
+    int enqueue_steps_direct(int nsteps, bool timing);
+    int enqueue_group_graph();
+    int capture_graphs();
+    void retire_graphs();
+    void reap_graphs(bool wait);
     int finish(const std::vector<int> &done);
     int next_chunk_len() const;
     bool refresh_may_be_due(int step) const;
@@ -132,6 +145,17 @@ private:
     int stage_next_ = 0;
     // offsets of the four tables and the mixed-problem list in d_tab_/d_probs_list_
     int off_t3_64 = 0, off_t1_64 = 0, off_t3_32 = 0, off_t1_32 = 0;
+    // graph replay: step counter on the device, two captured groups of
+    // kRefreshGrain steps (without / with the residual replacement)
+    int *d_step_ = nullptr;
+    int dstep_value_ = 0;              // value of *d_step_ after the queued work
+    bool graphs_ = true;
+    int chunk_ = 0;
+    bool graph_valid_ = false;
+    hipGraphExec_t gx_[2] = {nullptr, nullptr};
+    int captures_ = 0;
+    // replaced graphs, destroyed once the stream has passed this event
+    std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired_;
     // timing
     std::vector<hipEvent_t> ev_spmv_;
     int *d_ran_ = nullptr;

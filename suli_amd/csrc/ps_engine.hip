@@ -146,6 +146,9 @@ Engine::~Engine()
     for (auto &e : ev_spmv_)
         if (e) (void)hipEventDestroy(e);
     if (d_ran_) (void)hipFree(d_ran_);
+    retire_graphs();
+    reap_graphs(true);
+    if (d_step_) (void)hipFree(d_step_);
 }
 
 int Engine::init(hipStream_t stream, int max_slots)
@@ -163,6 +166,9 @@ int Engine::init(hipStream_t stream, int max_slots)
     ev_spmv_.resize(2 * kMaxTimed);
     for (auto &e : ev_spmv_) ECHK(hipEventCreate(&e));
     ECHK(hipMalloc(&d_ran_, sizeof(int) * kMaxTimed));
+    ECHK(hipMalloc(&d_step_, sizeof(int)));
+    ECHK(hipMemset(d_step_, 0, sizeof(int)));
+    dstep_value_ = 0;
     return 0;
 }
 
@@ -324,6 +330,7 @@ int Engine::rebuild_tables()
     if (nprobs) ECHK(hipMemcpyAsync(d_probs_list_, hp, sizeof(int) * nprobs, hipMemcpyHostToDevice, s_));
     ECHK(hipEventRecord(ev_stage_[stg], s_));
     tables_dirty_ = false;
+    graph_valid_ = false;   // captured launches name the old table sizes
     return 0;
 }
 
@@ -334,6 +341,7 @@ int Engine::rebuild_tables()
 // every iteration and only need the host to notice it.
 int Engine::next_chunk_len() const
 {
+    if (chunk_ > 0) return (chunk_ + kRefreshGrain - 1) / kRefreshGrain * kRefreshGrain;
     bool any_mixed = false;
     long long biggest = 0;
     int best = kMaxRefreshGap;
@@ -367,29 +375,134 @@ bool Engine::refresh_may_be_due(int s) const
     return false;
 }
 
-int Engine::enqueue_chunk(int K, bool timing)
+// Kernels of global steps step_+1 .. step_+nsteps launched from the host,
+// each with its step number as an argument.
+int Engine::enqueue_steps_direct(int nsteps, bool timing)
 {
     const int n3_64 = (int)tab_.t3_64.size(), n1_64 = (int)tab_.t1_64.size();
     const int n3_32 = (int)tab_.t3_32.size(), n1_32 = (int)tab_.t1_32.size();
     const int nmix = (int)tab_.mixed_probs.size();
-    for (int it = 0; it < K; ++it) {
+    for (int it = 0; it < nsteps; ++it) {
         ++step_;
+        const StepRef sr{nullptr, step_};
         const bool timed = timing && (step_ % kTimeStride == 2) && ntimed_ < kMaxTimed;
         int *ran = timed ? d_ran_ + ntimed_ : nullptr;
         if (timed) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
-        if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), step_, ran, s_);
-        if (n3_32) launch_cg_a(true, d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), step_, ran, s_);
+        if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, ran, s_);
+        if (n3_32) launch_cg_a(true, d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, ran, s_);
         if (timed) {
             ECHK(hipEventRecord(ev_spmv_[2 * ntimed_ + 1], s_));
             ++ntimed_;
         }
-        if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), step_, s_);
-        if (n1_32) launch_cg_b(true, d_probs_, d_tab_ + off_t1_32, n1_32, std::min(n1_32, kMaxGrid), step_, s_);
+        if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), sr, s_);
+        if (n1_32) launch_cg_b(true, d_probs_, d_tab_ + off_t1_32, n1_32, std::min(n1_32, kMaxGrid), sr, s_);
         if (nmix && step_ % kRefreshGrain == 0 && refresh_may_be_due(step_))
             launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,
-                           kMaxGrid, step_, s_);
+                           kMaxGrid, sr, s_);
     }
     ECHK(hipGetLastError());
+    return 0;
+}
+
+// Launches of the current graphs may still be queued: keep them until the
+// stream has passed its present tail.
+void Engine::retire_graphs()
+{
+    for (auto &g : gx_)
+        if (g) {
+            hipEvent_t ev = nullptr;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+                hipEventRecord(ev, s_) == hipSuccess) {
+                retired_.emplace_back(g, ev);
+            } else {
+                if (ev) (void)hipEventDestroy(ev);
+                (void)hipStreamSynchronize(s_);
+                (void)hipGraphExecDestroy(g);
+            }
+            g = nullptr;
+        }
+    graph_valid_ = false;
+}
+
+void Engine::reap_graphs(bool wait)
+{
+    size_t keep = 0;
+    for (size_t i = 0; i < retired_.size(); ++i) {
+        auto &r = retired_[i];
+        if (wait) (void)hipEventSynchronize(r.second);
+        if (wait || hipEventQuery(r.second) == hipSuccess) {
+            (void)hipGraphExecDestroy(r.first);
+            (void)hipEventDestroy(r.second);
+        } else {
+            retired_[keep++] = r;
+        }
+    }
+    retired_.resize(keep);
+}
+
+// Captures one group of kRefreshGrain steps of the current tables as a
+// graph (and a second one that ends with the residual replacement).  The
+// kernels read the step from d_step_; the group's last node advances it.
+int Engine::capture_graphs()
+{
+    retire_graphs();
+    reap_graphs(false);
+    const int n3_64 = (int)tab_.t3_64.size(), n1_64 = (int)tab_.t1_64.size();
+    const int n3_32 = (int)tab_.t3_32.size(), n1_32 = (int)tab_.t1_32.size();
+    const int nmix = (int)tab_.mixed_probs.size();
+    for (int v = 0; v < 2; ++v) {
+        if (v == 1 && !nmix) break;
+        ECHK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        for (int off = 1; off <= kRefreshGrain; ++off) {
+            const StepRef sr{d_step_, off};
+            if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, nullptr, s_);
+            if (n3_32) launch_cg_a(true, d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, nullptr, s_);
+            if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), sr, s_);
+            if (n1_32) launch_cg_b(true, d_probs_, d_tab_ + off_t1_32, n1_32, std::min(n1_32, kMaxGrid), sr, s_);
+        }
+        if (v == 1)
+            launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,
+                           kMaxGrid, StepRef{d_step_, kRefreshGrain}, s_);
+        launch_advance(d_step_, kRefreshGrain, s_);
+        hipGraph_t g = nullptr;
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(s_, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(SULI_PS_EHIP, std::string("graph capture failed: ") +
+                                          hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&gx_[v], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(SULI_PS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    graph_valid_ = true;
+    ++captures_;
+    return 0;
+}
+
+// Steps step_+1 .. step_+kRefreshGrain by replaying a captured group.
+int Engine::enqueue_group_graph()
+{
+    int rc;
+    if (!graph_valid_ && (rc = capture_graphs())) return rc;
+    if (dstep_value_ != step_) launch_set_step(d_step_, step_, s_);
+    const bool refresh = !tab_.mixed_probs.empty() && refresh_may_be_due(step_ + kRefreshGrain);
+    ECHK(hipGraphLaunch(gx_[refresh ? 1 : 0], s_));
+    step_ += kRefreshGrain;
+    dstep_value_ = step_;
+    return 0;
+}
+
+int Engine::enqueue_chunk(int K, bool timing)
+{
+    int rc;
+    if (graphs_ && !timing) {
+        for (int it = 0; it < K; it += kRefreshGrain)
+            if ((rc = enqueue_group_graph())) return rc;
+    } else if ((rc = enqueue_steps_direct(K, timing))) {
+        return rc;
+    }
     const int slot = chunk_idx_ & 1;
     int hi = 0;
     for (int i = 0; i < max_slots_; ++i)

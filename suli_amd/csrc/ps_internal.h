@@ -105,6 +105,13 @@ struct ProbDev {
     CgScalars *S;
 };
 
+// Global step of a launch: *base + off when base is set (kernels replayed
+// from a captured HIP graph read the step from device memory), else off.
+struct StepRef {
+    const int *base;
+    int off;
+};
+
 // Entry of a launch table: tile `tile` of problem `prob`.
 struct TileRef {
     int prob;
@@ -151,16 +158,19 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
 // cg_a: deferred e -= alpha p; stop rule on r.r; p_n = r + beta p_{n-1}
 // (p buffers alternate); q = A p_n; p.q partials.  ran (nullable): set to 1
 // if problem 0's tile 0 did the SpMV (timing samples).
-void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, int step, int *ran,
+void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran,
                  hipStream_t s);
 // cg_b: alpha = r.r / p.q; r -= alpha q; r.r partials.
-void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, int step, hipStream_t s);
+void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, StepRef step, hipStream_t s);
 // Mixed precision residual replacement after global step `step` for the
 // problems of the tables (`probs` lists them, one entry per problem): e += c
 // (with the deferred alpha p term), r = b - A e in FP64 handed to the FP32
 // recurrence; decide converged / capped / continue.
 void launch_refresh(const ProbDev *P, const TileRef *t1, int nt1, const TileRef *t3, int nt3, const int *probs,
-                    int nprobs, int grid, int step, hipStream_t s);
+                    int nprobs, int grid, StepRef step, hipStream_t s);
+// *base += by (one thread), *base = value: the device step counter.
+void launch_advance(int *base, int by, hipStream_t s);
+void launch_set_step(int *base, int value, hipStream_t s);
 // After a problem's solve: FP64 true residual r.r into S->rr_true and the
 // gauge shift that restores e0's null-space coordinate.
 void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *probs, int nprobs, int grid,

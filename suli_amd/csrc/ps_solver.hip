@@ -144,6 +144,12 @@ SolveOpts solve_opts(const suli_ps_opts &o)
     return so;
 }
 
+void apply_engine_opts(suli_ps_ctx *ctx)
+{
+    ctx->eng.set_graphs(ctx->opts.no_graphs == 0);
+    ctx->eng.set_chunk(ctx->opts.chunk);
+}
+
 void release_last(suli_ps_ctx *ctx)
 {
     for (int s : ctx->last_slots) ctx->eng.release(s);
@@ -271,6 +277,7 @@ suli_ps_ctx *suli_ps_create(const suli_ps_opts *opts)
         suli_ps_destroy(ctx);
         return nullptr;
     }
+    apply_engine_opts(ctx);
     return ctx;
 }
 
@@ -299,6 +306,7 @@ int suli_ps_set_opts(suli_ps_ctx *ctx, const suli_ps_opts *opts)
         return fail(ctx, SULI_PS_EINVAL, "relative stop rule needs tol > 0");
     ctx->opts = *opts;
     if (ctx->opts.max_iter <= 0) ctx->opts.max_iter = 10000;
+    apply_engine_opts(ctx);
     return 0;
 }
 

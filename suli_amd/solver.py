@@ -97,15 +97,16 @@ class DeviceFields:
 class PressureSolver:
     def __init__(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
                  max_iter: int = 10000, chunk: int = 0, verbose: bool = True, timing: bool = False,
-                 keep_null: bool = False):
+                 keep_null: bool = False, graphs: bool = True):
         self.lib = _lib.load()
         self.opts = _lib.Opts()
-        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null)
+        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs)
         self.ctx = C.c_void_p(self.lib.suli_ps_create(C.byref(self.opts)))
         if not self.ctx:
             raise SuliPsError("suli_ps_create failed (no HIP device?)")
 
-    def _set(self, precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null):
+    def _set(self, precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null,
+             graphs=True):
         self.opts.precision = PRECISIONS[precision]
         self.opts.stop_rule = STOP_RULES[stop_rule]
         self.opts.tol = tol
@@ -114,12 +115,13 @@ class PressureSolver:
         self.opts.verbose = 1 if verbose else 0
         self.opts.timing = 1 if timing else 0
         self.opts.keep_null = 1 if keep_null else 0
+        self.opts.no_graphs = 0 if graphs else 1
 
     def configure(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
                   max_iter: int = 10000, chunk: int = 0, verbose: bool = True,
-                  timing: bool = False, keep_null: bool = False) -> None:
+                  timing: bool = False, keep_null: bool = False, graphs: bool = True) -> None:
         """Solver options (include/suli_ps.h, suli_ps_opts)."""
-        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null)
+        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs)
         self._check(self.lib.suli_ps_set_opts(self.ctx, C.byref(self.opts)))
 
     def _check(self, rc: int) -> int:

@@ -113,3 +113,18 @@ def test_batch_of_one_and_empty(solver):
     alone, _ = solve_alone(solver, c)
     assert np.array_equal(got[0], alone)
     assert solver.graddin_batch_dev([]) == []
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-11)])
+def test_graph_replay_equals_direct_launches(solver, precision, tol):
+    """Captured-graph replay and host launches run the same kernels on the
+    same steps: identical bits, batch and single."""
+    cs = mixed_cases(seed=11)[:4]
+    solver.configure(precision=precision, stop_rule="relative", tol=tol, verbose=False, graphs=True)
+    got_g, sts_g = solve_batch(solver, cs)
+    solver.configure(precision=precision, stop_rule="relative", tol=tol, verbose=False, graphs=False)
+    got_d, sts_d = solve_batch(solver, cs)
+    solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
+    for i in range(len(cs)):
+        assert sts_g[i].iters == sts_d[i].iters
+        assert np.array_equal(got_g[i], got_d[i]), f"problem {i}"
