@@ -46,7 +46,8 @@ enum suli_ps_status {
     SULI_PS_NOT_CONVERGED = 1,
     SULI_PS_EINVAL = -1,
     SULI_PS_EHIP = -2,
-    SULI_PS_ENOMEM = -3
+    SULI_PS_ENOMEM = -3,
+    SULI_PS_EAGAIN = -4     /* server: the request is not finished yet */
 };
 
 /* Scalars graddin/posdin read from SuLi's modules: disc (nx..dt, t_press,
@@ -150,6 +151,53 @@ typedef struct suli_ps_problem {
  * operation applied to many.  Returns the largest per-problem status (0 or
  * 1) or an error; stats.solve_ms is the time of the whole batch. */
 int suli_ps_graddin_batch_dev(suli_ps_ctx *ctx, suli_ps_problem *probs, int32_t count);
+
+/* ---- Serving: pressure solves submitted as they arrive -------------------
+ *
+ * A server owns a device stream, a copy stream and two threads.  Requests
+ * (host arrays in the layouts above) are admitted in arrival order while
+ * their device memory (fields plus solver state) fits the HBM budget; their
+ * inputs are copied while other requests iterate, and every admitted
+ * request joins the running batch at the next chunk of CG steps (the
+ * kernels of suli_ps_graddin_batch_dev).  Each result is bit-identical to
+ * solving the request alone.  The reference has no counterpart: it solves
+ * the one grid of its time loop (1_principal.f90:102). */
+typedef struct suli_ps_server suli_ps_server;
+
+typedef struct suli_ps_server_opts {
+    suli_ps_opts solve;     /* options of every request (precision, stop rule, ...) */
+    uint64_t hbm_budget;    /* device bytes for resident requests; 0: 80% of free memory */
+    int32_t max_active;     /* requests resident at once; <= 0 or > 256: 256 */
+    int32_t reserved;
+} suli_ps_server_opts;
+
+typedef struct suli_ps_server_info {
+    int64_t submitted, completed;
+    int32_t queued, resident;      /* waiting for memory / holding device memory */
+    uint64_t resident_bytes, peak_resident_bytes, budget;
+    int64_t chunks;                /* chunks of CG steps launched */
+} suli_ps_server_info;
+
+suli_ps_server *suli_ps_server_create(const suli_ps_server_opts *opts);
+/* Finishes every submitted request, then stops the threads. */
+void suli_ps_server_destroy(suli_ps_server *srv);
+const char *suli_ps_server_last_error(const suli_ps_server *srv);
+/* Queues a solve of graddin's problem (host pointers, kept valid by the
+ * caller until wait returns; prd1 is read as the initial guess and written
+ * with the result).  Returns a ticket >= 0, or < 0 (a request larger than
+ * the budget is SULI_PS_ENOMEM).  Thread-safe. */
+int64_t suli_ps_server_submit(suli_ps_server *srv, const suli_ps_grid *g, const double *u,
+                              const double *v, const double *w, const double *rho,
+                              const double *obs_lsx, const double *obs_lsy,
+                              const double *obs_lsz, const double *dz, const double *dzz,
+                              double *prd1);
+/* Waits up to timeout_s seconds (< 0: no limit) for a request; on return 0
+ * or 1 its prd1 holds the result and stats (if non-NULL) its statistics,
+ * with solve_ms the time from submit to result on the host.  Returns
+ * SULI_PS_EAGAIN on timeout.  Each ticket is waited for once. */
+int suli_ps_server_wait(suli_ps_server *srv, int64_t ticket, double timeout_s,
+                        suli_ps_stats *stats);
+int suli_ps_server_get_info(suli_ps_server *srv, suli_ps_server_info *info);
 
 /* Device-memory helpers for callers without their own HIP code. */
 void *suli_ps_dev_alloc(suli_ps_ctx *ctx, size_t bytes);

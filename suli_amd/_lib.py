@@ -16,6 +16,7 @@ SULI_PS_STOP_REFERENCE = 0
 SULI_PS_STOP_RELATIVE = 1
 SULI_PS_OK = 0
 SULI_PS_NOT_CONVERGED = 1
+SULI_PS_EAGAIN = -4
 
 _dp = C.POINTER(C.c_double)
 
@@ -25,7 +26,9 @@ EXPORTS = (
     "suli_ps_last_error", "suli_ps_graddin", "suli_ps_posdin", "suli_ps_graddin_dev",
     "suli_ps_posdin_dev", "suli_ps_dev_alloc", "suli_ps_dev_free", "suli_ps_memcpy_h2d",
     "suli_ps_memcpy_d2h", "suli_ps_dev_zero", "suli_ps_synchronize", "suli_ps_time_spmv",
-    "suli_ps_graddin_batch_dev",
+    "suli_ps_graddin_batch_dev", "suli_ps_server_create", "suli_ps_server_destroy",
+    "suli_ps_server_last_error", "suli_ps_server_submit", "suli_ps_server_wait",
+    "suli_ps_server_get_info",
 )
 
 
@@ -65,6 +68,20 @@ class Problem(C.Structure):
         ("stats", Stats)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("solve", Opts), ("hbm_budget", C.c_uint64), ("max_active", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("queued", C.c_int32),
+                ("resident", C.c_int32), ("resident_bytes", C.c_uint64),
+                ("peak_resident_bytes", C.c_uint64), ("budget", C.c_uint64), ("chunks", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -100,5 +117,14 @@ def load(path: str = LIB_PATH):
     L.suli_ps_synchronize.argtypes = [vp]
     L.suli_ps_time_spmv.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.suli_ps_graddin_batch_dev.argtypes = [vp, C.POINTER(Problem), C.c_int32]
+    L.suli_ps_server_create.restype = vp
+    L.suli_ps_server_create.argtypes = [C.POINTER(ServerOpts)]
+    L.suli_ps_server_destroy.argtypes = [vp]
+    L.suli_ps_server_last_error.restype = C.c_char_p
+    L.suli_ps_server_last_error.argtypes = [vp]
+    L.suli_ps_server_submit.restype = C.c_int64
+    L.suli_ps_server_submit.argtypes = [vp, C.POINTER(Grid)] + [vp] * 10
+    L.suli_ps_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(Stats)]
+    L.suli_ps_server_get_info.argtypes = [vp, C.POINTER(ServerInfo)]
     _lib = L
     return L

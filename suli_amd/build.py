@@ -30,7 +30,9 @@ UNITS = {
     "ps_setup.hip": ["-ffp-contract=off"],
     "ps_cg.hip": [],
     "ps_engine.hip": [],
+    "ps_common.hip": [],
     "ps_solver.hip": [],
+    "ps_server.hip": [],
 }
 
 

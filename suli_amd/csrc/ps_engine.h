@@ -47,6 +47,7 @@ struct Slot {
     ProbDev pd{};
     void *block = nullptr;     // the problem's device memory
     size_t bytes = 0;
+    bool own_block = true;     // taken from the engine's pool (else the caller's)
     double *prd1_out = nullptr;  // device prd1 (halo box) written at the end
     int n0 = 0;
     int user = -1;             // caller's index for this problem
@@ -66,8 +67,11 @@ public:
     // Admits a problem: allocates its slot, enqueues the setup kernels on the
     // stream and schedules its iteration 1 at the next global step.
     // prd1_out receives the solution (device, reference layout) when done.
+    // block (optional): problem_bytes() of device memory the caller owns, used
+    // instead of the engine's pool; wait_ev (optional): the setup kernels
+    // wait for it (inputs copied on another stream).
     int admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, const SolveOpts &o, int user,
-              int *slot_out);
+              int *slot_out, void *block = nullptr, hipEvent_t wait_ev = nullptr);
     // Runs chunks until every admitted problem has finished; each finished
     // problem's prd1_out holds its solution and status its final state.
     int run_all(bool timing);
@@ -79,6 +83,9 @@ public:
     // Copies the final scalars of these slots to their status (synchronises).
     int read_final(const std::vector<int> &slots);
     int pending_chunks() const { return npend_; }
+    int chunks_enqueued() const { return chunk_idx_; }
+    // Device address of a slot's scalars (valid while the slot is held).
+    const CgScalars *device_status(int slot) const { return d_status_ + slot; }
     hipStream_t stream() const { return s_; }
     // Frees a finished slot (its memory returns to the pool).
     void release(int slot);

@@ -128,7 +128,7 @@ Engine::~Engine()
 {
     if (s_) (void)hipStreamSynchronize(s_);
     for (auto &sl : slots_)
-        if (sl.block) (void)hipFree(sl.block);
+        if (sl.block && sl.own_block) (void)hipFree(sl.block);
     pool_.release_all();
     if (d_probs_) (void)hipFree(d_probs_);
     if (d_status_) (void)hipFree(d_status_);
@@ -180,7 +180,7 @@ int Engine::active_count() const
 }
 
 int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, const SolveOpts &o, int user,
-                  int *slot_out)
+                  int *slot_out, void *block, hipEvent_t wait_ev)
 {
     int si = -1;
     for (int i = 0; i < max_slots_; ++i)
@@ -196,7 +196,8 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     sl.o = o;
     const int nt3 = spmv_tiles(sl.g), nt1 = vec_tiles(sl.g.n);
     sl.bytes = problem_bytes(gr.nx, gr.ny, gr.nz, o.mixed, o.kz);
-    sl.block = pool_.get(sl.bytes, err_);
+    sl.own_block = block == nullptr;
+    sl.block = block ? block : pool_.get(sl.bytes, err_);
     if (!sl.block) return SULI_PS_ENOMEM;
     Carver cv{static_cast<char *>(sl.block)};
     ProbDev &pd = sl.pd;
@@ -251,7 +252,8 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
         ECHK(hipEventRecord(ev_stage_[stg], s_));
     }
 
-    // setup kernels (ps_setup.hip)
+    // setup kernels (ps_setup.hip), after the inputs have arrived
+    if (wait_ev) ECHK(hipStreamWaitEvent(s_, wait_ev, 0));
     double *sx = const_cast<double *>(pd.sx), *sy = const_cast<double *>(pd.sy), *sz = const_cast<double *>(pd.sz);
     launch_face_coefs(sl.g, in, sx, sy, sz, s_);
     launch_diag_rhs(sl.g, in, sx, sy, sz, const_cast<double *>(pd.d), pd.b, s_);
@@ -269,7 +271,7 @@ void Engine::release(int si)
 {
     Slot &sl = slots_[si];
     if (!sl.used) return;
-    pool_.put(sl.block, sl.bytes);
+    if (sl.own_block) pool_.put(sl.block, sl.bytes);
     sl = Slot{};
     tables_dirty_ = true;
 }

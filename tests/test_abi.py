@@ -39,6 +39,8 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 7 * 8
     assert ctypes.sizeof(_lib.Problem) == ctypes.sizeof(_lib.Grid) + 10 * 8 + ctypes.sizeof(_lib.Stats)
     assert _lib.Problem.stats.offset == ctypes.sizeof(_lib.Grid) + 10 * 8
+    assert ctypes.sizeof(_lib.ServerOpts) == ctypes.sizeof(_lib.Opts) + 8 + 4 + 4
+    assert ctypes.sizeof(_lib.ServerInfo) == 8 + 8 + 4 + 4 + 8 + 8 + 8 + 8
 
 
 def test_fortran_shim_declares_same_entry_points():
