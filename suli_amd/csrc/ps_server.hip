@@ -197,11 +197,13 @@ void suli_ps_server::copy_loop()
             }
         }
         if (down) {
-            // result and final scalars, after the finish kernels
+            // result and final scalars, after the finish kernels.  A host
+            // wait: the device stream may be capturing a graph meanwhile,
+            // and a stream may not wait on work of a capturing stream.
             Request &r = *down;
             const size_t nbox = (size_t)(r.g.nx + 2) * (r.g.ny + 2) * (r.g.nz + 2);
             CgScalars fin{};
-            hipError_t e = hipStreamWaitEvent(s_copy, r.ev_done, 0);
+            hipError_t e = hipEventSynchronize(r.ev_done);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(r.prd1, r.dprd1, nbox * sizeof(double), hipMemcpyDeviceToHost, s_copy);
             if (e == hipSuccess)
