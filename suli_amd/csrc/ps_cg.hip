@@ -224,6 +224,155 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
     }
 }
 
+// One 3-D tile of the one-launch FP32 iteration n (cg_m).  With UPD the
+// residual of iteration n-1 is formed where needed,
+//   r_{n-1} = r_{n-2} - alpha_{n-1} q_{n-1}            (9_grad.f90:470-480)
+// else (iteration 1, or the step after a residual replacement) r_old is
+// r_{n-1} itself; then as cg_a: p_n = r_{n-1} + beta p_{n-1}, q_n = A p_n,
+// e -= alpha_{n-1} p_{n-1} on the tile's own cells (with UPD).  Writes
+// r_{n-1}, p_n, q_n; partials p_n.q_n, q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}.
+template <bool FIRST, bool UPD>
+__device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__restrict__ c32,
+                                          const float *__restrict__ r_old, const float *__restrict__ q_old,
+                                          const float *__restrict__ p_old, float *__restrict__ r_new,
+                                          float *__restrict__ p_new, float *__restrict__ q_new,
+                                          const float *__restrict__ sx, const float *__restrict__ sy,
+                                          const float *__restrict__ sz, float a, float b, double acc[4])
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long plane = (long long)nx * ny;
+    if (t.i >= nx || t.j >= ny) return;
+    long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
+    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
+    const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
+    const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
+    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
+    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+    auto rn = [&](long long x) -> float { return UPD ? r_old[x] - a * q_old[x] : r_old[x]; };
+    auto pn = [&](long long x) -> float { return FIRST ? rn(x) : rn(x) + b * p_old[x]; };
+    float poc = FIRST ? 0.0f : p_old[c];
+    float rc = rn(c);
+    float pc = FIRST ? rc : rc + b * poc;
+    float pb = t.k0 > 0 ? pn(c - plane) : pc;
+    float szb = sz[c];
+    for (int k = t.k0; k < t.k1; ++k) {
+        const long long r0 = c - t.i;
+        const long long up = (k < nz - 1) ? plane : 0;
+        const float pot = FIRST ? 0.0f : p_old[c + up];
+        const float rt = rn(c + up);
+        const float pt = FIRST ? rt : rt + b * pot;
+        const float szt = sz[c + plane];
+        const float pw = pn(r0 + iw), pe = pn(r0 + ie);
+        const float ps = pn(c + dS), pnn = pn(c + dN);
+        const float sxw = sx[fx], sxe = sx[fx + 1];
+        const float sys = sy[fy], syn = sy[fy + nx];
+        const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * pt - szb * pb;
+        q_new[c] = y;
+        p_new[c] = pc;
+        r_new[c] = rc;
+        if (UPD) c32[c] = c32[c] - a * poc;   // 9_grad.f90:458-468
+        acc[0] += (double)pc * (double)y;
+        acc[1] += (double)y * (double)rc;
+        acc[2] += (double)y * (double)y;
+        acc[3] += (double)rc * (double)rc;
+        pb = pc;
+        pc = pt;
+        rc = rt;
+        poc = pot;
+        szb = szt;
+        c += plane;
+        fx += (long long)ny * (nx + 1);
+        fy += (long long)(ny + 1) * nx;
+    }
+}
+
+__device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return parity ? pd.r32b : pd.r32; }
+__device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
+__device__ __forceinline__ float *pbuf(const ProbDev &pd, int parity) { return parity ? pd.p32_1 : pd.p32_0; }
+
+// The mixed solver's FP32 iteration, one launch per global step.  Scalars
+// from the totals of the previous step's partials (slot (n-1) & 1):
+//   alpha_{n-1} = rho_{n-2} / p_{n-1}.q_{n-1}               (9_grad.f90:450-455)
+//   rho_{n-1}   = rho_{n-2} - 2 alpha q.r + alpha^2 q.q      (r.r of r_{n-1})
+//   beta_n      = rho_{n-1} / rho_{n-2}                      (:510-515)
+// rho_{n-2} is summed directly by the previous step, so the identity only
+// spans one iteration.  After a residual replacement rho_{n-1} is the
+// replaced residual's r.r and nothing of iteration n-1 is left to finish.
+// The stop test is the FP64 one at replacements; here only the cap.
+__global__ __launch_bounds__(kThreads) void k_cg_m(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
+                                                   int ntab, StepRef sref, int *ran)
+{
+    const Run run = my_run(ntab);
+    const int step = step_of(sref);
+    int cached = -1;
+    double T[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = run.begin; t < run.end; ++t) {
+        const TileRef tr = tab[t];
+        const ProbDev &pd = P[tr.prob];
+        CgScalars *S = pd.S;
+        const int n = step - pd.n0 + 1;
+        if (n < 1 || n > S->done_at) continue;
+        const bool first = n == 1;
+        const bool upd = !first && step != S->skip_e_at;
+        if (!first && tr.prob != cached) {
+            total4_of(pd.P.m4 + (size_t)((n - 1) & 1) * 4 * pd.nt3, pd.nt3, T);
+            cached = tr.prob;
+        }
+        double alpha = 0.0, rho, beta = 0.0;
+        if (first) {
+            rho = S->rr[1];
+        } else if (upd) {
+            alpha = T[3] / T[0];
+            rho = T[3] - 2.0 * alpha * T[1] + alpha * alpha * T[2];
+            beta = rho / T[3];
+        } else {
+            rho = S->rr_after;
+            beta = S->beta_after;
+        }
+        const bool cap = n - 1 >= S->max_iter;
+        if (tr.tile == 0 && threadIdx.x == 0) {
+            S->rr[n & 1] = rho;
+            S->iters = n - 1;
+            S->alpha = alpha;
+            if (cap) S->done_at = n;
+            if (ran && tr.prob == 0 && !cap) *ran = 1;
+        }
+        const int o = (n - 1) & 1, w = n & 1;
+        const float a = (float)alpha;
+        const Tile tl = tile3(pd.g, tr.tile);
+        if (cap) {
+            // finish iteration n-1 only (the e update; the replacement that
+            // follows recomputes the residual)
+            if (upd && tl.i < pd.g.nx && tl.j < pd.g.ny) {
+                const float *p_old = pbuf(pd, o);
+                const long long plane = (long long)pd.g.nx * pd.g.ny;
+                long long c = (long long)tl.k0 * plane + (long long)tl.j * pd.g.nx + tl.i;
+                for (int k = tl.k0; k < tl.k1; ++k, c += plane) pd.c32[c] = pd.c32[c] - a * p_old[c];
+            }
+            continue;
+        }
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const float b = (float)beta;
+        if (first)
+            cg_m_tile<true, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
+                                   qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+        else if (upd)
+            cg_m_tile<false, true>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
+                                   qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+        else
+            cg_m_tile<false, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
+                                    qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+        block_sum4(acc);
+        if (threadIdx.x == 0) {
+            double *m = pd.P.m4 + (size_t)w * 4 * pd.nt3 + 4 * (size_t)tr.tile;
+            m[0] = acc[0];
+            m[1] = acc[1];
+            m[2] = acc[2];
+            m[3] = acc[3];
+        }
+    }
+}
+
 // cg_b: alpha = r.r / p.Ap (9_grad.f90:450-455); r -= alpha q (:470-480);
 // r.r partials (:482-508).  1-D tile k of a problem with nt1 tiles handles
 // the 16-byte vectors k*256 + lane + m*nt1*256.
@@ -289,13 +438,16 @@ __device__ __forceinline__ bool refresh_due(const ProbDev &pd, int step)
 }
 
 // Mixed precision, residual replacement step 1: fold the FP32 correction
-// into the FP64 iterate, e += c - alpha p (the deferred update of the last
-// iteration, when cg_b ran last), c = 0.
+// into the FP64 iterate, e += c - alpha_n p_n (the update of the last
+// iteration n, which its cg_m left to the next step; alpha_n from that
+// step's partials: r_{n-1}.r_{n-1} / p_n.q_n), c = 0.
 __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restrict__ P,
                                                         const TileRef *__restrict__ tab, int ntab, StepRef sref)
 {
     const Run run = my_run(ntab);
     const int step = step_of(sref);
+    int cached = -1;
+    double T[4] = {0.0, 0.0, 0.0, 0.0};
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
         const ProbDev &pd = P[tr.prob];
@@ -303,8 +455,12 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
         const int n = step - pd.n0 + 1;
         if (!refresh_due(pd, step)) continue;
         const bool pending = S->done_at == kNotDone;
-        const float a = (float)S->alpha;
-        const float *p_last = (n & 1) ? pd.p32_1 : pd.p32_0;
+        if (tr.prob != cached) {
+            total4_of(pd.P.m4 + (size_t)(n & 1) * 4 * pd.nt3, pd.nt3, T);
+            cached = tr.prob;
+        }
+        const float a = (float)(T[3] / T[0]);
+        const float *p_last = pbuf(pd, n & 1);
         double *e = pd.e;
         float *c = pd.c32;
         const long long len = pd.g.n;
@@ -360,6 +516,8 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
         const double *__restrict__ sy = pd.sy;
         const double *__restrict__ sz = pd.sz;
         const double *__restrict__ b = pd.b;
+        // the replaced residual goes where the next cg_m reads r_{n}
+        float *__restrict__ r32 = WRITE32 ? rbuf(pd, (step - pd.n0 + 1) & 1) : nullptr;
         double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
         if (t.i < nx && t.j < ny) {
             const long long plane = (long long)nx * ny;
@@ -378,7 +536,7 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
                 acc += r * r;
                 if (WRITE32) {
                     const float rf = (float)r;
-                    pd.r32[c] = rf;
+                    r32[c] = rf;
                     acc32 += (double)rf * (double)rf;
                 }
                 if (GAUGE) acc_en += ec * sqrt(pd.d[c]);
@@ -430,12 +588,12 @@ __global__ __launch_bounds__(kThreads) void k_refresh_decide(const ProbDev *__re
     const int n_last = step - pd.n0 + 1;
     const double rrt = total_of(pd.P.tres, pd.nt3);
     const double rrs = total_of(pd.P.tres + pd.nt3, pd.nt3);
+    double T[4];
+    total4_of(pd.P.m4 + (size_t)(n_last & 1) * 4 * pd.nt3, pd.nt3, T);
     const bool pending = S->done_at == kNotDone;
     const int iters = pending ? n_last : S->iters;
     const bool conv = !(fabs(rrt) > S->thresh);
     const bool cap = iters >= S->max_iter;
-    if (!conv && !cap)
-        for (int t = threadIdx.x; t < pd.nt1; t += kThreads) pd.P.upd[t] = t == 0 ? rrs : 0.0;
     if (threadIdx.x == 0) {
         S->rr_true = rrt;
         S->refreshes += 1;
@@ -447,6 +605,17 @@ __global__ __launch_bounds__(kThreads) void k_refresh_decide(const ProbDev *__re
             if (pending) S->done_at = n_last;
         } else {
             S->skip_e_at = step + 1;
+            S->rr_after = rrs;
+            // The direction p_n carries on with beta = r'.r' / r_{n-1}.r_{n-1}
+            // (the replaced residual in CG's beta).  After a cycle that cut
+            // the true r.r by less than 100x, beta = r'.r' / r_n.r_n
+            // instead (r_n.r_n of the FP32 recurrence, from the identity):
+            // the plain continuation can stall there at ~1e-9 relative.
+            const double rho_nm1 = T[3];
+            const double alpha_n = T[3] / T[0];
+            const double rho_n = T[3] - 2.0 * alpha_n * T[1] + alpha_n * alpha_n * T[2];
+            const bool stalled = S->prev_rr > 0.0 && rrt > 1e-2 * S->prev_rr;
+            S->beta_after = rrs / (stalled ? rho_n : rho_nm1);
             if (S->prev_iters > 0 && S->prev_rr > 0.0 && rrt < S->prev_rr)
                 S->slope = log(rrt / S->prev_rr) / (double)(n_last - S->prev_iters);
             S->prev_iters = n_last;
@@ -572,6 +741,11 @@ void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int gr
         hipLaunchKernelGGL(k_cg_a<float>, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
     else
         hipLaunchKernelGGL(k_cg_a<double>, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
+}
+
+void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_m, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
 }
 
 void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, StepRef step, hipStream_t s)

@@ -49,6 +49,46 @@ __device__ __forceinline__ double total_of(const double *partial, int n)
     return r;
 }
 
+// Four workgroup sums at once (fixed order), valid in thread 0 only.
+__device__ __forceinline__ void block_sum4(double v[4])
+{
+    __shared__ double red4[4][kThreads / 64];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = wave_sum(v[c]);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red4[c][threadIdx.x >> 6] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double s = red4[c][0];
+#pragma unroll
+            for (int w = 1; w < kThreads / 64; ++w) s += red4[c][w];
+            v[c] = s;
+        }
+}
+
+// Totals of n partial 4-vectors (m[4t + c]), returned to every thread; the
+// order depends only on n.
+__device__ __forceinline__ void total4_of(const double *m, int n, double out[4])
+{
+    __shared__ double tot4[4];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < n; t += kThreads)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += m[4 * t + c];
+    block_sum4(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) tot4[c] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c] = tot4[c];
+    __syncthreads();
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
 // XCD b % 8).  Map them so that each XCD walks one contiguous range of
 // tiles: neighbouring tiles (which share halo planes) then share an L2.

@@ -53,6 +53,9 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
         pd.q32 = cv.take<float>(n);
         pd.p32_0 = cv.take<float>(n);
         pd.p32_1 = cv.take<float>(n);
+        pd.r32b = cv.take<float>(n);
+        pd.q32b = cv.take<float>(n);
+        pd.P.m4 = cv.take<double>(8 * (size_t)nt3);
     }
     pd.P.spmv = cv.take<double>(nt3);
     pd.P.upd = cv.take<double>(nt1);
@@ -391,13 +394,12 @@ int Engine::enqueue_steps_direct(int nsteps, bool timing)
         int *ran = timed ? d_ran_ + ntimed_ : nullptr;
         if (timed) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
         if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, ran, s_);
-        if (n3_32) launch_cg_a(true, d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, ran, s_);
+        if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, ran, s_);
         if (timed) {
             ECHK(hipEventRecord(ev_spmv_[2 * ntimed_ + 1], s_));
             ++ntimed_;
         }
         if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), sr, s_);
-        if (n1_32) launch_cg_b(true, d_probs_, d_tab_ + off_t1_32, n1_32, std::min(n1_32, kMaxGrid), sr, s_);
         if (nmix && step_ % kRefreshGrain == 0 && refresh_may_be_due(step_))
             launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,
                            kMaxGrid, sr, s_);
@@ -458,9 +460,8 @@ int Engine::capture_graphs()
         for (int off = 1; off <= kRefreshGrain; ++off) {
             const StepRef sr{d_step_, off};
             if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, nullptr, s_);
-            if (n3_32) launch_cg_a(true, d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, nullptr, s_);
+            if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, nullptr, s_);
             if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), sr, s_);
-            if (n1_32) launch_cg_b(true, d_probs_, d_tab_ + off_t1_32, n1_32, std::min(n1_32, kMaxGrid), sr, s_);
         }
         if (v == 1)
             launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,

@@ -73,6 +73,8 @@ struct CgScalars {
     double thresh_ref;
     double prev_rr;   // true r.r at the previous replacement
     double slope;     // d log(r.r) / d iteration between replacements (< 0)
+    double rr_after;  // r.r of the replaced residual (mixed), used by the next step
+    double beta_after;  // beta of the step after a replacement (mixed)
 };
 
 // Per-tile partial sums of one problem.
@@ -82,6 +84,8 @@ struct Partials {
     double *tres;     // [3*nt3]    true residual r.r (FP64, FP32 copy) and e.sqrt(d)
     double *init;     // [5*nt1]    setup: r0.r0, b.b, r0.sqrt(d), d, e0.sqrt(d)
     double *proj;     // [2*nt1]    setup after projection: r0.r0, FP32 r0.r0
+    double *m4;       // [2][nt3][4] one-launch FP32 iteration n, slot n & 1:
+                      //            p_n.q_n, q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}
 };
 
 // Device descriptor of one problem of a batch: geometry, tiling, where its
@@ -98,9 +102,11 @@ struct ProbDev {
     double *b;
     // FP64 iterate and, for FP64 solves, the CG vectors (p alternates)
     double *e, *r, *q, *p0, *p1;
-    // FP32 inner solve (mixed): operator copy, correction c, vectors
+    // FP32 inner solve (mixed): operator copy, correction c, vectors.  The
+    // one-launch iteration alternates r, p and q between two buffers each
+    // (r32/r32b, p32_0/p32_1, q32/q32b by iteration parity).
     const float *sx32, *sy32, *sz32;
-    float *c32, *r32, *q32, *p32_0, *p32_1;
+    float *c32, *r32, *q32, *p32_0, *p32_1, *r32b, *q32b;
     Partials P;
     CgScalars *S;
 };
@@ -162,6 +168,13 @@ void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int gr
                  hipStream_t s);
 // cg_b: alpha = r.r / p.q; r -= alpha q; r.r partials.
 void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, StepRef step, hipStream_t s);
+// The FP32 inner iteration of the mixed solver in one launch per step:
+// finishes iteration n-1 (r and e updates, formed where needed) and does
+// iteration n's direction update and SpMV, with r.r of the finished
+// iteration from the exact identity r_{n-1}.r_{n-1} = r_{n-2}.r_{n-2}
+// - 2 alpha q.r_{n-2} + alpha^2 q.q anchored on the directly summed
+// r_{n-2}.r_{n-2}.  ran: as for cg_a.
+void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, hipStream_t s);
 // Mixed precision residual replacement after global step `step` for the
 // problems of the tables (`probs` lists them, one entry per problem): e += c
 // (with the deferred alpha p term), r = b - A e in FP64 handed to the FP32
