@@ -10,12 +10,18 @@ the same steps for the comparison.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-solves C]
 
-Prints one JSON line.  `roofline` is for the dominant kernel, the fused
-FP32 CG phase k_cg_a<float> (SpMV q = A p fused with the deferred update of
-the iterate and the direction update): algorithmic bytes 9 words/cell (reads
-c, p_{n-1}, r, sx, sy, sz; writes c, p_n, q) divided by its mean launch
-duration, timed with HIP events on the solver's stream during the timed
-solves (every 16th iteration).  `cpu_baseline` runs the reference's own Fortran
+Prints one JSON line.  `roofline` is for the dominant kernel, k_cg_m (the
+whole FP32 CG step of the mixed solver in one launch: residual and iterate
+updates of the previous iteration, direction update, SpMV, four dot-product
+partials): algorithmic bytes 11 words/cell (reads r, q, p of the previous
+step, the correction c and the three face-coefficient arrays; writes r, p,
+q and c) divided by its mean launch time, from HIP events on the solver's
+stream around windows of 7 consecutive launches (one window per 16 steps)
+during two directly launched solves.  The FP64 solver's step (k_cg_a +
+k_cg_b, 12 FP64 words/cell) is reported beside it.  `traffic` is the HBM
+bytes per launch from rocprofv3's FETCH_SIZE and WRITE_SIZE counters
+(profiles/hbm_traffic.json), calibrated on the copy kernel k_copy.
+`cpu_baseline` runs the reference's own Fortran
 graddin (oracle/_ref, compiled from the SuLi sources) with OpenMP on the
 host cores, on one solve of the same case.
 """
@@ -71,15 +77,17 @@ def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
         st = solver.graddin_dev(grid, df)
         if st.spmv_timed:
             spmv.append(st.spmv_ms)
+    # the plain SpMV q = A p of this solve's operator (k_spmv), 50 launches
+    spmv_only = solver.time_spmv(precision, reps=50)
     return dict(solves_per_s=steps / total, ms_per_step=total * 1e3 / steps,
                 p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
                 iters=int(np.median(iters)), refreshes=int(np.median(refreshes)),
                 rel_true_max=float(max(rel)), graphs=graphs,
-                spmv_ms=float(np.mean(spmv)) if spmv else None)
+                spmv_ms=float(np.mean(spmv)) if spmv else None, spmv_only_ms=spmv_only)
 
 
 def load_traffic(name):
-    path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -127,6 +135,10 @@ def main():
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=96)
+    ap.add_argument("--only", choices=("both", "mixed", "fp64"), default="both",
+                    help="solver(s) to time (profiling runs)")
+    ap.add_argument("--copy-mib", type=int, default=256,
+                    help="size of the plain copy kernel that calibrates bandwidth and counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
@@ -142,24 +154,43 @@ def main():
     solver = PressureSolver(verbose=False)
     df = solver.upload(g, case.fields)
     print(f"# case {case.name} n={n}", file=sys.stderr, flush=True)
-    mixed = run_solves(solver, g, df, "mixed", args.steps, args.warmup, args.tol)
-    print(f"# mixed {mixed}", file=sys.stderr, flush=True)
-    fp64 = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol)
-    print(f"# fp64 {fp64}", file=sys.stderr, flush=True)
+    copy_bytes = args.copy_mib << 20
+    copy_ms = solver.copy_bench(copy_bytes, reps=10)
+    copy = {"kernel": "k_copy", "bytes_read": copy_bytes, "bytes_written": copy_bytes,
+            "ms": copy_ms, "gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
+    print(f"# copy {copy}", file=sys.stderr, flush=True)
+    mixed = fp64 = None
+    if args.only in ("both", "mixed"):
+        mixed = run_solves(solver, g, df, "mixed", args.steps, args.warmup, args.tol)
+        print(f"# mixed {mixed}", file=sys.stderr, flush=True)
+    if args.only in ("both", "fp64"):
+        fp64 = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol)
+        print(f"# fp64 {fp64}", file=sys.stderr, flush=True)
     df.free()
     solver.close()
+    if mixed is None or fp64 is None:
+        print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy}))
+        return
 
-    bytes32 = 9 * 4 * n
+    bytes32 = 11 * 4 * n          # k_cg_m: 11 FP32 words per cell
+    bytes64 = 12 * 8 * n          # FP64 step (k_cg_a + k_cg_b): 12 FP64 words per cell
     spmv_ms = mixed["spmv_ms"]
     achieved = bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                "traffic": load_traffic(f"k_cg_a<float> {g.nx}x{g.ny}x{g.nz}"),
-                "kernel": "k_cg_a<float>", "bytes_per_launch": bytes32,
+                "traffic": load_traffic(f"k_cg_m {g.nx}x{g.ny}x{g.nz}"),
+                "kernel": "k_cg_m", "bytes_per_launch": bytes32,
                 "launch_ms": spmv_ms,
-                "fp64_kernel": {"kernel": "k_cg_a<double>", "launch_ms": fp64["spmv_ms"],
-                                "achieved": (2 * bytes32 / (fp64["spmv_ms"] * 1e-3) / 1e9)
-                                if fp64["spmv_ms"] else None}}
+                "spmv": {"kernel": "k_spmv<float>", "words_per_cell": 5,
+                         "launch_ms": mixed["spmv_only_ms"],
+                         "achieved": 5 * 4 * n / (mixed["spmv_only_ms"] * 1e-3) / 1e9,
+                         "frac": 5 * 4 * n / (mixed["spmv_only_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "fp64_launch_ms": fp64["spmv_only_ms"],
+                         "fp64_achieved": 5 * 8 * n / (fp64["spmv_only_ms"] * 1e-3) / 1e9},
+                "fp64_step": {"kernels": "k_cg_a<double> + k_cg_b<double>",
+                              "step_ms": fp64["spmv_ms"], "bytes_per_step": bytes64,
+                              "achieved": (bytes64 / (fp64["spmv_ms"] * 1e-3) / 1e9)
+                              if fp64["spmv_ms"] else None}}
 
     cpu = None
     if not args.no_cpu:
@@ -187,6 +218,7 @@ def main():
         "speedup_mixed_vs_fp64": mixed["solves_per_s"] / fp64["solves_per_s"],
         "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
         "roofline": roofline,
+        "copy_kernel": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

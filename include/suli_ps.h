@@ -69,7 +69,7 @@ typedef struct suli_ps_opts {
     int32_t max_iter;      /* <= 0: 10000, the reference cap (9_grad.f90:398) */
     int32_t chunk;         /* CG iterations between host polls; <= 0: automatic */
     int32_t verbose;       /* 1: print the reference's non-convergence line */
-    int32_t timing;        /* 1: time SpMV launches with HIP events (stats) */
+    int32_t timing;        /* 1: time CG steps with HIP events (stats) */
     int32_t keep_null;     /* 1: keep the initial residual's component along the
                               operator's null space (constant pressure), as the
                               reference does; 0 (default): remove it, which
@@ -82,14 +82,16 @@ typedef struct suli_ps_stats {
     int32_t status;        /* enum suli_ps_status */
     int32_t iters;         /* CG iterations (reference `cont`) */
     int32_t refreshes;     /* FP64 residual replacements (mixed) */
-    int32_t spmv_timed;    /* SpMV launches timed (opts.timing) */
+    int32_t spmv_timed;    /* timing windows measured (opts.timing) */
     double rr0;            /* r.r of the initial scaled residual */
     double bb;             /* b.b of the scaled right-hand side */
     double rr;             /* final r.r the stop rule saw */
     double thresh;         /* the stop rule's threshold on r.r */
     double rel_true;       /* FP64 ||b - A e|| / ||b|| recomputed at exit */
     double solve_ms;       /* device time of the solve, setup included */
-    double spmv_ms;        /* mean duration of a timed SpMV launch */
+    double spmv_ms;        /* mean time of one CG step (its launches, SpMV
+                              included) over windows of 7 consecutive steps,
+                              HIP events on the context stream (opts.timing) */
 } suli_ps_stats;
 
 typedef struct suli_ps_ctx suli_ps_ctx;
@@ -211,6 +213,12 @@ int suli_ps_synchronize(suli_ps_ctx *ctx);
  * operator of the last graddin call (precision as given) and returns the
  * mean launch duration in ms, timed with HIP events on the context stream. */
 int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double *ms);
+
+/* Measurement hook with no reference counterpart: a plain device copy of
+ * `bytes` (16-byte vectors, grid-stride) run `reps` times between two
+ * device buffers the library allocates; returns the mean launch time in ms.
+ * Its known traffic (bytes read, bytes written) calibrates HBM counters. */
+int suli_ps_copy_bench(suli_ps_ctx *ctx, size_t bytes, int32_t reps, double *ms);
 
 #ifdef __cplusplus
 }

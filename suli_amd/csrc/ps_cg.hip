@@ -641,6 +641,13 @@ __global__ __launch_bounds__(kThreads) void k_finish(const ProbDev *__restrict__
     }
 }
 
+// Plain copy (calibration of HBM counters).
+__global__ __launch_bounds__(kThreads) void k_copy(const float4 *__restrict__ x, float4 *__restrict__ y, long long n)
+{
+    for (long long v = (long long)blockIdx.x * kThreads + threadIdx.x; v < n; v += (long long)gridDim.x * kThreads)
+        y[v] = x[v];
+}
+
 __global__ void k_advance(int *base, int by)
 {
     if (threadIdx.x == 0) *base += by;
@@ -771,6 +778,12 @@ void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *prob
     hipLaunchKernelGGL((k_true_residual<false, true>), dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3,
                        StepRef{nullptr, 0});
     hipLaunchKernelGGL(k_finish, dim3(nprobs), dim3(kThreads), 0, s, P, probs);
+}
+
+void launch_copy(const void *src, void *dst, long long n16, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(kThreads), 0, s, static_cast<const float4 *>(src),
+                       static_cast<float4 *>(dst), n16);
 }
 
 void launch_advance(int *base, int by, hipStream_t s)

@@ -10,8 +10,9 @@ namespace sps {
 namespace {
 
 constexpr int kMaxGrid = 2048;     // persistent workgroups (8 per CU)
-constexpr int kMaxTimed = 256;     // SpMV launches timed per run_all
-constexpr int kTimeStride = 16;    // time every 16th step
+constexpr int kMaxTimed = 256;     // timing windows per run_all
+constexpr int kTimeStride = 16;    // one window per 16 steps ...
+constexpr int kTimeSteps = 7;      // ... over steps 16k+1 .. 16k+7 (no replacement inside)
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -168,7 +169,7 @@ int Engine::init(hipStream_t stream, int max_slots)
     for (auto &e : ev_stage_) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ev_spmv_.resize(2 * kMaxTimed);
     for (auto &e : ev_spmv_) ECHK(hipEventCreate(&e));
-    ECHK(hipMalloc(&d_ran_, sizeof(int) * kMaxTimed));
+    ECHK(hipMalloc(&d_ran_, sizeof(int) * 2 * kMaxTimed));
     ECHK(hipMalloc(&d_step_, sizeof(int)));
     ECHK(hipMemset(d_step_, 0, sizeof(int)));
     dstep_value_ = 0;
@@ -390,12 +391,16 @@ int Engine::enqueue_steps_direct(int nsteps, bool timing)
     for (int it = 0; it < nsteps; ++it) {
         ++step_;
         const StepRef sr{nullptr, step_};
-        const bool timed = timing && (step_ % kTimeStride == 2) && ntimed_ < kMaxTimed;
-        int *ran = timed ? d_ran_ + ntimed_ : nullptr;
-        if (timed) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
+        // timing windows of kTimeSteps consecutive steps (events only at
+        // their ends, so the per-step figure is the launches' own time)
+        const int ph = step_ % kTimeStride;
+        const bool t_open = timing && ph == 1 && ntimed_ < kMaxTimed;
+        const bool t_close = timing && ph == kTimeSteps && ntimed_ < kMaxTimed;
+        int *ran = t_open ? d_ran_ + 2 * ntimed_ : t_close ? d_ran_ + 2 * ntimed_ + 1 : nullptr;
+        if (t_open) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
         if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, ran, s_);
         if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, ran, s_);
-        if (timed) {
+        if (t_close) {
             ECHK(hipEventRecord(ev_spmv_[2 * ntimed_ + 1], s_));
             ++ntimed_;
         }
@@ -609,7 +614,7 @@ int Engine::run_all(bool timing)
 {
     int rc;
     ntimed_ = 0;
-    if (timing) ECHK(hipMemsetAsync(d_ran_, 0, sizeof(int) * kMaxTimed, s_));
+    if (timing) ECHK(hipMemsetAsync(d_ran_, 0, sizeof(int) * 2 * kMaxTimed, s_));
     std::vector<int> all;
     for (int i = 0; i < max_slots_; ++i)
         if (slots_[i].used && slots_[i].active && !slots_[i].finished) all.push_back(i);
@@ -625,14 +630,15 @@ int Engine::run_all(bool timing)
     spmv_ms_ = 0.0;
     spmv_n_ = 0;
     if (timing && ntimed_ > 0) {
-        std::vector<int> ran(ntimed_);
-        ECHK(hipMemcpy(ran.data(), d_ran_, sizeof(int) * ntimed_, hipMemcpyDeviceToHost));
+        // windows in which problem 0 iterated at both ends
+        std::vector<int> ran(2 * ntimed_);
+        ECHK(hipMemcpy(ran.data(), d_ran_, sizeof(int) * 2 * ntimed_, hipMemcpyDeviceToHost));
         double sum = 0.0;
         for (int t = 0; t < ntimed_; ++t) {
-            if (!ran[t]) continue;
+            if (!ran[2 * t] || !ran[2 * t + 1]) continue;
             float ms = 0.f;
             ECHK(hipEventElapsedTime(&ms, ev_spmv_[2 * t], ev_spmv_[2 * t + 1]));
-            sum += ms;
+            sum += ms / kTimeSteps;
             ++spmv_n_;
         }
         spmv_ms_ = spmv_n_ ? sum / spmv_n_ : 0.0;

@@ -191,6 +191,9 @@ void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *prob
 // Stand-alone SpMV q = A p of one problem (no vector updates), for timing.
 void launch_spmv_only(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int grid, hipStream_t s);
 
+// Plain copy of n16 16-byte vectors (measurement only).
+void launch_copy(const void *src, void *dst, long long n16, hipStream_t s);
+
 int spmv_tiles(const Geo &g);
 int vec_tiles(long long n);
 Geo make_geo(int nx, int ny, int nz, int px, int py, int kz = 0);

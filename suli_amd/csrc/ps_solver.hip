@@ -423,4 +423,31 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
     return 0;
 }
 
+int suli_ps_copy_bench(suli_ps_ctx *ctx, size_t bytes, int32_t reps, double *ms)
+{
+    if (!ctx || !ms || reps <= 0 || bytes < 16) return SULI_PS_EINVAL;
+    const size_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    hipError_t e = hipMemsetAsync(a, 0, n16 * 16, s);
+    launch_copy(a, b, (long long)n16, s);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) launch_copy(a, b, (long long)n16, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]);
+    if (e != hipSuccess) rc = fail(ctx, SULI_PS_EHIP, "copy bench: %s", hipGetErrorString(e));
+    *ms = (double)t / reps;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 }  // extern "C"
