@@ -76,6 +76,9 @@ typedef struct suli_ps_opts {
                               changes the solution by a constant only */
     int32_t no_graphs;     /* 1: launch every kernel from the host instead of
                               replaying captured HIP graphs (same results) */
+    int32_t autotune;      /* 1: pick each grid size's tile depth by timing the
+                              CG step kernel on its first solve (see
+                              suli_ps_autotune); 0: a fixed rule */
 } suli_ps_opts;
 
 typedef struct suli_ps_stats {
@@ -134,14 +137,33 @@ int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, doubl
                        double *w, const double *rho, const double *prd1,
                        const double *dz, const double *dzz, double *prd);
 
-/* One problem of a batch: its grid, device arrays (layouts as above) and,
- * on return, its statistics. */
+/* graddin / posdin with a solid-cell mask, an extension with no reference
+ * counterpart (the reference's IBM only masks the right-hand side,
+ * 9_grad.f90:45-88).  solid (nx*ny*nz bytes, x fastest, device; NULL = none)
+ * marks cells that are masked out: every face with a solid cell on either
+ * side (ghosts by the boundary rule) carries no flux, in the operator, in
+ * the divergence and in the correction; solid cells get pressure 0.
+ * Otherwise as suli_ps_graddin_dev / _posdin_dev. */
+int suli_ps_graddin_masked_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u,
+                               const double *v, const double *w, const double *rho,
+                               const double *obs_lsx, const double *obs_lsy,
+                               const double *obs_lsz, const double *dz, const double *dzz,
+                               const uint8_t *solid, double *prd1, suli_ps_stats *stats);
+int suli_ps_posdin_masked_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v,
+                              double *w, const double *rho, const double *prd1,
+                              const double *dz, const double *dzz, const uint8_t *solid,
+                              double *prd);
+
+/* One problem of a batch: its grid, device arrays (layouts as above; solid
+ * as for suli_ps_graddin_masked_dev, may be NULL) and, on return, its
+ * statistics. */
 typedef struct suli_ps_problem {
     suli_ps_grid grid;
     const double *u, *v, *w, *rho;
     const double *obs_lsx, *obs_lsy, *obs_lsz;
     const double *dz, *dzz;
     double *prd1;
+    const uint8_t *solid;
     suli_ps_stats stats;
 } suli_ps_problem;
 
@@ -213,6 +235,16 @@ int suli_ps_synchronize(suli_ps_ctx *ctx);
  * operator of the last graddin call (precision as given) and returns the
  * mean launch duration in ms, timed with HIP events on the context stream. */
 int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double *ms);
+
+/* Tile depth of the CG kernels for a grid size, no reference counterpart:
+ * times the per-step kernel (FP64 cg_a, or the mixed solver's FP32 step)
+ * on a scratch problem of this size for depths 2..48 planes and keeps the
+ * fastest for this context (returned in kz, with its time per step).
+ * Solves of that size then use it when opts.autotune is set.  A result
+ * depends on the tiling (the order of its sums), so solves of one size
+ * agree bit for bit only under the same choice. */
+int suli_ps_autotune(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t *kz,
+                     double *step_us);
 
 /* Measurement hook with no reference counterpart: a plain device copy of
  * `bytes` (16-byte vectors, grid-stride) run `reps` times between two

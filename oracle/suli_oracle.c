@@ -148,6 +148,28 @@ static double dot_interior(const oracle_grid *g, const double *a, const double *
     return s;
 }
 
+/* Solid-cell mask (extension, not in the reference).  Fortran halo index
+ * -> 1-based cell by the mirror/periodic rule; a face is closed when the
+ * cell on either side is solid. */
+static int cell_map(int I, int n, int per)
+{
+    if (I == 0) return per ? n : 1;
+    if (I == n + 1) return per ? 1 : n;
+    return I;
+}
+
+static int solid_at(const oracle_grid *g, int i, int j, int k)
+{
+    const int nx = g->nx, ny = g->ny;
+    return g->solid && g->solid[IC(cell_map(i, nx, g->ccx0 == 0), cell_map(j, ny, g->ccy0 == 0),
+                                   cell_map(k, g->nz, 0))];
+}
+
+/* x face i lies between cells i-1 and i (Fortran indices), likewise y, z. */
+static int closed_x(const oracle_grid *g, int i, int j, int k) { return solid_at(g, i - 1, j, k) || solid_at(g, i, j, k); }
+static int closed_y(const oracle_grid *g, int i, int j, int k) { return solid_at(g, i, j - 1, k) || solid_at(g, i, j, k); }
+static int closed_z(const oracle_grid *g, int i, int j, int k) { return solid_at(g, i, j, k - 1) || solid_at(g, i, j, k); }
+
 /* Everything graddin computes before the CG loop that tests want to see:
  * face coefficients, diagonal with ghosts, q = -div u (9_grad.f90:45-239). */
 static void build_unscaled(const oracle_grid *g, const double *u, const double *v, const double *w,
@@ -223,6 +245,22 @@ static void build_unscaled(const oracle_grid *g, const double *u, const double *
     free(rhoy);
     free(rhoz);
 
+    /* solid mask: closed faces carry no flux */
+    if (g->solid) {
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx + 1; i++)
+                    if (closed_x(g, i, j, k)) sx[IFX(i, j, k)] = 0.0;
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny + 1; j++)
+                for (i = 1; i <= nx; i++)
+                    if (closed_y(g, i, j, k)) sy[IFY(i, j, k)] = 0.0;
+        for (k = 1; k <= nz + 1; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    if (closed_z(g, i, j, k)) sz[IFZ(i, j, k)] = 0.0;
+    }
+
     /* diagonal (9_grad.f90:152-160) */
 #pragma omp parallel for num_threads(nt) private(i, j) if (nt > 1)
     for (k = 1; k <= nz; k++)
@@ -246,8 +284,24 @@ static void build_unscaled(const oracle_grid *g, const double *u, const double *
                     dv = v[IV(i, j, k)] - v[IV(i, j + 1, k)];
                     dw = w[IW(i, j, k)] - w[IW(i, j, k + 1)];
                 }
+                if (g->solid) {
+                    /* closed faces count with zero velocity; solid cells: d = 1, q = 0 */
+                    const double u0 = closed_x(g, i, j, k) ? 0.0 : u[IU(i, j, k)];
+                    const double u1 = closed_x(g, i + 1, j, k) ? 0.0 : u[IU(i + 1, j, k)];
+                    const double v0 = closed_y(g, i, j, k) ? 0.0 : v[IV(i, j, k)];
+                    const double v1 = closed_y(g, i, j + 1, k) ? 0.0 : v[IV(i, j + 1, k)];
+                    const double w0 = closed_z(g, i, j, k) ? 0.0 : w[IW(i, j, k)];
+                    const double w1 = closed_z(g, i, j, k + 1) ? 0.0 : w[IW(i, j, k + 1)];
+                    du = u0 - u1;
+                    dv = v0 - v1;
+                    dw = w0 - w1;
+                }
                 const double dzk = g->dzzv ? g->dzzv[k] : g->dz;
                 q[IC(i, j, k)] = du / g->dx + dv / g->dy + dw / dzk;
+                if (solid_at(g, i, j, k)) {
+                    d[IP(i, j, k)] = 1.0;
+                    q[IC(i, j, k)] = 0.0;
+                }
             }
     free(u1);
     free(v1);
@@ -324,6 +378,12 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
     /* e = prd1 * sqrt(d) on the whole box (9_grad.f90:242-252) */
 #pragma omp parallel for num_threads(nt) if (nt > 1)
     for (size_t c = 0; c < nbox; c++) e[c] = prd1[c] * sqrt(d[c]);
+    /* solid mask: solid cells (and their ghost images) start at 0 */
+    if (g->solid)
+        for (k = 0; k <= nz + 1; k++)
+            for (j = 0; j <= ny + 1; j++)
+                for (i = 0; i <= nx + 1; i++)
+                    if (solid_at(g, i, j, k)) e[IP(i, j, k)] = 0.0;
 
     scale_faces(g, d, sx, sy, sz);
 
@@ -342,17 +402,22 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
      * vector sqrt(d); remove the residual's component along it, making the
      * floating-point right-hand side exactly consistent.  Serial sums. */
     if (g->project_null) {
+        /* with a solid mask the null vector is sqrt(d) on fluid cells only */
         double rn = 0.0, dd = 0.0;
         for (k = 1; k <= nz; k++)
             for (j = 1; j <= ny; j++)
                 for (i = 1; i <= nx; i++) {
-                    rn = rn + r[IP(i, j, k)] * sqrt(d[IP(i, j, k)]);
-                    dd = dd + d[IP(i, j, k)];
+                    const double dc = solid_at(g, i, j, k) ? 0.0 : d[IP(i, j, k)];
+                    rn = rn + r[IP(i, j, k)] * sqrt(dc);
+                    dd = dd + dc;
                 }
         const double gamma = rn / dd;
         for (k = 1; k <= nz; k++)
             for (j = 1; j <= ny; j++)
-                for (i = 1; i <= nx; i++) r[IP(i, j, k)] = r[IP(i, j, k)] - gamma * sqrt(d[IP(i, j, k)]);
+                for (i = 1; i <= nx; i++) {
+                    const double dc = solid_at(g, i, j, k) ? 0.0 : d[IP(i, j, k)];
+                    r[IP(i, j, k)] = r[IP(i, j, k)] - gamma * sqrt(dc);
+                }
     }
 
     /* r.r, always serial (9_grad.f90:303-319) */
@@ -456,19 +521,23 @@ void oracle_posdin(const oracle_grid *g, double *u, double *v, double *w, const 
     oracle_interp_faces(g, rho, rhox, rhoy, rhoz);
     const double aux1 = g->dt / g->dx;
     const double aux2 = g->dt / g->dy;
+    /* with a solid mask, closed faces are left alone (extension) */
     for (k = 1; k <= nz; k++) {
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx + 1; i++)
-                u[IU(i, j, k)] = u[IU(i, j, k)] - aux1 * (prd1[IP(i, j, k)] - prd1[IP(i - 1, j, k)]) / rhox[IFX(i, j, k)];
+                if (!closed_x(g, i, j, k))
+                    u[IU(i, j, k)] = u[IU(i, j, k)] - aux1 * (prd1[IP(i, j, k)] - prd1[IP(i - 1, j, k)]) / rhox[IFX(i, j, k)];
         for (j = 1; j <= ny + 1; j++)
             for (i = 1; i <= nx; i++)
-                v[IV(i, j, k)] = v[IV(i, j, k)] - aux2 * (prd1[IP(i, j, k)] - prd1[IP(i, j - 1, k)]) / rhoy[IFY(i, j, k)];
+                if (!closed_y(g, i, j, k))
+                    v[IV(i, j, k)] = v[IV(i, j, k)] - aux2 * (prd1[IP(i, j, k)] - prd1[IP(i, j - 1, k)]) / rhoy[IFY(i, j, k)];
     }
     for (k = 1; k <= nz + 1; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
                 const double dzk = g->dzzv ? g->dzzv[k] : g->dz;
-                w[IW(i, j, k)] = w[IW(i, j, k)] - g->dt / dzk * (prd1[IP(i, j, k)] - prd1[IP(i, j, k - 1)]) / rhoz[IFZ(i, j, k)];
+                if (!closed_z(g, i, j, k))
+                    w[IW(i, j, k)] = w[IW(i, j, k)] - g->dt / dzk * (prd1[IP(i, j, k)] - prd1[IP(i, j, k - 1)]) / rhoz[IFZ(i, j, k)];
             }
     memcpy(prd, prd1, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
     free(rhox);

@@ -42,6 +42,9 @@ typedef struct oracle_grid {
     int threads;            /* OpenMP threads for the sweeps; <= 0 = runtime default */
     int project_null;       /* not in the reference: remove the initial residual's
                                component along the null vector sqrt(d) */
+    const unsigned char *solid; /* not in the reference: cells (nx,ny,nz, x fastest)
+                                   masked out; faces touching one carry no flux;
+                                   NULL = none */
 } oracle_grid;
 
 typedef struct oracle_stats {

@@ -28,7 +28,8 @@ EXPORTS = (
     "suli_ps_memcpy_d2h", "suli_ps_dev_zero", "suli_ps_synchronize", "suli_ps_time_spmv",
     "suli_ps_graddin_batch_dev", "suli_ps_server_create", "suli_ps_server_destroy",
     "suli_ps_server_last_error", "suli_ps_server_submit", "suli_ps_server_wait",
-    "suli_ps_server_get_info", "suli_ps_copy_bench",
+    "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
+    "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev",
 )
 
 
@@ -47,6 +48,7 @@ class Opts(C.Structure):
         ("precision", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
         ("max_iter", C.c_int32), ("chunk", C.c_int32), ("verbose", C.c_int32),
         ("timing", C.c_int32), ("keep_null", C.c_int32), ("no_graphs", C.c_int32),
+        ("autotune", C.c_int32),
     ]
 
 
@@ -64,7 +66,7 @@ class Stats(C.Structure):
 
 class Problem(C.Structure):
     _fields_ = [("grid", Grid)] + [(n, C.c_void_p) for n in (
-        "u", "v", "w", "rho", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "prd1")] + [
+        "u", "v", "w", "rho", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "prd1", "solid")] + [
         ("stats", Stats)]
 
 
@@ -108,6 +110,8 @@ def load(path: str = LIB_PATH):
     pargs = [vp, C.POINTER(Grid)] + [vp] * 8
     L.suli_ps_posdin.argtypes = pargs
     L.suli_ps_posdin_dev.argtypes = pargs
+    L.suli_ps_graddin_masked_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 11 + [C.POINTER(Stats)]
+    L.suli_ps_posdin_masked_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 9
     L.suli_ps_dev_alloc.restype = vp
     L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]
     L.suli_ps_dev_free.argtypes = [vp, vp]
@@ -118,6 +122,8 @@ def load(path: str = LIB_PATH):
     L.suli_ps_time_spmv.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.suli_ps_graddin_batch_dev.argtypes = [vp, C.POINTER(Problem), C.c_int32]
     L.suli_ps_copy_bench.argtypes = [vp, C.c_size_t, C.c_int32, C.POINTER(C.c_double)]
+    L.suli_ps_autotune.argtypes = [vp, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]
     L.suli_ps_server_create.restype = vp
     L.suli_ps_server_create.argtypes = [C.POINTER(ServerOpts)]
     L.suli_ps_server_destroy.argtypes = [vp]

@@ -70,6 +70,9 @@ class Fields:
     obs_lsx: Optional[np.ndarray] = None
     obs_lsy: Optional[np.ndarray] = None
     obs_lsz: Optional[np.ndarray] = None
+    # solid-cell mask (uint8, (nx, ny, nz), Fortran order; not a reference
+    # mode): faces touching a solid cell carry no flux
+    solid: Optional[np.ndarray] = None
 
     def copy(self) -> "Fields":
         return Fields(*(None if a is None else a.copy(order="F") for a in dataclasses.astuple(self)))
@@ -161,6 +164,37 @@ def config2(seed: int = 1, nx: int = 256, ny: int = 128, nz: int = 96,
     height = 0.25 * (nz * dx) / 0.48
     f.rho[...] = sloshing_rho(g, height, 0.02 * (nz * dx) / 0.48, 1.5, rho_air, rho_water)
     return Case(f"cfg2-{nx}x{ny}x{nz}", g, f, seed)
+
+
+def cylinder_cells(g: Grid, radius: float) -> np.ndarray:
+    """Solid-cell mask (uint8, Fortran order) of a full-height vertical
+    cylinder at the domain centre: cells whose centre lies inside."""
+    xc, yc = 0.5 * g.nx * g.dx, 0.5 * g.ny * g.dy
+    x = cell_centres(g.nx, g.dx)
+    y = cell_centres(g.ny, g.dy)
+    inside = (x[:, None] - xc) ** 2 + (y[None, :] - yc) ** 2 < radius * radius
+    return np.asfortranarray(np.repeat(inside[:, :, None], g.nz, axis=2).astype(np.uint8))
+
+
+def config3(seed: int = 1, nx: int = 384, ny: int = 192, nz: int = 128,
+            dx: float = 5e-3) -> Case:
+    """Stretched vertical grid with an immersed solid: dz finest at the free
+    surface z = 0.6 H and growing away from it by at most 1.05 per cell
+    (tanh profile), H = nz dx; water (rho 1000) below the surface, air (1.2)
+    above, blended over 1.5 max(dx, dy, mean dz); a full-height cylinder of
+    radius 0.1 Ly at the centre masked out (faces touching it carry no flux);
+    seeded U(-0.05, 0.05) velocities.  Not a reference mode (the reference's
+    IBM masks only the right-hand side)."""
+    rng = np.random.default_rng(seed)
+    H = nz * dx
+    g = Grid(nx, ny, nz, dx, dx, dx, dt=1e-3, rho_f1=1.2, ccx0=1, ccy0=1)
+    g.dzv = stretched_dz(nz, H, 0.6, 1.05)
+    g.dzzv = dzz_from_dz(g.dzv)
+    f = zeros_fields(g)
+    random_velocity(g, f, 0.05, rng)
+    f.rho[...] = sloshing_rho(g, 0.6 * H, 0.0, 1.5, 1.2, 1000.0)
+    f.solid = cylinder_cells(g, 0.1 * ny * dx)
+    return Case(f"cfg3-{nx}x{ny}x{nz}", g, f, seed)
 
 
 def stretched_dz(nz: int, height: float, z_focus_frac: float = 0.6,

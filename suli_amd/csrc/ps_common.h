@@ -18,7 +18,7 @@ int validate_problem(const suli_ps_grid *g, const double *u, const double *v, co
 // Setup inputs of graddin's problem (pointers as given: host or device).
 SetupIn make_setup(const suli_ps_grid *g, const double *u, const double *v, const double *w, const double *rho,
                    const double *prd1, const double *ox, const double *oy, const double *oz, const double *dz,
-                   const double *dzz);
+                   const double *dzz, const uint8_t *solid = nullptr);
 SolveOpts solve_opts(const suli_ps_opts &o);
 // Statistics of a finished problem from its final scalars.
 void fill_stats(const Slot &sl, suli_ps_stats *st);

@@ -48,9 +48,10 @@ int validate_problem(const suli_ps_grid *g, const double *u, const double *v, co
 
 SetupIn make_setup(const suli_ps_grid *g, const double *u, const double *v, const double *w, const double *rho,
                    const double *prd1, const double *ox, const double *oy, const double *oz, const double *dz,
-                   const double *dzz)
+                   const double *dzz, const uint8_t *solid)
 {
     SetupIn in{};
+    in.solid = solid;
     in.u = u;
     in.v = v;
     in.w = w;

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/suli_ps.h"
@@ -34,7 +36,7 @@ struct SolveOpts {
     double tol = 1e-8;
     int max_iter = 10000;
     bool keep_null = false;
-    int kz = 0;          // planes per 3-D tile (0: default for the size)
+    int kz = 0;          // planes per 3-D tile (0: tuned or default for the size)
 };
 
 // One problem resident on the device.
@@ -49,13 +51,17 @@ struct Slot {
     size_t bytes = 0;
     bool own_block = true;     // taken from the engine's pool (else the caller's)
     double *prd1_out = nullptr;  // device prd1 (halo box) written at the end
+    const uint8_t *solid = nullptr;  // solid-cell mask (device), if any
     int n0 = 0;
     int user = -1;             // caller's index for this problem
     CgScalars status{};        // last status read on the host
 };
 
-// Bytes of device memory a problem of this size and precision needs.
-size_t problem_bytes(int nx, int ny, int nz, bool mixed, int kz);
+// Bytes of device memory a problem of this size and precision needs (for
+// any tile depth kz >= kMinKz).
+size_t problem_bytes(int nx, int ny, int nz, bool mixed);
+
+constexpr int kMinKz = 2;
 
 class Engine {
 public:
@@ -101,6 +107,16 @@ public:
     // Replay captured HIP graphs of kRefreshGrain steps (default) or launch
     // every kernel from the host.  Timed runs always launch directly.
     void set_graphs(bool on) { graphs_ = on; }
+    // Held while a graph is captured: other threads of the process that
+    // allocate or free device memory (which synchronises the device and
+    // breaks a capture in progress) take it too.
+    void set_capture_mutex(std::mutex *m) { capture_mutex_ = m; }
+    // Tile depth per problem size: with autotune on, a size's first
+    // admission times the per-step kernel over candidate depths on a
+    // scratch problem and keeps the fastest for every later problem of that
+    // size (results then depend on the size only, as without tuning).
+    void set_autotune(bool on) { autotune_ = on; }
+    int tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out);
     // Fixed chunk length (steps between host polls; rounded up to the
     // refresh grain), or automatic when <= 0.
     void set_chunk(int steps) { chunk_ = steps; }
@@ -157,7 +173,14 @@ private:
     int *d_step_ = nullptr;
     int dstep_value_ = 0;              // value of *d_step_ after the queued work
     bool graphs_ = true;
+    bool autotune_ = false;
+    struct Tuned {
+        int kz;
+        double us;
+    };
+    std::map<std::tuple<int, int, int, bool>, Tuned> tuned_;
     int chunk_ = 0;
+    std::mutex *capture_mutex_ = nullptr;
     bool graph_valid_ = false;
     hipGraphExec_t gx_[2] = {nullptr, nullptr};
     int captures_ = 0;

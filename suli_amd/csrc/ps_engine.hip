@@ -106,9 +106,11 @@ void Pool::release_all()
     held_ = 0;
 }
 
-size_t problem_bytes(int nx, int ny, int nz, bool mixed, int kz)
+size_t problem_bytes(int nx, int ny, int nz, bool mixed)
 {
-    const Geo g = make_geo(nx, ny, nz, 0, 0, kz);
+    // the partial-sum arrays sized for the most tiles (the thinnest slabs);
+    // everything else does not depend on the tiling
+    const Geo g = make_geo(nx, ny, nz, 0, 0, kMinKz);
     Carver cv{nullptr};
     ProbDev pd{};
     carve(cv, g, mixed, spmv_tiles(g), vec_tiles(g.n), pd);
@@ -196,10 +198,17 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     if (step_ % kRefreshGrain != 0) return fail(SULI_PS_EINVAL, "engine: admission off the refresh grain");
     Slot &sl = slots_[si];
     sl = Slot{};
-    sl.g = make_geo(gr.nx, gr.ny, gr.nz, gr.ccx0 == 0, gr.ccy0 == 0, o.kz);
+    int kz = o.kz > 0 ? std::max(o.kz, kMinKz) : 0;
+    if (kz == 0 && autotune_) {
+        double us = 0.0;
+        int rc_t = tune(gr.nx, gr.ny, gr.nz, o.mixed, &kz, &us);
+        if (rc_t) return rc_t;
+    }
+    sl.g = make_geo(gr.nx, gr.ny, gr.nz, gr.ccx0 == 0, gr.ccy0 == 0, kz);
     sl.o = o;
+    sl.o.kz = sl.g.kz;
     const int nt3 = spmv_tiles(sl.g), nt1 = vec_tiles(sl.g.n);
-    sl.bytes = problem_bytes(gr.nx, gr.ny, gr.nz, o.mixed, o.kz);
+    sl.bytes = problem_bytes(gr.nx, gr.ny, gr.nz, o.mixed);
     sl.own_block = block == nullptr;
     sl.block = block ? block : pool_.get(sl.bytes, err_);
     if (!sl.block) return SULI_PS_ENOMEM;
@@ -215,6 +224,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     sl.n0 = step_ + 1;   // iteration 1 runs at the next global step
     pd.n0 = sl.n0;
     sl.prd1_out = prd1_out;
+    sl.solid = in.solid;
     sl.user = user;
     sl.used = true;
     sl.active = true;
@@ -454,6 +464,8 @@ void Engine::reap_graphs(bool wait)
 // kernels read the step from d_step_; the group's last node advances it.
 int Engine::capture_graphs()
 {
+    std::unique_lock<std::mutex> lk;
+    if (capture_mutex_) lk = std::unique_lock<std::mutex>(*capture_mutex_);
     retire_graphs();
     reap_graphs(false);
     const int n3_64 = (int)tab_.t3_64.size(), n1_64 = (int)tab_.t1_64.size();
@@ -560,7 +572,7 @@ int Engine::finish(const std::vector<int> &done)
     launch_finish(d_probs_, dt, (int)t3.size(), dp, (int)done.size(), kMaxGrid, s_);
     for (int si : done) {
         Slot &sl = slots_[si];
-        launch_unscale(sl.g, sl.pd.e, sl.pd.d, sl.prd1_out, sl.pd.S, s_);
+        launch_unscale(sl.g, sl.pd.e, sl.pd.d, sl.prd1_out, sl.pd.S, sl.solid, s_);
         sl.finished = true;
     }
     ECHK(hipGetLastError());
@@ -643,6 +655,119 @@ int Engine::run_all(bool timing)
         }
         spmv_ms_ = spmv_n_ ? sum / spmv_n_ : 0.0;
     }
+    return 0;
+}
+
+int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out)
+{
+    const auto key = std::make_tuple(nx, ny, nz, mixed);
+    auto hit = tuned_.find(key);
+    if (hit != tuned_.end()) {
+        *kz_out = hit->second.kz;
+        *us_out = hit->second.us;
+        return 0;
+    }
+    const Geo g_def = make_geo(nx, ny, nz, 0, 0, 0);
+    std::vector<int> cands = {2, 3, 4, 6, 8, 12, 16, 24, 32, 48};
+    if (std::find(cands.begin(), cands.end(), g_def.kz) == cands.end()) cands.push_back(g_def.kz);
+    const size_t bytes = problem_bytes(nx, ny, nz, mixed);
+    void *blk = pool_.get(bytes, err_);
+    if (!blk) return SULI_PS_ENOMEM;
+    ProbDev *dP = nullptr;
+    CgScalars *dS = nullptr;
+    TileRef *dT = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = 0;
+    const Geo g_min = make_geo(nx, ny, nz, 0, 0, kMinKz);
+    const int max_nt3 = spmv_tiles(g_min);
+    auto cleanup = [&] {
+        (void)hipStreamSynchronize(s_);
+        if (dP) (void)hipFree(dP);
+        if (dS) (void)hipFree(dS);
+        if (dT) (void)hipFree(dT);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        pool_.put(blk, bytes);
+    };
+#define TCHK(call)                                                                                       \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) {                                                                          \
+            cleanup();                                                                                   \
+            return fail(SULI_PS_EHIP, std::string("autotune: ") + #call + ": " + hipGetErrorString(e_)); \
+        }                                                                                                \
+    } while (0)
+    TCHK(hipMalloc(&dP, sizeof(ProbDev)));
+    TCHK(hipMalloc(&dS, sizeof(CgScalars)));
+    TCHK(hipMalloc(&dT, sizeof(TileRef) * max_nt3));
+    TCHK(hipEventCreate(&a));
+    TCHK(hipEventCreate(&b));
+    // state that keeps the per-step kernel on its working path with finite
+    // values: zero vectors; FP64: entering r.r 0 over a previous 1 (beta 0),
+    // threshold -1; mixed: previous partials p.q = r.r = 1 (alpha = beta = 1)
+    CgScalars s0{};
+    s0.rr[0] = s0.rr[1] = 1.0;
+    s0.thresh = -1.0;
+    s0.done_at = kNotDone;
+    s0.skip_e_at = -1;
+    s0.max_iter = 1 << 30;
+    s0.mixed = mixed ? 1 : 0;
+    s0.next_refresh = kNotDone;
+    std::vector<TileRef> tab(max_nt3);
+    const int reps = 20, step = 2;   // n = 2 every launch: same inputs each time
+    double best_us = 1e300;
+    int best_kz = g_def.kz;
+    for (int kz : cands) {
+        if (kz > nz && kz != cands.front()) continue;
+        const Geo g = make_geo(nx, ny, nz, 0, 0, kz);
+        ProbDev pd{};
+        Carver cv{static_cast<char *>(blk)};
+        const int nt3 = spmv_tiles(g), nt1 = vec_tiles(g.n);
+        carve(cv, g, mixed, nt3, nt1, pd);
+        pd.g = g;
+        pd.nt3 = nt3;
+        pd.nt1 = nt1;
+        pd.mixed = mixed ? 1 : 0;
+        pd.n0 = 1;
+        pd.S = dS;
+        for (int t = 0; t < nt3; ++t) tab[t] = {0, t};
+        TCHK(hipMemsetAsync(blk, 0, cv.off, s_));
+        if (mixed) {
+            const double m1[4] = {1.0, 0.0, 0.0, 1.0};
+            TCHK(hipMemcpyAsync(pd.P.m4 + 4 * (size_t)nt3, m1, sizeof m1, hipMemcpyHostToDevice, s_));
+        }
+        TCHK(hipMemcpyAsync(dP, &pd, sizeof pd, hipMemcpyHostToDevice, s_));
+        TCHK(hipMemcpyAsync(dS, &s0, sizeof s0, hipMemcpyHostToDevice, s_));
+        TCHK(hipMemcpyAsync(dT, tab.data(), sizeof(TileRef) * nt3, hipMemcpyHostToDevice, s_));
+        TCHK(hipStreamSynchronize(s_));
+        const int grid = std::min(nt3, kMaxGrid);
+        auto launch = [&] {
+            if (mixed)
+                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step}, nullptr, s_);
+            else
+                launch_cg_a(false, dP, dT, nt3, grid, StepRef{nullptr, step}, nullptr, s_);
+        };
+        launch();
+        launch();
+        TCHK(hipEventRecord(a, s_));
+        for (int r = 0; r < reps; ++r) launch();
+        TCHK(hipEventRecord(b, s_));
+        TCHK(hipEventSynchronize(b));
+        TCHK(hipGetLastError());
+        float ms = 0.f;
+        TCHK(hipEventElapsedTime(&ms, a, b));
+        const double us = 1e3 * ms / reps;
+        if (us < best_us) {
+            best_us = us;
+            best_kz = g.kz;
+        }
+    }
+#undef TCHK
+    cleanup();
+    tuned_[key] = Tuned{best_kz, best_us};
+    *kz_out = best_kz;
+    *us_out = best_us;
+    (void)rc;
     return 0;
 }
 

@@ -133,6 +133,7 @@ struct SetupIn {
     double aux1, aux2, aux3;            // dt/dx^2, dt/dy^2, dt/dz^2
     double rho_f1;
     int ipa;                            // faces use rho_f1 (t_press == 2)
+    const uint8_t *solid;               // cells (x fastest, no halo) masked out; nullptr: none
 };
 
 // Operator storage (scaled face coefficients, x-face array (nx+1)*ny*nz,
@@ -153,9 +154,9 @@ void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, d
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
                  double *r0, double *r64, float *r32, float *c32, const Partials &P, int nb0, CgScalars *S,
                  hipStream_t s);
-// prd1 = e / sqrt(d) - S->gshift on the whole halo box.
+// prd1 = e / sqrt(d) - S->gshift on the whole halo box (0 in solid cells).
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
-                    hipStream_t s);
+                    const uint8_t *solid, hipStream_t s);
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s);
 
 // ---- batched CG (ps_cg.hip) ----

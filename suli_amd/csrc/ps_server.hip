@@ -110,6 +110,7 @@ struct suli_ps_server {
     uint64_t held = 0;
     bool stop = false;
     bool broken = false;     // the engine failed: no further solves
+    std::mutex mcap;         // device allocation and free vs graph capture
     std::string err;
     std::thread t_copy, t_dev;
 
@@ -155,9 +156,15 @@ void *suli_ps_server::alloc_block(size_t bytes, std::string &e)
         }
         held += bytes;
     }
-    for (auto &d : drop) (void)hipFree(d.second);
     void *p = nullptr;
-    const hipError_t he = hipMalloc(&p, bytes);
+    hipError_t he;
+    {
+        // a free synchronises the device, which would break a graph the
+        // device thread is capturing
+        std::lock_guard<std::mutex> cl(mcap);
+        for (auto &d : drop) (void)hipFree(d.second);
+        he = hipMalloc(&p, bytes);
+    }
     if (he != hipSuccess) {
         std::lock_guard<std::mutex> lk(m);
         held -= bytes;
@@ -387,6 +394,8 @@ suli_ps_server *suli_ps_server_create(const suli_ps_server_opts *opts)
     }
     srv->budget = srv->opts.hbm_budget ? srv->opts.hbm_budget : (uint64_t)(0.8 * (double)free_b);
     srv->eng.set_graphs(srv->opts.solve.no_graphs == 0);
+    srv->eng.set_capture_mutex(&srv->mcap);
+    srv->eng.set_autotune(srv->opts.solve.autotune != 0);
     srv->eng.set_chunk(srv->opts.solve.chunk);
     srv->t_copy = std::thread([srv] { srv->copy_loop(); });
     srv->t_dev = std::thread([srv] { srv->dev_loop(); });
@@ -455,7 +464,7 @@ int64_t suli_ps_server_submit(suli_ps_server *srv, const suli_ps_grid *g, const 
     r->dzz = dzz;
     r->prd1 = prd1;
     r->field_bytes = carve_fields(*r, nullptr);
-    r->bytes = r->field_bytes + problem_bytes(g->nx, g->ny, g->nz, srv->opts.solve.precision == SULI_PS_MIXED, 0);
+    r->bytes = r->field_bytes + problem_bytes(g->nx, g->ny, g->nz, srv->opts.solve.precision == SULI_PS_MIXED);
     if (hipEventCreateWithFlags(&r->ev_up, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&r->ev_done, hipEventDisableTiming) != hipSuccess) {
         if (r->ev_up) (void)hipEventDestroy(r->ev_up);

@@ -59,6 +59,32 @@ __device__ __forceinline__ double rho_face_z(const Geo &g, const SetupIn &in, in
     return (r[a * plane] + r[(a - 1) * plane]) * 0.5;
 }
 
+// Solid-cell mask (an extension with no reference counterpart): a face
+// with a solid cell on either side (ghost cells by the mirror/periodic rule)
+// carries no flux: its coefficient is 0, so it drops out of the diagonal as
+// well, and its velocity counts as 0 in the divergence; posdin leaves it
+// alone.  A solid cell's own row becomes the identity with zero right-hand
+// side (d = 1, b = 0, e = 0).
+__device__ __forceinline__ bool solid_cell(const Geo &g, const SetupIn &in, int i, int j, int k)
+{
+    return in.solid && in.solid[cidx(g, i, j, k)] != 0;
+}
+__device__ __forceinline__ bool closed_x(const Geo &g, const SetupIn &in, int f, int j, int k)
+{
+    return in.solid && (solid_cell(g, in, box_to_cell(f + 1, g.nx, g.px), j, k) ||
+                        solid_cell(g, in, box_to_cell(f, g.nx, g.px), j, k));
+}
+__device__ __forceinline__ bool closed_y(const Geo &g, const SetupIn &in, int i, int f, int k)
+{
+    return in.solid && (solid_cell(g, in, i, box_to_cell(f + 1, g.ny, g.py), k) ||
+                        solid_cell(g, in, i, box_to_cell(f, g.ny, g.py), k));
+}
+__device__ __forceinline__ bool closed_z(const Geo &g, const SetupIn &in, int i, int j, int f)
+{
+    return in.solid && (solid_cell(g, in, i, j, box_to_cell(f + 1, g.nz, 0)) ||
+                        solid_cell(g, in, i, j, box_to_cell(f, g.nz, 0)));
+}
+
 // s = dt/(delta^2 rho_face) (9_grad.f90:118-148; SuLi-ICdz/9_grad.f90:141)
 __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, double *sx, double *sy, double *sz)
 {
@@ -72,21 +98,23 @@ __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, doub
             const long long jk = t / (g.nx + 1);
             const int j = (int)(jk % g.ny), k = (int)(jk / g.ny);
             const double rf = in.ipa ? in.rho_f1 : rho_face_x(g, in, f, j, k);
-            sx[t] = in.aux1 / rf;
+            sx[t] = closed_x(g, in, f, j, k) ? 0.0 : in.aux1 / rf;
         } else if (t < nfx + nfy) {
             const long long u = t - nfx;
             const int i = (int)(u % g.nx);
             const long long jk = u / g.nx;
             const int f = (int)(jk % (g.ny + 1)), k = (int)(jk / (g.ny + 1));
             const double rf = in.ipa ? in.rho_f1 : rho_face_y(g, in, i, f, k);
-            sy[u] = in.aux2 / rf;
+            sy[u] = closed_y(g, in, i, f, k) ? 0.0 : in.aux2 / rf;
         } else {
             const long long u = t - nfx - nfy;
             const int i = (int)(u % g.nx);
             const long long jk = u / g.nx;
             const int j = (int)(jk % g.ny), f = (int)(jk / g.ny);
             const double rf = in.ipa ? in.rho_f1 : rho_face_z(g, in, i, j, f);
-            if (in.dz) {
+            if (closed_z(g, in, i, j, f)) {
+                sz[u] = 0.0;
+            } else if (in.dz) {
                 const double dzk = in.dz[f + 1];
                 sz[u] = in.dt / (dzk * dzk * rf);
             } else {
@@ -130,9 +158,23 @@ __global__ __launch_bounds__(kThreads) void k_diag_rhs(Geo g, SetupIn in, const 
             if (in.obsz[fz] >= 0.0) w0 = 0.0;
             if (in.obsz[fz + pl] >= 0.0) w1 = 0.0;
         }
+        if (in.solid) {
+            // closed faces carry no flux in the right-hand side either
+            if (closed_x(g, in, i, j, k)) u0 = 0.0;
+            if (closed_x(g, in, i + 1, j, k)) u1 = 0.0;
+            if (closed_y(g, in, i, j, k)) v0 = 0.0;
+            if (closed_y(g, in, i, j + 1, k)) v1 = 0.0;
+            if (closed_z(g, in, i, j, k)) w0 = 0.0;
+            if (closed_z(g, in, i, j, k + 1)) w1 = 0.0;
+        }
         const double dzk = in.dzz ? in.dzz[K] : in.dzu;
         const double q = (u0 - u1) / in.dx + (v0 - v1) / in.dy + (w0 - w1) / dzk;
-        b[c] = q / sqrt(dd);
+        if (solid_cell(g, in, i, j, k)) {
+            d[c] = 1.0;
+            b[c] = 0.0;
+        } else {
+            b[c] = q / sqrt(dd);
+        }
     }
 }
 
@@ -183,11 +225,13 @@ __global__ __launch_bounds__(kThreads) void k_scale_faces(Geo g, const double *d
 
 // Scaled unknown e = prd1 sqrt(d) at Fortran box index (I, J, K), ghosts of
 // d taken from the interior by the mirror/periodic rule (9_grad.f90:242-252).
-__device__ __forceinline__ double e_box(const Geo &g, const double *prd1, const double *d, int I, int J, int K)
+// Solid cells (and their ghost images) start at e = 0.
+__device__ __forceinline__ double e_box(const Geo &g, const SetupIn &in, const double *d, int I, int J, int K)
 {
-    const double p = prd1[I + (long long)(g.nx + 2) * (J + (long long)(g.ny + 2) * K)];
-    const double dd = d[cidx(g, box_to_cell(I, g.nx, g.px), box_to_cell(J, g.ny, g.py), box_to_cell(K, g.nz, 0))];
-    return p * sqrt(dd);
+    const int i = box_to_cell(I, g.nx, g.px), j = box_to_cell(J, g.ny, g.py), k = box_to_cell(K, g.nz, 0);
+    if (solid_cell(g, in, i, j, k)) return 0.0;
+    const double p = in.prd1[I + (long long)(g.nx + 2) * (J + (long long)(g.ny + 2) * K)];
+    return p * sqrt(d[cidx(g, i, j, k)]);
 }
 
 // Initial residual r = A e - b with the caller's prd1 ghost values
@@ -212,20 +256,20 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
         const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
         const long long fz = ((long long)k * ny + j) * nx + i;
         const long long pl = (long long)nx * ny;
-        const double ec = e_box(g, in.prd1, d, I, J, K);
+        const double ec = e_box(g, in, d, I, J, K);
         // neighbours with the caller's ghosts
-        const double eE = e_box(g, in.prd1, d, I + 1, J, K), eW = e_box(g, in.prd1, d, I - 1, J, K);
-        const double eN = e_box(g, in.prd1, d, I, J + 1, K), eS = e_box(g, in.prd1, d, I, J - 1, K);
-        const double eT = e_box(g, in.prd1, d, I, J, K + 1), eB = e_box(g, in.prd1, d, I, J, K - 1);
+        const double eE = e_box(g, in, d, I + 1, J, K), eW = e_box(g, in, d, I - 1, J, K);
+        const double eN = e_box(g, in, d, I, J + 1, K), eS = e_box(g, in, d, I, J - 1, K);
+        const double eT = e_box(g, in, d, I, J, K + 1), eB = e_box(g, in, d, I, J, K - 1);
         const double a_in = ec - op.sx[fx + 1] * eE - op.sx[fx] * eW - op.sy[fy + nx] * eN - op.sy[fy] * eS -
                             op.sz[fz + pl] * eT - op.sz[fz] * eB;
         // neighbours with mirrored/periodic ghosts
         const int iE = box_to_cell(I + 1, nx, g.px) + 1, iW = box_to_cell(I - 1, nx, g.px) + 1;
         const int jN = box_to_cell(J + 1, ny, g.py) + 1, jS = box_to_cell(J - 1, ny, g.py) + 1;
         const int kT = box_to_cell(K + 1, g.nz, 0) + 1, kB = box_to_cell(K - 1, g.nz, 0) + 1;
-        const double mE = e_box(g, in.prd1, d, iE, J, K), mW = e_box(g, in.prd1, d, iW, J, K);
-        const double mN = e_box(g, in.prd1, d, I, jN, K), mS = e_box(g, in.prd1, d, I, jS, K);
-        const double mT = e_box(g, in.prd1, d, I, J, kT), mB = e_box(g, in.prd1, d, I, J, kB);
+        const double mE = e_box(g, in, d, iE, J, K), mW = e_box(g, in, d, iW, J, K);
+        const double mN = e_box(g, in, d, I, jN, K), mS = e_box(g, in, d, I, jS, K);
+        const double mT = e_box(g, in, d, I, J, kT), mB = e_box(g, in, d, I, J, kB);
         const double a_mir = ec - op.sx[fx + 1] * mE - op.sx[fx] * mW - op.sy[fy + nx] * mN - op.sy[fy] * mS -
                              op.sz[fz + pl] * mT - op.sz[fz] * mB;
         const double bc = b[c];
@@ -233,7 +277,8 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
         b[c] = bc + (a_mir - a_in);
         e[c] = ec;
         r0[c] = r;
-        const double dc = d[c];
+        // null vector: sqrt(d) on fluid cells, 0 on solid ones
+        const double dc = solid_cell(g, in, i, j, k) ? 0.0 : d[c];
         acc_rr += r * r;
         acc_bb += bc * bc;
         acc_rn += r * sqrt(dc);
@@ -260,9 +305,9 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
 // residual is removed (gamma = r.n / n.n) and b_eff adjusted to match; this
 // changes the solution only by a constant (the pressure gauge).  The first
 // search direction p = r (9_grad.f90:372-380) is formed by the first cg_a.
-__global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const double *d, double *b, const double *r0,
-                                                           double *r64, float *r32, float *c32, Partials P, int nb0,
-                                                           CgScalars *S)
+__global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const uint8_t *solid, const double *d, double *b,
+                                                           const double *r0, double *r64, float *r32, float *c32,
+                                                           Partials P, int nb0, CgScalars *S)
 {
     const double rr = total_of(P.init, nb0);
     const double bb = total_of(P.init + nb0, nb0);
@@ -273,7 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const double *
     const long long stride = (long long)gridDim.x * kThreads;
     double acc = 0.0, acc32 = 0.0;
     for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
-        const double sq = sqrt(d[c]);
+        const double sq = (solid && solid[c]) ? 0.0 : sqrt(d[c]);
         const double r = gamma != 0.0 ? r0[c] - gamma * sq : r0[c];
         if (gamma != 0.0) b[c] = b[c] + gamma * sq;
         acc += r * r;
@@ -334,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0,
 // mirror/periodic rule in x, then y, then z order (9_grad.f90:594-659).
 // gshift (0 unless the null space is projected) restores the initial gauge.
 __global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, const double *d, double *prd1,
-                                                      const CgScalars *S)
+                                                      const CgScalars *S, const uint8_t *solid)
 {
     const double gshift = S->gshift;
     const int bx = g.nx + 2, by = g.ny + 2;
@@ -346,7 +391,8 @@ __global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, co
         const int J = (int)(jk % by), K = (int)(jk / by);
         const long long c = cidx(g, box_to_cell(I, g.nx, g.px), box_to_cell(J, g.ny, g.py), box_to_cell(K, g.nz, 0));
         const double p = e[c] / sqrt(d[c]);
-        prd1[t] = gshift != 0.0 ? p - gshift : p;
+        // solid cells (and their ghost images) are 0 exactly
+        prd1[t] = (solid && solid[c]) ? 0.0 : (gshift != 0.0 ? p - gshift : p);
     }
 }
 
@@ -372,7 +418,7 @@ __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *
             const long long ip = I + pb * J + pbb * K;
             const long long iu = I + (long long)(nx + 3) * (J + (long long)(ny + 2) * K);
             const double rf = rho_face_x(g, in, f, j, k);
-            u[iu] = u[iu] - aux1 * (in.prd1[ip] - in.prd1[ip - 1]) / rf;
+            if (!closed_x(g, in, f, j, k)) u[iu] = u[iu] - aux1 * (in.prd1[ip] - in.prd1[ip - 1]) / rf;
         } else if (t < nfx + nfy) {
             const long long s = t - nfx;
             const int i = (int)(s % nx);
@@ -382,7 +428,7 @@ __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *
             const long long ip = I + pb * J + pbb * K;
             const long long iv = I + (long long)(nx + 2) * (J + (long long)(ny + 3) * K);
             const double rf = rho_face_y(g, in, i, f, k);
-            v[iv] = v[iv] - aux2 * (in.prd1[ip] - in.prd1[ip - pb]) / rf;
+            if (!closed_y(g, in, i, f, k)) v[iv] = v[iv] - aux2 * (in.prd1[ip] - in.prd1[ip - pb]) / rf;
         } else {
             const long long s = t - nfx - nfy;
             const int i = (int)(s % nx);
@@ -393,7 +439,7 @@ __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *
             const long long iw = I + (long long)(nx + 2) * (J + (long long)(ny + 2) * K);
             const double rf = rho_face_z(g, in, i, j, f);
             const double dzk = in.dzz ? in.dzz[K] : in.dzu;
-            w[iw] = w[iw] - in.dt / dzk * (in.prd1[ip] - in.prd1[ip - pbb]) / rf;
+            if (!closed_z(g, in, i, j, f)) w[iw] = w[iw] - in.dt / dzk * (in.prd1[ip] - in.prd1[ip - pbb]) / rf;
         }
     }
 }
@@ -435,15 +481,16 @@ void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d
                  hipStream_t s)
 {
     hipLaunchKernelGGL(k_init_residual, dim3(nb0), dim3(kThreads), 0, s, g, in, op, d, b, e, r0, P);
-    hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, d, b, r0, r64, r32, c32, P, nb0, S);
+    hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, in.solid, d, b, r0, r64, r32, c32, P, nb0,
+                       S);
     hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kThreads), 0, s, P, nb0, S);
 }
 
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
-                    hipStream_t s)
+                    const uint8_t *solid, hipStream_t s)
 {
     const long long nbox = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
-    hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1, S);
+    hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1, S, solid);
 }
 
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s)

@@ -93,6 +93,7 @@ int check_problem(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, cons
 void apply_engine_opts(suli_ps_ctx *ctx)
 {
     ctx->eng.set_graphs(ctx->opts.no_graphs == 0);
+    ctx->eng.set_autotune(ctx->opts.autotune != 0);
     ctx->eng.set_chunk(ctx->opts.chunk);
 }
 
@@ -113,14 +114,14 @@ void warn_capped(const suli_ps_ctx *ctx, const suli_ps_stats &st)
 
 int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, const double *v, const double *w,
                     const double *rho, const double *ox, const double *oy, const double *oz, const double *dz,
-                    const double *dzz, double *prd1, suli_ps_stats *stats)
+                    const double *dzz, double *prd1, suli_ps_stats *stats, const uint8_t *solid = nullptr)
 {
     int rc = check_problem(ctx, gr, u, v, w, rho, ox, oy, oz, dz, dzz, prd1);
     if (rc) return rc;
     release_last(ctx);
     hipStream_t s = ctx->stream;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
-    const SetupIn in = make_setup(gr, u, v, w, rho, prd1, ox, oy, oz, dz, dzz);
+    const SetupIn in = make_setup(gr, u, v, w, rho, prd1, ox, oy, oz, dz, dzz, solid);
     int slot = -1;
     if ((rc = ctx->eng.admit(*gr, in, prd1, solve_opts(ctx->opts), 0, &slot)))
         return fail(ctx, rc, "%s", ctx->eng.error().c_str());
@@ -141,13 +142,14 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
 }
 
 int run_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, double *v, double *w, const double *rho,
-                   const double *prd1, const double *dz, const double *dzz, double *prd)
+                   const double *prd1, const double *dz, const double *dzz, double *prd,
+                   const uint8_t *solid = nullptr)
 {
     int rc = check_grid(ctx, gr);
     if (rc) return rc;
     if (!u || !v || !w || !rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
     Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
-    SetupIn in = make_setup(gr, nullptr, nullptr, nullptr, rho, prd1, nullptr, nullptr, nullptr, dz, dzz);
+    SetupIn in = make_setup(gr, nullptr, nullptr, nullptr, rho, prd1, nullptr, nullptr, nullptr, dz, dzz, solid);
     launch_posdin(g, in, u, v, w, ctx->stream);
     HIPCHK(ctx, hipGetLastError());
     if (prd && prd != prd1) {
@@ -257,6 +259,23 @@ int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, doubl
     return run_posdin_dev(ctx, g, u, v, w, rho, prd1, dz, dzz, prd);
 }
 
+int suli_ps_graddin_masked_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, const double *v,
+                               const double *w, const double *rho, const double *obs_lsx, const double *obs_lsy,
+                               const double *obs_lsz, const double *dz, const double *dzz, const uint8_t *solid,
+                               double *prd1, suli_ps_stats *stats)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_graddin_dev(ctx, g, u, v, w, rho, obs_lsx, obs_lsy, obs_lsz, dz, dzz, prd1, stats, solid);
+}
+
+int suli_ps_posdin_masked_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v, double *w,
+                              const double *rho, const double *prd1, const double *dz, const double *dzz,
+                              const uint8_t *solid, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_posdin_dev(ctx, g, u, v, w, rho, prd1, dz, dzz, prd, solid);
+}
+
 int suli_ps_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, const double *v, const double *w,
                     const double *rho, const double *obs_lsx, const double *obs_lsy, const double *obs_lsz,
                     const double *dz, const double *dzz, double *prd1, suli_ps_stats *stats)
@@ -349,7 +368,7 @@ int suli_ps_graddin_batch_dev(suli_ps_ctx *ctx, suli_ps_problem *probs, int32_t 
     for (int i = 0; i < count; ++i) {
         const suli_ps_problem &p = probs[i];
         const SetupIn in = make_setup(&p.grid, p.u, p.v, p.w, p.rho, p.prd1, p.obs_lsx, p.obs_lsy, p.obs_lsz, p.dz,
-                                      p.dzz);
+                                      p.dzz, p.solid);
         if ((rc = ctx->eng.admit(p.grid, in, p.prd1, so, i, &slots[i]))) {
             release_last(ctx);
             for (int s2 : slots)
@@ -420,6 +439,20 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
     if (!ctx || !ms || reps <= 0) return SULI_PS_EINVAL;
     int rc = ctx->eng.time_spmv(precision == SULI_PS_MIXED, reps, ms);
     if (rc) return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    return 0;
+}
+
+int suli_ps_autotune(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t *kz, double *step_us)
+{
+    if (!ctx || !kz || !step_us) return SULI_PS_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    int k = 0;
+    double us = 0.0;
+    if ((rc = ctx->eng.tune(g->nx, g->ny, g->nz, precision == SULI_PS_MIXED, &k, &us)))
+        return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    *kz = k;
+    *step_us = us;
     return 0;
 }
 

@@ -31,6 +31,7 @@ module suli_ps_iface
       integer(c_int32_t) :: timing = 0
       integer(c_int32_t) :: keep_null = 0
       integer(c_int32_t) :: no_graphs = 0
+      integer(c_int32_t) :: autotune = 0
    end type suli_ps_opts
 
    ! struct suli_ps_stats

@@ -25,7 +25,7 @@ from .solver import PRECISIONS, STOP_RULES, SuliPsError, _c_grid, _host_ptr
 class PressureServer:
     def __init__(self, precision: str = "mixed", stop_rule: str = "relative", tol: float = 1e-8,
                  max_iter: int = 10000, hbm_budget: int = 0, max_active: int = 0,
-                 graphs: bool = True):
+                 graphs: bool = True, autotune: bool = False):
         self.lib = _lib.load()
         o = _lib.ServerOpts()
         o.solve.precision = PRECISIONS[precision]
@@ -34,6 +34,7 @@ class PressureServer:
         o.solve.max_iter = max_iter
         o.solve.verbose = 0
         o.solve.no_graphs = 0 if graphs else 1
+        o.solve.autotune = 1 if autotune else 0
         o.hbm_budget = hbm_budget
         o.max_active = max_active
         self.srv = C.c_void_p(self.lib.suli_ps_server_create(C.byref(o)))

@@ -47,7 +47,7 @@ STOP_RULES = {"reference": _lib.SULI_PS_STOP_REFERENCE, "relative": _lib.SULI_PS
 class DeviceFields:
     """Device copies of the pressure-step arrays (reference layouts)."""
 
-    NAMES = ("u", "v", "w", "rho", "prd1", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz")
+    NAMES = ("u", "v", "w", "rho", "prd1", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "solid")
 
     def __init__(self, solver: "PressureSolver", grid: Grid, fields: Fields):
         self.solver = solver
@@ -55,12 +55,15 @@ class DeviceFields:
         self.ptrs = {}
         host = dict(u=fields.u, v=fields.v, w=fields.w, rho=fields.rho, prd1=fields.prd1,
                     obs_lsx=fields.obs_lsx, obs_lsy=fields.obs_lsy, obs_lsz=fields.obs_lsz,
-                    dz=grid.dzv, dzz=grid.dzzv)
+                    dz=grid.dzv, dzz=grid.dzzv, solid=getattr(fields, "solid", None))
+        self.dtypes = {}
         for name, arr in host.items():
             if arr is None:
                 self.ptrs[name] = None
                 continue
-            arr = np.asfortranarray(arr, dtype=np.float64)
+            dt = np.uint8 if name == "solid" else np.float64
+            arr = np.asfortranarray(arr, dtype=dt)
+            self.dtypes[name] = dt
             p = solver.lib.suli_ps_dev_alloc(solver.ctx, arr.nbytes)
             if not p:
                 self.free()
@@ -71,20 +74,20 @@ class DeviceFields:
                                                          C.c_void_p(arr.ctypes.data), arr.nbytes))
 
     def download(self, name: str) -> np.ndarray:
-        out = np.zeros(self.shapes[name], dtype=np.float64, order="F")
+        out = np.zeros(self.shapes[name], dtype=self.dtypes.get(name, np.float64), order="F")
         self.solver._check(self.solver.lib.suli_ps_memcpy_d2h(
             self.solver.ctx, C.c_void_p(out.ctypes.data), self.ptrs[name], out.nbytes))
         return out
 
     def upload(self, name: str, arr: np.ndarray) -> None:
-        arr = np.asfortranarray(arr, dtype=np.float64)
+        arr = np.asfortranarray(arr, dtype=self.dtypes.get(name, np.float64))
         assert arr.shape == self.shapes[name]
         self.solver._check(self.solver.lib.suli_ps_memcpy_h2d(
             self.solver.ctx, self.ptrs[name], C.c_void_p(arr.ctypes.data), arr.nbytes))
 
     def zero(self, name: str) -> None:
         """Asynchronously zero a device array (e.g. prd1 for a cold start)."""
-        nbytes = int(np.prod(self.shapes[name])) * 8
+        nbytes = int(np.prod(self.shapes[name])) * np.dtype(self.dtypes.get(name, np.float64)).itemsize
         self.solver._check(self.solver.lib.suli_ps_dev_zero(self.solver.ctx, self.ptrs[name], nbytes))
 
     def free(self) -> None:
@@ -97,16 +100,17 @@ class DeviceFields:
 class PressureSolver:
     def __init__(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
                  max_iter: int = 10000, chunk: int = 0, verbose: bool = True, timing: bool = False,
-                 keep_null: bool = False, graphs: bool = True):
+                 keep_null: bool = False, graphs: bool = True, autotune: bool = False):
         self.lib = _lib.load()
         self.opts = _lib.Opts()
-        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs)
+        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs,
+                  autotune)
         self.ctx = C.c_void_p(self.lib.suli_ps_create(C.byref(self.opts)))
         if not self.ctx:
             raise SuliPsError("suli_ps_create failed (no HIP device?)")
 
     def _set(self, precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null,
-             graphs=True):
+             graphs=True, autotune=False):
         self.opts.precision = PRECISIONS[precision]
         self.opts.stop_rule = STOP_RULES[stop_rule]
         self.opts.tol = tol
@@ -116,12 +120,15 @@ class PressureSolver:
         self.opts.timing = 1 if timing else 0
         self.opts.keep_null = 1 if keep_null else 0
         self.opts.no_graphs = 0 if graphs else 1
+        self.opts.autotune = 1 if autotune else 0
 
     def configure(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
                   max_iter: int = 10000, chunk: int = 0, verbose: bool = True,
-                  timing: bool = False, keep_null: bool = False, graphs: bool = True) -> None:
+                  timing: bool = False, keep_null: bool = False, graphs: bool = True,
+                  autotune: bool = False) -> None:
         """Solver options (include/suli_ps.h, suli_ps_opts)."""
-        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs)
+        self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs,
+                  autotune)
         self._check(self.lib.suli_ps_set_opts(self.ctx, C.byref(self.opts)))
 
     def _check(self, rc: int) -> int:
@@ -160,9 +167,14 @@ class PressureSolver:
         st = _lib.Stats()
         cg = _c_grid(grid)
         p = df.ptrs
-        self._check(self.lib.suli_ps_graddin_dev(
-            self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["obs_lsx"], p["obs_lsy"],
-            p["obs_lsz"], p["dz"], p["dzz"], p["prd1"], C.byref(st)))
+        if p.get("solid") is not None:
+            self._check(self.lib.suli_ps_graddin_masked_dev(
+                self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["obs_lsx"], p["obs_lsy"],
+                p["obs_lsz"], p["dz"], p["dzz"], p["solid"], p["prd1"], C.byref(st)))
+        else:
+            self._check(self.lib.suli_ps_graddin_dev(
+                self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["obs_lsx"], p["obs_lsy"],
+                p["obs_lsz"], p["dz"], p["dzz"], p["prd1"], C.byref(st)))
         return st
 
     def graddin_batch_dev(self, items) -> list:
@@ -172,8 +184,9 @@ class PressureSolver:
         arr = (_lib.Problem * n)()
         for i, (grid, df) in enumerate(items):
             arr[i].grid = _c_grid(grid)
-            for name in ("u", "v", "w", "rho", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "prd1"):
-                setattr(arr[i], name, df.ptrs[name])
+            for name in ("u", "v", "w", "rho", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "prd1",
+                         "solid"):
+                setattr(arr[i], name, df.ptrs.get(name))
         self._check(self.lib.suli_ps_graddin_batch_dev(self.ctx, arr, n))
         out = []
         for i in range(n):
@@ -185,14 +198,28 @@ class PressureSolver:
     def posdin_dev(self, grid: Grid, df: DeviceFields) -> None:
         cg = _c_grid(grid)
         p = df.ptrs
-        self._check(self.lib.suli_ps_posdin_dev(
-            self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["prd1"], p["dz"], p["dzz"],
-            None))
+        if p.get("solid") is not None:
+            self._check(self.lib.suli_ps_posdin_masked_dev(
+                self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["prd1"], p["dz"], p["dzz"],
+                p["solid"], None))
+        else:
+            self._check(self.lib.suli_ps_posdin_dev(
+                self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["prd1"], p["dz"], p["dzz"],
+                None))
 
     def time_spmv(self, precision: str = "fp64", reps: int = 50) -> float:
         ms = C.c_double()
         self._check(self.lib.suli_ps_time_spmv(self.ctx, PRECISIONS[precision], reps, C.byref(ms)))
         return ms.value
+
+    def autotune(self, grid: Grid, precision: str = "mixed"):
+        """(kz, us per CG step) chosen for this grid size (tuned on first use)."""
+        kz = C.c_int32()
+        us = C.c_double()
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_autotune(self.ctx, C.byref(cg), PRECISIONS[precision],
+                                              C.byref(kz), C.byref(us)))
+        return kz.value, us.value
 
     def copy_bench(self, nbytes: int, reps: int = 20) -> float:
         """Mean time (ms) of the library's plain device copy of nbytes."""

@@ -128,3 +128,24 @@ def test_graph_replay_equals_direct_launches(solver, precision, tol):
     for i in range(len(cs)):
         assert sts_g[i].iters == sts_d[i].iters
         assert np.array_equal(got_g[i], got_d[i]), f"problem {i}"
+
+
+def test_autotune_picks_a_depth_and_keeps_bit_identity(solver):
+    """The tuner returns one of its candidate depths with a positive time,
+    caches it, and tuned solves stay bit-identical alone and in a batch."""
+    cs = mixed_cases(seed=17)[:3]
+    kz1, us1 = solver.autotune(cs[1].grid, "mixed")
+    kz2, us2 = solver.autotune(cs[1].grid, "mixed")
+    assert (kz1, us1) == (kz2, us2)
+    assert 2 <= kz1 <= max(48, cs[1].grid.nz) and us1 > 0
+    solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False,
+                     autotune=True)
+    got, sts = solve_batch(solver, cs)
+    for i, c in enumerate(cs):
+        alone, st1 = solve_alone(solver, c)
+        assert sts[i].status == 0 and sts[i].iters == st1.iters
+        assert np.array_equal(got[i], alone), f"problem {i}"
+        p_or, _, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
+                                     project_null=True)
+        assert helpers.rel_diff(got[i], p_or, helpers.face_mask(got[i].shape)) <= PRESSURE_RTOL
+    solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
