@@ -40,7 +40,7 @@ using Clock = std::chrono::steady_clock;
 constexpr int kMaxResident = 256;   // engine slots
 
 struct Request {
-    enum State { QUEUED, UPLOADING, READY, RUNNING, DOWNLOADING, DONE, FAILED };
+    enum State { QUEUED, UPLOADING, READY, RUNNING, FINISHING, DOWNLOADING, DONE, FAILED };
     int64_t ticket = 0;
     State state = QUEUED;
     suli_ps_grid g{};
@@ -111,6 +111,10 @@ struct suli_ps_server {
     bool stop = false;
     bool broken = false;     // the engine failed: no further solves
     std::mutex mcap;         // device allocation and free vs graph capture
+    // events of waited-for requests, reused by later ones (destroying an
+    // event recorded on the device stream is not allowed while it captures)
+    // (kept apart: copy-stream events and device-stream events)
+    std::vector<hipEvent_t> spare_up, spare_done;
     std::string err;
     std::thread t_copy, t_dev;
 
@@ -204,15 +208,14 @@ void suli_ps_server::copy_loop()
             }
         }
         if (down) {
-            // result and final scalars, after the finish kernels.  A host
-            // wait: the device stream may be capturing a graph meanwhile,
-            // and a stream may not wait on work of a capturing stream.
+            // result and final scalars; the device thread hands a request
+            // over once its finish kernels have completed (it alone touches
+            // the device stream's events: that stream may be capturing a
+            // graph, and no other thread may wait on its events then)
             Request &r = *down;
             const size_t nbox = (size_t)(r.g.nx + 2) * (r.g.ny + 2) * (r.g.nz + 2);
             CgScalars fin{};
-            hipError_t e = hipEventSynchronize(r.ev_done);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(r.prd1, r.dprd1, nbox * sizeof(double), hipMemcpyDeviceToHost, s_copy);
+            hipError_t e = hipMemcpyAsync(r.prd1, r.dprd1, nbox * sizeof(double), hipMemcpyDeviceToHost, s_copy);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(&fin, eng.device_status(r.slot), sizeof fin, hipMemcpyDeviceToHost, s_copy);
             if (e == hipSuccess) e = hipStreamSynchronize(s_copy);
@@ -291,17 +294,39 @@ void suli_ps_server::dev_loop()
     (void)hipSetDevice(device);
     const SolveOpts so = solve_opts(opts.solve);
     std::vector<int> finished;
+    std::deque<Request *> finishing;   // finish kernels queued, not yet known complete
+    // hands requests whose finish kernels have completed to the copy thread
+    auto promote = [&](bool block) {
+        while (!finishing.empty()) {
+            Request *r = finishing.front();
+            const hipError_t q = block ? hipEventSynchronize(r->ev_done) : hipEventQuery(r->ev_done);
+            if (q == hipErrorNotReady) break;
+            finishing.pop_front();
+            std::lock_guard<std::mutex> lk(m);
+            if (q != hipSuccess) {
+                to_release.push_back(r->slot);
+                fail_resident(r, SULI_PS_EHIP, std::string("finish: ") + hipGetErrorString(q));
+                cv_done.notify_all();
+                continue;
+            }
+            r->state = Request::DOWNLOADING;
+            downloads.push_back(r);
+            cv_copy.notify_all();
+            block = false;
+        }
+    };
     for (;;) {
+        promote(false);
         std::vector<Request *> incoming;
         std::vector<int> rel;
         {
             std::unique_lock<std::mutex> lk(m);
             cv_dev.wait(lk, [&] {
-                return !ready.empty() || !to_release.empty() ||
+                return !ready.empty() || !to_release.empty() || !finishing.empty() ||
                        (!broken && (eng.active_count() > 0 || eng.pending_chunks() > 0)) ||
                        (stop && resident == 0 && queue.empty());
             });
-            if (stop && resident == 0 && queue.empty() && ready.empty() &&
+            if (stop && resident == 0 && queue.empty() && ready.empty() && finishing.empty() &&
                 (broken || (eng.active_count() == 0 && eng.pending_chunks() == 0))) {
                 for (int s : to_release) eng.release(s);
                 to_release.clear();
@@ -328,7 +353,11 @@ void suli_ps_server::dev_loop()
             r->slot = slot;
             r->state = Request::RUNNING;
         }
-        if (broken || (eng.active_count() == 0 && eng.pending_chunks() == 0)) continue;
+        if (broken || (eng.active_count() == 0 && eng.pending_chunks() == 0)) {
+            // nothing to launch: wait for the oldest finishing request
+            if (incoming.empty() && !finishing.empty()) promote(true);
+            continue;
+        }
         finished.clear();
         const int before = eng.chunks_enqueued();
         int rc = eng.pump(false, finished);
@@ -349,8 +378,8 @@ void suli_ps_server::dev_loop()
                 Request *r = kv.second;
                 if (r->state != Request::RUNNING || r->slot != s) continue;
                 r->final_slot = eng.slot(s);
-                r->state = Request::DOWNLOADING;
-                downloads.push_back(r);
+                r->state = Request::FINISHING;
+                finishing.push_back(r);
             }
         if (rc) {
             // an engine failure ends every request on the device; later
@@ -364,7 +393,6 @@ void suli_ps_server::dev_loop()
             cv_done.notify_all();
             cv_copy.notify_all();
         }
-        if (!finished.empty()) cv_copy.notify_all();
     }
 }
 
@@ -429,6 +457,8 @@ void suli_ps_server_destroy(suli_ps_server *srv)
         delete r;
     }
     for (auto &kv : srv->cache) (void)hipFree(kv.second);
+    for (hipEvent_t e : srv->spare_up) (void)hipEventDestroy(e);
+    for (hipEvent_t e : srv->spare_done) (void)hipEventDestroy(e);
     srv->eng.~Engine();
     new (&srv->eng) Engine();
     (void)hipStreamDestroy(srv->s_dev);
@@ -465,28 +495,35 @@ int64_t suli_ps_server_submit(suli_ps_server *srv, const suli_ps_grid *g, const 
     r->prd1 = prd1;
     r->field_bytes = carve_fields(*r, nullptr);
     r->bytes = r->field_bytes + problem_bytes(g->nx, g->ny, g->nz, srv->opts.solve.precision == SULI_PS_MIXED);
-    if (hipEventCreateWithFlags(&r->ev_up, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&r->ev_done, hipEventDisableTiming) != hipSuccess) {
-        if (r->ev_up) (void)hipEventDestroy(r->ev_up);
-        if (r->ev_done) (void)hipEventDestroy(r->ev_done);
+    std::lock_guard<std::mutex> lk(srv->m);
+    for (auto pr : {std::make_pair(&r->ev_up, &srv->spare_up), std::make_pair(&r->ev_done, &srv->spare_done)}) {
+        if (!pr.second->empty()) {
+            *pr.first = pr.second->back();
+            pr.second->pop_back();
+        } else if (hipEventCreateWithFlags(pr.first, hipEventDisableTiming) != hipSuccess) {
+            *pr.first = nullptr;
+        }
+    }
+    if (!r->ev_up || !r->ev_done) {
+        if (r->ev_up) srv->spare_up.push_back(r->ev_up);
+        if (r->ev_done) srv->spare_done.push_back(r->ev_done);
         delete r;
         return SULI_PS_EHIP;
     }
-    std::lock_guard<std::mutex> lk(srv->m);
     if (r->bytes > srv->budget) {
         char buf[160];
         std::snprintf(buf, sizeof buf, "request needs %zu bytes of device memory, the budget is %llu", r->bytes,
                       (unsigned long long)srv->budget);
         srv->err = buf;
-        (void)hipEventDestroy(r->ev_up);
-        (void)hipEventDestroy(r->ev_done);
+        srv->spare_up.push_back(r->ev_up);
+        srv->spare_done.push_back(r->ev_done);
         delete r;
         return SULI_PS_ENOMEM;
     }
     if (srv->stop) {
         srv->err = "server is stopping";
-        (void)hipEventDestroy(r->ev_up);
-        (void)hipEventDestroy(r->ev_done);
+        srv->spare_up.push_back(r->ev_up);
+        srv->spare_done.push_back(r->ev_done);
         delete r;
         return SULI_PS_EINVAL;
     }
@@ -526,9 +563,9 @@ int suli_ps_server_wait(suli_ps_server *srv, int64_t ticket, double timeout_s, s
         srv->err = r->err;
     }
     srv->reqs.erase(it);
+    srv->spare_up.push_back(r->ev_up);
+    srv->spare_done.push_back(r->ev_done);
     lk.unlock();
-    (void)hipEventDestroy(r->ev_up);
-    (void)hipEventDestroy(r->ev_done);
     delete r;
     return rc;
 }
