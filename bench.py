@@ -47,10 +47,11 @@ def percentile(xs, q):
 
 
 def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
-    """Timed solves (kernels replayed from captured HIP graphs), then two
-    solves launched directly with HIP events around the fused CG kernel."""
+    """Timed solves (kernels replayed from captured HIP graphs, tile shape
+    from the autotuner, which runs during the first warm-up solve), then two
+    solves launched directly with HIP events around the CG step."""
     solver.configure(precision=precision, stop_rule="relative", tol=tol, verbose=False,
-                     graphs=graphs)
+                     graphs=graphs, autotune=True)
     for _ in range(warmup):
         df.zero("prd1")
         solver.graddin_dev(grid, df)
@@ -70,7 +71,7 @@ def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
     solver.synchronize()
     total = time.perf_counter() - t0
     solver.configure(precision=precision, stop_rule="relative", tol=tol, verbose=False,
-                     timing=True)
+                     timing=True, autotune=True)
     spmv = []
     for _ in range(2):
         df.zero("prd1")
@@ -79,11 +80,135 @@ def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
             spmv.append(st.spmv_ms)
     # the plain SpMV q = A p of this solve's operator (k_spmv), 50 launches
     spmv_only = solver.time_spmv(precision, reps=50)
+    kz, vx, _ = solver.autotune(grid, precision)
     return dict(solves_per_s=steps / total, ms_per_step=total * 1e3 / steps,
                 p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
                 iters=int(np.median(iters)), refreshes=int(np.median(refreshes)),
-                rel_true_max=float(max(rel)), graphs=graphs,
+                rel_true_max=float(max(rel)), graphs=graphs, tiles={"kz": kz, "vx": vx},
                 spmv_ms=float(np.mean(spmv)) if spmv else None, spmv_only_ms=spmv_only)
+
+
+def rel_diff(a, b):
+    """max |a - b| / max |b| over the interior and face ghosts (edge and
+    corner ghosts of the halo box excluded)."""
+    m = np.ones(a.shape, dtype=bool)
+    for ax in range(3):
+        for other in range(3):
+            if other == ax:
+                continue
+            for e in (0, -1):
+                for e2 in (0, -1):
+                    idx = [slice(None)] * 3
+                    idx[ax] = e
+                    idx[other] = e2
+                    m[tuple(idx)] = False
+    return float(np.abs(a - b)[m].max() / np.abs(b[m]).max())
+
+
+def bench_stretched_cylinder(steps):
+    """384x192x128 stretched grid with the masked cylinder (not a reference
+    mode): solves/s of both solvers, and the mixed result against the FP64
+    one (the FP64 solver is pinned to the CPU oracle at small sizes)."""
+    from suli_amd import cases
+    from suli_amd.solver import PressureSolver
+    c = cases.config3(seed=1)
+    g = c.grid
+    solver = PressureSolver(verbose=False)
+    df = solver.upload(g, c.fields)
+    out, prd = {}, {}
+    for precision in ("mixed", "fp64"):
+        solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
+                         autotune=True)
+        df.zero("prd1")
+        solver.graddin_dev(g, df)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            df.zero("prd1")
+            st = solver.graddin_dev(g, df)
+        solver.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        if st.status != 0:
+            raise RuntimeError(f"cylinder {precision} did not converge: {st.as_dict()}")
+        prd[precision] = df.download("prd1")
+        kz, vx, _ = solver.autotune(g, precision)
+        out[precision] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iters": st.iters,
+                          "rel_true": st.rel_true, "tiles": {"kz": kz, "vx": vx}}
+    df.free()
+    solver.close()
+    out["mixed_vs_fp64_rel_pressure_diff"] = rel_diff(prd["mixed"], prd["fp64"])
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["workload"] = (f"stretched dz (tanh, ratio 1.05, finest at 0.6H) {g.nx}x{g.ny}x{g.nz}, masked "
+                       "full-height cylinder r = 0.1 Ly, rho 1000/1.2, 1e-8 relative residual")
+    return out
+
+
+def bench_batch(seed=1, count=64):
+    """64 uneven problems (sizes log-uniform in [32, 128] per axis) solved
+    in one batch, against solving them one after another."""
+    from suli_amd import cases
+    from suli_amd.solver import PressureSolver
+    cs = cases.batch_cases(seed=seed, count=count)
+    cells = sum(c.grid.ncell for c in cs)
+    out = {"problems": count, "cells": cells,
+           "workload": f"{count} two-phase problems, nx,ny,nz log-uniform in [32,128] (x8), "
+                       "flat surface at U(0.3,0.7)H, density ratio in {1,10,1000}, 1e-8 relative"}
+    solver = PressureSolver(verbose=False)
+    dfs = [solver.upload(c.grid, c.fields) for c in cs]
+    items = [(c.grid, df) for c, df in zip(cs, dfs)]
+    for precision in ("mixed", "fp64"):
+        solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
+                         autotune=True)
+        for df in dfs:
+            df.zero("prd1")
+        solver.graddin_batch_dev(items)      # warm-up: tunes each size
+        for df in dfs:
+            df.zero("prd1")
+        solver.synchronize()
+        t0 = time.perf_counter()
+        sts = solver.graddin_batch_dev(items)
+        t_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for (g, df) in items:
+            df.zero("prd1")
+            solver.graddin_dev(g, df)
+        t_seq = time.perf_counter() - t0
+        out[precision] = {"batch_s": t_batch, "solves_per_s": count / t_batch,
+                          "mcells_per_s": cells / t_batch / 1e6,
+                          "one_by_one_solves_per_s": count / t_seq,
+                          "batching_speedup": t_seq / t_batch,
+                          "max_iters": max(s.iters for s in sts),
+                          "not_converged": sum(1 for s in sts if s.status != 0)}
+        print(f"# batch {precision} {out[precision]}", file=sys.stderr, flush=True)
+    for df in dfs:
+        df.free()
+    solver.close()
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    return out
+
+
+def bench_serving(seed=1, count=48, rates=(60.0, 240.0)):
+    """Requests of uneven sizes arriving as a Poisson process, served with
+    continuous batching; latency from arrival to the result on the host."""
+    from suli_amd import cases
+    from suli_amd.server import PressureServer, poisson_stream
+    base = cases.serving_cases(seed=seed, count=count)
+    out = {"requests": count, "rates_per_s": list(rates),
+           "workload": f"{count} requests, nx log-uniform [32,256], ny,nz [32,128] (x8), "
+                       "two-phase tank fields, Poisson arrivals, 1e-8 relative"}
+    for precision in ("mixed", "fp64"):
+        out[precision] = []
+        for rate in rates:
+            cs = [cases.Case(c.name, c.grid, c.fields.copy(), c.seed) for c in base]
+            with PressureServer(precision=precision, stop_rule="relative", tol=1e-8) as srv:
+                rep = poisson_stream(srv, cs, rate=rate, seed=seed)
+            rep.pop("stats")
+            info = rep.pop("info")
+            rep["peak_resident_mb"] = info["peak_resident_bytes"] / 1e6
+            rep["chunks"] = info["chunks"]
+            out[precision].append(rep)
+            print(f"# serving {precision} {rep}", file=sys.stderr, flush=True)
+    return out
 
 
 def load_traffic(name):
@@ -139,6 +264,8 @@ def main():
                     help="solver(s) to time (profiling runs)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel that calibrates bandwidth and counters")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the stretched-cylinder, batch and serving workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
@@ -192,6 +319,17 @@ def main():
                               "achieved": (bytes64 / (fp64["spmv_ms"] * 1e-3) / 1e9)
                               if fp64["spmv_ms"] else None}}
 
+    extra = None
+    if not args.no_extra:
+        extra = {}
+        print("# stretched cylinder", file=sys.stderr, flush=True)
+        extra["stretched_cylinder"] = bench_stretched_cylinder(2)
+        print(f"# {extra['stretched_cylinder']}", file=sys.stderr, flush=True)
+        print("# batch", file=sys.stderr, flush=True)
+        extra["batch_uneven"] = bench_batch()
+        print("# serving", file=sys.stderr, flush=True)
+        extra["serving"] = bench_serving()
+
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
@@ -219,6 +357,7 @@ def main():
         "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
         "roofline": roofline,
         "copy_kernel": copy,
+        "workloads": extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

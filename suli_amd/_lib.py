@@ -29,7 +29,7 @@ EXPORTS = (
     "suli_ps_graddin_batch_dev", "suli_ps_server_create", "suli_ps_server_destroy",
     "suli_ps_server_last_error", "suli_ps_server_submit", "suli_ps_server_wait",
     "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
-    "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev",
+    "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
 )
 
 
@@ -123,7 +123,8 @@ def load(path: str = LIB_PATH):
     L.suli_ps_graddin_batch_dev.argtypes = [vp, C.POINTER(Problem), C.c_int32]
     L.suli_ps_copy_bench.argtypes = [vp, C.c_size_t, C.c_int32, C.POINTER(C.c_double)]
     L.suli_ps_autotune.argtypes = [vp, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
-                                   C.POINTER(C.c_double)]
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.suli_ps_set_tiles.argtypes = [vp, C.POINTER(Grid), C.c_int32, C.c_int32, C.c_int32]
     L.suli_ps_server_create.restype = vp
     L.suli_ps_server_create.argtypes = [C.POINTER(ServerOpts)]
     L.suli_ps_server_destroy.argtypes = [vp]

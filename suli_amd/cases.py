@@ -308,3 +308,35 @@ def log_uniform_sizes(count: int, seed: int, lo: int = 32, hi: int = 128, mult: 
             dims.append(int(max(lo, min(hi, mult * round(n / mult)))))
         out.append(tuple(dims))
     return out
+
+
+def batch_cases(seed: int = 1, count: int = 64, dx: float = 5e-3):
+    """Uneven batch: nx, ny, nz each log-uniform in [32, 128] rounded to
+    multiples of 8; a flat free surface at a height drawn from U(0.3, 0.7) H;
+    density ratio drawn from {1, 10, 1000} (water 1000); U(-0.05, 0.05)
+    velocities; walls.  Seeds seed*1000 + i."""
+    rng = np.random.default_rng(seed)
+    sizes = log_uniform_sizes(count, seed, 32, 128, 8)
+    out = []
+    for i, (nx, ny, nz) in enumerate(sizes):
+        ratio = float(rng.choice([1.0, 10.0, 1000.0]))
+        h = float(rng.uniform(0.3, 0.7))
+        c = random_case(nx, ny, nz, seed=seed * 1000 + i, dx=dx, ratio=ratio, surface=h)
+        out.append(c)
+    return out
+
+
+def serving_cases(seed: int = 1, count: int = 32, dx: float = 5e-3):
+    """Request stream: nx log-uniform in [32, 256], ny and nz in [32, 128]
+    (multiples of 8), two-phase sinusoidal-surface tank fields as the
+    256x128x96 workload, seeded per request (seed*1000 + i)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        dims = []
+        for lo, hi in ((32, 256), (32, 128), (32, 128)):
+            n = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+            dims.append(int(max(lo, min(hi, 8 * round(n / 8)))))
+        nx, ny, nz = dims
+        out.append(config2(seed=seed * 1000 + i, nx=nx, ny=ny, nz=nz))
+    return out

@@ -38,7 +38,12 @@ struct alignas(16) Vec {
     T v[W];
 };
 
-__device__ __forceinline__ int step_of(StepRef s) { return s.base ? *s.base + s.off : s.off; }
+__device__ __forceinline__ int step_of(StepRef s)
+{
+    const int step = s.base ? *s.base + s.off : s.off;
+    if (s.next && blockIdx.x == 0 && threadIdx.x == 0) *s.next = *s.base + kRefreshGrain;
+    return step;
+}
 
 // Contiguous run [begin, end) of the launch table for this workgroup.
 struct Run {
@@ -55,7 +60,8 @@ __device__ __forceinline__ Run my_run(int ntab)
     return r;
 }
 
-// 3-D tile `tile` of a problem: 64 x 4 columns, planes [k0, k1).
+// 3-D tile `tile` of a problem: (64 vx) x 4 columns, planes [k0, k1); a
+// lane owns x cells i .. i + vx - 1.
 struct Tile {
     int i, j, k0, k1;
 };
@@ -67,7 +73,7 @@ __device__ __forceinline__ Tile tile3(const Geo &g, int tile)
     const int ty = rest % g.tiles_y;
     const int tz = rest / g.tiles_y;
     Tile t;
-    t.i = tx * kTileX + (threadIdx.x & (kTileX - 1));
+    t.i = (tx * kTileX + (threadIdx.x & (kTileX - 1))) * g.vx;
     t.j = ty * kTileY + (threadIdx.x / kTileX);
     t.k0 = tz * g.kz;
     t.k1 = min(t.k0 + g.kz, g.nz);
@@ -286,6 +292,98 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
     }
 }
 
+// Two x cells per lane (FP32, even nx): the lane's cells i, i+1 are one
+// 8-byte vector in every cell-layout array (all 256-byte aligned, i even).
+__device__ __forceinline__ float2 ld2(const float *p, long long x) { return *reinterpret_cast<const float2 *>(p + x); }
+__device__ __forceinline__ void st2(float *p, long long x, float2 v) { *reinterpret_cast<float2 *>(p + x) = v; }
+
+// cg_m_tile with two cells per lane: the same arithmetic per cell (and the
+// same term order), with the lane's x neighbours shared in registers and
+// 8-byte loads and stores.
+template <bool FIRST, bool UPD>
+__device__ __forceinline__ void cg_m_tile2(const Geo &g, const Tile &t, float *__restrict__ c32,
+                                           const float *__restrict__ r_old, const float *__restrict__ q_old,
+                                           const float *__restrict__ p_old, float *__restrict__ r_new,
+                                           float *__restrict__ p_new, float *__restrict__ q_new,
+                                           const float *__restrict__ sx, const float *__restrict__ sy,
+                                           const float *__restrict__ sz, float a, float b, double acc[4])
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long plane = (long long)nx * ny;
+    if (t.i >= nx || t.j >= ny) return;
+    long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
+    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + 1, nx, g.px);
+    const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
+    const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
+    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
+    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+    auto rn1 = [&](long long x) -> float { return UPD ? r_old[x] - a * q_old[x] : r_old[x]; };
+    auto pn1 = [&](long long x) -> float { return FIRST ? rn1(x) : rn1(x) + b * p_old[x]; };
+    auto rn2 = [&](long long x) -> float2 {
+        float2 r = ld2(r_old, x);
+        if (UPD) {
+            const float2 q = ld2(q_old, x);
+            r.x = r.x - a * q.x;
+            r.y = r.y - a * q.y;
+        }
+        return r;
+    };
+    auto pn2 = [&](long long x) -> float2 {
+        float2 r = rn2(x);
+        if (!FIRST) {
+            const float2 po = ld2(p_old, x);
+            r.x = r.x + b * po.x;
+            r.y = r.y + b * po.y;
+        }
+        return r;
+    };
+    float2 poc = FIRST ? make_float2(0.0f, 0.0f) : ld2(p_old, c);
+    float2 rc = rn2(c);
+    float2 pc = FIRST ? rc : make_float2(rc.x + b * poc.x, rc.y + b * poc.y);
+    float2 pb = t.k0 > 0 ? pn2(c - plane) : pc;
+    float2 szb = ld2(sz, c);
+    for (int k = t.k0; k < t.k1; ++k) {
+        const long long r0 = c - t.i;
+        const long long up = (k < nz - 1) ? plane : 0;
+        const float2 pot = FIRST ? make_float2(0.0f, 0.0f) : ld2(p_old, c + up);
+        const float2 rt = rn2(c + up);
+        const float2 pt = FIRST ? rt : make_float2(rt.x + b * pot.x, rt.y + b * pot.y);
+        const float2 szt = ld2(sz, c + plane);
+        const float pw = pn1(r0 + iw), pe = pn1(r0 + ie);
+        const float2 ps = pn2(c + dS), pnn = pn2(c + dN);
+        const float sx0 = sx[fx], sx1 = sx[fx + 1], sx2 = sx[fx + 2];
+        const float2 sys = ld2(sy, fy), syn = ld2(sy, fy + nx);
+        float2 y;
+        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * pt.x - szb.x * pb.x;
+        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * pt.y - szb.y * pb.y;
+        st2(q_new, c, y);
+        st2(p_new, c, pc);
+        st2(r_new, c, rc);
+        if (UPD) {
+            float2 cv = ld2(c32, c);
+            cv.x = cv.x - a * poc.x;
+            cv.y = cv.y - a * poc.y;
+            st2(c32, c, cv);
+        }
+        acc[0] += (double)pc.x * (double)y.x;
+        acc[0] += (double)pc.y * (double)y.y;
+        acc[1] += (double)y.x * (double)rc.x;
+        acc[1] += (double)y.y * (double)rc.y;
+        acc[2] += (double)y.x * (double)y.x;
+        acc[2] += (double)y.y * (double)y.y;
+        acc[3] += (double)rc.x * (double)rc.x;
+        acc[3] += (double)rc.y * (double)rc.y;
+        pb = pc;
+        pc = pt;
+        rc = rt;
+        poc = pot;
+        szb = szt;
+        c += plane;
+        fx += (long long)ny * (nx + 1);
+        fy += (long long)(ny + 1) * nx;
+    }
+}
+
 __device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return parity ? pd.r32b : pd.r32; }
 __device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
 __device__ __forceinline__ float *pbuf(const ProbDev &pd, int parity) { return parity ? pd.p32_1 : pd.p32_0; }
@@ -343,17 +441,29 @@ __global__ __launch_bounds__(kThreads) void k_cg_m(const ProbDev *__restrict__ P
         if (cap) {
             // finish iteration n-1 only (the e update; the replacement that
             // follows recomputes the residual)
-            if (upd && tl.i < pd.g.nx && tl.j < pd.g.ny) {
+            for (int sub = 0; upd && sub < pd.g.vx; ++sub) {
+                const int ci = tl.i + sub;
+                if (ci >= pd.g.nx || tl.j >= pd.g.ny) break;
                 const float *p_old = pbuf(pd, o);
                 const long long plane = (long long)pd.g.nx * pd.g.ny;
-                long long c = (long long)tl.k0 * plane + (long long)tl.j * pd.g.nx + tl.i;
+                long long c = (long long)tl.k0 * plane + (long long)tl.j * pd.g.nx + ci;
                 for (int k = tl.k0; k < tl.k1; ++k, c += plane) pd.c32[c] = pd.c32[c] - a * p_old[c];
             }
             continue;
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const float b = (float)beta;
-        if (first)
+        if (pd.g.vx == 2) {
+            if (first)
+                cg_m_tile2<true, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w),
+                                        pbuf(pd, w), qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+            else if (upd)
+                cg_m_tile2<false, true>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w),
+                                        pbuf(pd, w), qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+            else
+                cg_m_tile2<false, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w),
+                                         pbuf(pd, w), qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+        } else if (first)
             cg_m_tile<true, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
                                    qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
         else if (upd)
@@ -519,16 +629,18 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
         // the replaced residual goes where the next cg_m reads r_{n}
         float *__restrict__ r32 = WRITE32 ? rbuf(pd, (step - pd.n0 + 1) & 1) : nullptr;
         double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
-        if (t.i < nx && t.j < ny) {
+        for (int sub = 0; sub < g.vx; ++sub) {
+            const int ci = t.i + sub;
+            if (ci >= nx || t.j >= ny) break;
             const long long plane = (long long)nx * ny;
-            const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
+            const int iw = nb_lo(ci, nx, g.px), ie = nb_hi(ci, nx, g.px);
             const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
             const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
-            long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-            long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
-            long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+            long long c = (long long)t.k0 * plane + (long long)t.j * nx + ci;
+            long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + ci;
+            long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + ci;
             for (int k = t.k0; k < t.k1; ++k) {
-                const long long r0 = c - t.i;
+                const long long r0 = c - ci;
                 const long long up = (k < nz - 1) ? plane : 0, dn = (k > 0) ? plane : 0;
                 const double ec = e[c];
                 const double r = ec - sx[fx + 1] * e[r0 + ie] - sx[fx] * e[r0 + iw] - sy[fy + nx] * e[c + dN] -
@@ -648,14 +760,52 @@ __global__ __launch_bounds__(kThreads) void k_copy(const float4 *__restrict__ x,
         y[v] = x[v];
 }
 
-__global__ void k_advance(int *base, int by)
-{
-    if (threadIdx.x == 0) *base += by;
-}
-
 __global__ void k_set_step(int *base, int value)
 {
     if (threadIdx.x == 0) *base = value;
+}
+
+// Plain FP32 SpMV of one tile with two cells per lane (timing only).
+__device__ __forceinline__ double spmv_tile2(const Geo &g, const Tile &t, const float *__restrict__ p,
+                                             float *__restrict__ q, const float *__restrict__ sx,
+                                             const float *__restrict__ sy, const float *__restrict__ sz)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    double acc = 0.0;
+    if (t.i >= nx || t.j >= ny) return acc;
+    const long long plane = (long long)nx * ny;
+    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + 1, nx, g.px);
+    const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
+    const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
+    long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
+    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
+    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+    float2 pc = ld2(p, c);
+    float2 pb = t.k0 > 0 ? ld2(p, c - plane) : pc;
+    float2 szb = ld2(sz, c);
+    for (int k = t.k0; k < t.k1; ++k) {
+        const long long r0 = c - t.i;
+        const long long up = (k < nz - 1) ? plane : 0;
+        const float2 pt = ld2(p, c + up);
+        const float2 szt = ld2(sz, c + plane);
+        const float pw = p[r0 + iw], pe = p[r0 + ie];
+        const float2 ps = ld2(p, c + dS), pnn = ld2(p, c + dN);
+        const float sx0 = sx[fx], sx1 = sx[fx + 1], sx2 = sx[fx + 2];
+        const float2 sys = ld2(sy, fy), syn = ld2(sy, fy + nx);
+        float2 y;
+        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * pt.x - szb.x * pb.x;
+        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * pt.y - szb.y * pb.y;
+        st2(q, c, y);
+        acc += (double)pc.x * (double)y.x;
+        acc += (double)pc.y * (double)y.y;
+        pb = pc;
+        pc = pt;
+        szb = szt;
+        c += plane;
+        fx += (long long)ny * (nx + 1);
+        fy += (long long)(ny + 1) * nx;
+    }
+    return acc;
 }
 
 // Plain SpMV q = A p of the FP64 or FP32 operator (timing only).
@@ -674,7 +824,11 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
         const Tile t = tile3(g, tr.tile);
         const int nx = g.nx, ny = g.ny, nz = g.nz;
         double acc = 0.0;
-        if (t.i < nx && t.j < ny) {
+        if (sizeof(T) == 4 && g.vx == 2) {
+            acc = spmv_tile2(g, t, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(q),
+                             reinterpret_cast<const float *>(V.sx), reinterpret_cast<const float *>(V.sy),
+                             reinterpret_cast<const float *>(V.sz));
+        } else if (t.i < nx && t.j < ny) {
             const long long plane = (long long)nx * ny;
             const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
             const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
@@ -709,7 +863,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
 
 }  // namespace
 
-Geo make_geo(int nx, int ny, int nz, int px, int py, int kz)
+Geo make_geo(int nx, int ny, int nz, int px, int py, int kz, int vx)
 {
     Geo g{};
     g.nx = nx;
@@ -718,7 +872,8 @@ Geo make_geo(int nx, int ny, int nz, int px, int py, int kz)
     g.px = px;
     g.py = py;
     g.n = (long long)nx * ny * nz;
-    g.tiles_x = (nx + kTileX - 1) / kTileX;
+    g.vx = (vx == 2 && nx % 2 == 0) ? 2 : 1;
+    g.tiles_x = (nx + kTileX * g.vx - 1) / (kTileX * g.vx);
     g.tiles_y = (ny + kTileY - 1) / kTileY;
     if (kz <= 0) {
         // about kSpmvBlocks tiles (4 per CU) for a large grid, at least 4
@@ -776,7 +931,7 @@ void launch_finish(const ProbDev *P, const TileRef *t3, int nt3, const int *prob
                    hipStream_t s)
 {
     hipLaunchKernelGGL((k_true_residual<false, true>), dim3(std::min(grid, nt3)), dim3(kThreads), 0, s, P, t3, nt3,
-                       StepRef{nullptr, 0});
+                       StepRef{nullptr, 0, nullptr});
     hipLaunchKernelGGL(k_finish, dim3(nprobs), dim3(kThreads), 0, s, P, probs);
 }
 
@@ -784,11 +939,6 @@ void launch_copy(const void *src, void *dst, long long n16, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(kThreads), 0, s, static_cast<const float4 *>(src),
                        static_cast<float4 *>(dst), n16);
-}
-
-void launch_advance(int *base, int by, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, base, by);
 }
 
 void launch_set_step(int *base, int value, hipStream_t s)

@@ -37,6 +37,7 @@ struct SolveOpts {
     int max_iter = 10000;
     bool keep_null = false;
     int kz = 0;          // planes per 3-D tile (0: tuned or default for the size)
+    int vx = 0;          // x cells per lane, FP32 only (0: tuned or 1)
 };
 
 // One problem resident on the device.
@@ -116,7 +117,9 @@ public:
     // scratch problem and keeps the fastest for every later problem of that
     // size (results then depend on the size only, as without tuning).
     void set_autotune(bool on) { autotune_ = on; }
-    int tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out);
+    int tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, double *us_out);
+    // Fixes the tile shape of a size (as if tuned).
+    void set_tiles(int nx, int ny, int nz, bool mixed, int kz, int vx);
     // Fixed chunk length (steps between host polls; rounded up to the
     // refresh grain), or automatic when <= 0.
     void set_chunk(int steps) { chunk_ = steps; }
@@ -170,19 +173,19 @@ private:
     int off_t3_64 = 0, off_t1_64 = 0, off_t3_32 = 0, off_t1_32 = 0;
     // graph replay: step counter on the device, two captured groups of
     // kRefreshGrain steps (without / with the residual replacement)
-    int *d_step_ = nullptr;
-    int dstep_value_ = 0;              // value of *d_step_ after the queued work
+    int *d_step_ = nullptr;            // two counters, alternating by group parity
+    bool dstep_valid_ = true;          // the counter of the next group holds step_
     bool graphs_ = true;
     bool autotune_ = false;
     struct Tuned {
-        int kz;
+        int kz, vx;
         double us;
     };
     std::map<std::tuple<int, int, int, bool>, Tuned> tuned_;
     int chunk_ = 0;
     std::mutex *capture_mutex_ = nullptr;
     bool graph_valid_ = false;
-    hipGraphExec_t gx_[2] = {nullptr, nullptr};
+    hipGraphExec_t gx_[4] = {nullptr, nullptr, nullptr, nullptr};   // [parity][replacement]
     int captures_ = 0;
     // replaced graphs, destroyed once the stream has passed this event
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired_;

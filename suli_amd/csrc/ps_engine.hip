@@ -172,9 +172,9 @@ int Engine::init(hipStream_t stream, int max_slots)
     ev_spmv_.resize(2 * kMaxTimed);
     for (auto &e : ev_spmv_) ECHK(hipEventCreate(&e));
     ECHK(hipMalloc(&d_ran_, sizeof(int) * 2 * kMaxTimed));
-    ECHK(hipMalloc(&d_step_, sizeof(int)));
-    ECHK(hipMemset(d_step_, 0, sizeof(int)));
-    dstep_value_ = 0;
+    ECHK(hipMalloc(&d_step_, 2 * sizeof(int)));
+    ECHK(hipMemset(d_step_, 0, 2 * sizeof(int)));
+    dstep_valid_ = true;
     return 0;
 }
 
@@ -199,14 +199,19 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     Slot &sl = slots_[si];
     sl = Slot{};
     int kz = o.kz > 0 ? std::max(o.kz, kMinKz) : 0;
-    if (kz == 0 && autotune_) {
+    int vx = o.vx > 0 ? o.vx : 0;
+    if ((kz == 0 || vx == 0) && autotune_) {
+        int kz_t = 0, vx_t = 1;
         double us = 0.0;
-        int rc_t = tune(gr.nx, gr.ny, gr.nz, o.mixed, &kz, &us);
+        int rc_t = tune(gr.nx, gr.ny, gr.nz, o.mixed, &kz_t, &vx_t, &us);
         if (rc_t) return rc_t;
+        if (kz == 0) kz = kz_t;
+        if (vx == 0) vx = vx_t;
     }
-    sl.g = make_geo(gr.nx, gr.ny, gr.nz, gr.ccx0 == 0, gr.ccy0 == 0, kz);
+    sl.g = make_geo(gr.nx, gr.ny, gr.nz, gr.ccx0 == 0, gr.ccy0 == 0, kz, o.mixed ? vx : 1);
     sl.o = o;
     sl.o.kz = sl.g.kz;
+    sl.o.vx = sl.g.vx;
     const int nt3 = spmv_tiles(sl.g), nt1 = vec_tiles(sl.g.n);
     sl.bytes = problem_bytes(gr.nx, gr.ny, gr.nz, o.mixed);
     sl.own_block = block == nullptr;
@@ -400,7 +405,7 @@ int Engine::enqueue_steps_direct(int nsteps, bool timing)
     const int nmix = (int)tab_.mixed_probs.size();
     for (int it = 0; it < nsteps; ++it) {
         ++step_;
-        const StepRef sr{nullptr, step_};
+        const StepRef sr{nullptr, step_, nullptr};
         // timing windows of kTimeSteps consecutive steps (events only at
         // their ends, so the per-step figure is the launches' own time)
         const int ph = step_ % kTimeStride;
@@ -419,6 +424,7 @@ int Engine::enqueue_steps_direct(int nsteps, bool timing)
             launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,
                            kMaxGrid, sr, s_);
     }
+    dstep_valid_ = false;   // the device counters no longer follow step_
     ECHK(hipGetLastError());
     return 0;
 }
@@ -460,8 +466,9 @@ void Engine::reap_graphs(bool wait)
 }
 
 // Captures one group of kRefreshGrain steps of the current tables as a
-// graph (and a second one that ends with the residual replacement).  The
-// kernels read the step from d_step_; the group's last node advances it.
+// graph, with and without the residual replacement at its end, for each
+// of the two device step counters: a group of parity p reads counter p and
+// its first launch stores the next group's step in counter 1 - p.
 int Engine::capture_graphs()
 {
     std::unique_lock<std::mutex> lk;
@@ -471,19 +478,25 @@ int Engine::capture_graphs()
     const int n3_64 = (int)tab_.t3_64.size(), n1_64 = (int)tab_.t1_64.size();
     const int n3_32 = (int)tab_.t3_32.size(), n1_32 = (int)tab_.t1_32.size();
     const int nmix = (int)tab_.mixed_probs.size();
-    for (int v = 0; v < 2; ++v) {
-        if (v == 1 && !nmix) break;
+    for (int gi = 0; gi < 4; ++gi) {
+        const int par = gi >> 1, v = gi & 1;
+        if (v == 1 && !nmix) continue;
+        const int *base = d_step_ + par;
+        int *next = d_step_ + (1 - par);
         ECHK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         for (int off = 1; off <= kRefreshGrain; ++off) {
-            const StepRef sr{d_step_, off};
+            // the group's first launch (cg_a, else cg_m) carries the counter update
+            const StepRef sr{base, off, off == 1 ? next : nullptr};
+            const StepRef sr_m{base, off, (off == 1 && !n3_64) ? next : nullptr};
             if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, nullptr, s_);
-            if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, nullptr, s_);
-            if (n1_64) launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid), sr, s_);
+            if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr_m, nullptr, s_);
+            if (n1_64)
+                launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid),
+                            StepRef{base, off, nullptr}, s_);
         }
         if (v == 1)
             launch_refresh(d_probs_, d_tab_ + off_t1_32, n1_32, d_tab_ + off_t3_32, n3_32, d_probs_list_, nmix,
-                           kMaxGrid, StepRef{d_step_, kRefreshGrain}, s_);
-        launch_advance(d_step_, kRefreshGrain, s_);
+                           kMaxGrid, StepRef{base, kRefreshGrain, nullptr}, s_);
         hipGraph_t g = nullptr;
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(s_, &g);
@@ -492,7 +505,7 @@ int Engine::capture_graphs()
             return fail(SULI_PS_EHIP, std::string("graph capture failed: ") +
                                           hipGetErrorString(le != hipSuccess ? le : ce));
         }
-        const hipError_t ie = hipGraphInstantiate(&gx_[v], g, nullptr, nullptr, 0);
+        const hipError_t ie = hipGraphInstantiate(&gx_[gi], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ie != hipSuccess) return fail(SULI_PS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     }
@@ -506,11 +519,13 @@ int Engine::enqueue_group_graph()
 {
     int rc;
     if (!graph_valid_ && (rc = capture_graphs())) return rc;
-    if (dstep_value_ != step_) launch_set_step(d_step_, step_, s_);
+    const int par = (step_ / kRefreshGrain) & 1;
+    // counter par holds step_ after a replayed group; after direct launches set it
+    if (!dstep_valid_) launch_set_step(d_step_ + par, step_, s_);
     const bool refresh = !tab_.mixed_probs.empty() && refresh_may_be_due(step_ + kRefreshGrain);
-    ECHK(hipGraphLaunch(gx_[refresh ? 1 : 0], s_));
+    ECHK(hipGraphLaunch(gx_[2 * par + (refresh ? 1 : 0)], s_));
     step_ += kRefreshGrain;
-    dstep_value_ = step_;
+    dstep_valid_ = true;
     return 0;
 }
 
@@ -658,12 +673,19 @@ int Engine::run_all(bool timing)
     return 0;
 }
 
-int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out)
+void Engine::set_tiles(int nx, int ny, int nz, bool mixed, int kz, int vx)
+{
+    const Geo g = make_geo(nx, ny, nz, 0, 0, std::max(kz, kMinKz), mixed ? vx : 1);
+    tuned_[std::make_tuple(nx, ny, nz, mixed)] = Tuned{g.kz, g.vx, 0.0};
+}
+
+int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, double *us_out)
 {
     const auto key = std::make_tuple(nx, ny, nz, mixed);
     auto hit = tuned_.find(key);
     if (hit != tuned_.end()) {
         *kz_out = hit->second.kz;
+        *vx_out = hit->second.vx;
         *us_out = hit->second.us;
         return 0;
     }
@@ -716,10 +738,16 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out
     std::vector<TileRef> tab(max_nt3);
     const int reps = 20, step = 2;   // n = 2 every launch: same inputs each time
     double best_us = 1e300;
-    int best_kz = g_def.kz;
-    for (int kz : cands) {
-        if (kz > nz && kz != cands.front()) continue;
-        const Geo g = make_geo(nx, ny, nz, 0, 0, kz);
+    int best_kz = g_def.kz, best_vx = 1;
+    std::vector<std::pair<int, int>> shapes;
+    for (int vx : {1, 2}) {
+        if (vx == 2 && (!mixed || nx % 2)) continue;
+        for (int kz : cands)
+            if (kz <= nz || kz == cands.front()) shapes.emplace_back(kz, vx);
+    }
+    for (const auto &shape : shapes) {
+        const int kz = shape.first;
+        const Geo g = make_geo(nx, ny, nz, 0, 0, kz, shape.second);
         ProbDev pd{};
         Carver cv{static_cast<char *>(blk)};
         const int nt3 = spmv_tiles(g), nt1 = vec_tiles(g.n);
@@ -743,9 +771,9 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out
         const int grid = std::min(nt3, kMaxGrid);
         auto launch = [&] {
             if (mixed)
-                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step}, nullptr, s_);
+                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
             else
-                launch_cg_a(false, dP, dT, nt3, grid, StepRef{nullptr, step}, nullptr, s_);
+                launch_cg_a(false, dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
         };
         launch();
         launch();
@@ -760,12 +788,14 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, double *us_out
         if (us < best_us) {
             best_us = us;
             best_kz = g.kz;
+            best_vx = g.vx;
         }
     }
 #undef TCHK
     cleanup();
-    tuned_[key] = Tuned{best_kz, best_us};
+    tuned_[key] = Tuned{best_kz, best_vx, best_us};
     *kz_out = best_kz;
+    *vx_out = best_vx;
     *us_out = best_us;
     (void)rc;
     return 0;

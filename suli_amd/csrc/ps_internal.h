@@ -31,7 +31,8 @@ struct Geo {
     int nx, ny, nz;
     int px, py;               // periodic in x / y (z is always mirrored)
     long long n;              // nx*ny*nz
-    int tiles_x, tiles_y, tiles_z, kz;   // 3-D tiling: 64 x 4 columns, kz planes
+    int tiles_x, tiles_y, tiles_z, kz;   // 3-D tiling: (64 vx) x 4 columns, kz planes
+    int vx;                   // x cells per lane in 3-D tiles: 1, or 2 (FP32, even nx)
 };
 
 // Device-resident CG state of one problem.
@@ -113,9 +114,13 @@ struct ProbDev {
 
 // Global step of a launch: *base + off when base is set (kernels replayed
 // from a captured HIP graph read the step from device memory), else off.
+// next (nullable): the launch stores *base + kRefreshGrain there (from one
+// thread) for the next replayed group, which reads that counter instead:
+// two counters alternate, so no launch ever writes the one it reads.
 struct StepRef {
     const int *base;
     int off;
+    int *next;
 };
 
 // Entry of a launch table: tile `tile` of problem `prob`.
@@ -182,8 +187,7 @@ void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef
 // recurrence; decide converged / capped / continue.
 void launch_refresh(const ProbDev *P, const TileRef *t1, int nt1, const TileRef *t3, int nt3, const int *probs,
                     int nprobs, int grid, StepRef step, hipStream_t s);
-// *base += by (one thread), *base = value: the device step counter.
-void launch_advance(int *base, int by, hipStream_t s);
+// *base = value: sets a device step counter.
 void launch_set_step(int *base, int value, hipStream_t s);
 // After a problem's solve: FP64 true residual r.r into S->rr_true and the
 // gauge shift that restores e0's null-space coordinate.
@@ -197,6 +201,6 @@ void launch_copy(const void *src, void *dst, long long n16, hipStream_t s);
 
 int spmv_tiles(const Geo &g);
 int vec_tiles(long long n);
-Geo make_geo(int nx, int ny, int nz, int px, int py, int kz = 0);
+Geo make_geo(int nx, int ny, int nz, int px, int py, int kz = 0, int vx = 1);
 
 }  // namespace sps

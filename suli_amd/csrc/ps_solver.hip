@@ -442,17 +442,29 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
     return 0;
 }
 
-int suli_ps_autotune(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t *kz, double *step_us)
+int suli_ps_autotune(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t *kz, int32_t *vx,
+                     double *step_us)
 {
-    if (!ctx || !kz || !step_us) return SULI_PS_EINVAL;
+    if (!ctx || !kz || !vx || !step_us) return SULI_PS_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
-    int k = 0;
+    int k = 0, v = 1;
     double us = 0.0;
-    if ((rc = ctx->eng.tune(g->nx, g->ny, g->nz, precision == SULI_PS_MIXED, &k, &us)))
+    if ((rc = ctx->eng.tune(g->nx, g->ny, g->nz, precision == SULI_PS_MIXED, &k, &v, &us)))
         return fail(ctx, rc, "%s", ctx->eng.error().c_str());
     *kz = k;
+    *vx = v;
     *step_us = us;
+    return 0;
+}
+
+int suli_ps_set_tiles(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t kz, int32_t vx)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (kz < 1 || (vx != 1 && vx != 2)) return fail(ctx, SULI_PS_EINVAL, "tile shape kz=%d vx=%d", kz, vx);
+    ctx->eng.set_tiles(g->nx, g->ny, g->nz, precision == SULI_PS_MIXED, kz, vx);
     return 0;
 }
 

@@ -213,13 +213,19 @@ class PressureSolver:
         return ms.value
 
     def autotune(self, grid: Grid, precision: str = "mixed"):
-        """(kz, us per CG step) chosen for this grid size (tuned on first use)."""
+        """(kz, vx, us per CG step) chosen for this grid size (tuned on first use)."""
         kz = C.c_int32()
+        vx = C.c_int32()
         us = C.c_double()
         cg = _c_grid(grid)
         self._check(self.lib.suli_ps_autotune(self.ctx, C.byref(cg), PRECISIONS[precision],
-                                              C.byref(kz), C.byref(us)))
-        return kz.value, us.value
+                                              C.byref(kz), C.byref(vx), C.byref(us)))
+        return kz.value, vx.value, us.value
+
+    def set_tiles(self, grid: Grid, precision: str, kz: int, vx: int) -> None:
+        """Fix the tile shape used for this grid size (with autotune on)."""
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_set_tiles(self.ctx, C.byref(cg), PRECISIONS[precision], kz, vx))
 
     def copy_bench(self, nbytes: int, reps: int = 20) -> float:
         """Mean time (ms) of the library's plain device copy of nbytes."""

@@ -134,10 +134,11 @@ def test_autotune_picks_a_depth_and_keeps_bit_identity(solver):
     """The tuner returns one of its candidate depths with a positive time,
     caches it, and tuned solves stay bit-identical alone and in a batch."""
     cs = mixed_cases(seed=17)[:3]
-    kz1, us1 = solver.autotune(cs[1].grid, "mixed")
-    kz2, us2 = solver.autotune(cs[1].grid, "mixed")
-    assert (kz1, us1) == (kz2, us2)
-    assert 2 <= kz1 <= max(48, cs[1].grid.nz) and us1 > 0
+    kz1, vx1, us1 = solver.autotune(cs[1].grid, "mixed")
+    kz2, vx2, us2 = solver.autotune(cs[1].grid, "mixed")
+    assert (kz1, vx1, us1) == (kz2, vx2, us2)
+    assert 2 <= kz1 <= max(48, cs[1].grid.nz) and vx1 in (1, 2) and us1 > 0
+    assert solver.autotune(cs[1].grid, "fp64")[1] == 1
     solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False,
                      autotune=True)
     got, sts = solve_batch(solver, cs)
@@ -148,4 +149,27 @@ def test_autotune_picks_a_depth_and_keeps_bit_identity(solver):
         p_or, _, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
                                      project_null=True)
         assert helpers.rel_diff(got[i], p_or, helpers.face_mask(got[i].shape)) <= PRESSURE_RTOL
+    solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
+
+
+@pytest.mark.parametrize("kz", [3, 8])
+def test_two_cells_per_lane_vs_oracle(solver, kz):
+    """FP32 tiles with two x cells per lane (even nx; walls, x-periodic,
+    y-periodic, stretched): same tolerance against the oracle, and batch
+    results still equal solo ones."""
+    cs = [cases.random_case(24, 20, 16, seed=61), cases.random_case(70, 33, 19, seed=62, periodic_x=True),
+          cases.random_case(64, 4, 8, seed=63, periodic_y=True), cases.random_case(40, 24, 20, seed=64,
+                                                                                    stretched=True)]
+    cases.make_stretched_compatible(cs[3])
+    for c in cs:
+        solver.set_tiles(c.grid, "mixed", kz, 2)
+    solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True)
+    got, sts = solve_batch(solver, cs)
+    for i, c in enumerate(cs):
+        p_or, _, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
+                                     project_null=True)
+        assert rc == 0 and sts[i].status == 0
+        assert helpers.rel_diff(got[i], p_or, helpers.face_mask(got[i].shape)) <= PRESSURE_RTOL, i
+        alone, st1 = solve_alone(solver, c)
+        assert np.array_equal(got[i], alone), f"problem {i}"
     solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
