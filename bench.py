@@ -264,6 +264,8 @@ def main():
                     help="solver(s) to time (profiling runs)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel that calibrates bandwidth and counters")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel directly instead of replaying HIP graphs")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the stretched-cylinder, batch and serving workloads")
     args = ap.parse_args()
@@ -288,10 +290,12 @@ def main():
     print(f"# copy {copy}", file=sys.stderr, flush=True)
     mixed = fp64 = None
     if args.only in ("both", "mixed"):
-        mixed = run_solves(solver, g, df, "mixed", args.steps, args.warmup, args.tol)
+        mixed = run_solves(solver, g, df, "mixed", args.steps, args.warmup, args.tol,
+                           graphs=not args.no_graphs)
         print(f"# mixed {mixed}", file=sys.stderr, flush=True)
     if args.only in ("both", "fp64"):
-        fp64 = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol)
+        fp64 = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol,
+                          graphs=not args.no_graphs)
         print(f"# fp64 {fp64}", file=sys.stderr, flush=True)
     df.free()
     solver.close()
