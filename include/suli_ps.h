@@ -58,6 +58,8 @@ typedef struct suli_ps_grid {
     int32_t ccx0, ccy0;     /* 0 = periodic in x / y */
     int32_t t_press;        /* 1 = projection; 2 = IPA, faces use rho_f1 (9_grad.f90:109-116) */
     int32_t obst_t, ibm_t;  /* obst_t != 0 && ibm_t == 2: masked RHS (9_grad.f90:45-88) */
+    int32_t top_bc;         /* 0: zero-flux lid, as the reference; 1: p = 0 on the top
+                               face (an extension: ghost = -interior, w free there) */
     double dx, dy, dz, dt;  /* dz unused when a dz(0:nz+1) array is passed */
     double rho_f1;
 } suli_ps_grid;

@@ -37,7 +37,7 @@ class OracleGrid(C.Structure):
         ("obsx", _dp), ("obsy", _dp), ("obsz", _dp),
         ("omp_t", C.c_int), ("stop_rule", C.c_int), ("tol", C.c_double),
         ("max_iter", C.c_int), ("threads", C.c_int), ("project_null", C.c_int),
-        ("solid", C.c_void_p),
+        ("solid", C.c_void_p), ("top_dirichlet", C.c_int),
     ]
 
 
@@ -91,6 +91,7 @@ def make_grid(g, fields=None, *, stop_rule=STOP_REFERENCE, tol=1e-8, max_iter=10
     og.omp_t, og.stop_rule, og.tol = omp_t, stop_rule, tol
     og.max_iter, og.threads = max_iter, threads
     og.project_null = 1 if project_null else 0
+    og.top_dirichlet = 1 if getattr(g, "top_bc", 0) == 1 else 0
     solid = getattr(fields, "solid", None) if fields is not None else None
     if solid is not None:
         assert solid.dtype == np.uint8 and solid.flags.f_contiguous

@@ -92,8 +92,10 @@ void oracle_interp_faces(const oracle_grid *g, const double *rho, double *rhox,
 
 /* Ghost fill of a halo-box array in the reference's order: x faces (mirror
  * or periodic), then y, then z (always mirror).  9_grad.f90:188-239 (d),
- * :322-370 (r), :531-579 (p), :594-643 (e). */
-static void fill_ghosts(const oracle_grid *g, double *a)
+ * :322-370 (r), :531-579 (p), :594-643 (e).  With top_sign = -1 the top
+ * ghost plane is minus the interior (the Dirichlet lid extension; used for
+ * r, p and e, never for d). */
+static void fill_ghosts_s(const oracle_grid *g, double *a, double top_sign)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     int i, j, k;
@@ -120,9 +122,14 @@ static void fill_ghosts(const oracle_grid *g, double *a)
     for (j = 0; j <= ny + 1; j++)
         for (i = 0; i <= nx + 1; i++) {
             a[IP(i, j, 0)] = a[IP(i, j, 1)];
-            a[IP(i, j, nz + 1)] = a[IP(i, j, nz)];
+            a[IP(i, j, nz + 1)] = top_sign * a[IP(i, j, nz)];
         }
 }
+
+static void fill_ghosts(const oracle_grid *g, double *a) { fill_ghosts_s(g, a, 1.0); }
+
+/* ghosts of a pressure-like array: the lid is Dirichlet when top_dirichlet */
+static void fill_ghosts_p(const oracle_grid *g, double *a) { fill_ghosts_s(g, a, g->top_dirichlet ? -1.0 : 1.0); }
 
 /* Sum of a[c]*b[c] over interior cells.  Serial k-j-i order (omp_t = 0,
  * 9_grad.f90:422-433, :482-493) or an OpenMP reduction (omp_t = 1,
@@ -378,6 +385,10 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
     /* e = prd1 * sqrt(d) on the whole box (9_grad.f90:242-252) */
 #pragma omp parallel for num_threads(nt) if (nt > 1)
     for (size_t c = 0; c < nbox; c++) e[c] = prd1[c] * sqrt(d[c]);
+    /* Dirichlet lid (extension): the top ghosts are minus the interior */
+    if (g->top_dirichlet)
+        for (j = 0; j <= ny + 1; j++)
+            for (i = 0; i <= nx + 1; i++) e[IP(i, j, nz + 1)] = -e[IP(i, j, nz)];
     /* solid mask: solid cells (and their ghost images) start at 0 */
     if (g->solid)
         for (k = 0; k <= nz + 1; k++)
@@ -401,7 +412,7 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
      * leaves A singular with null vector 1, so the scaled operator has null
      * vector sqrt(d); remove the residual's component along it, making the
      * floating-point right-hand side exactly consistent.  Serial sums. */
-    if (g->project_null) {
+    if (g->project_null && !g->top_dirichlet) {   /* a Dirichlet lid leaves no null space */
         /* with a solid mask the null vector is sqrt(d) on fluid cells only */
         double rn = 0.0, dd = 0.0;
         for (k = 1; k <= nz; k++)
@@ -433,7 +444,7 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
             }
 
     /* ghosts of r, p = r (9_grad.f90:322-380) */
-    fill_ghosts(g, r);
+    fill_ghosts_p(g, r);
     memcpy(p, r, nbox * sizeof(double));
 
     /* stop rule (9_grad.f90:384-398) */
@@ -486,13 +497,13 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
             for (j = 1; j <= ny; j++)
                 for (i = 1; i <= nx; i++)
                     p[IP(i, j, k)] = r[IP(i, j, k)] + beta * p[IP(i, j, k)];
-        fill_ghosts(g, p);
+        fill_ghosts_p(g, p);
 
         aux1 = fabs(rr);
     }
 
     /* ghosts of e and un-scaling on the whole box (9_grad.f90:594-659) */
-    fill_ghosts(g, e);
+    fill_ghosts_p(g, e);
 #pragma omp parallel for num_threads(nt) if (nt > 1)
     for (size_t c = 0; c < nbox; c++) prd1[c] = e[c] / sqrt(d[c]);
 

@@ -45,6 +45,9 @@ typedef struct oracle_grid {
     const unsigned char *solid; /* not in the reference: cells (nx,ny,nz, x fastest)
                                    masked out; faces touching one carry no flux;
                                    NULL = none */
+    int top_dirichlet;      /* not in the reference: p = 0 on the top face (ghost =
+                               -interior for the pressure-like arrays) instead of
+                               the zero-flux lid */
 } oracle_grid;
 
 typedef struct oracle_stats {

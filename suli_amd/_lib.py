@@ -38,6 +38,7 @@ class Grid(C.Structure):
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("ccx0", C.c_int32), ("ccy0", C.c_int32),
         ("t_press", C.c_int32), ("obst_t", C.c_int32), ("ibm_t", C.c_int32),
+        ("top_bc", C.c_int32),
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
         ("rho_f1", C.c_double),
     ]

@@ -46,6 +46,7 @@ class Grid:
     ibm_t: int = 0
     dzv: Optional[np.ndarray] = None    # dz(0:nz+1) for the stretched (ICdz) grid
     dzzv: Optional[np.ndarray] = None   # dzz(0:nz+1)
+    top_bc: int = 0        # 1: p = 0 on the top face (not a reference mode)
 
     @property
     def ncell(self) -> int:
@@ -141,15 +142,21 @@ def sloshing_rho(g: Grid, height: float, amp: float, alpha1: float,
     return np.asfortranarray(heaviside_rho(-phi, eps, rho_air, rho_water))
 
 
-def config1(seed: int = 1, nx: int = 64, ny: int = 64, nz: int = 32) -> Case:
+def config1(seed: int = 1, nx: int = 64, ny: int = 64, nz: int = 32,
+            top_dirichlet: bool = False) -> Case:
     """Single-phase Poisson: 64x64x32, dx=dy=dz=0.01 m, rho=1000,
-    u,v,w ~ U(-0.1, 0.1), dt = 1e-3, all walls Neumann (the reference BC)."""
-    g = Grid(nx, ny, nz, 0.01, 0.01, 0.01, dt=1e-3, rho_f1=1000.0)
+    u,v,w ~ U(-0.1, 0.1), dt = 1e-3.  All walls Neumann (the reference BC),
+    or with top_dirichlet p = 0 on the top face (an extension), where w is
+    then free (drawn like the interior faces)."""
+    g = Grid(nx, ny, nz, 0.01, 0.01, 0.01, dt=1e-3, rho_f1=1000.0,
+             top_bc=1 if top_dirichlet else 0)
     f = zeros_fields(g)
     rng = np.random.default_rng(seed)
     random_velocity(g, f, 0.1, rng)
+    if top_dirichlet:
+        f.w[1:nx + 1, 1:ny + 1, nz + 1] = rng.uniform(-0.1, 0.1, (nx, ny))
     f.rho[...] = 1000.0
-    return Case(f"cfg1-{nx}x{ny}x{nz}", g, f, seed)
+    return Case(f"cfg1{'-dirichlet' if top_dirichlet else ''}-{nx}x{ny}x{nz}", g, f, seed)
 
 
 def config2(seed: int = 1, nx: int = 256, ny: int = 128, nz: int = 96,

@@ -137,7 +137,8 @@ __device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, T *__re
         const T ps = pn(c + dS), pnn = pn(c + dN);
         const T sxw = sx[fx], sxe = sx[fx + 1];
         const T sys = sy[fy], syn = sy[fy + nx];
-        const T y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * pt - szb * pb;
+        const T ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid: ghost = -cell
+        const T y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
         q[c] = y;
         p_new[c] = pc;
         if (DO_E) e[c] = e[c] - a_prev * poc;   // 9_grad.f90:458-468
@@ -272,7 +273,8 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         const float ps = pn(c + dS), pnn = pn(c + dN);
         const float sxw = sx[fx], sxe = sx[fx + 1];
         const float sys = sy[fy], syn = sy[fy + nx];
-        const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * pt - szb * pb;
+        const float ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid
+        const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
         q_new[c] = y;
         p_new[c] = pc;
         r_new[c] = rc;
@@ -353,9 +355,11 @@ __device__ __forceinline__ void cg_m_tile2(const Geo &g, const Tile &t, float *_
         const float2 ps = pn2(c + dS), pnn = pn2(c + dN);
         const float sx0 = sx[fx], sx1 = sx[fx + 1], sx2 = sx[fx + 2];
         const float2 sys = ld2(sy, fy), syn = ld2(sy, fy + nx);
+        const bool lid = g.dtop && k == nz - 1;   // Dirichlet lid
+        const float2 ptu = lid ? make_float2(-pc.x, -pc.y) : pt;
         float2 y;
-        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * pt.x - szb.x * pb.x;
-        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * pt.y - szb.y * pb.y;
+        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * ptu.x - szb.x * pb.x;
+        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * ptu.y - szb.y * pb.y;
         st2(q_new, c, y);
         st2(p_new, c, pc);
         st2(r_new, c, rc);
@@ -643,8 +647,9 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
                 const long long r0 = c - ci;
                 const long long up = (k < nz - 1) ? plane : 0, dn = (k > 0) ? plane : 0;
                 const double ec = e[c];
+                const double et = (g.dtop && k == nz - 1) ? -ec : e[c + up];   // Dirichlet lid
                 const double r = ec - sx[fx + 1] * e[r0 + ie] - sx[fx] * e[r0 + iw] - sy[fy + nx] * e[c + dN] -
-                                 sy[fy] * e[c + dS] - sz[c + plane] * e[c + up] - sz[c] * e[c - dn] - b[c];
+                                 sy[fy] * e[c + dS] - sz[c + plane] * et - sz[c] * e[c - dn] - b[c];
                 acc += r * r;
                 if (WRITE32) {
                     const float rf = (float)r;
@@ -792,9 +797,11 @@ __device__ __forceinline__ double spmv_tile2(const Geo &g, const Tile &t, const 
         const float2 ps = ld2(p, c + dS), pnn = ld2(p, c + dN);
         const float sx0 = sx[fx], sx1 = sx[fx + 1], sx2 = sx[fx + 2];
         const float2 sys = ld2(sy, fy), syn = ld2(sy, fy + nx);
+        const bool lid = g.dtop && k == nz - 1;
+        const float2 ptu = lid ? make_float2(-pc.x, -pc.y) : pt;
         float2 y;
-        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * pt.x - szb.x * pb.x;
-        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * pt.y - szb.y * pb.y;
+        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * ptu.x - szb.x * pb.x;
+        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * ptu.y - szb.y * pb.y;
         st2(q, c, y);
         acc += (double)pc.x * (double)y.x;
         acc += (double)pc.y * (double)y.y;
@@ -844,8 +851,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
                 const long long up = (k < nz - 1) ? plane : 0;
                 const T pt = p[c + up];
                 const T szt = V.sz[c + plane];
+                const T ptu = (g.dtop && k == nz - 1) ? -pc : pt;
                 const T y = pc - V.sx[fx + 1] * p[r0 + ie] - V.sx[fx] * p[r0 + iw] - V.sy[fy + nx] * p[c + dN] -
-                            V.sy[fy] * p[c + dS] - szt * pt - szb * pb;
+                            V.sy[fy] * p[c + dS] - szt * ptu - szb * pb;
                 q[c] = y;
                 acc += (double)pc * (double)y;
                 pb = pc;

@@ -28,6 +28,7 @@ int validate_grid(const suli_ps_grid *g, std::string &err)
     if (g->t_press != 1 && g->t_press != 2)
         return bad(err, "t_press = %d: only 1 (projection) and 2 (IPA) solve for pressure", g->t_press);
     if (g->t_press == 2 && !(g->rho_f1 > 0)) return bad(err, "IPA needs rho_f1 > 0");
+    if (g->top_bc != 0 && g->top_bc != 1) return bad(err, "top_bc = %d: 0 (zero flux) or 1 (p = 0)", g->top_bc);
     return 0;
 }
 

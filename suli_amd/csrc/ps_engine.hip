@@ -209,6 +209,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
         if (vx == 0) vx = vx_t;
     }
     sl.g = make_geo(gr.nx, gr.ny, gr.nz, gr.ccx0 == 0, gr.ccy0 == 0, kz, o.mixed ? vx : 1);
+    sl.g.dtop = gr.top_bc == 1 ? 1 : 0;
     sl.o = o;
     sl.o.kz = sl.g.kz;
     sl.o.vx = sl.g.vx;
@@ -238,7 +239,8 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     CgScalars s0{};
     s0.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
     s0.mixed = pd.mixed;
-    s0.project = o.keep_null ? 0 : 1;
+    // a Dirichlet lid leaves the operator without a null space
+    s0.project = (o.keep_null || sl.g.dtop) ? 0 : 1;
     s0.rel_rule = o.stop_rule == SULI_PS_STOP_RELATIVE;
     s0.tol = o.tol;
     // threshold of the reference stop rule: 0.0001/(nx*ny*nz) (9_grad.f90:393)

@@ -33,6 +33,7 @@ struct Geo {
     long long n;              // nx*ny*nz
     int tiles_x, tiles_y, tiles_z, kz;   // 3-D tiling: (64 vx) x 4 columns, kz planes
     int vx;                   // x cells per lane in 3-D tiles: 1, or 2 (FP32, even nx)
+    int dtop;                 // 1: p = 0 on the top face (top ghost = -interior)
 };
 
 // Device-resident CG state of one problem.

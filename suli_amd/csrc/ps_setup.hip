@@ -225,13 +225,20 @@ __global__ __launch_bounds__(kThreads) void k_scale_faces(Geo g, const double *d
 
 // Scaled unknown e = prd1 sqrt(d) at Fortran box index (I, J, K), ghosts of
 // d taken from the interior by the mirror/periodic rule (9_grad.f90:242-252).
-// Solid cells (and their ghost images) start at e = 0.
-__device__ __forceinline__ double e_box(const Geo &g, const SetupIn &in, const double *d, int I, int J, int K)
+// Solid cells (and their ghost images) start at e = 0; with a Dirichlet
+// lid the top ghost plane is minus the cell below.
+__device__ __forceinline__ double e_box0(const Geo &g, const SetupIn &in, const double *d, int I, int J, int K)
 {
     const int i = box_to_cell(I, g.nx, g.px), j = box_to_cell(J, g.ny, g.py), k = box_to_cell(K, g.nz, 0);
     if (solid_cell(g, in, i, j, k)) return 0.0;
     const double p = in.prd1[I + (long long)(g.nx + 2) * (J + (long long)(g.ny + 2) * K)];
     return p * sqrt(d[cidx(g, i, j, k)]);
+}
+
+__device__ __forceinline__ double e_box(const Geo &g, const SetupIn &in, const double *d, int I, int J, int K)
+{
+    if (g.dtop && K == g.nz + 1) return -e_box0(g, in, d, I, J, g.nz);
+    return e_box0(g, in, d, I, J, K);
 }
 
 // Initial residual r = A e - b with the caller's prd1 ghost values
@@ -269,7 +276,8 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
         const int kT = box_to_cell(K + 1, g.nz, 0) + 1, kB = box_to_cell(K - 1, g.nz, 0) + 1;
         const double mE = e_box(g, in, d, iE, J, K), mW = e_box(g, in, d, iW, J, K);
         const double mN = e_box(g, in, d, I, jN, K), mS = e_box(g, in, d, I, jS, K);
-        const double mT = e_box(g, in, d, I, J, kT), mB = e_box(g, in, d, I, J, kB);
+        const double mT = (g.dtop && K == g.nz) ? -ec : e_box(g, in, d, I, J, kT);
+        const double mB = e_box(g, in, d, I, J, kB);
         const double a_mir = ec - op.sx[fx + 1] * mE - op.sx[fx] * mW - op.sy[fy + nx] * mN - op.sy[fy] * mS -
                              op.sz[fz + pl] * mT - op.sz[fz] * mB;
         const double bc = b[c];
@@ -391,8 +399,10 @@ __global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, co
         const int J = (int)(jk % by), K = (int)(jk / by);
         const long long c = cidx(g, box_to_cell(I, g.nx, g.px), box_to_cell(J, g.ny, g.py), box_to_cell(K, g.nz, 0));
         const double p = e[c] / sqrt(d[c]);
-        // solid cells (and their ghost images) are 0 exactly
-        prd1[t] = (solid && solid[c]) ? 0.0 : (gshift != 0.0 ? p - gshift : p);
+        // solid cells (and their ghost images) are 0 exactly; a Dirichlet
+        // lid's ghost plane is minus the cell below
+        const double v = (solid && solid[c]) ? 0.0 : (gshift != 0.0 ? p - gshift : p);
+        prd1[t] = (g.dtop && K == g.nz + 1) ? -v : v;
     }
 }
 

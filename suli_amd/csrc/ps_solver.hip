@@ -149,6 +149,7 @@ int run_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, double *
     if (rc) return rc;
     if (!u || !v || !w || !rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
     Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
+    g.dtop = gr->top_bc == 1 ? 1 : 0;
     SetupIn in = make_setup(gr, nullptr, nullptr, nullptr, rho, prd1, nullptr, nullptr, nullptr, dz, dzz, solid);
     launch_posdin(g, in, u, v, w, ctx->stream);
     HIPCHK(ctx, hipGetLastError());

@@ -16,6 +16,7 @@ module suli_ps_iface
       integer(c_int32_t) :: ccx0, ccy0
       integer(c_int32_t) :: t_press
       integer(c_int32_t) :: obst_t, ibm_t
+      integer(c_int32_t) :: top_bc = 0
       real(c_double) :: dx, dy, dz, dt
       real(c_double) :: rho_f1
    end type suli_ps_grid

@@ -29,6 +29,7 @@ def _c_grid(g: Grid) -> _lib.Grid:
     cg.t_press, cg.obst_t, cg.ibm_t = g.t_press, g.obst_t, g.ibm_t
     cg.dx, cg.dy, cg.dz, cg.dt = g.dx, g.dy, g.dz, g.dt
     cg.rho_f1 = g.rho_f1
+    cg.top_bc = getattr(g, "top_bc", 0)
     return cg
 
 

@@ -33,7 +33,8 @@ def test_library_exports_every_header_symbol():
 
 def test_struct_layouts_match_header():
     """ctypes mirrors of the ABI structs have the C sizes (natural alignment)."""
-    assert ctypes.sizeof(_lib.Grid) == 8 * 4 + 5 * 8
+    assert ctypes.sizeof(_lib.Grid) == 9 * 4 + 4 + 5 * 8     # padded before the doubles
+    assert _lib.Grid.dx.offset == 40
     assert ctypes.sizeof(_lib.Opts) == 4 + 4 + 8 + 7 * 4 + 4   # trailing pad to 8
     assert _lib.Opts.no_graphs.offset == 36 and _lib.Opts.autotune.offset == 40
     assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 7 * 8

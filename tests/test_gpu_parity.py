@@ -240,3 +240,82 @@ def test_spmv_timing_hook(solver, cfg2):
     solve(solver, g, f, precision="mixed", stop_rule="relative", tol=1e-6, timing=True)
     ms = solver.time_spmv("mixed", reps=20)
     assert 0 < ms < 5.0
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 2, 5), (2, 7, 3), (65, 5, 2), (1 + 64 * 2, 3, 4)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tiny_and_ragged_grids(solver, shape, precision):
+    """The smallest grids the reference allows (2 cells per axis), odd and
+    tile-ragged sizes (nx = 64k + 1), against the oracle."""
+    c = cases.random_case(*shape, seed=71 + sum(shape))
+    tol = 1e-12 if precision == "fp64" else 1e-11
+    p_or, st_or, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
+                                     project_null=True)
+    assert rc == 0
+    prd1, st = solve(solver, c.grid, c.fields, precision=precision, stop_rule="relative", tol=tol)
+    assert st.status == 0, st.as_dict()
+    mask = helpers.face_mask(prd1.shape)
+    assert helpers.rel_diff(prd1, p_or, mask) <= PRESSURE_RTOL
+
+
+def test_config1_reference_mode(solver):
+    """64x64x32 single-phase Poisson (dx = 0.01, rho = 1000, U(-0.1, 0.1)),
+    all-Neumann as the reference: FP64 with the reference stop rule against
+    the oracle, and mixed precision at 1e-8 relative."""
+    c = cases.config1(seed=1)
+    p_or, st_or, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_REFERENCE,
+                                     project_null=True, threads=8)
+    assert rc == 0
+    prd1, st = solve(solver, c.grid, c.fields)
+    assert st.status == 0 and abs(st.iters - st_or.iters) <= 1
+    mask = helpers.face_mask(prd1.shape)
+    assert helpers.rel_diff(prd1, p_or, mask) <= 1e-6   # reference rule: r.r <= 1e-4/N only
+    prd1m, stm = solve(solver, c.grid, c.fields, precision="mixed", stop_rule="relative", tol=1e-8)
+    assert stm.status == 0 and stm.rel_true <= 1e-8
+
+
+def lid_case(shape, seed, **kw):
+    c = cases.random_case(*shape, seed=seed, **kw)
+    c.grid.top_bc = 1
+    g = c.grid
+    c.fields.w[1:g.nx + 1, 1:g.ny + 1, g.nz + 1] = np.random.default_rng(seed).uniform(
+        -0.05, 0.05, (g.nx, g.ny))
+    return c
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kw", [dict(), dict(periodic_x=True), dict(ratio=1.0), dict(stretched=True)],
+                         ids=["walls", "px", "single-phase", "stretched"])
+def test_dirichlet_lid_vs_oracle(solver, precision, kw):
+    """p = 0 on the top face (config 1's boundary condition, an extension):
+    pressure and posdin against the oracle."""
+    c = lid_case((28, 20, 14), 81, **kw)
+    p_or, st_or, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12)
+    assert rc == 0
+    tol = 1e-12 if precision == "fp64" else 1e-11
+    prd1, st = solve(solver, c.grid, c.fields, precision=precision, stop_rule="relative", tol=tol)
+    assert st.status == 0, st.as_dict()
+    mask = helpers.face_mask(prd1.shape)
+    assert helpers.rel_diff(prd1, p_or, mask) <= PRESSURE_RTOL
+    np.testing.assert_array_equal(prd1[1:-1, 1:-1, -1], -prd1[1:-1, 1:-1, -2])
+    u1, v1, w1, _ = oracle.posdin(c.grid, c.fields, p_or)
+    f2 = c.fields.copy()
+    f2.prd1 = p_or.copy(order="F")
+    solver.posdin(c.grid, f2)
+    np.testing.assert_array_equal(f2.w, w1)
+    np.testing.assert_array_equal(f2.u, u1)
+
+
+def test_config1_dirichlet_full_size(solver):
+    """64x64x32 single-phase Poisson with p = 0 on the top face, FP64 and
+    mixed to 1e-8, against the oracle."""
+    c = cases.config1(seed=1, top_dirichlet=True)
+    p_or, st_or, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-10,
+                                     threads=8)
+    assert rc == 0
+    mask = None
+    for precision in ("fp64", "mixed"):
+        prd1, st = solve(solver, c.grid, c.fields, precision=precision, stop_rule="relative", tol=1e-10)
+        assert st.status == 0 and st.rel_true <= 1.05e-10, st.as_dict()
+        mask = helpers.face_mask(prd1.shape)
+        assert helpers.rel_diff(prd1, p_or, mask) <= 1e-7, precision
