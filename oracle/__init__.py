@@ -72,6 +72,10 @@ def lib():
         _lib.oracle_scaled_operator.argtypes = [C.POINTER(OracleGrid)] + [_dp] * 10
         _lib.oracle_heaviside_rho.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                               C.c_double, _dp]
+        _lib.oracle_pressh.restype = None
+        _lib.oracle_pressh.argtypes = [C.POINTER(OracleGrid), _dp, C.c_double, _dp]
+        _lib.oracle_posdin_hydro.restype = None
+        _lib.oracle_posdin_hydro.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     return _lib
 
 
@@ -148,6 +152,26 @@ def scaled_operator(case_grid, fields):
     return shx, shy, shz, d, q, b
 
 
+def pressh(case_grid, rho, gz, prd1):
+    """C restatement of pressh (hydrostatic pressure): returns a copy of
+    prd1 with its interior written."""
+    out = np.asfortranarray(prd1).copy(order="F")
+    og = make_grid(case_grid)
+    lib().oracle_pressh(C.byref(og), _ptr(np.asfortranarray(rho, dtype=np.float64)), gz, _ptr(out))
+    return out
+
+
+def posdin_hydro(case_grid, fields, hs, prd1):
+    """C restatement of posdin's hydrostatic branch. Returns (u, v, w, prd)."""
+    u, v, w = (fields.u.copy(order="F"), fields.v.copy(order="F"), fields.w.copy(order="F"))
+    prd = np.zeros_like(prd1, order="F")
+    og = make_grid(case_grid)
+    lib().oracle_posdin_hydro(C.byref(og), _ptr(u), _ptr(v), _ptr(w), _ptr(fields.rho),
+                              _ptr(np.asfortranarray(hs, dtype=np.float64)),
+                              _ptr(np.asfortranarray(prd1)), _ptr(prd))
+    return u, v, w, prd
+
+
 def heaviside_rho(ls, eps, rho1, rho2):
     ls = np.ascontiguousarray(ls, dtype=np.float64).ravel(order="K")
     out = np.empty_like(ls)
@@ -176,6 +200,8 @@ def ref_lib(variant: str = "ic"):
                                      [C.c_double, _dp, _dp])
         L.suli_ref_graddin.argtypes = [_dp] * 4 + [C.c_int] + [_dp] * 4
         L.suli_ref_posdin.argtypes = [_dp] * 6
+        L.suli_ref_pressh.argtypes = [_dp, C.c_double, _dp]
+        L.suli_ref_posdin_hydro.argtypes = [_dp] * 7
         _ref_libs[variant] = L
     return _ref_libs[variant]
 
@@ -244,4 +270,28 @@ def ref_posdin(case_grid, fields, prd1):
     prd = np.zeros_like(prd1, order="F")
     _big_stack_call(_stack_bytes(g), L.suli_ref_posdin, _ptr(u), _ptr(v), _ptr(w),
                     _ptr(fields.rho), _ptr(np.asfortranarray(prd1)), _ptr(prd))
+    return u, v, w, prd
+
+
+def ref_pressh(case_grid, rho, gz, prd1):
+    g = case_grid
+    L = ref_lib("icdz" if g.stretched else "ic")
+    _ref_setup(L, g, 0)
+    out = np.asfortranarray(prd1).copy(order="F")
+    _big_stack_call(_stack_bytes(g), L.suli_ref_pressh, _ptr(np.asfortranarray(rho, dtype=np.float64)),
+                    gz, _ptr(out))
+    return out
+
+
+def ref_posdin_hydro(case_grid, fields, hs, prd1):
+    """The reference's posdin with t_press = 0 (its hydrostatic branch)."""
+    import dataclasses
+    g = dataclasses.replace(case_grid, t_press=0)
+    L = ref_lib("icdz" if g.stretched else "ic")
+    _ref_setup(L, g, 0)
+    u, v, w = (fields.u.copy(order="F"), fields.v.copy(order="F"), fields.w.copy(order="F"))
+    prd = np.zeros_like(prd1, order="F")
+    _big_stack_call(_stack_bytes(g), L.suli_ref_posdin_hydro, _ptr(u), _ptr(v), _ptr(w),
+                    _ptr(np.asfortranarray(fields.rho)), _ptr(np.asfortranarray(hs, dtype=np.float64)), _ptr(np.asfortranarray(prd1)),
+                    _ptr(prd))
     return u, v, w, prd

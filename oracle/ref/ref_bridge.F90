@@ -122,4 +122,52 @@ contains
       prdout = prd
    end subroutine ref_posdin
 
+   ! Runs the reference pressh() (hydrostatic pressure, t_press = 0) once:
+   ! prd1's interior is written from rho and gz; its halo is left as given.
+   subroutine ref_pressh(rhoin, gz_, prd1io) bind(C, name="suli_ref_pressh")
+      use disc
+      use velpre
+      use param
+      real(c_double), intent(in) :: rhoin(nx, ny, nz)
+      real(c_double), value :: gz_
+      real(c_double), intent(inout) :: prd1io(0:nx+1, 0:ny+1, 0:nz+1)
+
+      rho = rhoin
+      gz = gz_
+      prd1 = prd1io
+      call pressh()
+      prd1io = prd1
+   end subroutine ref_pressh
+
+   ! Runs the reference posdin() in its hydrostatic branch (t_press /= 1,
+   ! set up by suli_ref_setup): u, v corrected from prd1 and rho, w from
+   ! continuity, velocities zeroed where the face value of hs is below 1e-4,
+   ! prd = prd1.
+   subroutine ref_posdin_hydro(uio, vio, wio, rhoin, hsin, prd1in, prdout) bind(C, name="suli_ref_posdin_hydro")
+      use disc
+      use velpre
+      use ls_param
+      real(c_double), intent(inout) :: uio(0:nx+2, 0:ny+1, 0:nz+1)
+      real(c_double), intent(inout) :: vio(0:nx+1, 0:ny+2, 0:nz+1)
+      real(c_double), intent(inout) :: wio(0:nx+1, 0:ny+1, 0:nz+2)
+      real(c_double), intent(in) :: rhoin(nx, ny, nz)
+      real(c_double), intent(in) :: hsin(nx, ny, nz)
+      real(c_double), intent(in) :: prd1in(0:nx+1, 0:ny+1, 0:nz+1)
+      real(c_double), intent(out) :: prdout(0:nx+1, 0:ny+1, 0:nz+1)
+
+      if (allocated(hs)) deallocate(hs)
+      allocate(hs(nx, ny, nz))
+      hs = hsin
+      u = uio
+      v = vio
+      w = wio
+      rho = rhoin
+      prd1 = prd1in
+      call posdin()
+      uio = u
+      vio = v
+      wio = w
+      prdout = prd
+   end subroutine ref_posdin_hydro
+
 end module suli_ref_bridge

@@ -519,6 +519,29 @@ int oracle_graddin(const oracle_grid *g, const double *u, const double *v, const
 }
 
 /* SuLi-IC/10_pos.f90:17-46,95; SuLi-ICdz/10_pos.f90:21,41. */
+/* posdin's horizontal correction, taken in both branches
+ * (SuLi-IC/10_pos.f90:24-37): u and v lose dt/dx (dt/dy) times the
+ * pressure difference over the face density.  With a solid mask, closed
+ * faces are left alone (extension). */
+static void posdin_uv(const oracle_grid *g, double *u, double *v, const double *rhox,
+                      const double *rhoy, const double *prd1)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const double aux1 = g->dt / g->dx;
+    const double aux2 = g->dt / g->dy;
+    int i, j, k;
+    for (k = 1; k <= nz; k++) {
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                if (!closed_x(g, i, j, k))
+                    u[IU(i, j, k)] = u[IU(i, j, k)] - aux1 * (prd1[IP(i, j, k)] - prd1[IP(i - 1, j, k)]) / rhox[IFX(i, j, k)];
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                if (!closed_y(g, i, j, k))
+                    v[IV(i, j, k)] = v[IV(i, j, k)] - aux2 * (prd1[IP(i, j, k)] - prd1[IP(i, j - 1, k)]) / rhoy[IFY(i, j, k)];
+    }
+}
+
 void oracle_posdin(const oracle_grid *g, double *u, double *v, double *w, const double *rho,
                    const double *prd1, double *prd)
 {
@@ -530,19 +553,7 @@ void oracle_posdin(const oracle_grid *g, double *u, double *v, double *w, const 
     double *rhoy = malloc(nfy * sizeof(double));
     double *rhoz = malloc(nfz * sizeof(double));
     oracle_interp_faces(g, rho, rhox, rhoy, rhoz);
-    const double aux1 = g->dt / g->dx;
-    const double aux2 = g->dt / g->dy;
-    /* with a solid mask, closed faces are left alone (extension) */
-    for (k = 1; k <= nz; k++) {
-        for (j = 1; j <= ny; j++)
-            for (i = 1; i <= nx + 1; i++)
-                if (!closed_x(g, i, j, k))
-                    u[IU(i, j, k)] = u[IU(i, j, k)] - aux1 * (prd1[IP(i, j, k)] - prd1[IP(i - 1, j, k)]) / rhox[IFX(i, j, k)];
-        for (j = 1; j <= ny + 1; j++)
-            for (i = 1; i <= nx; i++)
-                if (!closed_y(g, i, j, k))
-                    v[IV(i, j, k)] = v[IV(i, j, k)] - aux2 * (prd1[IP(i, j, k)] - prd1[IP(i, j - 1, k)]) / rhoy[IFY(i, j, k)];
-    }
+    posdin_uv(g, u, v, rhox, rhoy, prd1);
     for (k = 1; k <= nz + 1; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -554,6 +565,89 @@ void oracle_posdin(const oracle_grid *g, double *u, double *v, double *w, const 
     free(rhox);
     free(rhoy);
     free(rhoz);
+}
+
+/* Hydrostatic pressure, t_press = 0 (SuLi-IC/9_grad.f90:667-702; dz(k)
+ * weights: SuLi-ICdz/9_grad.f90:665-700).  Column integral from the top
+ * (Simpson over two cells below nz-1); prd1's halo is not touched. */
+void oracle_pressh(const oracle_grid *g, const double *rho, double gz, double *prd1)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    int i, j, k;
+    for (k = nz; k >= 1; k--)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                if (g->dzv) {
+                    const double *dz = g->dzv;
+                    if (k == nz) {
+                        prd1[IP(i, j, k)] = dz[k] * gz * rho[IC(i, j, k)];
+                    } else if (k == nz - 1) {
+                        prd1[IP(i, j, k)] = gz * (rho[IC(i, j, k)] * dz[k] + rho[IC(i, j, k + 1)] * dz[k + 1]) * 0.5 +
+                                            prd1[IP(i, j, k + 1)];
+                    } else {
+                        const double intz = (rho[IC(i, j, k)] * dz[k] + 4. * rho[IC(i, j, k + 1)] * dz[k + 1] +
+                                             rho[IC(i, j, k + 2)] * dz[k + 2]) / 3.;
+                        prd1[IP(i, j, k)] = gz * intz + prd1[IP(i, j, k + 2)];
+                    }
+                } else {
+                    const double dz = g->dz;
+                    if (k == nz) {
+                        prd1[IP(i, j, k)] = dz * gz * rho[IC(i, j, k)];
+                    } else if (k == nz - 1) {
+                        prd1[IP(i, j, k)] = dz * gz * (rho[IC(i, j, k)] + rho[IC(i, j, k + 1)]) * 0.5 +
+                                            prd1[IP(i, j, k + 1)];
+                    } else {
+                        const double intz = dz * (rho[IC(i, j, k)] + 4. * rho[IC(i, j, k + 1)] + rho[IC(i, j, k + 2)]) / 3.;
+                        prd1[IP(i, j, k)] = gz * intz + prd1[IP(i, j, k + 2)];
+                    }
+                }
+            }
+}
+
+/* posdin's hydrostatic branch, t_press /= 1 (SuLi-IC/10_pos.f90:24-37 and
+ * :48-93; dz(k): SuLi-ICdz/10_pos.f90:48-56): the same u, v correction as
+ * the other branch, w from continuity column by column from w = 0 at the
+ * bottom, then u, v, w zeroed on the faces where the
+ * face-interpolated hs (8_derivadas.f90 interp*_cf) is below 1e-4; prd =
+ * prd1. */
+void oracle_posdin_hydro(const oracle_grid *g, double *u, double *v, double *w, const double *rho,
+                         const double *hs, const double *prd1, double *prd)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz,
+                 nfz = (size_t)nx * ny * (nz + 1);
+    int i, j, k;
+    double *hx = malloc(nfx * sizeof(double));
+    double *hy = malloc(nfy * sizeof(double));
+    double *hz = malloc(nfz * sizeof(double));
+    oracle_interp_faces(g, rho, hx, hy, hz);
+    posdin_uv(g, u, v, hx, hy, prd1);
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) w[IW(i, j, 1)] = 0.;
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double dzk = g->dzv ? g->dzv[k] : g->dz;
+                w[IW(i, j, k + 1)] = w[IW(i, j, k)] - (u[IU(i + 1, j, k)] - u[IU(i, j, k)]) * dzk / g->dx -
+                                     (v[IV(i, j + 1, k)] - v[IV(i, j, k)]) * dzk / g->dy;
+            }
+    oracle_interp_faces(g, hs, hx, hy, hz);
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                if (hx[IFX(i, j, k)] < 0.0001) u[IU(i, j, k)] = 0.;
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                if (hy[IFY(i, j, k)] < 0.0001) v[IV(i, j, k)] = 0.;
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                if (hz[IFZ(i, j, k)] < 0.0001) w[IW(i, j, k)] = 0.;
+    free(hx);
+    free(hy);
+    free(hz);
+    memcpy(prd, prd1, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
 }
 
 /* SuLi-IC/14_level_set.f90:555-587 (with dx1 = max(dx,dy,dz) from :41 and

@@ -89,6 +89,16 @@ void oracle_scaled_operator(const oracle_grid *g, const double *u,
 
 /* Smoothed Heaviside density, SuLi-IC/14_level_set.f90:41,555-587:
  * rho = rho1 (1-H) + rho2 H with H(ls) over half-width eps. */
+/* SuLi-IC/9_grad.f90:667-702 (SuLi-ICdz with dzv): hydrostatic prd1
+ * interior from rho, gz (module param) and dz; the halo is not touched. */
+void oracle_pressh(const oracle_grid *g, const double *rho, double gz, double *prd1);
+
+/* SuLi-IC/10_pos.f90:24-37 then :48-93 (t_press /= 1 branch): u, v
+ * corrected from prd1 and rho, w from continuity, then u, v, w zeroed where
+ * hs (nx,ny,nz) is below 1e-4 on the face; prd = prd1. */
+void oracle_posdin_hydro(const oracle_grid *g, double *u, double *v, double *w, const double *rho,
+                         const double *hs, const double *prd1, double *prd);
+
 void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1,
                           double rho2, double *rho);
 
