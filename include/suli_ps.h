@@ -156,6 +156,37 @@ int suli_ps_posdin_masked_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u
                               const double *dz, const double *dzz, const uint8_t *solid,
                               double *prd);
 
+/* Hydrostatic mode, t_press == 0 (SuLi-IC/1_principal.f90:97-100 calls
+ * pressh() then posdin()).
+ *
+ * pressh (SuLi-IC/9_grad.f90:667-702; SuLi-ICdz/9_grad.f90:665-700 when dz
+ * is given): prd1's interior becomes the column integral of gz*rho from the
+ * top (dz*gz*rho in the top cell, the trapezoid of the top two cells below
+ * it, Simpson's rule over two cells elsewhere); prd1's halo is not written.
+ * gz is the reference's module value (9.80665*cos(atan(decliv)),
+ * SuLi-IC/2_iniciais.f90:110).  t_press and dt are not read.
+ *
+ * posdin_hydro, posdin's t_press != 1 branch (SuLi-IC/10_pos.f90:24-37,
+ * 48-95; SuLi-ICdz/10_pos.f90 with dz): u, v -= dt grad(prd1)/rho_face as
+ * in suli_ps_posdin; w(:,:,1) = 0 and w from continuity up every column;
+ * then u, v, w set to 0 on the faces where the face value of hs (the
+ * Heaviside field, (nx,ny,nz) like rho) is below 1e-4; prd = prd1 (prd may
+ * be NULL).  The reference's posdin reads t_press to pick this branch;
+ * here the caller picks the entry point.
+ *
+ * Host pointers; the _dev forms take device pointers and run on the
+ * context's stream. */
+int suli_ps_pressh(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho,
+                   const double *dz, double gz, double *prd1);
+int suli_ps_pressh_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho,
+                       const double *dz, double gz, double *prd1);
+int suli_ps_posdin_hydro(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v,
+                         double *w, const double *rho, const double *hs,
+                         const double *prd1, const double *dz, double *prd);
+int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v,
+                             double *w, const double *rho, const double *hs,
+                             const double *prd1, const double *dz, double *prd);
+
 /* One problem of a batch: its grid, device arrays (layouts as above; solid
  * as for suli_ps_graddin_masked_dev, may be NULL) and, on return, its
  * statistics. */

@@ -30,6 +30,7 @@ EXPORTS = (
     "suli_ps_server_last_error", "suli_ps_server_submit", "suli_ps_server_wait",
     "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
     "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
+    "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
 )
 
 
@@ -113,6 +114,10 @@ def load(path: str = LIB_PATH):
     L.suli_ps_posdin_dev.argtypes = pargs
     L.suli_ps_graddin_masked_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 11 + [C.POINTER(Stats)]
     L.suli_ps_posdin_masked_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 9
+    L.suli_ps_pressh.argtypes = [vp, C.POINTER(Grid), vp, vp, C.c_double, vp]
+    L.suli_ps_pressh_dev.argtypes = [vp, C.POINTER(Grid), vp, vp, C.c_double, vp]
+    L.suli_ps_posdin_hydro.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
+    L.suli_ps_posdin_hydro_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
     L.suli_ps_dev_alloc.restype = vp
     L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]
     L.suli_ps_dev_free.argtypes = [vp, vp]

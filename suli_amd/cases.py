@@ -74,6 +74,8 @@ class Fields:
     # solid-cell mask (uint8, (nx, ny, nz), Fortran order; not a reference
     # mode): faces touching a solid cell carry no flux
     solid: Optional[np.ndarray] = None
+    # Heaviside field hs (nx, ny, nz), read by posdin's hydrostatic branch
+    hs: Optional[np.ndarray] = None
 
     def copy(self) -> "Fields":
         return Fields(*(None if a is None else a.copy(order="F") for a in dataclasses.astuple(self)))

@@ -11,6 +11,8 @@
 namespace sps {
 
 // 0, or SULI_PS_EINVAL with the reason in err.
+// Sizes, spacings and dt only: what the hydrostatic column sweeps need.
+int validate_shape(const suli_ps_grid *g, std::string &err);
 int validate_grid(const suli_ps_grid *g, std::string &err);
 int validate_problem(const suli_ps_grid *g, const double *u, const double *v, const double *w, const double *rho,
                      const double *ox, const double *oy, const double *oz, const double *dz, const double *dzz,

@@ -17,7 +17,7 @@ int bad(std::string &err, const char *fmt, int a = 0, int b = 0, int c = 0)
 }
 }  // namespace
 
-int validate_grid(const suli_ps_grid *g, std::string &err)
+int validate_shape(const suli_ps_grid *g, std::string &err)
 {
     if (!g) return bad(err, "grid is NULL");
     if (g->nx < 2 || g->ny < 2 || g->nz < 2)
@@ -25,6 +25,13 @@ int validate_grid(const suli_ps_grid *g, std::string &err)
     if ((long long)g->nx * g->ny * g->nz > (1LL << 31) - 1)
         return bad(err, "grid %dx%dx%d is too large", g->nx, g->ny, g->nz);
     if (!(g->dx > 0) || !(g->dy > 0) || !(g->dt > 0)) return bad(err, "dx, dy and dt must be positive");
+    return 0;
+}
+
+int validate_grid(const suli_ps_grid *g, std::string &err)
+{
+    const int rc = validate_shape(g, err);
+    if (rc) return rc;
     if (g->t_press != 1 && g->t_press != 2)
         return bad(err, "t_press = %d: only 1 (projection) and 2 (IPA) solve for pressure", g->t_press);
     if (g->t_press == 2 && !(g->rho_f1 > 0)) return bad(err, "IPA needs rho_f1 > 0");

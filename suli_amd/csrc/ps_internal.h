@@ -164,6 +164,13 @@ void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
                     const uint8_t *solid, hipStream_t s);
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s);
+// Hydrostatic mode (t_press = 0): pressh's column integral into prd1's
+// interior, and posdin's other branch (u, v correction, w from continuity,
+// velocities zeroed where hs < 1e-4 on the face).
+void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,
+                   hipStream_t s);
+void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, double *u, double *v, double *w,
+                         hipStream_t s);
 
 // ---- batched CG (ps_cg.hip) ----
 // Global step `step` runs iteration step - n0 + 1 of every problem in the

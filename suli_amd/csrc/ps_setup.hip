@@ -19,15 +19,16 @@ __device__ __forceinline__ long long cidx(const Geo &g, int i, int j, int k)
     return ((long long)k * g.ny + j) * g.nx + i;
 }
 
-// Face densities, interpx_cf / interpy_cf / interpz_cf
+// Face values of a cell field, interpx_cf / interpy_cf / interpz_cf
 // (SuLi-IC/8_derivadas.f90:305-357, :381-418, :443-481; dz-weighted z
 // faces: SuLi-ICdz/8_derivadas.f90:450-489).  Face index f is 0-based:
 // face f of a direction lies between cells f-1 and f.  Boundary faces copy
 // the adjacent interior face, except x-periodic where both take the mean of
-// the first and last cell.  y and z have no periodic branch.
-__device__ __forceinline__ double rho_face_x(const Geo &g, const SetupIn &in, int f, int j, int k)
+// the first and last cell.  y and z have no periodic branch.  Used for rho
+// and, in posdin's hydrostatic branch, for the Heaviside field hs.
+__device__ __forceinline__ double face_x(const Geo &g, const double *field, int f, int j, int k)
 {
-    const double *r = in.rho + ((long long)k * g.ny + j) * g.nx;
+    const double *r = field + ((long long)k * g.ny + j) * g.nx;
     const int nx = g.nx;
     if (f >= 1 && f <= nx - 1) return (r[f] + r[f - 1]) * 0.5;
     if (!g.px) {
@@ -37,26 +38,39 @@ __device__ __forceinline__ double rho_face_x(const Geo &g, const SetupIn &in, in
     return (r[0] + r[nx - 1]) * 0.5;
 }
 
-__device__ __forceinline__ double rho_face_y(const Geo &g, const SetupIn &in, int i, int f, int k)
+__device__ __forceinline__ double face_y(const Geo &g, const double *field, int i, int f, int k)
 {
-    const double *r = in.rho + (long long)k * g.ny * g.nx + i;
+    const double *r = field + (long long)k * g.ny * g.nx + i;
     const int ny = g.ny;
     const int a = f == 0 ? 1 : (f == ny ? ny - 1 : f);
     return (r[(long long)a * g.nx] + r[(long long)(a - 1) * g.nx]) * 0.5;
 }
 
-__device__ __forceinline__ double rho_face_z(const Geo &g, const SetupIn &in, int i, int j, int f)
+__device__ __forceinline__ double face_z(const Geo &g, const double *field, const double *dz, int i, int j, int f)
 {
     const long long plane = (long long)g.nx * g.ny;
-    const double *r = in.rho + (long long)j * g.nx + i;
+    const double *r = field + (long long)j * g.nx + i;
     const int nz = g.nz;
     const int a = f == 0 ? 1 : (f == nz ? nz - 1 : f);
-    if (in.dz) {
+    if (dz) {
         // Fortran face a+1 between cells a and a+1 (1-based): weights dz(a+1), dz(a)
-        const double d1 = in.dz[a + 1], d0 = in.dz[a];
+        const double d1 = dz[a + 1], d0 = dz[a];
         return (r[a * plane] * d1 + r[(a - 1) * plane] * d0) / (d1 + d0);
     }
     return (r[a * plane] + r[(a - 1) * plane]) * 0.5;
+}
+
+__device__ __forceinline__ double rho_face_x(const Geo &g, const SetupIn &in, int f, int j, int k)
+{
+    return face_x(g, in.rho, f, j, k);
+}
+__device__ __forceinline__ double rho_face_y(const Geo &g, const SetupIn &in, int i, int f, int k)
+{
+    return face_y(g, in.rho, i, f, k);
+}
+__device__ __forceinline__ double rho_face_z(const Geo &g, const SetupIn &in, int i, int j, int f)
+{
+    return face_z(g, in.rho, in.dz, i, j, f);
 }
 
 // Solid-cell mask (an extension with no reference counterpart): a face
@@ -410,12 +424,13 @@ __global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, co
 //   u(i,j,k) -= dt/dx (p(i)-p(i-1)) / rhox,   i = 1..nx+1, j = 1..ny, k = 1..nz
 //   v(i,j,k) -= dt/dy (p(j)-p(j-1)) / rhoy,   j = 1..ny+1
 //   w(i,j,k) -= dt/dz (p(k)-p(k-1)) / rhoz,   k = 1..nz+1 (dz -> dzz(k) on stretched grids)
-__global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w)
+// With_z false: the u and v faces only, the part both branches share.
+__global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w, bool with_z)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long nfx = (long long)(nx + 1) * ny * nz;
     const long long nfy = (long long)nx * (ny + 1) * nz;
-    const long long nfz = (long long)nx * ny * (nz + 1);
+    const long long nfz = with_z ? (long long)nx * ny * (nz + 1) : 0;
     const long long stride = (long long)gridDim.x * kThreads;
     const double aux1 = in.dt / in.dx, aux2 = in.dt / in.dy;
     const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
@@ -450,6 +465,117 @@ __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *
             const double rf = rho_face_z(g, in, i, j, f);
             const double dzk = in.dzz ? in.dzz[K] : in.dzu;
             if (!closed_z(g, in, i, j, f)) w[iw] = w[iw] - in.dt / dzk * (in.prd1[ip] - in.prd1[ip - pbb]) / rf;
+        }
+    }
+}
+
+// pressh, the hydrostatic pressure of t_press = 0 (SuLi-IC/9_grad.f90:
+// 667-702; dz(k) weights: SuLi-ICdz/9_grad.f90:665-700).  One lane per
+// (i, j) column walks k from the top: the top cell takes dz g rho, the one
+// below the trapezoid of the top two, every other cell Simpson's rule over
+// the two cells above it plus prd1(k+2).  The two values above stay in
+// registers; lanes of a wavefront are consecutive i, so every k step reads
+// and writes contiguous rows.  Only prd1's interior is written.
+__global__ __launch_bounds__(kThreads) void k_pressh(Geo g, const double *__restrict__ rho, const double *dz, double dzu,
+                                                     double gz, double *__restrict__ prd1)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long col = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (col >= (long long)nx * ny) return;
+    const int i = (int)(col % nx), j = (int)(col / nx);
+    const long long plane = (long long)nx * ny;
+    const long long pbb = (long long)(nx + 2) * (ny + 2);
+    const double *rc = rho + col;
+    double *pc = prd1 + (i + 1) + (long long)(nx + 2) * (j + 1);
+    double p1 = 0.0, p2 = 0.0;  // prd1 at k+1, k+2
+    double r1 = 0.0, r2 = 0.0;  // rho at k+1, k+2
+    for (int K = nz; K >= 1; --K) {
+        const double r0 = rc[(long long)(K - 1) * plane];
+        double p;
+        if (dz) {
+            if (K == nz) {
+                p = dz[K] * gz * r0;
+            } else if (K == nz - 1) {
+                p = gz * (r0 * dz[K] + r1 * dz[K + 1]) * 0.5 + p1;
+            } else {
+                const double intz = (r0 * dz[K] + 4. * r1 * dz[K + 1] + r2 * dz[K + 2]) / 3.;
+                p = gz * intz + p2;
+            }
+        } else {
+            if (K == nz) {
+                p = dzu * gz * r0;
+            } else if (K == nz - 1) {
+                p = dzu * gz * (r0 + r1) * 0.5 + p1;
+            } else {
+                const double intz = dzu * (r0 + 4. * r1 + r2) / 3.;
+                p = gz * intz + p2;
+            }
+        }
+        pc[(long long)K * pbb] = p;
+        p2 = p1;
+        p1 = p;
+        r2 = r1;
+        r1 = r0;
+    }
+}
+
+// posdin's hydrostatic branch, after the shared u, v correction
+// (SuLi-IC/10_pos.f90:48-57; dz(k): SuLi-ICdz/10_pos.f90:52): w(:,:,1) = 0
+// over the whole bottom plane, then w(k+1) = w(k) - du dz/dx - dv dz/dy up
+// each interior column from the corrected u, v.  The same lane then zeroes
+// its column's w faces where the face value of hs is below 1e-4 (:84-91);
+// the chain itself runs on the values before zeroing, as in the reference,
+// where the zeroing is a later sweep.  Lanes over the (nx+2)(ny+2) plane;
+// the ring only clears its bottom face.
+__global__ __launch_bounds__(kThreads) void k_posdin_hydro_w(Geo g, const double *u, const double *v,
+                                                             const double *hs, const double *dz, double dzu,
+                                                             double dx, double dy, double *w)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= (long long)(nx + 2) * (ny + 2)) return;
+    const int I = (int)(t % (nx + 2)), J = (int)(t / (nx + 2));
+    const long long wbb = (long long)(nx + 2) * (ny + 2);
+    double *wc = w + t;
+    wc[wbb] = 0.;  // w(:,:,1) = 0
+    if (I < 1 || I > nx || J < 1 || J > ny) return;
+    const long long ubb = (long long)(nx + 3) * (ny + 2), vbb = (long long)(nx + 2) * (ny + 3);
+    const double *uc = u + I + (long long)(nx + 3) * J;
+    const double *vc = v + I + (long long)(nx + 2) * J;
+    double wk = 0.;
+    if (face_z(g, hs, dz, I - 1, J - 1, 0) < 0.0001) wc[wbb] = 0.;
+    for (int K = 1; K <= nz; ++K) {
+        const double dzk = dz ? dz[K] : dzu;
+        const double du = uc[(long long)K * ubb + 1] - uc[(long long)K * ubb];
+        const double dv = vc[(long long)K * vbb + (nx + 2)] - vc[(long long)K * vbb];
+        wk = wk - du * dzk / dx - dv * dzk / dy;
+        wc[(long long)(K + 1) * wbb] = face_z(g, hs, dz, I - 1, J - 1, K) < 0.0001 ? 0. : wk;
+    }
+}
+
+// posdin's hydrostatic branch, last: u and v zeroed on the faces where hs
+// is below 1e-4 (SuLi-IC/10_pos.f90:64-82).  Runs after the w sweep, which
+// reads the values before zeroing.
+__global__ __launch_bounds__(kThreads) void k_posdin_hydro_uv(Geo g, const double *hs, double *u, double *v)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long nfx = (long long)(nx + 1) * ny * nz;
+    const long long nfy = (long long)nx * (ny + 1) * nz;
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy; t += stride) {
+        if (t < nfx) {
+            const int f = (int)(t % (nx + 1));
+            const long long jk = t / (nx + 1);
+            const int j = (int)(jk % ny), k = (int)(jk / ny);
+            if (face_x(g, hs, f, j, k) < 0.0001)
+                u[(f + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(ny + 2) * (k + 1))] = 0.;
+        } else {
+            const long long s = t - nfx;
+            const int i = (int)(s % nx);
+            const long long jk = s / nx;
+            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
+            if (face_y(g, hs, i, f, k) < 0.0001)
+                v[(i + 1) + (long long)(nx + 2) * ((f + 1) + (long long)(ny + 3) * (k + 1))] = 0.;
         }
     }
 }
@@ -507,7 +633,26 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
 {
     const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
                          (long long)g.nx * g.ny * (g.nz + 1);
-    hipLaunchKernelGGL(k_posdin, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, u, v, w);
+    hipLaunchKernelGGL(k_posdin, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, u, v, w, true);
+}
+
+void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,
+                   hipStream_t s)
+{
+    const long long ncol = (long long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_pressh, dim3((unsigned)((ncol + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, g, rho,
+                       dz, dzu, gz, prd1);
+}
+
+void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, double *u, double *v, double *w,
+                         hipStream_t s)
+{
+    const long long nfxy = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz;
+    hipLaunchKernelGGL(k_posdin, dim3(grid_for(nfxy)), dim3(kThreads), 0, s, g, in, u, v, w, false);
+    const long long nplane = (long long)(g.nx + 2) * (g.ny + 2);
+    hipLaunchKernelGGL(k_posdin_hydro_w, dim3((unsigned)((nplane + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       g, u, v, hs, in.dz, in.dzu, in.dx, in.dy, w);
+    hipLaunchKernelGGL(k_posdin_hydro_uv, dim3(grid_for(nfxy)), dim3(kThreads), 0, s, g, hs, u, v);
 }
 
 }  // namespace sps

@@ -27,6 +27,7 @@ struct Staging {
     void *block = nullptr;
     double *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *prd1 = nullptr;
     double *obsx = nullptr, *obsy = nullptr, *obsz = nullptr, *dz = nullptr, *dzz = nullptr;
+    double *hs = nullptr;
 };
 
 }  // namespace
@@ -161,6 +162,48 @@ int run_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, double *
     return 0;
 }
 
+int check_columns(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *dz)
+{
+    std::string e;
+    const int rc = validate_shape(gr, e);
+    if (rc) return fail(ctx, rc, "%s", e.c_str());
+    if (!dz && !(gr->dz > 0)) return fail(ctx, SULI_PS_EINVAL, "dz must be positive");
+    return 0;
+}
+
+int run_pressh_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *rho, const double *dz, double gz,
+                   double *prd1)
+{
+    int rc = check_columns(ctx, gr, dz);
+    if (rc) return rc;
+    if (!rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "rho and prd1 must be non-NULL");
+    const Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
+    launch_pressh(g, rho, dz, gr->dz, gz, prd1, ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int run_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, double *v, double *w,
+                         const double *rho, const double *hs, const double *prd1, const double *dz, double *prd)
+{
+    int rc = check_columns(ctx, gr, dz);
+    if (rc) return rc;
+    if (!u || !v || !w || !rho || !hs || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho, hs and prd1 must be non-NULL");
+    const Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
+    const SetupIn in =
+        make_setup(gr, nullptr, nullptr, nullptr, rho, prd1, nullptr, nullptr, nullptr, dz, nullptr, nullptr);
+    launch_posdin_hydro(g, in, hs, u, v, w, ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    if (prd && prd != prd1) {
+        const size_t nbox = (size_t)(gr->nx + 2) * (gr->ny + 2) * (gr->nz + 2);
+        HIPCHK(ctx, hipMemcpyAsync(prd, prd1, nbox * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 // Staging for the host-pointer API, kept while the grid shape is unchanged.
 int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
 {
@@ -172,19 +215,20 @@ int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
         st = Staging{};
     }
     const size_t n = (size_t)nx * ny * nz;
-    const size_t sizes[10] = {(size_t)(nx + 3) * (ny + 2) * (nz + 2), (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+    const size_t sizes[11] = {(size_t)(nx + 3) * (ny + 2) * (nz + 2), (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                               (size_t)(nx + 2) * (ny + 2) * (nz + 3), n, (size_t)(nx + 2) * (ny + 2) * (nz + 2),
                               (size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
-                              (size_t)nz + 2, (size_t)nz + 2};
-    size_t off[10], total = 0;
-    for (int i = 0; i < 10; ++i) {
+                              (size_t)nz + 2, (size_t)nz + 2, n};
+    size_t off[11], total = 0;
+    for (int i = 0; i < 11; ++i) {
         off[i] = total;
         total += (sizes[i] * sizeof(double) + 255) / 256 * 256;
     }
     hipError_t e = hipMalloc(&st.block, total);
     if (e != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
-    double **ptrs[10] = {&st.u, &st.v, &st.w, &st.rho, &st.prd1, &st.obsx, &st.obsy, &st.obsz, &st.dz, &st.dzz};
-    for (int i = 0; i < 10; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
+    double **ptrs[11] = {&st.u,    &st.v,    &st.w,  &st.rho, &st.prd1, &st.obsx,
+                         &st.obsy, &st.obsz, &st.dz, &st.dzz, &st.hs};
+    for (int i = 0; i < 11; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
     st.nx = nx;
     st.ny = ny;
     st.nz = nz;
@@ -341,6 +385,74 @@ int suli_ps_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v
     }
     rc = run_posdin_dev(ctx, g, st.u, st.v, st.w, st.rho, st.prd1, dz ? st.dz : nullptr, dz ? st.dzz : nullptr,
                         nullptr);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);  // prd = prd1 (10_pos.f90:95)
+    return 0;
+}
+
+int suli_ps_pressh_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, const double *dz, double gz,
+                       double *prd1)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_pressh_dev(ctx, g, rho, dz, gz, prd1);
+}
+
+int suli_ps_pressh(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, const double *dz, double gz,
+                   double *prd1)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_columns(ctx, g, dz);
+    if (rc) return rc;
+    if (!rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "rho and prd1 must be non-NULL");
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    const size_t nbox = (size_t)(g->nx + 2) * (g->ny + 2) * (g->nz + 2);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    if (dz) HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (g->nz + 2), hipMemcpyHostToDevice, s));
+    if ((rc = run_pressh_dev(ctx, g, st.rho, dz ? st.dz : nullptr, gz, st.prd1))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return 0;
+}
+
+int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v, double *w,
+                             const double *rho, const double *hs, const double *prd1, const double *dz, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_posdin_hydro_dev(ctx, g, u, v, w, rho, hs, prd1, dz, prd);
+}
+
+int suli_ps_posdin_hydro(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v, double *w,
+                         const double *rho, const double *hs, const double *prd1, const double *dz, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_columns(ctx, g, dz);
+    if (rc) return rc;
+    if (!u || !v || !w || !rho || !hs || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho, hs and prd1 must be non-NULL");
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                 nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.hs, hs, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    if (dz) HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+    rc = run_posdin_hydro_dev(ctx, g, st.u, st.v, st.w, st.rho, st.hs, st.prd1, dz ? st.dz : nullptr, nullptr);
     if (rc) return rc;
     HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));

@@ -1,10 +1,10 @@
-! Drop-in graddin() and posdin() for SuLi on the MI355X library.
+! Drop-in graddin(), pressh() and posdin() for SuLi on the MI355X library.
 !
-! Replaces SuLi-IC/9_grad.f90 (graddin, :10-663) and SuLi-IC/10_pos.f90
-! (posdin, :1-99) in a SuLi build: same names, no arguments, same module
-! variables read and written (velpre: u, v, w, rho, prd1, prd; disc: nx..dt,
-! t_press, obst_t, ibm_t; cond: ccx0, ccy0; obst: obs_ls*; ls_param:
-! rho_f1).  Compile with -DSULI_ICDZ for SuLi-ICdz, whose disc module holds
+! Replaces SuLi-IC/9_grad.f90 (graddin, :10-663; pressh, :667-702) and
+! SuLi-IC/10_pos.f90 (posdin, :1-99) in a SuLi build: same names, no
+! arguments, same module variables read and written (velpre: u, v, w, rho,
+! prd1, prd; disc: nx..dt, t_press, obst_t, ibm_t; cond: ccx0, ccy0; obst:
+! obs_ls*; ls_param: rho_f1, hs; param: gz).  Compile with -DSULI_ICDZ for SuLi-ICdz, whose disc module holds
 ! the dz(0:nz+1) and dzz(0:nz+1) arrays.
 !
 ! Run-time options (environment, read once):
@@ -110,6 +110,7 @@ END SUBROUTINE graddin
 SUBROUTINE posdin()
    use iso_c_binding
    use velpre
+   use ls_param, only: hs
    use suli_ps_iface
    use suli_ps_dropin_state
    implicit none
@@ -119,14 +120,39 @@ SUBROUTINE posdin()
    call suli_ps_dropin_init()
    call suli_ps_dropin_grid(g)
    if (t_press /= 1) then
-      ! hydrostatic branch (SuLi-IC/10_pos.f90:47-94) is not on the device
-      write (*, *) "suli_ps posdin: only the projection branch (t_press = 1) is provided"
-      error stop 1
-   end if
+      ! the hydrostatic branch (SuLi-IC/10_pos.f90:47-94) reads hs
 #ifdef SULI_ICDZ
-   rc = suli_ps_posdin(suli_ps_ctx, g, u, v, w, rho, prd1, dz, dzz, prd)
+      rc = suli_ps_posdin_hydro(suli_ps_ctx, g, u, v, w, rho, hs, prd1, dz, prd)
 #else
-   rc = suli_ps_posdin(suli_ps_ctx, g, u, v, w, rho, prd1, prd=prd)
+      rc = suli_ps_posdin_hydro(suli_ps_ctx, g, u, v, w, rho, hs, prd1, prd=prd)
 #endif
+   else
+#ifdef SULI_ICDZ
+      rc = suli_ps_posdin(suli_ps_ctx, g, u, v, w, rho, prd1, dz, dzz, prd)
+#else
+      rc = suli_ps_posdin(suli_ps_ctx, g, u, v, w, rho, prd1, prd=prd)
+#endif
+   end if
    call suli_ps_dropin_check(rc, "posdin")
 END SUBROUTINE posdin
+
+
+SUBROUTINE pressh()
+   use iso_c_binding
+   use velpre
+   use param, only: gz
+   use suli_ps_iface
+   use suli_ps_dropin_state
+   implicit none
+   type(suli_ps_grid) :: g
+   integer(c_int) :: rc
+
+   call suli_ps_dropin_init()
+   call suli_ps_dropin_grid(g)
+#ifdef SULI_ICDZ
+   rc = suli_ps_pressh(suli_ps_ctx, g, rho, dz, gz, prd1)
+#else
+   rc = suli_ps_pressh(suli_ps_ctx, g, rho, gz=gz, prd1=prd1)
+#endif
+   call suli_ps_dropin_check(rc, "pressh")
+END SUBROUTINE pressh

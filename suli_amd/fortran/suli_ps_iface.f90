@@ -42,6 +42,7 @@ module suli_ps_iface
    end type suli_ps_stats
 
    public :: suli_ps_create, suli_ps_destroy, suli_ps_graddin, suli_ps_posdin, suli_ps_last_error
+   public :: suli_ps_pressh, suli_ps_posdin_hydro
    public :: suli_ps_error_message
 
    interface
@@ -88,6 +89,30 @@ module suli_ps_iface
          real(c_double), intent(inout), optional :: prd(*)
          integer(c_int) :: rc
       end function suli_ps_posdin
+
+      ! Hydrostatic mode (t_press = 0): pressh and posdin's other branch.
+      function suli_ps_pressh(ctx, g, rho, dz, gz, prd1) bind(C, name="suli_ps_pressh") result(rc)
+         import :: c_ptr, c_double, c_int, suli_ps_grid
+         type(c_ptr), value :: ctx
+         type(suli_ps_grid), intent(in) :: g
+         real(c_double), intent(in) :: rho(*)
+         real(c_double), intent(in), optional :: dz(*)
+         real(c_double), value :: gz
+         real(c_double), intent(inout) :: prd1(*)
+         integer(c_int) :: rc
+      end function suli_ps_pressh
+
+      function suli_ps_posdin_hydro(ctx, g, u, v, w, rho, hs, prd1, dz, prd) &
+         bind(C, name="suli_ps_posdin_hydro") result(rc)
+         import :: c_ptr, c_double, c_int, suli_ps_grid
+         type(c_ptr), value :: ctx
+         type(suli_ps_grid), intent(in) :: g
+         real(c_double), intent(inout) :: u(*), v(*), w(*)
+         real(c_double), intent(in) :: rho(*), hs(*), prd1(*)
+         real(c_double), intent(in), optional :: dz(*)
+         real(c_double), intent(inout), optional :: prd(*)
+         integer(c_int) :: rc
+      end function suli_ps_posdin_hydro
    end interface
 
 contains

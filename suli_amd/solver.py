@@ -48,7 +48,8 @@ STOP_RULES = {"reference": _lib.SULI_PS_STOP_REFERENCE, "relative": _lib.SULI_PS
 class DeviceFields:
     """Device copies of the pressure-step arrays (reference layouts)."""
 
-    NAMES = ("u", "v", "w", "rho", "prd1", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "solid")
+    NAMES = ("u", "v", "w", "rho", "prd1", "obs_lsx", "obs_lsy", "obs_lsz", "dz", "dzz", "solid",
+             "hs")
 
     def __init__(self, solver: "PressureSolver", grid: Grid, fields: Fields):
         self.solver = solver
@@ -56,7 +57,8 @@ class DeviceFields:
         self.ptrs = {}
         host = dict(u=fields.u, v=fields.v, w=fields.w, rho=fields.rho, prd1=fields.prd1,
                     obs_lsx=fields.obs_lsx, obs_lsy=fields.obs_lsy, obs_lsz=fields.obs_lsz,
-                    dz=grid.dzv, dzz=grid.dzzv, solid=getattr(fields, "solid", None))
+                    dz=grid.dzv, dzz=grid.dzzv, solid=getattr(fields, "solid", None),
+                    hs=getattr(fields, "hs", None))
         self.dtypes = {}
         for name, arr in host.items():
             if arr is None:
@@ -207,6 +209,35 @@ class PressureSolver:
             self._check(self.lib.suli_ps_posdin_dev(
                 self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["prd1"], p["dz"], p["dzz"],
                 None))
+
+    def pressh(self, grid: Grid, fields: Fields, gz: float) -> np.ndarray:
+        """Hydrostatic prd1 (t_press = 0), host arrays; prd1 updated in place."""
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_pressh(self.ctx, C.byref(cg), _host_ptr(fields.rho),
+                                            _host_ptr(grid.dzv), gz, _host_ptr(fields.prd1)))
+        return fields.prd1
+
+    def posdin_hydro(self, grid: Grid, fields: Fields) -> np.ndarray:
+        """posdin's hydrostatic branch on host arrays (fields.hs required);
+        u, v, w updated in place, returns prd."""
+        cg = _c_grid(grid)
+        prd = np.zeros_like(fields.prd1, order="F")
+        self._check(self.lib.suli_ps_posdin_hydro(
+            self.ctx, C.byref(cg), _host_ptr(fields.u), _host_ptr(fields.v), _host_ptr(fields.w),
+            _host_ptr(fields.rho), _host_ptr(fields.hs), _host_ptr(fields.prd1), _host_ptr(grid.dzv),
+            _host_ptr(prd)))
+        return prd
+
+    def pressh_dev(self, grid: Grid, df: DeviceFields, gz: float) -> None:
+        cg = _c_grid(grid)
+        p = df.ptrs
+        self._check(self.lib.suli_ps_pressh_dev(self.ctx, C.byref(cg), p["rho"], p["dz"], gz, p["prd1"]))
+
+    def posdin_hydro_dev(self, grid: Grid, df: DeviceFields) -> None:
+        cg = _c_grid(grid)
+        p = df.ptrs
+        self._check(self.lib.suli_ps_posdin_hydro_dev(
+            self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["hs"], p["prd1"], p["dz"], None))
 
     def time_spmv(self, precision: str = "fp64", reps: int = 50) -> float:
         ms = C.c_double()

@@ -33,3 +33,21 @@ def test_fortran_dropin_matches_reference(variant, args, precision):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert re.search(r"ok =\s*T", r.stdout)
+
+
+@pytest.mark.parametrize("variant,args", [
+    ("ic", ["24", "16", "12", "0", "0", "1"]),
+    ("ic", ["33", "20", "15", "1", "0", "1"]),
+    ("icdz", ["24", "16", "13", "0", "0", "1"]),
+])
+def test_fortran_dropin_hydrostatic_matches_reference(variant, args):
+    """t_press = 0: the drop-in pressh() and posdin()'s hydrostatic branch
+    against the reference's, to 1e-14 relative."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{variant}")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in check program not built (needs the reference sources)")
+    print(f"dropin hydro {variant} {args}", flush=True)
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"ok =\s*T", r.stdout)
