@@ -187,6 +187,19 @@ int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u,
                              double *w, const double *rho, const double *hs,
                              const double *prd1, const double *dz, double *prd);
 
+/* Density from the level set on the device: heaviside with t_tens = 0
+ * (SuLi-IC/14_level_set.f90:508-594; the t_tens = 1 surface-tension
+ * gradients hsx/hsy/hsz are not computed).  For each of the nx*ny*nz cells
+ * (rho's layout): hs = 0 below -eps, 1 above eps, else
+ * 0.5*(1 + ls/eps + sin(pi*ls/eps)/pi); rho = rho_f1*(1-hs) + rho_f2*hs
+ * with rho_f1 from the grid; mu = mu_f1*(1-hs) + mu_f2*hs (the reference's
+ * ls_mu).  eps = alpha1*dx1, dx1 = max(dx,dy,dz) (:41).  hs and mu may be
+ * NULL.  Device pointers, context stream; returns once written.  With it a
+ * caller holding the level set feeds suli_ps_graddin_dev directly. */
+int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *ls,
+                          double eps, double rho_f2, double mu_f1, double mu_f2,
+                          double *hs, double *rho, double *mu);
+
 /* One problem of a batch: its grid, device arrays (layouts as above; solid
  * as for suli_ps_graddin_masked_dev, may be NULL) and, on return, its
  * statistics. */

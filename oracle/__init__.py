@@ -72,6 +72,8 @@ def lib():
         _lib.oracle_scaled_operator.argtypes = [C.POINTER(OracleGrid)] + [_dp] * 10
         _lib.oracle_heaviside_rho.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                               C.c_double, _dp]
+        _lib.oracle_heaviside.restype = None
+        _lib.oracle_heaviside.argtypes = [C.c_int, _dp] + [C.c_double] * 5 + [_dp] * 3
         _lib.oracle_pressh.restype = None
         _lib.oracle_pressh.argtypes = [C.POINTER(OracleGrid), _dp, C.c_double, _dp]
         _lib.oracle_posdin_hydro.restype = None
@@ -172,6 +174,15 @@ def posdin_hydro(case_grid, fields, hs, prd1):
     return u, v, w, prd
 
 
+def heaviside(ls, eps, rho1, rho2, mu1, mu2):
+    """C restatement of heaviside (t_tens = 0). Returns (hs, rho, mu), each
+    shaped like ls (Fortran order)."""
+    ls = np.asfortranarray(ls, dtype=np.float64)
+    hs, rho, mu = (np.empty_like(ls, order="F") for _ in range(3))
+    lib().oracle_heaviside(ls.size, _ptr(ls), eps, rho1, rho2, mu1, mu2, _ptr(hs), _ptr(rho), _ptr(mu))
+    return hs, rho, mu
+
+
 def heaviside_rho(ls, eps, rho1, rho2):
     ls = np.ascontiguousarray(ls, dtype=np.float64).ravel(order="K")
     out = np.empty_like(ls)
@@ -202,6 +213,8 @@ def ref_lib(variant: str = "ic"):
         L.suli_ref_posdin.argtypes = [_dp] * 6
         L.suli_ref_pressh.argtypes = [_dp, C.c_double, _dp]
         L.suli_ref_posdin_hydro.argtypes = [_dp] * 7
+        if variant == "ic":
+            L.suli_ref_heaviside.argtypes = [_dp] + [C.c_double] * 6 + [_dp] * 3
         _ref_libs[variant] = L
     return _ref_libs[variant]
 
@@ -281,6 +294,18 @@ def ref_pressh(case_grid, rho, gz, prd1):
     _big_stack_call(_stack_bytes(g), L.suli_ref_pressh, _ptr(np.asfortranarray(rho, dtype=np.float64)),
                     gz, _ptr(out))
     return out
+
+
+def ref_heaviside(case_grid, ls, alpha1, dx1, rho1, rho2, mu1, mu2):
+    """The reference's heaviside (SuLi-IC build, t_tens = 0). Returns
+    (hs, rho, mu)."""
+    g = case_grid
+    L = ref_lib("ic")
+    _ref_setup(L, g, 0)
+    ls = np.asfortranarray(ls, dtype=np.float64)
+    hs, rho, mu = (np.empty_like(ls, order="F") for _ in range(3))
+    L.suli_ref_heaviside(_ptr(ls), alpha1, dx1, rho1, rho2, mu1, mu2, _ptr(hs), _ptr(rho), _ptr(mu))
+    return hs, rho, mu
 
 
 def ref_posdin_hydro(case_grid, fields, hs, prd1):

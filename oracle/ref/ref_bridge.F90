@@ -170,4 +170,34 @@ contains
       prdout = prd
    end subroutine ref_posdin_hydro
 
+#ifndef SULI_ICDZ
+   ! Runs the reference heaviside() (SuLi-IC/14_level_set.f90:508-594) with
+   ! t_tens = 0: hs, rho and ls_mu from the level set ls, alpha1 and dx1.
+   subroutine ref_heaviside(lsin, alpha1_, dx1_, rho1, rho2, mu1, mu2, hsout, rhoout, muout) &
+      bind(C, name="suli_ref_heaviside")
+      use disc
+      use velpre
+      use ls_param
+      real(c_double), intent(in) :: lsin(nx, ny, nz)
+      real(c_double), value :: alpha1_, dx1_, rho1, rho2, mu1, mu2
+      real(c_double), intent(out) :: hsout(nx, ny, nz), rhoout(nx, ny, nz), muout(nx, ny, nz)
+
+      if (allocated(ls)) deallocate(ls)
+      if (allocated(hs)) deallocate(hs)
+      if (allocated(ls_mu)) deallocate(ls_mu)
+      if (allocated(rho)) deallocate(rho)
+      allocate(ls(nx, ny, nz), hs(nx, ny, nz), ls_mu(nx, ny, nz), rho(nx, ny, nz))
+      ls = lsin
+      t_tens = 0
+      alpha1 = alpha1_
+      dx1 = dx1_
+      rho_f1 = rho1; rho_f2 = rho2
+      mu_f1 = mu1; mu_f2 = mu2
+      call heaviside()
+      hsout = hs
+      rhoout = rho
+      muout = ls_mu
+   end subroutine ref_heaviside
+#endif
+
 end module suli_ref_bridge

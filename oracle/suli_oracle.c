@@ -650,9 +650,12 @@ void oracle_posdin_hydro(const oracle_grid *g, double *u, double *v, double *w, 
     memcpy(prd, prd1, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
 }
 
-/* SuLi-IC/14_level_set.f90:555-587 (with dx1 = max(dx,dy,dz) from :41 and
- * eps = alpha1 * dx1 supplied by the caller). */
-void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1, double rho2, double *rho)
+/* heaviside with t_tens = 0 (SuLi-IC/14_level_set.f90:555-590, with
+ * dx1 = max(dx,dy,dz) from :41 and eps = alpha1 * dx1 supplied by the
+ * caller): hs from the level set, then rho and ls_mu as its linear blends.
+ * mu and hs may be NULL. */
+void oracle_heaviside(int n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
+                      double *hs_out, double *rho, double *mu)
 {
     const double pi = acos(-1.0);
     for (int c = 0; c < n; c++) {
@@ -663,6 +666,13 @@ void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1, doub
             hs = 1.0;
         else
             hs = 0.5 * (1.0 + ls[c] / eps + 1.0 / pi * sin(pi * ls[c] / eps));
+        if (hs_out) hs_out[c] = hs;
         rho[c] = rho1 * (1.0 - hs) + rho2 * hs;
+        if (mu) mu[c] = mu1 * (1.0 - hs) + mu2 * hs;
     }
+}
+
+void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1, double rho2, double *rho)
+{
+    oracle_heaviside(n, ls, eps, rho1, rho2, 0.0, 0.0, NULL, rho, NULL);
 }

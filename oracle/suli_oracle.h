@@ -99,6 +99,10 @@ void oracle_pressh(const oracle_grid *g, const double *rho, double gz, double *p
 void oracle_posdin_hydro(const oracle_grid *g, double *u, double *v, double *w, const double *rho,
                          const double *hs, const double *prd1, double *prd);
 
+/* SuLi-IC/14_level_set.f90:555-590 (t_tens = 0): hs, rho and ls_mu from
+ * the level set; hs_out and mu may be NULL. */
+void oracle_heaviside(int n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
+                      double *hs_out, double *rho, double *mu);
 void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1,
                           double rho2, double *rho);
 

@@ -31,6 +31,7 @@ EXPORTS = (
     "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
     "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
     "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
+    "suli_ps_heaviside_dev",
 )
 
 
@@ -118,6 +119,7 @@ def load(path: str = LIB_PATH):
     L.suli_ps_pressh_dev.argtypes = [vp, C.POINTER(Grid), vp, vp, C.c_double, vp]
     L.suli_ps_posdin_hydro.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
     L.suli_ps_posdin_hydro_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
+    L.suli_ps_heaviside_dev.argtypes = [vp, C.POINTER(Grid), vp] + [C.c_double] * 4 + [vp] * 3
     L.suli_ps_dev_alloc.restype = vp
     L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]
     L.suli_ps_dev_free.argtypes = [vp, vp]

@@ -127,10 +127,9 @@ def cell_centres(n: int, d: float) -> np.ndarray:
     return (np.arange(1, n + 1) - 0.5) * d
 
 
-def sloshing_rho(g: Grid, height: float, amp: float, alpha1: float,
-                 rho_air: float, rho_water: float) -> np.ndarray:
+def sloshing_level_set(g: Grid, height: float, amp: float) -> np.ndarray:
     """Two-phase tank: phi = z - height - amp sin(2 pi x / Lx), water below.
-    The level set passed to the Heaviside is ls = -phi (ls > 0 is water)."""
+    Returns the level set ls = -phi (ls > 0 is water), (nx, ny, nz)."""
     lx = g.nx * g.dx
     x = cell_centres(g.nx, g.dx)
     if g.stretched:
@@ -139,9 +138,15 @@ def sloshing_rho(g: Grid, height: float, amp: float, alpha1: float,
     else:
         z = cell_centres(g.nz, g.dz)
     phi = z[None, None, :] - height - amp * np.sin(2.0 * np.pi * x[:, None, None] / lx)
-    phi = np.broadcast_to(phi, (g.nx, g.ny, g.nz))
+    return np.asfortranarray(-np.broadcast_to(phi, (g.nx, g.ny, g.nz)))
+
+
+def sloshing_rho(g: Grid, height: float, amp: float, alpha1: float,
+                 rho_air: float, rho_water: float) -> np.ndarray:
+    """Density of the two-phase tank (sloshing_level_set) through the
+    smoothed Heaviside of half-width alpha1 max(dx, dy, dz)."""
     eps = alpha1 * max(g.dx, g.dy, g.dz)
-    return np.asfortranarray(heaviside_rho(-phi, eps, rho_air, rho_water))
+    return np.asfortranarray(heaviside_rho(sloshing_level_set(g, height, amp), eps, rho_air, rho_water))
 
 
 def config1(seed: int = 1, nx: int = 64, ny: int = 64, nz: int = 32,

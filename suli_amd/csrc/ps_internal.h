@@ -169,6 +169,9 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
 // velocities zeroed where hs < 1e-4 on the face).
 void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,
                    hipStream_t s);
+// rho (and hs, mu when non-null) from the level set, heaviside with t_tens = 0.
+void launch_heaviside(long long n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
+                      double *hs, double *rho, double *mu, hipStream_t s);
 void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, double *u, double *v, double *w,
                          hipStream_t s);
 

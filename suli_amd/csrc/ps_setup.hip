@@ -580,6 +580,32 @@ __global__ __launch_bounds__(kThreads) void k_posdin_hydro_uv(Geo g, const doubl
     }
 }
 
+// heaviside with t_tens = 0 (SuLi-IC/14_level_set.f90:555-590): the
+// smoothed Heaviside of the level set over a band of half-width eps =
+// alpha1*dx1 (dx1 = max(dx,dy,dz), :41), then rho and the viscosity as its
+// linear blends between the two fluids.  hs and mu may be null.
+__global__ __launch_bounds__(kThreads) void k_heaviside(long long n, const double *__restrict__ ls, double eps,
+                                                        double rho1, double rho2, double mu1, double mu2,
+                                                        double *__restrict__ hs_out, double *__restrict__ rho,
+                                                        double *__restrict__ mu)
+{
+    const double pi = 3.141592653589793;  // acos(-1.) in FP64
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < n; c += stride) {
+        const double l = ls[c];
+        double hs;
+        if (l < -eps)
+            hs = 0.;
+        else if (l > eps)
+            hs = 1.;
+        else
+            hs = 0.5 * (1. + l / eps + 1. / pi * sin(pi * l / eps));
+        if (hs_out) hs_out[c] = hs;
+        rho[c] = rho1 * (1. - hs) + rho2 * hs;
+        if (mu) mu[c] = mu1 * (1. - hs) + mu2 * hs;
+    }
+}
+
 int grid_for(long long n)
 {
     long long b = (n + kThreads - 1) / kThreads;
@@ -634,6 +660,13 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
     const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
                          (long long)g.nx * g.ny * (g.nz + 1);
     hipLaunchKernelGGL(k_posdin, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, u, v, w, true);
+}
+
+void launch_heaviside(long long n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
+                      double *hs, double *rho, double *mu, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_heaviside, dim3(grid_for(n)), dim3(kThreads), 0, s, n, ls, eps, rho1, rho2, mu1, mu2, hs,
+                       rho, mu);
 }
 
 void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,

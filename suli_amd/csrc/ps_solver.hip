@@ -422,6 +422,22 @@ int suli_ps_pressh(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, c
     return 0;
 }
 
+int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *ls, double eps, double rho_f2,
+                          double mu_f1, double mu_f2, double *hs, double *rho, double *mu)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    std::string e;
+    const int rc = validate_shape(g, e);
+    if (rc) return fail(ctx, rc, "%s", e.c_str());
+    if (!ls || !rho) return fail(ctx, SULI_PS_EINVAL, "ls and rho must be non-NULL");
+    if (!(eps > 0)) return fail(ctx, SULI_PS_EINVAL, "eps = alpha1*dx1 must be positive");
+    launch_heaviside((long long)g->nx * g->ny * g->nz, ls, eps, g->rho_f1, rho_f2, mu_f1, mu_f2, hs, rho, mu,
+                     ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v, double *w,
                              const double *rho, const double *hs, const double *prd1, const double *dz, double *prd)
 {

@@ -239,6 +239,33 @@ class PressureSolver:
         self._check(self.lib.suli_ps_posdin_hydro_dev(
             self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["hs"], p["prd1"], p["dz"], None))
 
+    def heaviside_dev(self, grid: Grid, ls_ptr, eps: float, rho_f2: float, mu_f1: float, mu_f2: float,
+                      rho_ptr, hs_ptr=None, mu_ptr=None) -> None:
+        """rho (and hs, mu) from a device-resident level set (raw device
+        pointers, e.g. DeviceFields.ptrs entries or suli_ps_dev_alloc)."""
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_heaviside_dev(self.ctx, C.byref(cg), ls_ptr, eps, rho_f2, mu_f1, mu_f2,
+                                                   hs_ptr, rho_ptr, mu_ptr))
+
+    def dev_array(self, arr: np.ndarray):
+        """Copy a float64 array to a new device allocation; returns the pointer
+        (free with dev_free)."""
+        arr = np.asfortranarray(arr, dtype=np.float64)
+        p = self.lib.suli_ps_dev_alloc(self.ctx, arr.nbytes)
+        if not p:
+            raise SuliPsError(f"device allocation of {arr.nbytes} B failed")
+        self._check(self.lib.suli_ps_memcpy_h2d(self.ctx, C.c_void_p(p), C.c_void_p(arr.ctypes.data),
+                                                arr.nbytes))
+        return C.c_void_p(p)
+
+    def dev_download(self, ptr, shape) -> np.ndarray:
+        out = np.zeros(shape, dtype=np.float64, order="F")
+        self._check(self.lib.suli_ps_memcpy_d2h(self.ctx, C.c_void_p(out.ctypes.data), ptr, out.nbytes))
+        return out
+
+    def dev_free(self, ptr) -> None:
+        self.lib.suli_ps_dev_free(self.ctx, ptr)
+
     def time_spmv(self, precision: str = "fp64", reps: int = 50) -> float:
         ms = C.c_double()
         self._check(self.lib.suli_ps_time_spmv(self.ctx, PRECISIONS[precision], reps, C.byref(ms)))
