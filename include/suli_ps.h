@@ -285,12 +285,13 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
 /* Tile shape of the CG kernels for a grid size, no reference counterpart:
  * times the per-step kernel (FP64 cg_a, or the mixed solver's FP32 step)
  * on a scratch problem of this size over tile depths of 2..48 planes and,
- * for the FP32 step on even nx, one or two x cells per lane, and keeps the
+ * for the FP32 step, one, two or four x cells per lane (two and four where
+ * nx is a multiple), and keeps the
  * fastest for this context (kz, vx, and its time per step).  Solves of
  * that size then use it when opts.autotune is set.  A result depends on
  * the tiling (the order of its sums), so solves of one size agree bit for
  * bit only under the same choice.  suli_ps_set_tiles fixes a shape
- * instead (vx is 1 for FP64 and for odd nx). */
+ * instead (vx 1, 2 or 4; 1 for FP64 and where nx is not a multiple). */
 int suli_ps_autotune(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t *kz,
                      int32_t *vx, double *step_us);
 int suli_ps_set_tiles(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t kz,

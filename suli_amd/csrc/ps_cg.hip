@@ -294,16 +294,38 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
     }
 }
 
-// Two x cells per lane (FP32, even nx): the lane's cells i, i+1 are one
-// 8-byte vector in every cell-layout array (all 256-byte aligned, i even).
-__device__ __forceinline__ float2 ld2(const float *p, long long x) { return *reinterpret_cast<const float2 *>(p + x); }
-__device__ __forceinline__ void st2(float *p, long long x, float2 v) { *reinterpret_cast<float2 *>(p + x) = v; }
+// V x cells per lane (FP32, V = 2 or 4, nx a multiple of V): the lane's
+// cells i .. i+V-1 are one 8- or 16-byte vector in every cell-layout array
+// (all 256-byte aligned, i a multiple of V).
+template <int V>
+__device__ __forceinline__ void ldv(const float *p, long long x, float (&o)[V])
+{
+    if constexpr (V == 2) {
+        const float2 t = *reinterpret_cast<const float2 *>(p + x);
+        o[0] = t.x;
+        o[1] = t.y;
+    } else {
+        const float4 t = *reinterpret_cast<const float4 *>(p + x);
+        o[0] = t.x;
+        o[1] = t.y;
+        o[2] = t.z;
+        o[3] = t.w;
+    }
+}
+template <int V>
+__device__ __forceinline__ void stv(float *p, long long x, const float (&v)[V])
+{
+    if constexpr (V == 2)
+        *reinterpret_cast<float2 *>(p + x) = make_float2(v[0], v[1]);
+    else
+        *reinterpret_cast<float4 *>(p + x) = make_float4(v[0], v[1], v[2], v[3]);
+}
 
-// cg_m_tile with two cells per lane: the same arithmetic per cell (and the
-// same term order), with the lane's x neighbours shared in registers and
-// 8-byte loads and stores.
-template <bool FIRST, bool UPD>
-__device__ __forceinline__ void cg_m_tile2(const Geo &g, const Tile &t, float *__restrict__ c32,
+// cg_m_tile with V cells per lane: the same arithmetic per cell (and the
+// same term order), with the lane's inner x neighbours shared in registers
+// and 8- or 16-byte loads and stores.
+template <int V, bool FIRST, bool UPD>
+__device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
                                            const float *__restrict__ r_old, const float *__restrict__ q_old,
                                            const float *__restrict__ p_old, float *__restrict__ r_new,
                                            float *__restrict__ p_new, float *__restrict__ q_new,
@@ -314,77 +336,137 @@ __device__ __forceinline__ void cg_m_tile2(const Geo &g, const Tile &t, float *_
     const long long plane = (long long)nx * ny;
     if (t.i >= nx || t.j >= ny) return;
     long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + 1, nx, g.px);
+    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + V - 1, nx, g.px);
     const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
     long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
     long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
     auto rn1 = [&](long long x) -> float { return UPD ? r_old[x] - a * q_old[x] : r_old[x]; };
     auto pn1 = [&](long long x) -> float { return FIRST ? rn1(x) : rn1(x) + b * p_old[x]; };
-    auto rn2 = [&](long long x) -> float2 {
-        float2 r = ld2(r_old, x);
+    auto rnv = [&](long long x, float (&r)[V]) {
+        ldv<V>(r_old, x, r);
         if (UPD) {
-            const float2 q = ld2(q_old, x);
-            r.x = r.x - a * q.x;
-            r.y = r.y - a * q.y;
+            float q[V];
+            ldv<V>(q_old, x, q);
+#pragma unroll
+            for (int m = 0; m < V; ++m) r[m] = r[m] - a * q[m];
         }
-        return r;
     };
-    auto pn2 = [&](long long x) -> float2 {
-        float2 r = rn2(x);
+    auto pnv = [&](long long x, float (&pv)[V]) {
+        rnv(x, pv);
         if (!FIRST) {
-            const float2 po = ld2(p_old, x);
-            r.x = r.x + b * po.x;
-            r.y = r.y + b * po.y;
+            float po[V];
+            ldv<V>(p_old, x, po);
+#pragma unroll
+            for (int m = 0; m < V; ++m) pv[m] = pv[m] + b * po[m];
         }
-        return r;
     };
-    float2 poc = FIRST ? make_float2(0.0f, 0.0f) : ld2(p_old, c);
-    float2 rc = rn2(c);
-    float2 pc = FIRST ? rc : make_float2(rc.x + b * poc.x, rc.y + b * poc.y);
-    float2 pb = t.k0 > 0 ? pn2(c - plane) : pc;
-    float2 szb = ld2(sz, c);
+    float poc[V], rc[V], pc[V], pb[V], szb[V];
+    if (FIRST) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) poc[m] = 0.0f;
+    } else {
+        ldv<V>(p_old, c, poc);
+    }
+    rnv(c, rc);
+#pragma unroll
+    for (int m = 0; m < V; ++m) pc[m] = FIRST ? rc[m] : rc[m] + b * poc[m];
+    if (t.k0 > 0) {
+        pnv(c - plane, pb);
+    } else {
+#pragma unroll
+        for (int m = 0; m < V; ++m) pb[m] = pc[m];
+    }
+    ldv<V>(sz, c, szb);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
-        const float2 pot = FIRST ? make_float2(0.0f, 0.0f) : ld2(p_old, c + up);
-        const float2 rt = rn2(c + up);
-        const float2 pt = FIRST ? rt : make_float2(rt.x + b * pot.x, rt.y + b * pot.y);
-        const float2 szt = ld2(sz, c + plane);
-        const float pw = pn1(r0 + iw), pe = pn1(r0 + ie);
-        const float2 ps = pn2(c + dS), pnn = pn2(c + dN);
-        const float sx0 = sx[fx], sx1 = sx[fx + 1], sx2 = sx[fx + 2];
-        const float2 sys = ld2(sy, fy), syn = ld2(sy, fy + nx);
-        const bool lid = g.dtop && k == nz - 1;   // Dirichlet lid
-        const float2 ptu = lid ? make_float2(-pc.x, -pc.y) : pt;
-        float2 y;
-        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * ptu.x - szb.x * pb.x;
-        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * ptu.y - szb.y * pb.y;
-        st2(q_new, c, y);
-        st2(p_new, c, pc);
-        st2(r_new, c, rc);
-        if (UPD) {
-            float2 cv = ld2(c32, c);
-            cv.x = cv.x - a * poc.x;
-            cv.y = cv.y - a * poc.y;
-            st2(c32, c, cv);
+        float pot[V], rt[V], pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1];
+        if (FIRST) {
+#pragma unroll
+            for (int m = 0; m < V; ++m) pot[m] = 0.0f;
+        } else {
+            ldv<V>(p_old, c + up, pot);
         }
-        acc[0] += (double)pc.x * (double)y.x;
-        acc[0] += (double)pc.y * (double)y.y;
-        acc[1] += (double)y.x * (double)rc.x;
-        acc[1] += (double)y.y * (double)rc.y;
-        acc[2] += (double)y.x * (double)y.x;
-        acc[2] += (double)y.y * (double)y.y;
-        acc[3] += (double)rc.x * (double)rc.x;
-        acc[3] += (double)rc.y * (double)rc.y;
-        pb = pc;
-        pc = pt;
-        rc = rt;
-        poc = pot;
-        szb = szt;
+        rnv(c + up, rt);
+#pragma unroll
+        for (int m = 0; m < V; ++m) pt[m] = FIRST ? rt[m] : rt[m] + b * pot[m];
+        ldv<V>(sz, c + plane, szt);
+        const float pw = pn1(r0 + iw), pe = pn1(r0 + ie);
+        pnv(c + dS, ps);
+        pnv(c + dN, pnn);
+#pragma unroll
+        for (int m = 0; m <= V; ++m) sxv[m] = sx[fx + m];
+        ldv<V>(sy, fy, sys);
+        ldv<V>(sy, fy + nx, syn);
+        const bool lid = g.dtop && k == nz - 1;   // Dirichlet lid
+        float y[V];
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const float east = m == V - 1 ? pe : pc[m + 1];
+            const float west = m == 0 ? pw : pc[m - 1];
+            const float top = lid ? -pc[m] : pt[m];
+            y[m] = pc[m] - sxv[m + 1] * east - sxv[m] * west - syn[m] * pnn[m] - sys[m] * ps[m] - szt[m] * top -
+                   szb[m] * pb[m];
+        }
+        stv<V>(q_new, c, y);
+        stv<V>(p_new, c, pc);
+        stv<V>(r_new, c, rc);
+        if (UPD) {
+            float cv[V];
+            ldv<V>(c32, c, cv);
+#pragma unroll
+            for (int m = 0; m < V; ++m) cv[m] = cv[m] - a * poc[m];   // 9_grad.f90:458-468
+            stv<V>(c32, c, cv);
+        }
+#pragma unroll
+        for (int m = 0; m < V; ++m) acc[0] += (double)pc[m] * (double)y[m];
+#pragma unroll
+        for (int m = 0; m < V; ++m) acc[1] += (double)y[m] * (double)rc[m];
+#pragma unroll
+        for (int m = 0; m < V; ++m) acc[2] += (double)y[m] * (double)y[m];
+#pragma unroll
+        for (int m = 0; m < V; ++m) acc[3] += (double)rc[m] * (double)rc[m];
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            pb[m] = pc[m];
+            pc[m] = pt[m];
+            rc[m] = rt[m];
+            poc[m] = pot[m];
+            szb[m] = szt[m];
+        }
         c += plane;
         fx += (long long)ny * (nx + 1);
         fy += (long long)(ny + 1) * nx;
+    }
+}
+
+// One tile of the mixed step with V cells per lane (V = 1: cg_m_tile).
+template <int V>
+__device__ __forceinline__ void cg_m_step(bool first, bool upd, const ProbDev &pd, const Tile &tl, float *r_o,
+                                          float *q_o, float *p_o, float *r_w, float *p_w, float *q_w, float a,
+                                          float b, double acc[4])
+{
+    if constexpr (V == 1) {
+        if (first)
+            cg_m_tile<true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a, b,
+                                   acc);
+        else if (upd)
+            cg_m_tile<false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a, b,
+                                   acc);
+        else
+            cg_m_tile<false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a, b,
+                                    acc);
+    } else {
+        if (first)
+            cg_m_tilev<V, true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a,
+                                       b, acc);
+        else if (upd)
+            cg_m_tilev<V, false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a,
+                                       b, acc);
+        else
+            cg_m_tilev<V, false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32,
+                                        a, b, acc);
     }
 }
 
@@ -457,25 +539,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_m(const ProbDev *__restrict__ P
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const float b = (float)beta;
-        if (pd.g.vx == 2) {
-            if (first)
-                cg_m_tile2<true, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w),
-                                        pbuf(pd, w), qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
-            else if (upd)
-                cg_m_tile2<false, true>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w),
-                                        pbuf(pd, w), qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
-            else
-                cg_m_tile2<false, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w),
-                                         pbuf(pd, w), qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
-        } else if (first)
-            cg_m_tile<true, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
-                                   qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
-        else if (upd)
-            cg_m_tile<false, true>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
-                                   qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+        float *r_o = rbuf(pd, o), *q_o = qbuf(pd, o), *p_o = pbuf(pd, o);
+        float *r_w = rbuf(pd, w), *p_w = pbuf(pd, w), *q_w = qbuf(pd, w);
+        if (pd.g.vx == 4)
+            cg_m_step<4>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, a, b, acc);
+        else if (pd.g.vx == 2)
+            cg_m_step<2>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, a, b, acc);
         else
-            cg_m_tile<false, false>(pd.g, tl, pd.c32, rbuf(pd, o), qbuf(pd, o), pbuf(pd, o), rbuf(pd, w), pbuf(pd, w),
-                                    qbuf(pd, w), pd.sx32, pd.sy32, pd.sz32, a, b, acc);
+            cg_m_step<1>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, a, b, acc);
         block_sum4(acc);
         if (threadIdx.x == 0) {
             double *m = pd.P.m4 + (size_t)w * 4 * pd.nt3 + 4 * (size_t)tr.tile;
@@ -770,8 +841,9 @@ __global__ void k_set_step(int *base, int value)
     if (threadIdx.x == 0) *base = value;
 }
 
-// Plain FP32 SpMV of one tile with two cells per lane (timing only).
-__device__ __forceinline__ double spmv_tile2(const Geo &g, const Tile &t, const float *__restrict__ p,
+// Plain FP32 SpMV of one tile with V cells per lane (timing only).
+template <int V>
+__device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const float *__restrict__ p,
                                              float *__restrict__ q, const float *__restrict__ sx,
                                              const float *__restrict__ sy, const float *__restrict__ sz)
 {
@@ -779,35 +851,51 @@ __device__ __forceinline__ double spmv_tile2(const Geo &g, const Tile &t, const 
     double acc = 0.0;
     if (t.i >= nx || t.j >= ny) return acc;
     const long long plane = (long long)nx * ny;
-    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + 1, nx, g.px);
+    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + V - 1, nx, g.px);
     const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
     long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
     long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
     long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-    float2 pc = ld2(p, c);
-    float2 pb = t.k0 > 0 ? ld2(p, c - plane) : pc;
-    float2 szb = ld2(sz, c);
+    float pc[V], pb[V], szb[V];
+    ldv<V>(p, c, pc);
+    if (t.k0 > 0)
+        ldv<V>(p, c - plane, pb);
+    else
+#pragma unroll
+        for (int m = 0; m < V; ++m) pb[m] = pc[m];
+    ldv<V>(sz, c, szb);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
-        const float2 pt = ld2(p, c + up);
-        const float2 szt = ld2(sz, c + plane);
+        float pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1], y[V];
+        ldv<V>(p, c + up, pt);
+        ldv<V>(sz, c + plane, szt);
         const float pw = p[r0 + iw], pe = p[r0 + ie];
-        const float2 ps = ld2(p, c + dS), pnn = ld2(p, c + dN);
-        const float sx0 = sx[fx], sx1 = sx[fx + 1], sx2 = sx[fx + 2];
-        const float2 sys = ld2(sy, fy), syn = ld2(sy, fy + nx);
+        ldv<V>(p, c + dS, ps);
+        ldv<V>(p, c + dN, pnn);
+#pragma unroll
+        for (int m = 0; m <= V; ++m) sxv[m] = sx[fx + m];
+        ldv<V>(sy, fy, sys);
+        ldv<V>(sy, fy + nx, syn);
         const bool lid = g.dtop && k == nz - 1;
-        const float2 ptu = lid ? make_float2(-pc.x, -pc.y) : pt;
-        float2 y;
-        y.x = pc.x - sx1 * pc.y - sx0 * pw - syn.x * pnn.x - sys.x * ps.x - szt.x * ptu.x - szb.x * pb.x;
-        y.y = pc.y - sx2 * pe - sx1 * pc.x - syn.y * pnn.y - sys.y * ps.y - szt.y * ptu.y - szb.y * pb.y;
-        st2(q, c, y);
-        acc += (double)pc.x * (double)y.x;
-        acc += (double)pc.y * (double)y.y;
-        pb = pc;
-        pc = pt;
-        szb = szt;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const float east = m == V - 1 ? pe : pc[m + 1];
+            const float west = m == 0 ? pw : pc[m - 1];
+            const float top = lid ? -pc[m] : pt[m];
+            y[m] = pc[m] - sxv[m + 1] * east - sxv[m] * west - syn[m] * pnn[m] - sys[m] * ps[m] - szt[m] * top -
+                   szb[m] * pb[m];
+        }
+        stv<V>(q, c, y);
+#pragma unroll
+        for (int m = 0; m < V; ++m) acc += (double)pc[m] * (double)y[m];
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            pb[m] = pc[m];
+            pc[m] = pt[m];
+            szb[m] = szt[m];
+        }
         c += plane;
         fx += (long long)ny * (nx + 1);
         fy += (long long)(ny + 1) * nx;
@@ -831,10 +919,14 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
         const Tile t = tile3(g, tr.tile);
         const int nx = g.nx, ny = g.ny, nz = g.nz;
         double acc = 0.0;
-        if (sizeof(T) == 4 && g.vx == 2) {
-            acc = spmv_tile2(g, t, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(q),
-                             reinterpret_cast<const float *>(V.sx), reinterpret_cast<const float *>(V.sy),
-                             reinterpret_cast<const float *>(V.sz));
+        if (sizeof(T) == 4 && g.vx == 4) {
+            acc = spmv_tilev<4>(g, t, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(q),
+                                reinterpret_cast<const float *>(V.sx), reinterpret_cast<const float *>(V.sy),
+                                reinterpret_cast<const float *>(V.sz));
+        } else if (sizeof(T) == 4 && g.vx == 2) {
+            acc = spmv_tilev<2>(g, t, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(q),
+                                reinterpret_cast<const float *>(V.sx), reinterpret_cast<const float *>(V.sy),
+                                reinterpret_cast<const float *>(V.sz));
         } else if (t.i < nx && t.j < ny) {
             const long long plane = (long long)nx * ny;
             const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
@@ -880,7 +972,7 @@ Geo make_geo(int nx, int ny, int nz, int px, int py, int kz, int vx)
     g.px = px;
     g.py = py;
     g.n = (long long)nx * ny * nz;
-    g.vx = (vx == 2 && nx % 2 == 0) ? 2 : 1;
+    g.vx = ((vx == 2 || vx == 4) && nx % vx == 0) ? vx : 1;
     g.tiles_x = (nx + kTileX * g.vx - 1) / (kTileX * g.vx);
     g.tiles_y = (ny + kTileY - 1) / kTileY;
     if (kz <= 0) {

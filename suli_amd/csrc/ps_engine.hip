@@ -742,8 +742,8 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
     double best_us = 1e300;
     int best_kz = g_def.kz, best_vx = 1;
     std::vector<std::pair<int, int>> shapes;
-    for (int vx : {1, 2}) {
-        if (vx == 2 && (!mixed || nx % 2)) continue;
+    for (int vx : {1, 2, 4}) {
+        if (vx > 1 && (!mixed || nx % vx)) continue;
         for (int kz : cands)
             if (kz <= nz || kz == cands.front()) shapes.emplace_back(kz, vx);
     }

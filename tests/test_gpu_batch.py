@@ -137,7 +137,7 @@ def test_autotune_picks_a_depth_and_keeps_bit_identity(solver):
     kz1, vx1, us1 = solver.autotune(cs[1].grid, "mixed")
     kz2, vx2, us2 = solver.autotune(cs[1].grid, "mixed")
     assert (kz1, vx1, us1) == (kz2, vx2, us2)
-    assert 2 <= kz1 <= max(48, cs[1].grid.nz) and vx1 in (1, 2) and us1 > 0
+    assert 2 <= kz1 <= max(48, cs[1].grid.nz) and vx1 in (1, 2, 4) and us1 > 0
     assert solver.autotune(cs[1].grid, "fp64")[1] == 1
     solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False,
                      autotune=True)
@@ -152,17 +152,20 @@ def test_autotune_picks_a_depth_and_keeps_bit_identity(solver):
     solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
 
 
-@pytest.mark.parametrize("kz", [3, 8])
-def test_two_cells_per_lane_vs_oracle(solver, kz):
-    """FP32 tiles with two x cells per lane (even nx; walls, x-periodic,
-    y-periodic, stretched): same tolerance against the oracle, and batch
-    results still equal solo ones."""
-    cs = [cases.random_case(24, 20, 16, seed=61), cases.random_case(70, 33, 19, seed=62, periodic_x=True),
+@pytest.mark.parametrize("kz,vx", [(3, 2), (8, 2), (3, 4), (8, 4)])
+def test_wide_lanes_vs_oracle(solver, kz, vx):
+    """FP32 tiles with two or four x cells per lane (nx a multiple; walls,
+    x-periodic, y-periodic, stretched, Dirichlet lid): same tolerance
+    against the oracle, and batch results still equal solo ones."""
+    nxp = 70 if vx == 2 else 72
+    cs = [cases.random_case(24, 20, 16, seed=61), cases.random_case(nxp, 33, 19, seed=62, periodic_x=True),
           cases.random_case(64, 4, 8, seed=63, periodic_y=True), cases.random_case(40, 24, 20, seed=64,
-                                                                                    stretched=True)]
+                                                                                    stretched=True),
+          cases.config1(seed=65, nx=32, ny=12, nz=10, top_dirichlet=True)]
     cases.make_stretched_compatible(cs[3])
     for c in cs:
-        solver.set_tiles(c.grid, "mixed", kz, 2)
+        solver.set_tiles(c.grid, "mixed", kz, vx)
+        assert solver.autotune(c.grid, "mixed")[:2] == (kz, vx)
     solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True)
     got, sts = solve_batch(solver, cs)
     for i, c in enumerate(cs):
