@@ -200,6 +200,26 @@ int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double 
                           double eps, double rho_f2, double mu_f1, double mu_f2,
                           double *hs, double *rho, double *mu);
 
+/* SuLi restart file (dados/arquivorestart): restart_salva writes the model
+ * state as one list-directed record (SuLi-IC/11_diversos.f90:581-599) that
+ * restart_ini reads back (:500-577).  This reads the pressure step's arrays
+ * out of such a file for a grid of nx*ny*nz cells, in the layouts above
+ * (host memory, Fortran order): any non-NULL pointer receives its array,
+ * NULL ones are skipped.  Host code only; no context needed.  Returns 0, or
+ * SULI_PS_EINVAL with a message in err (when err is non-NULL) for a file
+ * that cannot be opened, holds a malformed value, or ends early.  Null
+ * values ("r*") leave the destination as it was, as Fortran's read does.
+ * The record holds byxf, never allocated by SuLi (0_modules.f90:153); when
+ * hs (after it) is asked for, the file is counted once to size it. */
+typedef struct suli_ps_restart {
+    double *u, *v, *w;          /* face layouts as suli_ps_graddin */
+    double *prd0, *prd1, *prd;  /* (0:nx+1, 0:ny+1, 0:nz+1) */
+    double *rho, *ls_mu, *ls;   /* (nx, ny, nz) */
+    double *hs;                 /* (nx, ny, nz), the Heaviside field */
+} suli_ps_restart;
+int suli_ps_restart_read(const char *path, int32_t nx, int32_t ny, int32_t nz,
+                         suli_ps_restart *out, char *err, size_t errlen);
+
 /* One problem of a batch: its grid, device arrays (layouts as above; solid
  * as for suli_ps_graddin_masked_dev, may be NULL) and, on return, its
  * statistics. */

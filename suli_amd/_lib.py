@@ -31,7 +31,7 @@ EXPORTS = (
     "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
     "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
     "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
-    "suli_ps_heaviside_dev",
+    "suli_ps_heaviside_dev", "suli_ps_restart_read",
 )
 
 
@@ -44,6 +44,11 @@ class Grid(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
         ("rho_f1", C.c_double),
     ]
+
+
+class Restart(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in
+                ("u", "v", "w", "prd0", "prd1", "prd", "rho", "ls_mu", "ls", "hs")]
 
 
 class Opts(C.Structure):
@@ -119,6 +124,8 @@ def load(path: str = LIB_PATH):
     L.suli_ps_pressh_dev.argtypes = [vp, C.POINTER(Grid), vp, vp, C.c_double, vp]
     L.suli_ps_posdin_hydro.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
     L.suli_ps_posdin_hydro_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
+    L.suli_ps_restart_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Restart),
+                                       C.c_char_p, C.c_size_t]
     L.suli_ps_heaviside_dev.argtypes = [vp, C.POINTER(Grid), vp] + [C.c_double] * 4 + [vp] * 3
     L.suli_ps_dev_alloc.restype = vp
     L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]

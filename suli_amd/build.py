@@ -33,6 +33,7 @@ UNITS = {
     "ps_common.hip": [],
     "ps_solver.hip": [],
     "ps_server.hip": [],
+    "ps_restart.cpp": [],
 }
 
 
@@ -44,7 +45,7 @@ def hipcc() -> str:
 
 
 def _compile(src: str, extra) -> str:
-    obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc(), *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
