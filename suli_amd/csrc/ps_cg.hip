@@ -80,6 +80,25 @@ __device__ __forceinline__ Tile tile3(const Geo &g, int tile)
     return t;
 }
 
+// Face coefficients of the scaled operator as the iteration kernels read
+// them: the three face arrays (index f of the face), or, for a constant-
+// density operator, the per-plane tables (every x and y face of plane k
+// holds tx[k], ty[k]; z face kf holds tz[kf]), which store the same values.
+template <typename T>
+struct FaceArrays {
+    const T *sx, *sy, *sz;
+    __device__ __forceinline__ T x(long long f, int) const { return sx[f]; }
+    __device__ __forceinline__ T y(long long f, int) const { return sy[f]; }
+    __device__ __forceinline__ T z(long long f, int) const { return sz[f]; }
+};
+template <typename T>
+struct PlaneTables {
+    const T *tx, *ty, *tz;
+    __device__ __forceinline__ T x(long long, int k) const { return tx[k]; }
+    __device__ __forceinline__ T y(long long, int k) const { return ty[k]; }
+    __device__ __forceinline__ T z(long long, int kf) const { return tz[kf]; }
+};
+
 // Per-workgroup cache of a problem's partial-sum total.
 struct TotalCache {
     int prob = -1;
@@ -103,12 +122,10 @@ struct TotalCache {
 // FIRST (iteration 1: p_1 = r_0, nothing to finish) and DO_E (apply the
 // deferred e -= alpha p) are template parameters so that the plane loop has
 // no branches: every load of a plane is issued before any is waited for.
-template <typename T, bool FIRST, bool DO_E>
+template <typename T, bool FIRST, bool DO_E, class CO>
 __device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, T *__restrict__ e,
                                             const T *__restrict__ p_old, T *__restrict__ p_new,
-                                            const T *__restrict__ r, const T *__restrict__ sx,
-                                            const T *__restrict__ sy, const T *__restrict__ sz, T *__restrict__ q,
-                                            T b, T a_prev)
+                                            const T *__restrict__ r, const CO &co, T *__restrict__ q, T b, T a_prev)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -125,18 +142,18 @@ __device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, T *__re
     T poc = FIRST ? T(0) : p_old[c];
     T pc = FIRST ? r[c] : r[c] + b * poc;
     T pb = t.k0 > 0 ? pn(c - plane) : pc;
-    T szb = sz[c];
+    T szb = co.z(c, t.k0);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         // top neighbour; at the lid the mirror ghost is the cell itself
         const long long up = (k < nz - 1) ? plane : 0;
         const T pot = FIRST ? T(0) : p_old[c + up];
         const T pt = FIRST ? r[c + up] : r[c + up] + b * pot;
-        const T szt = sz[c + plane];
+        const T szt = co.z(c + plane, k + 1);
         const T pw = pn(r0 + iw), pe = pn(r0 + ie);
         const T ps = pn(c + dS), pnn = pn(c + dN);
-        const T sxw = sx[fx], sxe = sx[fx + 1];
-        const T sys = sy[fy], syn = sy[fy + nx];
+        const T sxw = co.x(fx, k), sxe = co.x(fx + 1, k);
+        const T sys = co.y(fy, k), syn = co.y(fy + nx, k);
         const T ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid: ghost = -cell
         const T y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
         q[c] = y;
@@ -158,15 +175,24 @@ template <typename T>
 struct Vecs {
     T *e, *r, *q, *p0, *p1;
     const T *sx, *sy, *sz;
+    const T *tx, *ty, *tz;   // null unless the operator has plane tables
 };
 
 __device__ __forceinline__ Vecs<double> vecs(const ProbDev &pd, double)
 {
-    return {pd.e, pd.r, pd.q, pd.p0, pd.p1, pd.sx, pd.sy, pd.sz};
+    return {pd.e, pd.r, pd.q, pd.p0, pd.p1, pd.sx, pd.sy, pd.sz, pd.tx, pd.ty, pd.tz};
 }
 __device__ __forceinline__ Vecs<float> vecs(const ProbDev &pd, float)
 {
-    return {pd.c32, pd.r32, pd.q32, pd.p32_0, pd.p32_1, pd.sx32, pd.sy32, pd.sz32};
+    return {pd.c32, pd.r32, pd.q32, pd.p32_0, pd.p32_1, pd.sx32, pd.sy32, pd.sz32, pd.tx32, pd.ty32, pd.tz32};
+}
+
+// Calls f with the problem's coefficient source (tables or face arrays).
+template <typename T, class F>
+__device__ __forceinline__ auto with_coefs(const Vecs<T> &V, F &&f)
+{
+    if (V.tx) return f(PlaneTables<T>{V.tx, V.ty, V.tz});
+    return f(FaceArrays<T>{V.sx, V.sy, V.sz});
 }
 
 template <typename T>
@@ -219,13 +245,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
             continue;
         }
         const T b = (T)beta;
-        double acc;
-        if (first)
-            acc = cg_a_tile<T, true, false>(pd.g, tl, V.e, p_old, p_new, V.r, V.sx, V.sy, V.sz, V.q, b, a_prev);
-        else if (do_e)
-            acc = cg_a_tile<T, false, true>(pd.g, tl, V.e, p_old, p_new, V.r, V.sx, V.sy, V.sz, V.q, b, a_prev);
-        else
-            acc = cg_a_tile<T, false, false>(pd.g, tl, V.e, p_old, p_new, V.r, V.sx, V.sy, V.sz, V.q, b, a_prev);
+        const double acc = with_coefs(V, [&](const auto &co) {
+            if (first) return cg_a_tile<T, true, false>(pd.g, tl, V.e, p_old, p_new, V.r, co, V.q, b, a_prev);
+            if (do_e) return cg_a_tile<T, false, true>(pd.g, tl, V.e, p_old, p_new, V.r, co, V.q, b, a_prev);
+            return cg_a_tile<T, false, false>(pd.g, tl, V.e, p_old, p_new, V.r, co, V.q, b, a_prev);
+        });
         const double s = block_sum(acc);
         if (threadIdx.x == 0) pd.P.spmv[tr.tile] = s;
     }
@@ -238,13 +262,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
 // r_{n-1} itself; then as cg_a: p_n = r_{n-1} + beta p_{n-1}, q_n = A p_n,
 // e -= alpha_{n-1} p_{n-1} on the tile's own cells (with UPD).  Writes
 // r_{n-1}, p_n, q_n; partials p_n.q_n, q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}.
-template <bool FIRST, bool UPD>
+template <bool FIRST, bool UPD, class CO>
 __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__restrict__ c32,
                                           const float *__restrict__ r_old, const float *__restrict__ q_old,
                                           const float *__restrict__ p_old, float *__restrict__ r_new,
-                                          float *__restrict__ p_new, float *__restrict__ q_new,
-                                          const float *__restrict__ sx, const float *__restrict__ sy,
-                                          const float *__restrict__ sz, float a, float b, double acc[4])
+                                          float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
+                                          float a, float b, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -261,18 +284,18 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
     float rc = rn(c);
     float pc = FIRST ? rc : rc + b * poc;
     float pb = t.k0 > 0 ? pn(c - plane) : pc;
-    float szb = sz[c];
+    float szb = co.z(c, t.k0);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
         const float pot = FIRST ? 0.0f : p_old[c + up];
         const float rt = rn(c + up);
         const float pt = FIRST ? rt : rt + b * pot;
-        const float szt = sz[c + plane];
+        const float szt = co.z(c + plane, k + 1);
         const float pw = pn(r0 + iw), pe = pn(r0 + ie);
         const float ps = pn(c + dS), pnn = pn(c + dN);
-        const float sxw = sx[fx], sxe = sx[fx + 1];
-        const float sys = sy[fy], syn = sy[fy + nx];
+        const float sxw = co.x(fx, k), sxe = co.x(fx + 1, k);
+        const float sys = co.y(fy, k), syn = co.y(fy + nx, k);
         const float ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid
         const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
         q_new[c] = y;
@@ -321,16 +344,42 @@ __device__ __forceinline__ void stv(float *p, long long x, const float (&v)[V])
         *reinterpret_cast<float4 *>(p + x) = make_float4(v[0], v[1], v[2], v[3]);
 }
 
+// V consecutive y or z face coefficients of a lane (one vector load, or the
+// plane's value repeated).
+template <int V>
+__device__ __forceinline__ void yv(const FaceArrays<float> &co, long long f, int, float (&o)[V])
+{
+    ldv<V>(co.sy, f, o);
+}
+template <int V>
+__device__ __forceinline__ void yv(const PlaneTables<float> &co, long long, int k, float (&o)[V])
+{
+    const float v = co.ty[k];
+#pragma unroll
+    for (int m = 0; m < V; ++m) o[m] = v;
+}
+template <int V>
+__device__ __forceinline__ void zv(const FaceArrays<float> &co, long long f, int, float (&o)[V])
+{
+    ldv<V>(co.sz, f, o);
+}
+template <int V>
+__device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int kf, float (&o)[V])
+{
+    const float v = co.tz[kf];
+#pragma unroll
+    for (int m = 0; m < V; ++m) o[m] = v;
+}
+
 // cg_m_tile with V cells per lane: the same arithmetic per cell (and the
 // same term order), with the lane's inner x neighbours shared in registers
 // and 8- or 16-byte loads and stores.
-template <int V, bool FIRST, bool UPD>
+template <int V, bool FIRST, bool UPD, class CO>
 __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
                                            const float *__restrict__ r_old, const float *__restrict__ q_old,
                                            const float *__restrict__ p_old, float *__restrict__ r_new,
-                                           float *__restrict__ p_new, float *__restrict__ q_new,
-                                           const float *__restrict__ sx, const float *__restrict__ sy,
-                                           const float *__restrict__ sz, float a, float b, double acc[4])
+                                           float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
+                                           float a, float b, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -377,7 +426,7 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 #pragma unroll
         for (int m = 0; m < V; ++m) pb[m] = pc[m];
     }
-    ldv<V>(sz, c, szb);
+    zv<V>(co, c, t.k0, szb);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
@@ -391,14 +440,14 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
         rnv(c + up, rt);
 #pragma unroll
         for (int m = 0; m < V; ++m) pt[m] = FIRST ? rt[m] : rt[m] + b * pot[m];
-        ldv<V>(sz, c + plane, szt);
+        zv<V>(co, c + plane, k + 1, szt);
         const float pw = pn1(r0 + iw), pe = pn1(r0 + ie);
         pnv(c + dS, ps);
         pnv(c + dN, pnn);
 #pragma unroll
-        for (int m = 0; m <= V; ++m) sxv[m] = sx[fx + m];
-        ldv<V>(sy, fy, sys);
-        ldv<V>(sy, fy + nx, syn);
+        for (int m = 0; m <= V; ++m) sxv[m] = co.x(fx + m, k);
+        yv<V>(co, fy, k, sys);
+        yv<V>(co, fy + nx, k, syn);
         const bool lid = g.dtop && k == nz - 1;   // Dirichlet lid
         float y[V];
 #pragma unroll
@@ -442,31 +491,25 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 }
 
 // One tile of the mixed step with V cells per lane (V = 1: cg_m_tile).
-template <int V>
+template <int V, class CO>
 __device__ __forceinline__ void cg_m_step(bool first, bool upd, const ProbDev &pd, const Tile &tl, float *r_o,
-                                          float *q_o, float *p_o, float *r_w, float *p_w, float *q_w, float a,
-                                          float b, double acc[4])
+                                          float *q_o, float *p_o, float *r_w, float *p_w, float *q_w, const CO &co,
+                                          float a, float b, double acc[4])
 {
     if constexpr (V == 1) {
         if (first)
-            cg_m_tile<true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a, b,
-                                   acc);
+            cg_m_tile<true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
         else if (upd)
-            cg_m_tile<false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a, b,
-                                   acc);
+            cg_m_tile<false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
         else
-            cg_m_tile<false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a, b,
-                                    acc);
+            cg_m_tile<false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
     } else {
         if (first)
-            cg_m_tilev<V, true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a,
-                                       b, acc);
+            cg_m_tilev<V, true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
         else if (upd)
-            cg_m_tilev<V, false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32, a,
-                                       b, acc);
+            cg_m_tilev<V, false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
         else
-            cg_m_tilev<V, false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, pd.sx32, pd.sy32, pd.sz32,
-                                        a, b, acc);
+            cg_m_tilev<V, false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
     }
 }
 
@@ -474,7 +517,8 @@ __device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return p
 __device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
 __device__ __forceinline__ float *pbuf(const ProbDev &pd, int parity) { return parity ? pd.p32_1 : pd.p32_0; }
 
-// The mixed solver's FP32 iteration, one launch per global step.  Scalars
+// The mixed solver's FP32 iteration, one launch per global step (register
+// budget held to four waves per SIMD).  Scalars
 // from the totals of the previous step's partials (slot (n-1) & 1):
 //   alpha_{n-1} = rho_{n-2} / p_{n-1}.q_{n-1}               (9_grad.f90:450-455)
 //   rho_{n-1}   = rho_{n-2} - 2 alpha q.r + alpha^2 q.q      (r.r of r_{n-1})
@@ -483,7 +527,7 @@ __device__ __forceinline__ float *pbuf(const ProbDev &pd, int parity) { return p
 // spans one iteration.  After a residual replacement rho_{n-1} is the
 // replaced residual's r.r and nothing of iteration n-1 is left to finish.
 // The stop test is the FP64 one at replacements; here only the cap.
-__global__ __launch_bounds__(kThreads) void k_cg_m(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
+__global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
                                                    int ntab, StepRef sref, int *ran)
 {
     const Run run = my_run(ntab);
@@ -541,12 +585,15 @@ __global__ __launch_bounds__(kThreads) void k_cg_m(const ProbDev *__restrict__ P
         const float b = (float)beta;
         float *r_o = rbuf(pd, o), *q_o = qbuf(pd, o), *p_o = pbuf(pd, o);
         float *r_w = rbuf(pd, w), *p_w = pbuf(pd, w), *q_w = qbuf(pd, w);
-        if (pd.g.vx == 4)
-            cg_m_step<4>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, a, b, acc);
-        else if (pd.g.vx == 2)
-            cg_m_step<2>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, a, b, acc);
-        else
-            cg_m_step<1>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, a, b, acc);
+        with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
+            if (pd.g.vx == 4)
+                cg_m_step<4>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+            else if (pd.g.vx == 2)
+                cg_m_step<2>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+            else
+                cg_m_step<1>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+            return 0;
+        });
         block_sum4(acc);
         if (threadIdx.x == 0) {
             double *m = pd.P.m4 + (size_t)w * 4 * pd.nt3 + 4 * (size_t)tr.tile;
@@ -697,13 +744,11 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
         const Tile t = tile3(g, tr.tile);
         const int nx = g.nx, ny = g.ny, nz = g.nz;
         const double *__restrict__ e = pd.e;
-        const double *__restrict__ sx = pd.sx;
-        const double *__restrict__ sy = pd.sy;
-        const double *__restrict__ sz = pd.sz;
         const double *__restrict__ b = pd.b;
         // the replaced residual goes where the next cg_m reads r_{n}
         float *__restrict__ r32 = WRITE32 ? rbuf(pd, (step - pd.n0 + 1) & 1) : nullptr;
         double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
+        with_coefs(vecs(pd, 0.0), [&](const auto &co) {
         for (int sub = 0; sub < g.vx; ++sub) {
             const int ci = t.i + sub;
             if (ci >= nx || t.j >= ny) break;
@@ -719,8 +764,9 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
                 const long long up = (k < nz - 1) ? plane : 0, dn = (k > 0) ? plane : 0;
                 const double ec = e[c];
                 const double et = (g.dtop && k == nz - 1) ? -ec : e[c + up];   // Dirichlet lid
-                const double r = ec - sx[fx + 1] * e[r0 + ie] - sx[fx] * e[r0 + iw] - sy[fy + nx] * e[c + dN] -
-                                 sy[fy] * e[c + dS] - sz[c + plane] * et - sz[c] * e[c - dn] - b[c];
+                const double r = ec - co.x(fx + 1, k) * e[r0 + ie] - co.x(fx, k) * e[r0 + iw] -
+                                 co.y(fy + nx, k) * e[c + dN] - co.y(fy, k) * e[c + dS] - co.z(c + plane, k + 1) * et -
+                                 co.z(c, k) * e[c - dn] - b[c];
                 acc += r * r;
                 if (WRITE32) {
                     const float rf = (float)r;
@@ -733,6 +779,8 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
                 fy += (long long)(ny + 1) * nx;
             }
         }
+        return 0;
+        });
         const double s = block_sum(acc);
         const double s32 = block_sum(acc32);
         const double sen = block_sum(acc_en);
@@ -842,10 +890,9 @@ __global__ void k_set_step(int *base, int value)
 }
 
 // Plain FP32 SpMV of one tile with V cells per lane (timing only).
-template <int V>
+template <int V, class CO>
 __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const float *__restrict__ p,
-                                             float *__restrict__ q, const float *__restrict__ sx,
-                                             const float *__restrict__ sy, const float *__restrict__ sz)
+                                             float *__restrict__ q, const CO &co)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     double acc = 0.0;
@@ -864,20 +911,20 @@ __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const 
     else
 #pragma unroll
         for (int m = 0; m < V; ++m) pb[m] = pc[m];
-    ldv<V>(sz, c, szb);
+    zv<V>(co, c, t.k0, szb);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
         float pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1], y[V];
         ldv<V>(p, c + up, pt);
-        ldv<V>(sz, c + plane, szt);
+        zv<V>(co, c + plane, k + 1, szt);
         const float pw = p[r0 + iw], pe = p[r0 + ie];
         ldv<V>(p, c + dS, ps);
         ldv<V>(p, c + dN, pnn);
 #pragma unroll
-        for (int m = 0; m <= V; ++m) sxv[m] = sx[fx + m];
-        ldv<V>(sy, fy, sys);
-        ldv<V>(sy, fy + nx, syn);
+        for (int m = 0; m <= V; ++m) sxv[m] = co.x(fx + m, k);
+        yv<V>(co, fy, k, sys);
+        yv<V>(co, fy + nx, k, syn);
         const bool lid = g.dtop && k == nz - 1;
 #pragma unroll
         for (int m = 0; m < V; ++m) {
@@ -919,15 +966,18 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
         const Tile t = tile3(g, tr.tile);
         const int nx = g.nx, ny = g.ny, nz = g.nz;
         double acc = 0.0;
-        if (sizeof(T) == 4 && g.vx == 4) {
-            acc = spmv_tilev<4>(g, t, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(q),
-                                reinterpret_cast<const float *>(V.sx), reinterpret_cast<const float *>(V.sy),
-                                reinterpret_cast<const float *>(V.sz));
-        } else if (sizeof(T) == 4 && g.vx == 2) {
-            acc = spmv_tilev<2>(g, t, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(q),
-                                reinterpret_cast<const float *>(V.sx), reinterpret_cast<const float *>(V.sy),
-                                reinterpret_cast<const float *>(V.sz));
-        } else if (t.i < nx && t.j < ny) {
+        with_coefs(V, [&](const auto &co) {
+        if constexpr (sizeof(T) == 4) {
+            if (g.vx == 4) {
+                acc = spmv_tilev<4>(g, t, p, q, co);
+                return 0;
+            }
+            if (g.vx == 2) {
+                acc = spmv_tilev<2>(g, t, p, q, co);
+                return 0;
+            }
+        }
+        if (t.i < nx && t.j < ny) {
             const long long plane = (long long)nx * ny;
             const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i, nx, g.px);
             const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
@@ -937,15 +987,15 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
             long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
             T pc = p[c];
             T pb = t.k0 > 0 ? p[c - plane] : pc;
-            T szb = V.sz[c];
+            T szb = co.z(c, t.k0);
             for (int k = t.k0; k < t.k1; ++k) {
                 const long long r0 = c - t.i;
                 const long long up = (k < nz - 1) ? plane : 0;
                 const T pt = p[c + up];
-                const T szt = V.sz[c + plane];
+                const T szt = co.z(c + plane, k + 1);
                 const T ptu = (g.dtop && k == nz - 1) ? -pc : pt;
-                const T y = pc - V.sx[fx + 1] * p[r0 + ie] - V.sx[fx] * p[r0 + iw] - V.sy[fy + nx] * p[c + dN] -
-                            V.sy[fy] * p[c + dS] - szt * ptu - szb * pb;
+                const T y = pc - co.x(fx + 1, k) * p[r0 + ie] - co.x(fx, k) * p[r0 + iw] -
+                            co.y(fy + nx, k) * p[c + dN] - co.y(fy, k) * p[c + dS] - szt * ptu - szb * pb;
                 q[c] = y;
                 acc += (double)pc * (double)y;
                 pb = pc;
@@ -956,6 +1006,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
                 fy += (long long)(ny + 1) * nx;
             }
         }
+        return 0;
+        });
         const double s = block_sum(acc);
         if (threadIdx.x == 0) pd.P.spmv[tr.tile] = s;
     }

@@ -58,6 +58,14 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
         pd.q32b = cv.take<float>(n);
         pd.P.m4 = cv.take<double>(8 * (size_t)nt3);
     }
+    pd.tx = cv.take<double>((size_t)g.nz + 1);
+    pd.ty = cv.take<double>((size_t)g.nz + 1);
+    pd.tz = cv.take<double>((size_t)g.nz + 1);
+    if (mixed) {
+        pd.tx32 = cv.take<float>((size_t)g.nz + 1);
+        pd.ty32 = cv.take<float>((size_t)g.nz + 1);
+        pd.tz32 = cv.take<float>((size_t)g.nz + 1);
+    }
     pd.P.spmv = cv.take<double>(nt3);
     pd.P.upd = cv.take<double>(nt1);
     pd.P.tres = cv.take<double>(3 * (size_t)nt3);
@@ -223,6 +231,12 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     pd = ProbDev{};
     pd.g = sl.g;
     carve(cv, sl.g, o.mixed, nt3, nt1, pd);
+    // constant density (IPA, 9_grad.f90:109-116) and no solid mask: the
+    // iteration kernels read per-plane tables instead of the face arrays
+    if (gr.t_press != 2 || in.solid) {
+        pd.tx = pd.ty = pd.tz = nullptr;
+        pd.tx32 = pd.ty32 = pd.tz32 = nullptr;
+    }
     pd.nt3 = nt3;
     pd.nt1 = nt1;
     pd.mixed = o.mixed ? 1 : 0;
@@ -280,6 +294,10 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     launch_diag_rhs(sl.g, in, sx, sy, sz, const_cast<double *>(pd.d), pd.b, s_);
     launch_scale_faces(sl.g, pd.d, sx, sy, sz, const_cast<float *>(pd.sx32), const_cast<float *>(pd.sy32),
                        const_cast<float *>(pd.sz32), s_);
+    if (pd.tx)
+        launch_face_tables(sl.g, pd.sx, pd.sy, pd.sz, pd.sx32, pd.sy32, pd.sz32, const_cast<double *>(pd.tx),
+                           const_cast<double *>(pd.ty), const_cast<double *>(pd.tz), const_cast<float *>(pd.tx32),
+                           const_cast<float *>(pd.ty32), const_cast<float *>(pd.tz32), s_);
     launch_init(sl.g, in, Op<double>{pd.sx, pd.sy, pd.sz}, pd.d, pd.b, pd.e, pd.r, o.mixed ? nullptr : pd.r,
                 pd.r32, pd.c32, pd.P, nt1, pd.S, s_);
     ECHK(hipGetLastError());
@@ -754,6 +772,8 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         Carver cv{static_cast<char *>(blk)};
         const int nt3 = spmv_tiles(g), nt1 = vec_tiles(g.n);
         carve(cv, g, mixed, nt3, nt1, pd);
+        pd.tx = pd.ty = pd.tz = nullptr;   // tuned on the face-array operator
+        pd.tx32 = pd.ty32 = pd.tz32 = nullptr;
         pd.g = g;
         pd.nt3 = nt3;
         pd.nt1 = nt1;

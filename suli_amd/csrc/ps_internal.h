@@ -109,6 +109,12 @@ struct ProbDev {
     // (r32/r32b, p32_0/p32_1, q32/q32b by iteration parity).
     const float *sx32, *sy32, *sz32;
     float *c32, *r32, *q32, *p32_0, *p32_1, *r32b, *q32b;
+    // constant density (t_press = 2, no solid mask): every face of a plane
+    // carries the same scaled coefficient, so the iteration kernels read
+    // tx[k], ty[k] (x and y faces of plane k) and tz[f] (z face f) instead
+    // of the face arrays; nullptr otherwise
+    const double *tx, *ty, *tz;
+    const float *tx32, *ty32, *tz32;
     Partials P;
     CgScalars *S;
 };
@@ -155,6 +161,11 @@ void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const do
                      double *d, double *b, hipStream_t s);
 void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
                         float *sy32, float *sz32, hipStream_t s);
+// Per-plane coefficient tables of a constant-density operator, copied from
+// its (plane-constant) face arrays; the FP32 ones when sx32 is non-null.
+void launch_face_tables(const Geo &g, const double *sx, const double *sy, const double *sz, const float *sx32,
+                        const float *sy32, const float *sz32, double *tx, double *ty, double *tz, float *tx32,
+                        float *ty32, float *tz32, hipStream_t s);
 // Initial residual and null-space projection on nb0 workgroups; leaves r
 // (r64, or r32 with c32 = 0 for mixed precision) and S ready for iteration 1.
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,

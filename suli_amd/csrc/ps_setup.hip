@@ -606,6 +606,29 @@ __global__ __launch_bounds__(kThreads) void k_heaviside(long long n, const doubl
     }
 }
 
+// Tables of a constant-density operator: face 0 of row 0 of each plane
+// (every face of the plane holds the same value).
+__global__ __launch_bounds__(kThreads) void k_face_tables(Geo g, const double *sx, const double *sy,
+                                                          const double *sz, const float *sx32, const float *sy32,
+                                                          const float *sz32, double *tx, double *ty, double *tz,
+                                                          float *tx32, float *ty32, float *tz32)
+{
+    const long long plane = (long long)g.nx * g.ny;
+    for (int k = threadIdx.x; k <= g.nz; k += kThreads) {
+        const long long fx = (long long)k * g.ny * (g.nx + 1), fy = (long long)k * (g.ny + 1) * g.nx;
+        if (k < g.nz) {
+            tx[k] = sx[fx];
+            ty[k] = sy[fy];
+            if (sx32) {
+                tx32[k] = sx32[fx];
+                ty32[k] = sy32[fy];
+            }
+        }
+        tz[k] = sz[k * plane];
+        if (sz32) tz32[k] = sz32[k * plane];
+    }
+}
+
 int grid_for(long long n)
 {
     long long b = (n + kThreads - 1) / kThreads;
@@ -636,6 +659,14 @@ void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, d
                          (long long)g.nx * g.ny * (g.nz + 1);
     hipLaunchKernelGGL(k_scale_faces, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, d, sx, sy, sz, sx32, sy32,
                        sz32);
+}
+
+void launch_face_tables(const Geo &g, const double *sx, const double *sy, const double *sz, const float *sx32,
+                        const float *sy32, const float *sz32, double *tx, double *ty, double *tz, float *tx32,
+                        float *ty32, float *tz32, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_face_tables, dim3(1), dim3(kThreads), 0, s, g, sx, sy, sz, sx32, sy32, sz32, tx, ty, tz,
+                       tx32, ty32, tz32);
 }
 
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,

@@ -356,3 +356,20 @@ def test_project_null_repairs_inconsistent_rhs():
     p = p1[1:-1, 1:-1, 1:-1]
     gx = lambda a: np.diff(a, axis=0)
     assert np.abs(gx(p) - gx(p_dense)).max() <= 1e-7 * np.abs(gx(p_dense)).max()
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(periodic_x=True), dict(periodic_y=True), dict(stretched=True),
+                                dict(ibm=True)])
+def test_constant_density_coefficients_are_plane_constant(kw):
+    """With t_press = 2 every face uses rho_f1 (9_grad.f90:109-116): all x
+    and y faces of a plane, and all z faces of a level, carry the same
+    scaled coefficient, bit for bit (wall faces included).  The GPU's
+    per-plane tables rely on it."""
+    c = cases.random_case(10, 7, 6, seed=3, **kw)
+    c.grid.t_press = 2
+    shx, shy, shz, d, _, _ = oracle.scaled_operator(c.grid, c.fields)
+    for k in range(c.grid.nz):
+        assert np.all(shx[:, :, k] == shx[0, 0, k]) and np.all(shy[:, :, k] == shy[0, 0, k])
+        assert np.all(d[1:-1, 1:-1, k + 1] == d[1, 1, k + 1])
+    for f in range(c.grid.nz + 1):
+        assert np.all(shz[:, :, f] == shz[0, 0, f])
