@@ -72,6 +72,11 @@ def lib():
         _lib.oracle_scaled_operator.argtypes = [C.POINTER(OracleGrid)] + [_dp] * 10
         _lib.oracle_heaviside_rho.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                               C.c_double, _dp]
+        _lib.oracle_it_graddin.restype = C.c_int
+        _lib.oracle_it_graddin.argtypes = [C.POINTER(OracleGrid), C.c_double, C.c_double] + [_dp] * 9 + \
+            [C.POINTER(OracleStats)]
+        _lib.oracle_it_posdin.restype = None
+        _lib.oracle_it_posdin.argtypes = [C.POINTER(OracleGrid), C.c_double, C.c_double] + [_dp] * 7
         _lib.oracle_heaviside.restype = None
         _lib.oracle_heaviside.argtypes = [C.c_int, _dp] + [C.c_double] * 5 + [_dp] * 3
         _lib.oracle_pressh.restype = None
@@ -174,6 +179,37 @@ def posdin_hydro(case_grid, fields, hs, prd1):
     return u, v, w, prd
 
 
+def _it_grid(g, stop_rule, tol, max_iter):
+    og = OracleGrid()
+    og.nx, og.ny, og.nz = g.nx, g.ny, g.nz
+    og.dx, og.dy, og.dz, og.dt = g.dx, g.dy, 0.0, g.dt
+    og.ccx0, og.ccy0 = g.ccx0, g.ccy0
+    og.stop_rule, og.tol, og.max_iter = stop_rule, tol, max_iter
+    return og
+
+
+def it_graddin(g, f, *, stop_rule=STOP_REFERENCE, tol=1e-8, max_iter=10000):
+    """C restatement of SuLi-IT's graddin (g: cases.ITGrid, f: cases.ITFields).
+    Returns (prd1, stats, rc)."""
+    og = _it_grid(g, stop_rule, tol, max_iter)
+    prd1 = f.prd1.copy(order="F")
+    st = OracleStats()
+    rc = lib().oracle_it_graddin(C.byref(og), g.tetah, g.gz, _ptr(f.u), _ptr(f.v), _ptr(f.w), _ptr(f.dzx),
+                                 _ptr(f.dzy), _ptr(f.dzz), _ptr(f.eta1), _ptr(f.deltah), _ptr(prd1), C.byref(st))
+    return prd1, st, rc
+
+
+def it_posdin(g, f, prd1):
+    """C restatement of SuLi-IT's posdin without dz_it. Returns (u, v, w,
+    eta1, prd)."""
+    og = _it_grid(g, STOP_REFERENCE, 0.0, 0)
+    u, v, w, eta1 = (f.u.copy(order="F"), f.v.copy(order="F"), f.w.copy(order="F"), f.eta1.copy(order="F"))
+    prd = np.zeros_like(prd1, order="F")
+    lib().oracle_it_posdin(C.byref(og), g.tetah, g.gz, _ptr(u), _ptr(v), _ptr(w), _ptr(f.dzz), _ptr(eta1),
+                           _ptr(np.asfortranarray(prd1)), _ptr(prd))
+    return u, v, w, eta1, prd
+
+
 def heaviside(ls, eps, rho1, rho2, mu1, mu2):
     """C restatement of heaviside (t_tens = 0). Returns (hs, rho, mu), each
     shaped like ls (Fortran order)."""
@@ -206,6 +242,10 @@ def ref_lib(variant: str = "ic"):
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built (needs the reference sources)")
         L = C.CDLL(path)
+        if variant == "it":
+            L.suli_ref_it_graddin.argtypes = [_dp] * 9
+            _ref_libs[variant] = L
+            return L
         L.suli_ref_setup.restype = None
         L.suli_ref_setup.argtypes = ([C.c_int] * 3 + [C.c_double] * 4 + [C.c_int] * 6 +
                                      [C.c_double, _dp, _dp])
@@ -294,6 +334,30 @@ def ref_pressh(case_grid, rho, gz, prd1):
     _big_stack_call(_stack_bytes(g), L.suli_ref_pressh, _ptr(np.asfortranarray(rho, dtype=np.float64)),
                     gz, _ptr(out))
     return out
+
+
+def ref_it_available() -> bool:
+    return ref_available("it")
+
+
+def ref_it_info():
+    """(nx, ny, nz, dx, dy, dt, tetah, gz) that SuLi-IT's modules fix at
+    compile time."""
+    L = ref_lib("it")
+    dims = (C.c_int * 3)()
+    par = (C.c_double * 5)()
+    L.suli_ref_it_info(dims, par)
+    return tuple(dims) + tuple(par)
+
+
+def ref_it_graddin(f):
+    """The reference's SuLi-IT graddin on fields of its compiled size.
+    Returns prd1."""
+    L = ref_lib("it")
+    prd1 = f.prd1.copy(order="F")
+    L.suli_ref_it_graddin(_ptr(f.u), _ptr(f.v), _ptr(f.w), _ptr(f.dzx), _ptr(f.dzy), _ptr(f.dzz),
+                          _ptr(f.eta1), _ptr(f.deltah), _ptr(prd1))
+    return prd1
 
 
 def ref_heaviside(case_grid, ls, alpha1, dx1, rho1, rho2, mu1, mu2):

@@ -676,3 +676,212 @@ void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1, doub
 {
     oracle_heaviside(n, ls, eps, rho1, rho2, 0.0, 0.0, NULL, rho, NULL);
 }
+
+/* ------------------------------------------------------------------------
+ * SuLi-IT non-hydrostatic pressure (SuLi-IT/9_grad.f90:10-263, Casulli's
+ * semi-implicit scheme).  Same Jacobi-scaled CG as SuLi-IC, with:
+ *   - face coefficients from per-face layer thicknesses (:44-62):
+ *     sx = tetah dt/dx^2 dzx, sy = tetah dt/dy^2 dzy, sz = tetah dt/dzz,
+ *     and no top face (sz(nz+1) = 0);
+ *   - the continuity right-hand side in fluxes (:65-81): u dzx and v dzy
+ *     differences, w unscaled; the top plane adds the free surface,
+ *     d += 1/(gz tetah dt), q -= (eta1 - deltah)/(tetah dt);
+ *   - the stop rule |r.r| > 1e-11 on the r.r that entered the previous
+ *     iteration (:165-177: the loop tests alfamupr, summed at the top of
+ *     the iteration before), and no cap.
+ * u, dzx: (0:nx+2,0:ny+1,0:nz+1); v, dzy: (0:nx+1,0:ny+2,0:nz+1); w, dzz:
+ * (0:nx+1,0:ny+1,0:nz+2); eta1 (0:nx+1,0:ny+1); deltah (nx,ny); prd1 the
+ * halo box.  Ghost fills: fill_ghosts_s (x, then y, then z). */
+#define IE(i, j) ((size_t)(i) + (size_t)(nx + 2) * (size_t)(j))
+#define IDH(i, j) ((size_t)((i) - 1) + (size_t)nx * (size_t)((j) - 1))
+
+int oracle_it_graddin(const oracle_grid *g, double tetah, double gz, const double *u, const double *v,
+                      const double *w, const double *dzx, const double *dzy, const double *dzz,
+                      const double *eta1, const double *deltah, double *prd1, oracle_stats *st)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    if (nx < 2 || ny < 2 || nz < 2) return -1;
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz,
+                 nfz = (size_t)nx * ny * (nz + 1), ncell = (size_t)nx * ny * nz;
+    int i, j, k;
+    double *sx = malloc(nfx * sizeof(double));
+    double *sy = malloc(nfy * sizeof(double));
+    double *sz = malloc(nfz * sizeof(double));
+    double *q = malloc(ncell * sizeof(double));
+    double *d = calloc(nbox, sizeof(double));
+    double *e = calloc(nbox, sizeof(double));
+    double *r = calloc(nbox, sizeof(double));
+    double *p = calloc(nbox, sizeof(double));
+    double *mp = calloc(nbox, sizeof(double));
+    if (!sx || !sy || !sz || !q || !d || !e || !r || !p || !mp) {
+        free(sx); free(sy); free(sz); free(q); free(d); free(e); free(r); free(p); free(mp);
+        return -2;
+    }
+    const double dt = g->dt, dx = g->dx, dy = g->dy;
+
+    /* face coefficients (:44-62) */
+    const double aux1 = tetah * dt / (dx * dx);
+    const double aux2 = tetah * dt / (dy * dy);
+    for (k = 1; k <= nz; k++) {
+        for (j = 1; j <= ny; j++) {
+            for (i = 1; i <= nx + 1; i++) sx[IFX(i, j, k)] = aux1 * dzx[IU(i, j, k)];
+            for (i = 1; i <= nx; i++) {
+                sy[IFY(i, j, k)] = aux2 * dzy[IV(i, j, k)];
+                sz[IFZ(i, j, k)] = tetah * dt / dzz[IW(i, j, k)];
+            }
+        }
+        for (i = 1; i <= nx; i++) sy[IFY(i, ny + 1, k)] = aux2 * dzy[IV(i, ny + 1, k)];
+    }
+    for (j = 1; j <= ny; j++)
+        for (i = 1; i <= nx; i++) sz[IFZ(i, j, nz + 1)] = 0.;
+
+    /* diagonal and right-hand side (:65-81) */
+    for (k = 1; k <= nz - 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                d[IP(i, j, k)] = sx[IFX(i + 1, j, k)] + sx[IFX(i, j, k)] + sy[IFY(i, j + 1, k)] + sy[IFY(i, j, k)] +
+                                 sz[IFZ(i, j, k + 1)] + sz[IFZ(i, j, k)];
+                q[IC(i, j, k)] = (u[IU(i, j, k)] * dzx[IU(i, j, k)] - u[IU(i + 1, j, k)] * dzx[IU(i + 1, j, k)]) / dx +
+                                 (v[IV(i, j, k)] * dzy[IV(i, j, k)] - v[IV(i, j + 1, k)] * dzy[IV(i, j + 1, k)]) / dy +
+                                 w[IW(i, j, k)] - w[IW(i, j, k + 1)];
+            }
+    k = nz;
+    for (j = 1; j <= ny; j++)
+        for (i = 1; i <= nx; i++) {
+            d[IP(i, j, k)] = sx[IFX(i + 1, j, k)] + sx[IFX(i, j, k)] + sy[IFY(i, j + 1, k)] + sy[IFY(i, j, k)] +
+                             sz[IFZ(i, j, k)] + 1. / (gz * tetah * dt);
+            q[IC(i, j, k)] = (u[IU(i, j, k)] * dzx[IU(i, j, k)] - u[IU(i + 1, j, k)] * dzx[IU(i + 1, j, k)]) / dx +
+                             (v[IV(i, j, k)] * dzy[IV(i, j, k)] - v[IV(i, j + 1, k)] * dzy[IV(i, j + 1, k)]) / dy +
+                             w[IW(i, j, k)] - (eta1[IE(i, j)] - deltah[IDH(i, j)]) / (tetah * dt);
+        }
+    /* ghosts of d (:84-106), e = prd1 sqrt(d) (:108-114) */
+    fill_ghosts_s(g, d, 1.0);
+    for (size_t c = 0; c < nbox; c++) e[c] = prd1[c] * sqrt(d[c]);
+
+    /* scaled faces and the initial residual (:116-140); a face between
+     * cells a and b takes s / sqrt(d_a d_b) from either side */
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double dc = d[IP(i, j, k)];
+                const double api = sx[IFX(i + 1, j, k)] / sqrt(dc * d[IP(i + 1, j, k)]);
+                const double ani = sx[IFX(i, j, k)] / sqrt(dc * d[IP(i - 1, j, k)]);
+                const double apj = sy[IFY(i, j + 1, k)] / sqrt(dc * d[IP(i, j + 1, k)]);
+                const double anj = sy[IFY(i, j, k)] / sqrt(dc * d[IP(i, j - 1, k)]);
+                const double apk = sz[IFZ(i, j, k + 1)] / sqrt(dc * d[IP(i, j, k + 1)]);
+                const double ank = sz[IFZ(i, j, k)] / sqrt(dc * d[IP(i, j, k - 1)]);
+                r[IP(i, j, k)] = e[IP(i, j, k)] - api * e[IP(i + 1, j, k)] - ani * e[IP(i - 1, j, k)] -
+                                 apj * e[IP(i, j + 1, k)] - anj * e[IP(i, j - 1, k)] - apk * e[IP(i, j, k + 1)] -
+                                 ank * e[IP(i, j, k - 1)] - q[IC(i, j, k)] / sqrt(dc);
+            }
+    /* the scaled coefficients replace the unscaled ones for the iteration
+     * (the product d_a d_b is the same in either order) */
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++) {
+                const double da = d[IP(i - 1, j, k)], db = d[IP(i, j, k)];
+                sx[IFX(i, j, k)] = sx[IFX(i, j, k)] / sqrt(db * da);
+            }
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++) {
+                const double da = d[IP(i, j - 1, k)], db = d[IP(i, j, k)];
+                sy[IFY(i, j, k)] = sy[IFY(i, j, k)] / sqrt(db * da);
+            }
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double da = d[IP(i, j, k - 1)], db = d[IP(i, j, k)];
+                sz[IFZ(i, j, k)] = sz[IFZ(i, j, k)] / sqrt(db * da);
+            }
+
+    double rr = dot_interior(g, r, r, 1);
+    double bb = 0.0;
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double bi = q[IC(i, j, k)] / sqrt(d[IP(i, j, k)]);
+                bb = bb + bi * bi;
+            }
+    fill_ghosts_s(g, r, 1.0);
+    memcpy(p, r, nbox * sizeof(double));
+
+    const double thresh = g->stop_rule == ORACLE_STOP_RELATIVE ? g->tol * g->tol * bb : 0.00000000001;
+    const int max_iter = g->max_iter > 0 ? g->max_iter : 10000;
+    double alfamu = rr;   /* the loop tests this (:165) */
+    int cont = 0;
+    if (st) st->rr0 = rr;
+    while (fabs(alfamu) > thresh && cont < max_iter) {
+        cont = cont + 1;
+        /* mp = A p, alfamu = r.r, alfadi = p.mp in one serial sweep (:176-190) */
+        double adi = 0.0;
+        alfamu = 0.0;
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++) {
+                    mp[IP(i, j, k)] = p[IP(i, j, k)] - p[IP(i + 1, j, k)] * sx[IFX(i + 1, j, k)] -
+                                      p[IP(i - 1, j, k)] * sx[IFX(i, j, k)] - p[IP(i, j + 1, k)] * sy[IFY(i, j + 1, k)] -
+                                      p[IP(i, j - 1, k)] * sy[IFY(i, j, k)] - p[IP(i, j, k + 1)] * sz[IFZ(i, j, k + 1)] -
+                                      p[IP(i, j, k - 1)] * sz[IFZ(i, j, k)];
+                    alfamu = alfamu + r[IP(i, j, k)] * r[IP(i, j, k)];
+                    adi = adi + p[IP(i, j, k)] * mp[IP(i, j, k)];
+                }
+        const double alpha = alfamu / adi;
+        /* e -= alpha p, r -= alpha mp, beta (:194-208) */
+        double bmu = 0.0;
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++) {
+                    e[IP(i, j, k)] = e[IP(i, j, k)] - alpha * p[IP(i, j, k)];
+                    r[IP(i, j, k)] = r[IP(i, j, k)] - alpha * mp[IP(i, j, k)];
+                    bmu = bmu + r[IP(i, j, k)] * r[IP(i, j, k)];
+                }
+        const double beta = bmu / alfamu;
+        rr = bmu;
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++) p[IP(i, j, k)] = r[IP(i, j, k)] + beta * p[IP(i, j, k)];
+        /* ghosts of e and p (:220-253) */
+        fill_ghosts_s(g, e, 1.0);
+        fill_ghosts_s(g, p, 1.0);
+    }
+    /* un-scaling on the whole box (:257-263) */
+    for (size_t c = 0; c < nbox; c++) prd1[c] = e[c] / sqrt(d[c]);
+    if (st) {
+        st->iters = cont;
+        st->converged = !(fabs(alfamu) > thresh);
+        st->bb = bb;
+        st->rr = rr;
+        st->thresh = thresh;
+    }
+    free(sx); free(sy); free(sz); free(q); free(d); free(e); free(r); free(p); free(mp);
+    return st && !st->converged ? 1 : (fabs(alfamu) > thresh ? 1 : 0);
+}
+
+/* SuLi-IT posdin (SuLi-IT/10_pos.f90:73-132, without the layer-thickness
+ * update dz_it): eta1 += prd1(:,:,nz)/gz on its whole (0:nx+1,0:ny+1)
+ * box; u, v, w -= tetah dt grad(prd1) with w over dzz; prd = prd1. */
+void oracle_it_posdin(const oracle_grid *g, double tetah, double gz, double *u, double *v, double *w,
+                      const double *dzz, double *eta1, const double *prd1, double *prd)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    int i, j, k;
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) eta1[IE(i, j)] = eta1[IE(i, j)] + prd1[IP(i, j, nz)] / gz;
+    const double aux1 = tetah * g->dt / g->dx;
+    const double aux2 = tetah * g->dt / g->dy;
+    for (k = 1; k <= nz; k++) {
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                u[IU(i, j, k)] = u[IU(i, j, k)] - aux1 * (prd1[IP(i, j, k)] - prd1[IP(i - 1, j, k)]);
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                v[IV(i, j, k)] = v[IV(i, j, k)] - aux2 * (prd1[IP(i, j, k)] - prd1[IP(i, j - 1, k)]);
+    }
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                w[IW(i, j, k)] = w[IW(i, j, k)] - tetah * g->dt / dzz[IW(i, j, k)] * (prd1[IP(i, j, k)] - prd1[IP(i, j, k - 1)]);
+    memcpy(prd, prd1, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
+}

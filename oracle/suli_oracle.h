@@ -106,6 +106,21 @@ void oracle_heaviside(int n, const double *ls, double eps, double rho1, double r
 void oracle_heaviside_rho(int n, const double *ls, double eps, double rho1,
                           double rho2, double *rho);
 
+/* SuLi-IT/9_grad.f90:10-263: the non-hydrostatic pressure of SuLi-IT
+ * (coefficients from per-face layer thicknesses, free-surface top row,
+ * stop rule |r.r| > 1e-11 on the previous iteration's r.r).  g supplies
+ * nx..dt, ccx0, ccy0, stop_rule, tol, max_iter; u and dzx in u's layout,
+ * v and dzy in v's, w and dzz in w's, eta1 (0:nx+1,0:ny+1), deltah
+ * (nx,ny).  Returns 0 when converged, 1 at max_iter. */
+int oracle_it_graddin(const oracle_grid *g, double tetah, double gz, const double *u, const double *v,
+                      const double *w, const double *dzx, const double *dzy, const double *dzz,
+                      const double *eta1, const double *deltah, double *prd1, oracle_stats *st);
+
+/* SuLi-IT/10_pos.f90:73-132 without dz_it: eta1 += prd1(:,:,nz)/gz; u, v,
+ * w corrected in place; prd = prd1. */
+void oracle_it_posdin(const oracle_grid *g, double tetah, double gz, double *u, double *v, double *w,
+                      const double *dzz, double *eta1, const double *prd1, double *prd);
+
 #ifdef __cplusplus
 }
 #endif

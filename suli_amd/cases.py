@@ -354,3 +354,75 @@ def serving_cases(seed: int = 1, count: int = 32, dx: float = 5e-3):
         nx, ny, nz = dims
         out.append(config2(seed=seed * 1000 + i, nx=nx, ny=ny, nz=nz))
     return out
+
+
+# --------------------------------------------------------------------------
+# SuLi-IT (non-hydrostatic, Casulli's semi-implicit scheme): the pressure
+# system of SuLi-IT/9_grad.f90:10-263.
+
+
+@dataclasses.dataclass
+class ITGrid:
+    """Scalars of SuLi-IT's pressure solve (modules disc, cond, parametros)."""
+
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dt: float
+    tetah: float = 0.6          # implicitness (SuLi-IT/0_modules.f90:150)
+    gz: float = 9.80665
+    ccx0: int = 2               # 0 = periodic in x
+    ccy0: int = 2
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+@dataclasses.dataclass
+class ITFields:
+    """SuLi-IT arrays in its layouts: u, dzx (0:nx+2,0:ny+1,0:nz+1); v, dzy
+    (0:nx+1,0:ny+2,0:nz+1); w, dzz (0:nx+1,0:ny+1,0:nz+2); eta1
+    (0:nx+1,0:ny+1); deltah (nx,ny); prd1 (0:nx+1,0:ny+1,0:nz+1)."""
+
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    dzx: np.ndarray
+    dzy: np.ndarray
+    dzz: np.ndarray
+    eta1: np.ndarray
+    deltah: np.ndarray
+    prd1: np.ndarray
+
+    def copy(self) -> "ITFields":
+        return ITFields(*(a.copy(order="F") for a in dataclasses.astuple(self)))
+
+
+def it_case(nx: int, ny: int, nz: int, seed: int, *, dx: float = None, dy: float = None,
+            dt: float = None, dz: float = None, periodic_x: bool = False,
+            periodic_y: bool = False) -> "tuple[ITGrid, ITFields]":
+    """A seeded SuLi-IT state: layers of about dz with +-20 % variation per
+    face, velocities O(0.1), a free surface eta1 O(1e-2) and its deltah."""
+    rng = np.random.default_rng(seed)
+    pi = np.pi
+    dx = 2.0 * pi / 50.0 if dx is None else dx
+    dy = 2.0 * pi / 50.0 if dy is None else dy
+    dt = 0.001 * pi if dt is None else dt
+    dz = pi / 25.0 if dz is None else dz
+    g = ITGrid(nx, ny, nz, dx, dy, dt, ccx0=0 if periodic_x else 2, ccy0=0 if periodic_y else 2)
+    F = lambda *s: np.asfortranarray(np.zeros(s))
+    u, dzx = F(nx + 3, ny + 2, nz + 2), F(nx + 3, ny + 2, nz + 2)
+    v, dzy = F(nx + 2, ny + 3, nz + 2), F(nx + 2, ny + 3, nz + 2)
+    w, dzz = F(nx + 2, ny + 2, nz + 3), F(nx + 2, ny + 2, nz + 3)
+    for a in (dzx, dzy, dzz):
+        a[...] = dz * (1.0 + 0.2 * rng.uniform(-1, 1, a.shape))
+    u[1:nx + 2, 1:ny + 1, 1:nz + 1] = rng.uniform(-0.1, 0.1, (nx + 1, ny, nz))
+    v[1:nx + 1, 1:ny + 2, 1:nz + 1] = rng.uniform(-0.1, 0.1, (nx, ny + 1, nz))
+    w[1:nx + 1, 1:ny + 1, 1:nz + 2] = rng.uniform(-0.02, 0.02, (nx, ny, nz + 1))
+    eta1 = np.asfortranarray(rng.uniform(-0.01, 0.01, (nx + 2, ny + 2)))
+    deltah = np.asfortranarray(rng.uniform(-0.01, 0.01, (nx, ny)))
+    prd1 = F(nx + 2, ny + 2, nz + 2)
+    return g, ITFields(u, v, w, dzx, dzy, dzz, eta1, deltah, prd1)

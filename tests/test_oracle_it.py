@@ -1,0 +1,116 @@
+"""SuLi-IT's non-hydrostatic pressure (SuLi-IT/9_grad.f90:10-263) and
+velocity correction (SuLi-IT/10_pos.f90:73-132) restated in the C oracle.
+
+Pinned by: the reference's own compiled graddin, bit for bit (SuLi-IT fixes
+its grid, 50x50x25, and its walls at compile time); a dense solve of the
+system assembled face by face from first principles at other sizes and
+with periodic sides; and posdin's invariant: the continuity right-hand side
+recomputed from the corrected velocities and surface is the unscaled
+residual q - A p, so it vanishes as the solve converges."""
+import numpy as np
+import pytest
+
+import oracle
+from suli_amd import cases
+
+needs_ref = pytest.mark.skipif(not oracle.ref_it_available(), reason="reference not built")
+
+
+def ref_case(seed):
+    nx, ny, nz, dx, dy, dt, tetah, gz = oracle.ref_it_info()
+    g, f = cases.it_case(nx, ny, nz, seed=seed, dx=dx, dy=dy, dt=dt)
+    g.tetah, g.gz = tetah, gz
+    return g, f
+
+
+@needs_ref
+@pytest.mark.parametrize("seed,warm", [(5, False), (6, True)])
+def test_matches_compiled_reference(seed, warm):
+    g, f = ref_case(seed)
+    if warm:
+        f.prd1[...] = np.random.default_rng(seed).uniform(-1, 1, f.prd1.shape)
+    p_or, st, rc = oracle.it_graddin(g, f)
+    assert rc == 0 and st.iters > 10
+    np.testing.assert_array_equal(p_or, oracle.ref_it_graddin(f))
+
+
+def rhs_it(g, f, eta1=None):
+    """Continuity right-hand side of SuLi-IT/9_grad.f90:65-81 in numpy."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    eta1 = f.eta1 if eta1 is None else eta1
+    I, J, K = np.s_[1:nx + 1], np.s_[1:ny + 1], np.s_[1:nz + 1]
+    ux = f.u * f.dzx
+    vy = f.v * f.dzy
+    q = ((ux[1:nx + 1, J, K] - ux[2:nx + 2, J, K]) / g.dx + (vy[I, 1:ny + 1, K] - vy[I, 2:ny + 2, K]) / g.dy +
+         f.w[I, J, 1:nz + 1] - f.w[I, J, 2:nz + 2])
+    q[:, :, nz - 1] += f.w[I, J, nz + 1] - (eta1[I, J] - f.deltah) / (g.tetah * g.dt)
+    return q
+
+
+def dense_it(g, f):
+    """A p = q assembled face by face: each interior face couples its two
+    cells with s = tetah dt dz_face / dx^2 (x, y) or tetah dt / dzz (z);
+    periodic sides wrap; walls and the bottom carry no flux (their ghost is
+    the cell itself); the top row adds p / (gz tetah dt)."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    idx = lambda i, j, k: i + nx * (j + ny * k)
+    n = nx * ny * nz
+    A = np.zeros((n, n))
+    td = g.tetah * g.dt
+
+    def couple(a, b, s):
+        A[a, a] += s
+        A[b, b] += s
+        A[a, b] -= s
+        A[b, a] -= s
+
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = idx(i, j, k)
+                # x face between i and i+1 (Fortran face i+2), periodic wrap at the end
+                if i + 1 < nx:
+                    couple(c, idx(i + 1, j, k), td / g.dx ** 2 * f.dzx[i + 2, j + 1, k + 1])
+                elif g.ccx0 == 0:
+                    couple(c, idx(0, j, k), td / g.dx ** 2 * f.dzx[nx + 1, j + 1, k + 1])
+                if j + 1 < ny:
+                    couple(c, idx(i, j + 1, k), td / g.dy ** 2 * f.dzy[i + 1, j + 2, k + 1])
+                elif g.ccy0 == 0:
+                    couple(c, idx(i, 0, k), td / g.dy ** 2 * f.dzy[i + 1, ny + 1, k + 1])
+                if k + 1 < nz:
+                    couple(c, idx(i, j, k + 1), td / f.dzz[i + 1, j + 1, k + 2])
+                if k == nz - 1:
+                    A[c, c] += 1.0 / (g.gz * td)
+    return A
+
+
+@pytest.mark.parametrize("px,py", [(False, False), (True, False), (False, True)])
+def test_brute_force_solve(px, py):
+    g, f = cases.it_case(7, 5, 4, seed=11, periodic_x=px, periodic_y=py)
+    if px:   # a periodic side has one face for both ends: make the data agree
+        f.dzx[1, :, :] = f.dzx[g.nx + 1, :, :]
+        f.u[1, :, :] = f.u[g.nx + 1, :, :]
+    if py:
+        f.dzy[:, 1, :] = f.dzy[:, g.ny + 1, :]
+        f.v[:, 1, :] = f.v[:, g.ny + 1, :]
+    p_or, st, rc = oracle.it_graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-14)
+    assert rc == 0
+    A = dense_it(g, f)
+    q = rhs_it(g, f)
+    p = np.linalg.solve(A, q.ravel(order="F"))
+    got = p_or[1:-1, 1:-1, 1:-1].ravel(order="F")
+    assert np.abs(got - p).max() <= 1e-10 * np.abs(p).max()
+
+
+def test_posdin_restores_continuity():
+    g, f = cases.it_case(12, 9, 6, seed=12)
+    p, st, rc = oracle.it_graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-14)
+    assert rc == 0
+    u, v, w, eta1, prd = oracle.it_posdin(g, f, p)
+    after = cases.ITFields(u, v, w, f.dzx, f.dzy, f.dzz, eta1, f.deltah, p)
+    q0 = rhs_it(g, f)
+    q1 = rhs_it(g, after)
+    assert np.abs(q1).max() <= 1e-10 * np.abs(q0).max()
+    np.testing.assert_array_equal(prd, p)
+    # eta1 moved by p(nz)/gz on its whole box
+    np.testing.assert_allclose(eta1 - f.eta1, p[:, :, g.nz] / g.gz, rtol=0, atol=1e-15)
