@@ -200,6 +200,43 @@ int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double 
                           double eps, double rho_f2, double mu_f1, double mu_f2,
                           double *hs, double *rho, double *mu);
 
+/* SuLi-IT's non-hydrostatic pressure (Casulli's semi-implicit scheme):
+ * graddin, SuLi-IT/9_grad.f90:10-263, the same Jacobi-scaled CG with
+ *   sx = tetah dt/dx^2 dzx, sy = tetah dt/dy^2 dzy, sz = tetah dt/dzz on the
+ *   faces (no face above the top plane), the continuity right-hand side in
+ *   fluxes (u dzx, v dzy, w), and on the top plane the free surface:
+ *   d += 1/(gz tetah dt), q -= (eta1 - deltah)/(tetah dt).
+ * Grid: nx..nz, dx, dy, dt, ccx0, ccy0 (dz, t_press, obst_t, ibm_t and
+ * rho_f1 are not read; top_bc must be 0).  tetah and gz are module
+ * `parametros`'s (SuLi-IT/0_modules.f90:150-153).  Layouts: u and dzx
+ * (0:nx+2, 0:ny+1, 0:nz+1), v and dzy (0:nx+1, 0:ny+2, 0:nz+1), w and dzz
+ * (0:nx+1, 0:ny+1, 0:nz+2), eta1 (0:nx+1, 0:ny+1), deltah (nx, ny), prd1
+ * the halo box (in: initial guess; out: the whole box).  The reference stop
+ * rule is SuLi-IT's own: |r.r| > 1e-11 tested on the r.r that entered the
+ * previous iteration (:165-177); the relative rule and precisions are as
+ * for suli_ps_graddin.  The operator has no null space.
+ *
+ * posdin, SuLi-IT/10_pos.f90:73-132 without its layer-thickness update
+ * (dz(:,:,nz) and dz_it() stay with the caller): eta1 += prd1(:,:,nz)/gz over
+ * the whole eta1 box, u, v -= tetah dt/dx (dy) times the pressure
+ * difference, w -= tetah dt/dzz times it; prd = prd1 (prd may be NULL).
+ *
+ * Host pointers; the _dev forms take device pointers. */
+int suli_ps_it_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,
+                       const double *u, const double *v, const double *w, const double *dzx,
+                       const double *dzy, const double *dzz, const double *eta1,
+                       const double *deltah, double *prd1, suli_ps_stats *stats);
+int suli_ps_it_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,
+                           const double *u, const double *v, const double *w, const double *dzx,
+                           const double *dzy, const double *dzz, const double *eta1,
+                           const double *deltah, double *prd1, suli_ps_stats *stats);
+int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, double *u,
+                      double *v, double *w, const double *dzz, double *eta1, const double *prd1,
+                      double *prd);
+int suli_ps_it_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,
+                          double *u, double *v, double *w, const double *dzz, double *eta1,
+                          const double *prd1, double *prd);
+
 /* SuLi restart file (dados/arquivorestart): restart_salva writes the model
  * state as one list-directed record (SuLi-IC/11_diversos.f90:581-599) that
  * restart_ini reads back (:500-577).  This reads the pressure step's arrays

@@ -31,7 +31,8 @@ EXPORTS = (
     "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
     "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
     "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
-    "suli_ps_heaviside_dev", "suli_ps_restart_read",
+    "suli_ps_heaviside_dev", "suli_ps_restart_read", "suli_ps_it_graddin", "suli_ps_it_graddin_dev",
+    "suli_ps_it_posdin", "suli_ps_it_posdin_dev",
 )
 
 
@@ -126,6 +127,12 @@ def load(path: str = LIB_PATH):
     L.suli_ps_posdin_hydro_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
     L.suli_ps_restart_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Restart),
                                        C.c_char_p, C.c_size_t]
+    it_g = [vp, C.POINTER(Grid), C.c_double, C.c_double] + [vp] * 9 + [C.POINTER(Stats)]
+    L.suli_ps_it_graddin.argtypes = it_g
+    L.suli_ps_it_graddin_dev.argtypes = it_g
+    it_p = [vp, C.POINTER(Grid), C.c_double, C.c_double] + [vp] * 7
+    L.suli_ps_it_posdin.argtypes = it_p
+    L.suli_ps_it_posdin_dev.argtypes = it_p
     L.suli_ps_heaviside_dev.argtypes = [vp, C.POINTER(Grid), vp] + [C.c_double] * 4 + [vp] * 3
     L.suli_ps_dev_alloc.restype = vp
     L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]

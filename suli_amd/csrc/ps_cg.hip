@@ -215,7 +215,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
         const double beta = first ? 0.0 : rr_in / S->rr[(n - 1) & 1];   // 9_grad.f90:510-515
         // stop rule (9_grad.f90:398,582): FP64 tests the recurrence's r.r;
         // the mixed solver decides at its FP64 residual replacements
-        const bool conv = !(fabs(rr_in) > S->thresh);
+        const double rr_test = (S->lag_stop && !first) ? S->rr[(n - 1) & 1] : rr_in;
+        const bool conv = !(fabs(rr_test) > S->thresh);
         const bool cap = n - 1 >= S->max_iter;
         const bool stop = cap || (conv && !S->mixed);
         if (tr.tile == 0 && threadIdx.x == 0) {

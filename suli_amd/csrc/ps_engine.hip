@@ -253,12 +253,19 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     CgScalars s0{};
     s0.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
     s0.mixed = pd.mixed;
-    // a Dirichlet lid leaves the operator without a null space
-    s0.project = (o.keep_null || sl.g.dtop) ? 0 : 1;
+    // a Dirichlet lid, or SuLi-IT's free-surface row, leaves the operator
+    // without a null space
+    s0.project = (o.keep_null || sl.g.dtop || in.it) ? 0 : 1;
     s0.rel_rule = o.stop_rule == SULI_PS_STOP_RELATIVE;
     s0.tol = o.tol;
     // threshold of the reference stop rule: 0.0001/(nx*ny*nz) (9_grad.f90:393)
     s0.thresh_ref = 0.0001 / (double)(gr.nx * gr.ny * gr.nz);
+    if (in.it) {
+        // SuLi-IT: |r.r| > 1e-11, tested on the previous iteration's r.r
+        // (SuLi-IT/9_grad.f90:165)
+        s0.thresh_ref = 0.00000000001;
+        s0.lag_stop = 1;
+    }
     s0.done_at = kNotDone;
     s0.skip_e_at = -1;
     s0.next_refresh = kFirstRefresh;

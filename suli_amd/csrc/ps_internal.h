@@ -77,6 +77,8 @@ struct CgScalars {
     double slope;     // d log(r.r) / d iteration between replacements (< 0)
     double rr_after;  // r.r of the replaced residual (mixed), used by the next step
     double beta_after;  // beta of the step after a replacement (mixed)
+    int lag_stop;     // 1: the FP64 stop test reads the r.r that entered the previous
+                      // iteration (SuLi-IT's loop, SuLi-IT/9_grad.f90:165-177)
 };
 
 // Per-tile partial sums of one problem.
@@ -146,6 +148,13 @@ struct SetupIn {
     double rho_f1;
     int ipa;                            // faces use rho_f1 (t_press == 2)
     const uint8_t *solid;               // cells (x fastest, no halo) masked out; nullptr: none
+    // SuLi-IT's non-hydrostatic system (SuLi-IT/9_grad.f90:44-81) when it
+    // is set: coefficients from the layer thicknesses dzx (u layout), dzy
+    // (v layout), dz3 (w layout), a free-surface row on the top plane from
+    // eta1 (0:nx+1,0:ny+1) and deltah (nx,ny)
+    int it;
+    const double *dzx, *dzy, *dz3, *eta1, *deltah;
+    double tetah, gz;
 };
 
 // Operator storage (scaled face coefficients, x-face array (nx+1)*ny*nz,
@@ -181,6 +190,9 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
 void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,
                    hipStream_t s);
 // rho (and hs, mu when non-null) from the level set, heaviside with t_tens = 0.
+// SuLi-IT posdin without dz_it (SuLi-IT/10_pos.f90:73-132).
+void launch_it_posdin(const Geo &g, double dx, double dy, double dt, double tetah, double gz, const double *prd1,
+                      const double *dz3, double *eta1, double *u, double *v, double *w, hipStream_t s);
 void launch_heaviside(long long n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
                       double *hs, double *rho, double *mu, hipStream_t s);
 void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, double *u, double *v, double *w,

@@ -111,6 +111,11 @@ __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, doub
             const int f = (int)(t % (g.nx + 1));
             const long long jk = t / (g.nx + 1);
             const int j = (int)(jk % g.ny), k = (int)(jk / g.ny);
+            if (in.it) {
+                // SuLi-IT: tetah dt/dx^2 dzx (SuLi-IT/9_grad.f90:44-50)
+                sx[t] = in.aux1 * in.dzx[(f + 1) + (long long)(g.nx + 3) * ((j + 1) + (long long)(g.ny + 2) * (k + 1))];
+                continue;
+            }
             const double rf = in.ipa ? in.rho_f1 : rho_face_x(g, in, f, j, k);
             sx[t] = closed_x(g, in, f, j, k) ? 0.0 : in.aux1 / rf;
         } else if (t < nfx + nfy) {
@@ -118,6 +123,10 @@ __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, doub
             const int i = (int)(u % g.nx);
             const long long jk = u / g.nx;
             const int f = (int)(jk % (g.ny + 1)), k = (int)(jk / (g.ny + 1));
+            if (in.it) {
+                sy[u] = in.aux2 * in.dzy[(i + 1) + (long long)(g.nx + 2) * ((f + 1) + (long long)(g.ny + 3) * (k + 1))];
+                continue;
+            }
             const double rf = in.ipa ? in.rho_f1 : rho_face_y(g, in, i, f, k);
             sy[u] = closed_y(g, in, i, f, k) ? 0.0 : in.aux2 / rf;
         } else {
@@ -125,6 +134,13 @@ __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, doub
             const int i = (int)(u % g.nx);
             const long long jk = u / g.nx;
             const int j = (int)(jk % g.ny), f = (int)(jk / g.ny);
+            if (in.it) {
+                // tetah dt/dzz; no face above the top plane (:51-60)
+                sz[u] = f == g.nz ? 0.
+                                  : in.tetah * in.dt /
+                                        in.dz3[(i + 1) + (long long)(g.nx + 2) * ((j + 1) + (long long)(g.ny + 2) * (f + 1))];
+                continue;
+            }
             const double rf = in.ipa ? in.rho_f1 : rho_face_z(g, in, i, j, f);
             if (closed_z(g, in, i, j, f)) {
                 sz[u] = 0.0;
@@ -180,6 +196,23 @@ __global__ __launch_bounds__(kThreads) void k_diag_rhs(Geo g, SetupIn in, const 
             if (closed_y(g, in, i, j + 1, k)) v1 = 0.0;
             if (closed_z(g, in, i, j, k)) w0 = 0.0;
             if (closed_z(g, in, i, j, k + 1)) w1 = 0.0;
+        }
+        if (in.it) {
+            // SuLi-IT continuity in fluxes (SuLi-IT/9_grad.f90:65-81): the
+            // top plane adds the free surface to the diagonal and the rhs
+            const double ax = (u0 * in.dzx[iu] - u1 * in.dzx[iu + 1]) / in.dx;
+            const double ay = (v0 * in.dzy[iv] - v1 * in.dzy[iv + (nx + 2)]) / in.dy;
+            double q, dt = dd;
+            if (k < g.nz - 1) {
+                q = ax + ay + w0 - w1;
+            } else {
+                q = ax + ay + w0 - (in.eta1[I + (long long)(nx + 2) * J] - in.deltah[i + (long long)nx * j]) /
+                                       (in.tetah * in.dt);
+                dt = dd + 1. / (in.gz * in.tetah * in.dt);
+            }
+            d[c] = dt;
+            b[c] = q / sqrt(dt);
+            continue;
         }
         const double dzk = in.dzz ? in.dzz[K] : in.dzu;
         const double q = (u0 - u1) / in.dx + (v0 - v1) / in.dy + (w0 - w1) / dzk;
@@ -629,6 +662,53 @@ __global__ __launch_bounds__(kThreads) void k_face_tables(Geo g, const double *s
     }
 }
 
+// SuLi-IT posdin (SuLi-IT/10_pos.f90:73-132) without dz_it, which updates
+// the caller's layer thicknesses: eta1 += prd1(:,:,nz)/gz over its whole
+// (0:nx+1,0:ny+1) box; u, v -= tetah dt/dx (dy) times the pressure
+// difference; w -= tetah dt/dzz times it, k = 1..nz+1.
+__global__ __launch_bounds__(kThreads) void k_it_posdin(Geo g, double dx, double dy, double dt, double tetah,
+                                                        double gz, const double *prd1, const double *dz3,
+                                                        double *eta1, double *u, double *v, double *w)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long nfx = (long long)(nx + 1) * ny * nz;
+    const long long nfy = (long long)nx * (ny + 1) * nz;
+    const long long nfz = (long long)nx * ny * (nz + 1);
+    const long long neta = (long long)(nx + 2) * (ny + 2);
+    const long long stride = (long long)gridDim.x * kThreads;
+    const double aux1 = tetah * dt / dx, aux2 = tetah * dt / dy;
+    const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz + neta; t += stride) {
+        if (t < nfx) {
+            const int f = (int)(t % (nx + 1));
+            const long long jk = t / (nx + 1);
+            const int j = (int)(jk % ny), k = (int)(jk / ny);
+            const long long ip = (f + 1) + pb * (j + 1) + pbb * (k + 1);
+            const long long iu = (f + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(ny + 2) * (k + 1));
+            u[iu] = u[iu] - aux1 * (prd1[ip] - prd1[ip - 1]);
+        } else if (t < nfx + nfy) {
+            const long long s = t - nfx;
+            const int i = (int)(s % nx);
+            const long long jk = s / nx;
+            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
+            const long long ip = (i + 1) + pb * (f + 1) + pbb * (k + 1);
+            const long long iv = (i + 1) + (long long)(nx + 2) * ((f + 1) + (long long)(ny + 3) * (k + 1));
+            v[iv] = v[iv] - aux2 * (prd1[ip] - prd1[ip - pb]);
+        } else if (t < nfx + nfy + nfz) {
+            const long long s = t - nfx - nfy;
+            const int i = (int)(s % nx);
+            const long long jk = s / nx;
+            const int j = (int)(jk % ny), f = (int)(jk / ny);
+            const long long ip = (i + 1) + pb * (j + 1) + pbb * (f + 1);
+            const long long iw = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (f + 1));
+            w[iw] = w[iw] - tetah * dt / dz3[iw] * (prd1[ip] - prd1[ip - pbb]);
+        } else {
+            const long long e = t - nfx - nfy - nfz;   // eta1 box index I + (nx+2) J
+            eta1[e] = eta1[e] + prd1[e + pbb * nz] / gz;
+        }
+    }
+}
+
 int grid_for(long long n)
 {
     long long b = (n + kThreads - 1) / kThreads;
@@ -698,6 +778,15 @@ void launch_heaviside(long long n, const double *ls, double eps, double rho1, do
 {
     hipLaunchKernelGGL(k_heaviside, dim3(grid_for(n)), dim3(kThreads), 0, s, n, ls, eps, rho1, rho2, mu1, mu2, hs,
                        rho, mu);
+}
+
+void launch_it_posdin(const Geo &g, double dx, double dy, double dt, double tetah, double gz, const double *prd1,
+                      const double *dz3, double *eta1, double *u, double *v, double *w, hipStream_t s)
+{
+    const long long n = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
+                        (long long)g.nx * g.ny * (g.nz + 1) + (long long)(g.nx + 2) * (g.ny + 2);
+    hipLaunchKernelGGL(k_it_posdin, dim3(grid_for(n)), dim3(kThreads), 0, s, g, dx, dy, dt, tetah, gz, prd1, dz3,
+                       eta1, u, v, w);
 }
 
 void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,

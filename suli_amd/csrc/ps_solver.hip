@@ -28,6 +28,7 @@ struct Staging {
     double *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *prd1 = nullptr;
     double *obsx = nullptr, *obsy = nullptr, *obsz = nullptr, *dz = nullptr, *dzz = nullptr;
     double *hs = nullptr;
+    double *dzx = nullptr, *dzy = nullptr, *dz3 = nullptr, *eta1 = nullptr, *deltah = nullptr;   // SuLi-IT
 };
 
 }  // namespace
@@ -204,6 +205,97 @@ int run_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double *u, do
     return 0;
 }
 
+// SuLi-IT (SuLi-IT/9_grad.f90:10-263): the grid's sizes, spacings, dt and
+// periodic flags; t_press, the IBM mask, IPA and the Dirichlet lid do not
+// apply.
+int check_it(suli_ps_ctx *ctx, const suli_ps_grid *gr, double tetah, double gz)
+{
+    std::string e;
+    const int rc = validate_shape(gr, e);
+    if (rc) return fail(ctx, rc, "%s", e.c_str());
+    if (!(tetah > 0) || !(gz > 0)) return fail(ctx, SULI_PS_EINVAL, "tetah and gz must be positive");
+    if (gr->top_bc != 0) return fail(ctx, SULI_PS_EINVAL, "SuLi-IT has its own free-surface top row: top_bc must be 0");
+    return 0;
+}
+
+SetupIn make_setup_it(const suli_ps_grid *gr, double tetah, double gz, const double *u, const double *v,
+                      const double *w, const double *dzx, const double *dzy, const double *dz3, const double *eta1,
+                      const double *deltah, const double *prd1)
+{
+    SetupIn in{};
+    in.u = u;
+    in.v = v;
+    in.w = w;
+    in.prd1 = prd1;
+    in.dx = gr->dx;
+    in.dy = gr->dy;
+    in.dt = gr->dt;
+    // tetah dt/dx^2, tetah dt/dy^2 (SuLi-IT/9_grad.f90:44-45)
+    in.aux1 = tetah * gr->dt / (gr->dx * gr->dx);
+    in.aux2 = tetah * gr->dt / (gr->dy * gr->dy);
+    in.it = 1;
+    in.dzx = dzx;
+    in.dzy = dzy;
+    in.dz3 = dz3;
+    in.eta1 = eta1;
+    in.deltah = deltah;
+    in.tetah = tetah;
+    in.gz = gz;
+    return in;
+}
+
+int run_it_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr0, double tetah, double gz, const double *u,
+                       const double *v, const double *w, const double *dzx, const double *dzy, const double *dz3,
+                       const double *eta1, const double *deltah, double *prd1, suli_ps_stats *stats)
+{
+    int rc = check_it(ctx, gr0, tetah, gz);
+    if (rc) return rc;
+    if (!u || !v || !w || !dzx || !dzy || !dz3 || !eta1 || !deltah || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "every SuLi-IT array must be non-NULL");
+    suli_ps_grid gr = *gr0;
+    gr.t_press = 1;   // no IPA tables, no IBM mask: the operator is SuLi-IT's
+    gr.obst_t = 0;
+    release_last(ctx);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    const SetupIn in = make_setup_it(&gr, tetah, gz, u, v, w, dzx, dzy, dz3, eta1, deltah, prd1);
+    int slot = -1;
+    if ((rc = ctx->eng.admit(gr, in, prd1, solve_opts(ctx->opts), 0, &slot)))
+        return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    ctx->last_slots.push_back(slot);
+    if ((rc = ctx->eng.run_all(ctx->opts.timing != 0))) return fail(ctx, rc, "%s", ctx->eng.error().c_str());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    suli_ps_stats st;
+    fill_stats(ctx->eng.slot(slot), &st);
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    st.solve_ms = ms;
+    st.spmv_timed = ctx->eng.spmv_samples();
+    st.spmv_ms = ctx->eng.spmv_ms_mean();
+    if (stats) *stats = st;
+    warn_capped(ctx, st);
+    return st.status;
+}
+
+int run_it_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double tetah, double gz, double *u, double *v,
+                      double *w, const double *dz3, double *eta1, const double *prd1, double *prd)
+{
+    int rc = check_it(ctx, gr, tetah, gz);
+    if (rc) return rc;
+    if (!u || !v || !w || !dz3 || !eta1 || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz, eta1 and prd1 must be non-NULL");
+    const Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
+    launch_it_posdin(g, gr->dx, gr->dy, gr->dt, tetah, gz, prd1, dz3, eta1, u, v, w, ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    if (prd && prd != prd1) {
+        const size_t nbox = (size_t)(gr->nx + 2) * (gr->ny + 2) * (gr->nz + 2);
+        HIPCHK(ctx, hipMemcpyAsync(prd, prd1, nbox * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 // Staging for the host-pointer API, kept while the grid shape is unchanged.
 int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
 {
@@ -215,20 +307,22 @@ int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
         st = Staging{};
     }
     const size_t n = (size_t)nx * ny * nz;
-    const size_t sizes[11] = {(size_t)(nx + 3) * (ny + 2) * (nz + 2), (size_t)(nx + 2) * (ny + 3) * (nz + 2),
-                              (size_t)(nx + 2) * (ny + 2) * (nz + 3), n, (size_t)(nx + 2) * (ny + 2) * (nz + 2),
+    const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                 nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
+    const size_t sizes[16] = {nu, nv, nw, n, (size_t)(nx + 2) * (ny + 2) * (nz + 2),
                               (size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
-                              (size_t)nz + 2, (size_t)nz + 2, n};
-    size_t off[11], total = 0;
-    for (int i = 0; i < 11; ++i) {
+                              (size_t)nz + 2, (size_t)nz + 2, n, nu, nv, nw, (size_t)(nx + 2) * (ny + 2),
+                              (size_t)nx * ny};
+    size_t off[16], total = 0;
+    for (int i = 0; i < 16; ++i) {
         off[i] = total;
         total += (sizes[i] * sizeof(double) + 255) / 256 * 256;
     }
     hipError_t e = hipMalloc(&st.block, total);
     if (e != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
-    double **ptrs[11] = {&st.u,    &st.v,    &st.w,  &st.rho, &st.prd1, &st.obsx,
-                         &st.obsy, &st.obsz, &st.dz, &st.dzz, &st.hs};
-    for (int i = 0; i < 11; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
+    double **ptrs[16] = {&st.u,   &st.v,   &st.w,   &st.rho, &st.prd1, &st.obsx,   &st.obsy,   &st.obsz,
+                         &st.dz,  &st.dzz, &st.hs,  &st.dzx, &st.dzy,  &st.dz3, &st.eta1, &st.deltah};
+    for (int i = 0; i < 16; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
     st.nx = nx;
     st.ny = ny;
     st.nz = nz;
@@ -419,6 +513,85 @@ int suli_ps_pressh(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, c
     if ((rc = run_pressh_dev(ctx, g, st.rho, dz ? st.dz : nullptr, gz, st.prd1))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
+    return 0;
+}
+
+int suli_ps_it_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, const double *u,
+                           const double *v, const double *w, const double *dzx, const double *dzy, const double *dzz,
+                           const double *eta1, const double *deltah, double *prd1, suli_ps_stats *stats)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_it_graddin_dev(ctx, g, tetah, gz, u, v, w, dzx, dzy, dzz, eta1, deltah, prd1, stats);
+}
+
+int suli_ps_it_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, double *u, double *v,
+                          double *w, const double *dzz, double *eta1, const double *prd1, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_it_posdin_dev(ctx, g, tetah, gz, u, v, w, dzz, eta1, prd1, prd);
+}
+
+int suli_ps_it_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, const double *u,
+                       const double *v, const double *w, const double *dzx, const double *dzy, const double *dzz,
+                       const double *eta1, const double *deltah, double *prd1, suli_ps_stats *stats)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_it(ctx, g, tetah, gz);
+    if (rc) return rc;
+    if (!u || !v || !w || !dzx || !dzy || !dzz || !eta1 || !deltah || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "every SuLi-IT array must be non-NULL");
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                 nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.dzx, dzx, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.dzy, dzy, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.dz3, dzz, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.eta1, eta1, sizeof(double) * (nx + 2) * (ny + 2), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.deltah, deltah, sizeof(double) * nx * ny, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    rc = run_it_graddin_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.dzx, st.dzy, st.dz3, st.eta1, st.deltah,
+                            st.prd1, stats);
+    if (rc < 0) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return rc;
+}
+
+int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, double *u, double *v,
+                      double *w, const double *dzz, double *eta1, const double *prd1, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_it(ctx, g, tetah, gz);
+    if (rc) return rc;
+    if (!u || !v || !w || !dzz || !eta1 || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz, eta1 and prd1 must be non-NULL");
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                 nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3), ne = (size_t)(nx + 2) * (ny + 2);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.dz3, dzz, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.eta1, eta1, sizeof(double) * ne, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    if ((rc = run_it_posdin_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.dz3, st.eta1, st.prd1, nullptr))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(eta1, st.eta1, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);
     return 0;
 }
 

@@ -266,6 +266,59 @@ class PressureSolver:
     def dev_free(self, ptr) -> None:
         self.lib.suli_ps_dev_free(self.ctx, ptr)
 
+    # -- SuLi-IT -------------------------------------------------------------
+    @staticmethod
+    def _it_cgrid(g) -> _lib.Grid:
+        cg = _lib.Grid()
+        cg.nx, cg.ny, cg.nz = g.nx, g.ny, g.nz
+        cg.ccx0, cg.ccy0 = g.ccx0, g.ccy0
+        cg.t_press = 1
+        cg.dx, cg.dy, cg.dz, cg.dt = g.dx, g.dy, 0.0, g.dt
+        return cg
+
+    def it_graddin(self, g, f) -> _lib.Stats:
+        """SuLi-IT's pressure solve on host arrays (cases.ITGrid, ITFields);
+        f.prd1 updated in place."""
+        st = _lib.Stats()
+        cg = self._it_cgrid(g)
+        self._check(self.lib.suli_ps_it_graddin(
+            self.ctx, C.byref(cg), g.tetah, g.gz, _host_ptr(f.u), _host_ptr(f.v), _host_ptr(f.w),
+            _host_ptr(f.dzx), _host_ptr(f.dzy), _host_ptr(f.dzz), _host_ptr(f.eta1), _host_ptr(f.deltah),
+            _host_ptr(f.prd1), C.byref(st)))
+        return st
+
+    def it_posdin(self, g, f) -> np.ndarray:
+        """SuLi-IT's velocity correction and eta1 update on host arrays;
+        returns prd."""
+        cg = self._it_cgrid(g)
+        prd = np.zeros_like(f.prd1, order="F")
+        self._check(self.lib.suli_ps_it_posdin(
+            self.ctx, C.byref(cg), g.tetah, g.gz, _host_ptr(f.u), _host_ptr(f.v), _host_ptr(f.w),
+            _host_ptr(f.dzz), _host_ptr(f.eta1), _host_ptr(f.prd1), _host_ptr(prd)))
+        return prd
+
+    def it_upload(self, f) -> dict:
+        """Device copies of ITFields (name -> pointer); free with it_free."""
+        return {name: self.dev_array(getattr(f, name))
+                for name in ("u", "v", "w", "dzx", "dzy", "dzz", "eta1", "deltah", "prd1")}
+
+    def it_free(self, d: dict) -> None:
+        for p in d.values():
+            self.dev_free(p)
+
+    def it_graddin_dev(self, g, d: dict) -> _lib.Stats:
+        st = _lib.Stats()
+        cg = self._it_cgrid(g)
+        self._check(self.lib.suli_ps_it_graddin_dev(
+            self.ctx, C.byref(cg), g.tetah, g.gz, d["u"], d["v"], d["w"], d["dzx"], d["dzy"], d["dzz"],
+            d["eta1"], d["deltah"], d["prd1"], C.byref(st)))
+        return st
+
+    def it_posdin_dev(self, g, d: dict) -> None:
+        cg = self._it_cgrid(g)
+        self._check(self.lib.suli_ps_it_posdin_dev(
+            self.ctx, C.byref(cg), g.tetah, g.gz, d["u"], d["v"], d["w"], d["dzz"], d["eta1"], d["prd1"], None))
+
     def time_spmv(self, precision: str = "fp64", reps: int = 50) -> float:
         ms = C.c_double()
         self._check(self.lib.suli_ps_time_spmv(self.ctx, PRECISIONS[precision], reps, C.byref(ms)))
