@@ -220,6 +220,9 @@ int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double 
  * (dz(:,:,nz) and dz_it() stay with the caller): eta1 += prd1(:,:,nz)/gz over
  * the whole eta1 box, u, v -= tetah dt/dx (dy) times the pressure
  * difference, w -= tetah dt/dzz times it; prd = prd1 (prd may be NULL).
+ * The reference updates eta1, then dz and dzz (dz_it), then corrects w with
+ * the new dzz: a caller doing the same passes eta1 = NULL (its own eta1
+ * update already done) and the updated dzz.
  *
  * Host pointers; the _dev forms take device pointers. */
 int suli_ps_it_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,

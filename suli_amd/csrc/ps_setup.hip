@@ -664,8 +664,9 @@ __global__ __launch_bounds__(kThreads) void k_face_tables(Geo g, const double *s
 
 // SuLi-IT posdin (SuLi-IT/10_pos.f90:73-132) without dz_it, which updates
 // the caller's layer thicknesses: eta1 += prd1(:,:,nz)/gz over its whole
-// (0:nx+1,0:ny+1) box; u, v -= tetah dt/dx (dy) times the pressure
-// difference; w -= tetah dt/dzz times it, k = 1..nz+1.
+// (0:nx+1,0:ny+1) box (skipped when eta1 is null); u, v -= tetah dt/dx
+// (dy) times the pressure difference; w -= tetah dt/dzz times it,
+// k = 1..nz+1.
 __global__ __launch_bounds__(kThreads) void k_it_posdin(Geo g, double dx, double dy, double dt, double tetah,
                                                         double gz, const double *prd1, const double *dz3,
                                                         double *eta1, double *u, double *v, double *w)
@@ -674,7 +675,7 @@ __global__ __launch_bounds__(kThreads) void k_it_posdin(Geo g, double dx, double
     const long long nfx = (long long)(nx + 1) * ny * nz;
     const long long nfy = (long long)nx * (ny + 1) * nz;
     const long long nfz = (long long)nx * ny * (nz + 1);
-    const long long neta = (long long)(nx + 2) * (ny + 2);
+    const long long neta = eta1 ? (long long)(nx + 2) * (ny + 2) : 0;
     const long long stride = (long long)gridDim.x * kThreads;
     const double aux1 = tetah * dt / dx, aux2 = tetah * dt / dy;
     const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
@@ -784,7 +785,7 @@ void launch_it_posdin(const Geo &g, double dx, double dy, double dt, double teta
                       const double *dz3, double *eta1, double *u, double *v, double *w, hipStream_t s)
 {
     const long long n = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
-                        (long long)g.nx * g.ny * (g.nz + 1) + (long long)(g.nx + 2) * (g.ny + 2);
+                        (long long)g.nx * g.ny * (g.nz + 1) + (eta1 ? (long long)(g.nx + 2) * (g.ny + 2) : 0);
     hipLaunchKernelGGL(k_it_posdin, dim3(grid_for(n)), dim3(kThreads), 0, s, g, dx, dy, dt, tetah, gz, prd1, dz3,
                        eta1, u, v, w);
 }

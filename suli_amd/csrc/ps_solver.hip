@@ -283,8 +283,7 @@ int run_it_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double tetah, do
 {
     int rc = check_it(ctx, gr, tetah, gz);
     if (rc) return rc;
-    if (!u || !v || !w || !dz3 || !eta1 || !prd1)
-        return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz, eta1 and prd1 must be non-NULL");
+    if (!u || !v || !w || !dz3 || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz and prd1 must be non-NULL");
     const Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
     launch_it_posdin(g, gr->dx, gr->dy, gr->dt, tetah, gz, prd1, dz3, eta1, u, v, w, ctx->stream);
     HIPCHK(ctx, hipGetLastError());
@@ -570,8 +569,7 @@ int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, dou
     if (!ctx) return SULI_PS_EINVAL;
     int rc = check_it(ctx, g, tetah, gz);
     if (rc) return rc;
-    if (!u || !v || !w || !dzz || !eta1 || !prd1)
-        return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz, eta1 and prd1 must be non-NULL");
+    if (!u || !v || !w || !dzz || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz and prd1 must be non-NULL");
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -583,13 +581,15 @@ int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, dou
     HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(st.dz3, dzz, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.eta1, eta1, sizeof(double) * ne, hipMemcpyHostToDevice, s));
+    if (eta1) HIPCHK(ctx, hipMemcpyAsync(st.eta1, eta1, sizeof(double) * ne, hipMemcpyHostToDevice, s));
     HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    if ((rc = run_it_posdin_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.dz3, st.eta1, st.prd1, nullptr))) return rc;
+    if ((rc = run_it_posdin_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.dz3, eta1 ? st.eta1 : nullptr, st.prd1,
+                                nullptr)))
+        return rc;
     HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(eta1, st.eta1, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+    if (eta1) HIPCHK(ctx, hipMemcpyAsync(eta1, st.eta1, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);
     return 0;

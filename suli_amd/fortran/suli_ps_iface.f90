@@ -42,7 +42,7 @@ module suli_ps_iface
    end type suli_ps_stats
 
    public :: suli_ps_create, suli_ps_destroy, suli_ps_graddin, suli_ps_posdin, suli_ps_last_error
-   public :: suli_ps_pressh, suli_ps_posdin_hydro
+   public :: suli_ps_pressh, suli_ps_posdin_hydro, suli_ps_it_graddin, suli_ps_it_posdin
    public :: suli_ps_error_message
 
    interface
@@ -113,6 +113,31 @@ module suli_ps_iface
          real(c_double), intent(inout), optional :: prd(*)
          integer(c_int) :: rc
       end function suli_ps_posdin_hydro
+
+      ! SuLi-IT's non-hydrostatic pressure and velocity correction.
+      function suli_ps_it_graddin(ctx, g, tetah, gz, u, v, w, dzx, dzy, dzz, eta1, deltah, prd1, st) &
+         bind(C, name="suli_ps_it_graddin") result(rc)
+         import :: c_ptr, c_double, c_int, suli_ps_grid, suli_ps_stats
+         type(c_ptr), value :: ctx
+         type(suli_ps_grid), intent(in) :: g
+         real(c_double), value :: tetah, gz
+         real(c_double), intent(in) :: u(*), v(*), w(*), dzx(*), dzy(*), dzz(*), eta1(*), deltah(*)
+         real(c_double), intent(inout) :: prd1(*)
+         type(suli_ps_stats), intent(out) :: st
+         integer(c_int) :: rc
+      end function suli_ps_it_graddin
+
+      function suli_ps_it_posdin(ctx, g, tetah, gz, u, v, w, dzz, eta1, prd1, prd) &
+         bind(C, name="suli_ps_it_posdin") result(rc)
+         import :: c_ptr, c_double, c_int, suli_ps_grid
+         type(c_ptr), value :: ctx
+         type(suli_ps_grid), intent(in) :: g
+         real(c_double), value :: tetah, gz
+         real(c_double), intent(inout) :: u(*), v(*), w(*)
+         real(c_double), intent(in) :: dzz(*), prd1(*)
+         real(c_double), intent(inout), optional :: eta1(*), prd(*)
+         integer(c_int) :: rc
+      end function suli_ps_it_posdin
    end interface
 
 contains

@@ -51,3 +51,20 @@ def test_fortran_dropin_hydrostatic_matches_reference(variant, args):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert re.search(r"ok =\s*T", r.stdout)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fortran_dropin_it_matches_reference(precision):
+    """SuLi-IT: the drop-in graddin() (suli_ps_graddin_it.F90) inside a
+    program on SuLi-IT's own modules, against the reference's graddin."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "dropin_it")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in check program not built (needs the reference sources)")
+    env = dict(os.environ, SULI_PS_PRECISION=precision)
+    if precision == "mixed":
+        env["SULI_PS_TOL"] = "1e-11"
+    print(f"dropin_it {precision}", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"ok =\s*T", r.stdout)

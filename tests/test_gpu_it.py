@@ -115,3 +115,23 @@ def test_it_argument_errors(solver):
     g.tetah = 0.0
     with pytest.raises(SuliPsError, match="tetah"):
         solver.it_graddin(g, f.copy())
+
+
+def test_it_posdin_without_eta(solver):
+    """eta1 = NULL: the caller updates the surface itself (as SuLi-IT does
+    before dz_it); only the velocities change."""
+    from suli_amd import _lib
+    import ctypes as C
+    g, f = cases.it_case(16, 12, 8, seed=37)
+    p, _, _ = oracle.it_graddin(g, f, stop_rule=oracle.STOP_RELATIVE, tol=1e-10)
+    f.prd1[...] = p
+    u, v, w, _, _ = oracle.it_posdin(g, f, p)
+    f2 = f.copy()
+    cg = solver._it_cgrid(g)
+    solver._check(solver.lib.suli_ps_it_posdin(
+        solver.ctx, C.byref(cg), g.tetah, g.gz, C.c_void_p(f2.u.ctypes.data), C.c_void_p(f2.v.ctypes.data),
+        C.c_void_p(f2.w.ctypes.data), C.c_void_p(f2.dzz.ctypes.data), None, C.c_void_p(f2.prd1.ctypes.data),
+        None))
+    np.testing.assert_array_equal(f2.u, u)
+    np.testing.assert_array_equal(f2.w, w)
+    np.testing.assert_array_equal(f2.eta1, f.eta1)
