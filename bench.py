@@ -13,10 +13,12 @@ the same steps for the comparison.
 Prints one JSON line.  `roofline` is for the dominant kernel, k_cg_m (the
 whole FP32 CG step of the mixed solver in one launch: residual and iterate
 updates of the previous iteration, direction update, SpMV, four dot-product
-partials): algorithmic bytes 11 words/cell (reads r, q, p of the previous
-step, the correction c and the three face-coefficient arrays; writes r, p,
-q and c) divided by its mean launch time, from HIP events on the solver's
-stream around windows of 7 consecutive launches (one window per 16 steps)
+partials): algorithmic bytes 10.5 words/cell on average (every step reads
+r, q, p of the previous step and the three face-coefficient arrays and
+writes r, p and q; every other step also reads the correction c and the
+p of two steps back and writes c) divided by its mean launch time, from HIP
+events on the solver's stream around windows of 8 consecutive launches (one
+window per 16 steps)
 during two directly launched solves.  The FP64 solver's step (k_cg_a +
 k_cg_b, 12 FP64 words/cell) is reported beside it.  `traffic` is the HBM
 bytes per launch from rocprofv3's FETCH_SIZE and WRITE_SIZE counters
@@ -179,7 +181,7 @@ def bench_constant_density(steps):
         st = solver.graddin_dev(g, df)
         w = 4 if precision == "mixed" else 8
         spmv_ms = solver.time_spmv(precision, reps=50)
-        words = 8 if precision == "mixed" else 9
+        words = 7.5 if precision == "mixed" else 9   # mixed: 6 and 9 words in turn
         out[precision] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iters": st.iters,
                           "rel_true": st.rel_true, "step_ms": st.spmv_ms, "words_per_cell_step": words,
                           "step_achieved_gbs": words * w * n / (st.spmv_ms * 1e-3) / 1e9,
@@ -352,7 +354,7 @@ def main():
         print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy}))
         return
 
-    bytes32 = 11 * 4 * n          # k_cg_m: 11 FP32 words per cell
+    bytes32 = 42 * n              # k_cg_m: 10.5 FP32 words per cell on average (9 and 12)
     bytes64 = 12 * 8 * n          # FP64 step (k_cg_a + k_cg_b): 12 FP64 words per cell
     spmv_ms = mixed["spmv_ms"]
     achieved = bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None

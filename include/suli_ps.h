@@ -95,7 +95,7 @@ typedef struct suli_ps_stats {
     double rel_true;       /* FP64 ||b - A e|| / ||b|| recomputed at exit */
     double solve_ms;       /* device time of the solve, setup included */
     double spmv_ms;        /* mean time of one CG step (its launches, SpMV
-                              included) over windows of 7 consecutive steps,
+                              included) over windows of 8 consecutive steps,
                               HIP events on the context stream (opts.timing) */
 } suli_ps_stats;
 

@@ -263,12 +263,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
 // r_{n-1} itself; then as cg_a: p_n = r_{n-1} + beta p_{n-1}, q_n = A p_n,
 // e -= alpha_{n-1} p_{n-1} on the tile's own cells (with UPD).  Writes
 // r_{n-1}, p_n, q_n; partials p_n.q_n, q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}.
-template <bool FIRST, bool UPD, class CO>
+template <bool FIRST, bool UPD, bool E2, class CO>
 __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__restrict__ c32,
                                           const float *__restrict__ r_old, const float *__restrict__ q_old,
                                           const float *__restrict__ p_old, float *__restrict__ r_new,
                                           float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                          float a, float b, double acc[4])
+                                          float a, float a2, float b, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -299,10 +299,12 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         const float sys = co.y(fy, k), syn = co.y(fy + nx, k);
         const float ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid
         const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
+        // p_{n-2}, in the buffer p_n is about to replace (this lane's cell)
+        const float p2 = E2 ? p_new[c] : 0.0f;
         q_new[c] = y;
         p_new[c] = pc;
         r_new[c] = rc;
-        if (UPD) c32[c] = c32[c] - a * poc;   // 9_grad.f90:458-468
+        if (E2) c32[c] = c32[c] - (a2 * p2 + a * poc);   // 9_grad.f90:458-468, two iterations
         acc[0] += (double)pc * (double)y;
         acc[1] += (double)y * (double)rc;
         acc[2] += (double)y * (double)y;
@@ -375,12 +377,12 @@ __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int 
 // cg_m_tile with V cells per lane: the same arithmetic per cell (and the
 // same term order), with the lane's inner x neighbours shared in registers
 // and 8- or 16-byte loads and stores.
-template <int V, bool FIRST, bool UPD, class CO>
+template <int V, bool FIRST, bool UPD, bool E2, class CO>
 __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
                                            const float *__restrict__ r_old, const float *__restrict__ q_old,
                                            const float *__restrict__ p_old, float *__restrict__ r_new,
                                            float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                           float a, float b, double acc[4])
+                                           float a, float a2, float b, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -459,14 +461,16 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
             y[m] = pc[m] - sxv[m + 1] * east - sxv[m] * west - syn[m] * pnn[m] - sys[m] * ps[m] - szt[m] * top -
                    szb[m] * pb[m];
         }
+        float p2[V];
+        if (E2) ldv<V>(p_new, c, p2);   // p_{n-2}, before p_n replaces it
         stv<V>(q_new, c, y);
         stv<V>(p_new, c, pc);
         stv<V>(r_new, c, rc);
-        if (UPD) {
+        if (E2) {
             float cv[V];
             ldv<V>(c32, c, cv);
 #pragma unroll
-            for (int m = 0; m < V; ++m) cv[m] = cv[m] - a * poc[m];   // 9_grad.f90:458-468
+            for (int m = 0; m < V; ++m) cv[m] = cv[m] - (a2 * p2[m] + a * poc[m]);   // 9_grad.f90:458-468
             stv<V>(c32, c, cv);
         }
 #pragma unroll
@@ -492,26 +496,31 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 }
 
 // One tile of the mixed step with V cells per lane (V = 1: cg_m_tile).
-template <int V, class CO>
-__device__ __forceinline__ void cg_m_step(bool first, bool upd, const ProbDev &pd, const Tile &tl, float *r_o,
-                                          float *q_o, float *p_o, float *r_w, float *p_w, float *q_w, const CO &co,
-                                          float a, float b, double acc[4])
+// e2: fold the updates of the two previous iterations into c this step.
+template <int V, bool FIRST, bool UPD, bool E2, class CO>
+__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, const Tile &tl, float *r_o, float *q_o, float *p_o,
+                                              float *r_w, float *p_w, float *q_w, const CO &co, float a, float a2,
+                                              float b, double acc[4])
 {
-    if constexpr (V == 1) {
-        if (first)
-            cg_m_tile<true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-        else if (upd)
-            cg_m_tile<false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-        else
-            cg_m_tile<false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-    } else {
-        if (first)
-            cg_m_tilev<V, true, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-        else if (upd)
-            cg_m_tilev<V, false, true>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-        else
-            cg_m_tilev<V, false, false>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-    }
+    if constexpr (V == 1)
+        cg_m_tile<FIRST, UPD, E2>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+    else
+        cg_m_tilev<V, FIRST, UPD, E2>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+}
+
+template <int V, class CO>
+__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool e2, const ProbDev &pd, const Tile &tl,
+                                          float *r_o, float *q_o, float *p_o, float *r_w, float *p_w, float *q_w,
+                                          const CO &co, float a, float a2, float b, double acc[4])
+{
+    if (first)
+        cg_m_tile_any<V, true, false, false>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+    else if (upd && e2)
+        cg_m_tile_any<V, false, true, true>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+    else if (upd)
+        cg_m_tile_any<V, false, true, false>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+    else
+        cg_m_tile_any<V, false, false, false>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
 }
 
 __device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return parity ? pd.r32b : pd.r32; }
@@ -559,10 +568,16 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
             beta = S->beta_after;
         }
         const bool cap = n - 1 >= S->max_iter;
+        // c takes the updates of iterations n-2 and n-1 together at odd n
+        // (replacements fall after even iterations, so a pair never spans
+        // one); alpha_{n-2} was stored by the previous step
+        const bool e2 = upd && (n & 1);
+        const float a2 = e2 ? (float)S->alph[n & 1] : 0.0f;
         if (tr.tile == 0 && threadIdx.x == 0) {
             S->rr[n & 1] = rho;
             S->iters = n - 1;
             S->alpha = alpha;
+            if (upd) S->alph[(n - 1) & 1] = alpha;
             if (cap) S->done_at = n;
             if (ran && tr.prob == 0 && !cap) *ran = 1;
         }
@@ -570,15 +585,16 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         const float a = (float)alpha;
         const Tile tl = tile3(pd.g, tr.tile);
         if (cap) {
-            // finish iteration n-1 only (the e update; the replacement that
-            // follows recomputes the residual)
+            // finish what c has not taken yet: iteration n-1, and n-2 at odd
+            // n (the replacement that follows recomputes the residual)
             for (int sub = 0; upd && sub < pd.g.vx; ++sub) {
                 const int ci = tl.i + sub;
                 if (ci >= pd.g.nx || tl.j >= pd.g.ny) break;
-                const float *p_old = pbuf(pd, o);
+                const float *p_old = pbuf(pd, o), *p_two = pbuf(pd, w);
                 const long long plane = (long long)pd.g.nx * pd.g.ny;
                 long long c = (long long)tl.k0 * plane + (long long)tl.j * pd.g.nx + ci;
-                for (int k = tl.k0; k < tl.k1; ++k, c += plane) pd.c32[c] = pd.c32[c] - a * p_old[c];
+                for (int k = tl.k0; k < tl.k1; ++k, c += plane)
+                    pd.c32[c] = e2 ? pd.c32[c] - (a2 * p_two[c] + a * p_old[c]) : pd.c32[c] - a * p_old[c];
             }
             continue;
         }
@@ -588,11 +604,11 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         float *r_w = rbuf(pd, w), *p_w = pbuf(pd, w), *q_w = qbuf(pd, w);
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
             if (pd.g.vx == 4)
-                cg_m_step<4>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+                cg_m_step<4>(first, upd, e2, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
             else if (pd.g.vx == 2)
-                cg_m_step<2>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+                cg_m_step<2>(first, upd, e2, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
             else
-                cg_m_step<1>(first, upd, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+                cg_m_step<1>(first, upd, e2, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
             return 0;
         });
         block_sum4(acc);
@@ -671,9 +687,10 @@ __device__ __forceinline__ bool refresh_due(const ProbDev &pd, int step)
 }
 
 // Mixed precision, residual replacement step 1: fold the FP32 correction
-// into the FP64 iterate, e += c - alpha_n p_n (the update of the last
-// iteration n, which its cg_m left to the next step; alpha_n from that
-// step's partials: r_{n-1}.r_{n-1} / p_n.q_n), c = 0.
+// into the FP64 iterate, e += c - (alpha_{n-1} p_{n-1} + alpha_n p_n) (the
+// updates of the last two iterations, which c takes in pairs at the step
+// after the pair; alpha_n from step n's partials: r_{n-1}.r_{n-1} /
+// p_n.q_n), c = 0.
 __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restrict__ P,
                                                         const TileRef *__restrict__ tab, int ntab, StepRef sref)
 {
@@ -692,8 +709,11 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
             total4_of(pd.P.m4 + (size_t)(n & 1) * 4 * pd.nt3, pd.nt3, T);
             cached = tr.prob;
         }
+        // c has not taken iterations n-1 and n yet (n is even at a
+        // replacement): alpha_n from step n's partials, alpha_{n-1} stored
         const float a = (float)(T[3] / T[0]);
-        const float *p_last = pbuf(pd, n & 1);
+        const float a1 = (float)S->alph[(n - 1) & 1];
+        const float *p_last = pbuf(pd, n & 1), *p_prev = pbuf(pd, (n - 1) & 1);
         double *e = pd.e;
         float *c = pd.c32;
         const long long len = pd.g.n;
@@ -703,8 +723,9 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
             Vec<float> cv = reinterpret_cast<const Vec<float> *>(c)[v];
             if (pending) {
                 const Vec<float> pv = reinterpret_cast<const Vec<float> *>(p_last)[v];
+                const Vec<float> pp = reinterpret_cast<const Vec<float> *>(p_prev)[v];
 #pragma unroll
-                for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - a * pv.v[l];
+                for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - (a1 * pp.v[l] + a * pv.v[l]);
             }
             Vec<double> e0 = reinterpret_cast<Vec<double> *>(e)[2 * v];
             Vec<double> e1 = reinterpret_cast<Vec<double> *>(e)[2 * v + 1];
@@ -720,7 +741,7 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
             const long long x = nv * 4 + threadIdx.x;
             if (x < len) {
                 float cv = c[x];
-                if (pending) cv = cv - a * p_last[x];
+                if (pending) cv = cv - (a1 * p_prev[x] + a * p_last[x]);
                 e[x] += (double)cv;
                 c[x] = 0.0f;
             }

@@ -77,6 +77,9 @@ struct CgScalars {
     double slope;     // d log(r.r) / d iteration between replacements (< 0)
     double rr_after;  // r.r of the replaced residual (mixed), used by the next step
     double beta_after;  // beta of the step after a replacement (mixed)
+    double alph[2];   // mixed: alpha_k in slot k & 1, for the correction c, which
+                      // takes the updates of iterations k-1 and k together at the
+                      // step after k (k even)
     int lag_stop;     // 1: the FP64 stop test reads the r.r that entered the previous
                       // iteration (SuLi-IT's loop, SuLi-IT/9_grad.f90:165-177)
 };
