@@ -299,7 +299,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     if (wait_ev) ECHK(hipStreamWaitEvent(s_, wait_ev, 0));
     double *sx = const_cast<double *>(pd.sx), *sy = const_cast<double *>(pd.sy), *sz = const_cast<double *>(pd.sz);
     launch_face_coefs(sl.g, in, sx, sy, sz, s_);
-    launch_diag_rhs(sl.g, in, sx, sy, sz, const_cast<double *>(pd.d), pd.b, s_);
+    launch_diag_rhs(sl.g, in, sx, sy, sz, const_cast<double *>(pd.d), pd.b, pd.e, s_);
     launch_scale_faces(sl.g, pd.d, sx, sy, sz, const_cast<float *>(pd.sx32), const_cast<float *>(pd.sy32),
                        const_cast<float *>(pd.sz32), s_);
     if (pd.tx)

@@ -169,8 +169,9 @@ struct Op {
 
 // ---- setup, per problem (ps_setup.hip) ----
 void launch_face_coefs(const Geo &g, const SetupIn &in, double *sx, double *sy, double *sz, hipStream_t s);
+// Also writes the scaled unknown e = prd1 sqrt(d) of every cell.
 void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
-                     double *d, double *b, hipStream_t s);
+                     double *d, double *b, double *e, hipStream_t s);
 void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
                         float *sy32, float *sz32, hipStream_t s);
 // Per-plane coefficient tables of a constant-density operator, copied from

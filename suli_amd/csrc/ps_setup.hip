@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, doub
 // right-hand side b = q/sqrt(d) with q = -div u (9_grad.f90:163-183,297;
 // SuLi-ICdz/9_grad.f90:166,176; IBM face mask 9_grad.f90:45-88).
 __global__ __launch_bounds__(kThreads) void k_diag_rhs(Geo g, SetupIn in, const double *sx, const double *sy,
-                                                       const double *sz, double *d, double *b)
+                                                       const double *sz, double *d, double *b, double *e)
 {
     const int nx = g.nx, ny = g.ny;
     const long long stride = (long long)gridDim.x * kThreads;
@@ -212,6 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_diag_rhs(Geo g, SetupIn in, const 
             }
             d[c] = dt;
             b[c] = q / sqrt(dt);
+            e[c] = in.prd1[I + (long long)(nx + 2) * (J + (long long)(ny + 2) * K)] * sqrt(dt);
             continue;
         }
         const double dzk = in.dzz ? in.dzz[K] : in.dzu;
@@ -219,8 +220,11 @@ __global__ __launch_bounds__(kThreads) void k_diag_rhs(Geo g, SetupIn in, const 
         if (solid_cell(g, in, i, j, k)) {
             d[c] = 1.0;
             b[c] = 0.0;
+            e[c] = 0.0;
         } else {
             b[c] = q / sqrt(dd);
+            // the scaled unknown e = prd1 sqrt(d) (9_grad.f90:242-252), as e_box0
+            e[c] = in.prd1[I + (long long)(nx + 2) * (J + (long long)(ny + 2) * K)] * sqrt(dd);
         }
     }
 }
@@ -310,35 +314,39 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
         const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
         const long long fz = ((long long)k * ny + j) * nx + i;
         const long long pl = (long long)nx * ny;
-        const double ec = e_box(g, in, d, I, J, K);
-        // neighbours with the caller's ghosts
-        const double eE = e_box(g, in, d, I + 1, J, K), eW = e_box(g, in, d, I - 1, J, K);
-        const double eN = e_box(g, in, d, I, J + 1, K), eS = e_box(g, in, d, I, J - 1, K);
-        const double eT = e_box(g, in, d, I, J, K + 1), eB = e_box(g, in, d, I, J, K - 1);
+        // e of the cells (written with d by k_diag_rhs); ghost positions from
+        // the caller's prd1 ghosts, as the reference's e over the whole box
+        const double ec = e[c];
+        const double eE = i + 1 < nx ? e[c + 1] : e_box(g, in, d, I + 1, J, K);
+        const double eW = i > 0 ? e[c - 1] : e_box(g, in, d, I - 1, J, K);
+        const double eN = j + 1 < ny ? e[c + nx] : e_box(g, in, d, I, J + 1, K);
+        const double eS = j > 0 ? e[c - nx] : e_box(g, in, d, I, J - 1, K);
+        const double eT = k + 1 < g.nz ? e[c + pl] : e_box(g, in, d, I, J, K + 1);
+        const double eB = k > 0 ? e[c - pl] : e_box(g, in, d, I, J, K - 1);
         const double a_in = ec - op.sx[fx + 1] * eE - op.sx[fx] * eW - op.sy[fy + nx] * eN - op.sy[fy] * eS -
                             op.sz[fz + pl] * eT - op.sz[fz] * eB;
         // neighbours with mirrored/periodic ghosts
-        const int iE = box_to_cell(I + 1, nx, g.px) + 1, iW = box_to_cell(I - 1, nx, g.px) + 1;
-        const int jN = box_to_cell(J + 1, ny, g.py) + 1, jS = box_to_cell(J - 1, ny, g.py) + 1;
-        const int kT = box_to_cell(K + 1, g.nz, 0) + 1, kB = box_to_cell(K - 1, g.nz, 0) + 1;
-        const double mE = e_box(g, in, d, iE, J, K), mW = e_box(g, in, d, iW, J, K);
-        const double mN = e_box(g, in, d, I, jN, K), mS = e_box(g, in, d, I, jS, K);
-        const double mT = (g.dtop && K == g.nz) ? -ec : e_box(g, in, d, I, J, kT);
-        const double mB = e_box(g, in, d, I, J, kB);
+        const int iE = box_to_cell(I + 1, nx, g.px), iW = box_to_cell(I - 1, nx, g.px);
+        const int jN = box_to_cell(J + 1, ny, g.py), jS = box_to_cell(J - 1, ny, g.py);
+        const int kT = box_to_cell(K + 1, g.nz, 0), kB = box_to_cell(K - 1, g.nz, 0);
+        const double mE = e[cidx(g, iE, j, k)], mW = e[cidx(g, iW, j, k)];
+        const double mN = e[cidx(g, i, jN, k)], mS = e[cidx(g, i, jS, k)];
+        const double mT = (g.dtop && K == g.nz) ? -ec : e[cidx(g, i, j, kT)];
+        const double mB = e[cidx(g, i, j, kB)];
         const double a_mir = ec - op.sx[fx + 1] * mE - op.sx[fx] * mW - op.sy[fy + nx] * mN - op.sy[fy] * mS -
                              op.sz[fz + pl] * mT - op.sz[fz] * mB;
         const double bc = b[c];
         const double r = a_in - bc;
         b[c] = bc + (a_mir - a_in);
-        e[c] = ec;
         r0[c] = r;
         // null vector: sqrt(d) on fluid cells, 0 on solid ones
         const double dc = solid_cell(g, in, i, j, k) ? 0.0 : d[c];
+        const double sdc = sqrt(dc);
         acc_rr += r * r;
         acc_bb += bc * bc;
-        acc_rn += r * sqrt(dc);
+        acc_rn += r * sdc;
         acc_dd += dc;
-        acc_en += ec * sqrt(dc);
+        acc_en += ec * sdc;
     }
     const double s0 = block_sum(acc_rr), s1 = block_sum(acc_bb), s2 = block_sum(acc_rn), s3 = block_sum(acc_dd);
     const double s4 = block_sum(acc_en);
@@ -728,9 +736,9 @@ void launch_face_coefs(const Geo &g, const SetupIn &in, double *sx, double *sy, 
 }
 
 void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
-                     double *d, double *b, hipStream_t s)
+                     double *d, double *b, double *e, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_diag_rhs, dim3(grid_for(g.n)), dim3(kThreads), 0, s, g, in, sx, sy, sz, d, b);
+    hipLaunchKernelGGL(k_diag_rhs, dim3(grid_for(g.n)), dim3(kThreads), 0, s, g, in, sx, sy, sz, d, b, e);
 }
 
 void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, double *sz, float *sx32,
