@@ -13,10 +13,11 @@ the same steps for the comparison.
 Prints one JSON line.  `roofline` is for the dominant kernel, k_cg_m (the
 whole FP32 CG step of the mixed solver in one launch: residual and iterate
 updates of the previous iteration, direction update, SpMV, four dot-product
-partials): algorithmic bytes 10.5 words/cell on average (every step reads
-r, q, p of the previous step and the three face-coefficient arrays and
-writes r, p and q; every other step also reads the correction c and the
-p of two steps back and writes c) divided by its mean launch time, from HIP
+partials): algorithmic bytes 10.25 words/cell on average (every step
+reads r, q, p of the previous step and the three face-coefficient arrays
+and writes r, p and q; every fourth step also reads the correction c and
+the three older p of the direction ring and writes c) divided by its mean
+launch time, from HIP
 events on the solver's stream around windows of 8 consecutive launches (one
 window per 16 steps)
 during two directly launched solves.  The FP64 solver's step (k_cg_a +
@@ -354,7 +355,7 @@ def main():
         print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy}))
         return
 
-    bytes32 = 42 * n              # k_cg_m: 10.5 FP32 words per cell on average (9 and 12)
+    bytes32 = 41 * n              # k_cg_m: 10.25 FP32 words per cell on average (9, and 14 every 4th step)
     bytes64 = 12 * 8 * n          # FP64 step (k_cg_a + k_cg_b): 12 FP64 words per cell
     spmv_ms = mixed["spmv_ms"]
     achieved = bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None

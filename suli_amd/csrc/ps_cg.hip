@@ -184,7 +184,7 @@ __device__ __forceinline__ Vecs<double> vecs(const ProbDev &pd, double)
 }
 __device__ __forceinline__ Vecs<float> vecs(const ProbDev &pd, float)
 {
-    return {pd.c32, pd.r32, pd.q32, pd.p32_0, pd.p32_1, pd.sx32, pd.sy32, pd.sz32, pd.tx32, pd.ty32, pd.tz32};
+    return {pd.c32, pd.r32, pd.q32, pd.p32, pd.p32 + pd.pslot, pd.sx32, pd.sy32, pd.sz32, pd.tx32, pd.ty32, pd.tz32};
 }
 
 // Calls f with the problem's coefficient source (tables or face arrays).
@@ -261,14 +261,24 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
 //   r_{n-1} = r_{n-2} - alpha_{n-1} q_{n-1}            (9_grad.f90:470-480)
 // else (iteration 1, or the step after a residual replacement) r_old is
 // r_{n-1} itself; then as cg_a: p_n = r_{n-1} + beta p_{n-1}, q_n = A p_n,
-// e -= alpha_{n-1} p_{n-1} on the tile's own cells (with UPD).  Writes
-// r_{n-1}, p_n, q_n; partials p_n.q_n, q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}.
-template <bool FIRST, bool UPD, bool E2, class CO>
+// with FOLD, c -= alpha_j p_j for the kFold iterations j = n-kFold .. n-1
+// on the tile's own cells.  Writes r_{n-1}, p_n, q_n; partials p_n.q_n,
+// q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}.
+//
+// The fold's operands: p_{n-kFold} is in the ring slot that p_n replaces
+// (read through p_new before the store), p_{n-1} is p_old, and the slots in
+// between are listed in Fold; alphas oldest first.
+struct Fold {
+    const float *p[kFold - 2];   // p_{n-kFold+1} .. p_{n-2}
+    float a[kFold];              // alpha_{n-kFold} .. alpha_{n-1}
+};
+
+template <bool FIRST, bool UPD, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__restrict__ c32,
                                           const float *__restrict__ r_old, const float *__restrict__ q_old,
                                           const float *__restrict__ p_old, float *__restrict__ r_new,
                                           float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                          float a, float a2, float b, double acc[4])
+                                          float a, float b, const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -299,12 +309,19 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         const float sys = co.y(fy, k), syn = co.y(fy + nx, k);
         const float ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid
         const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
-        // p_{n-2}, in the buffer p_n is about to replace (this lane's cell)
-        const float p2 = E2 ? p_new[c] : 0.0f;
+        // p_{n-kFold}, in the slot p_n is about to replace (this lane's cell)
+        const float p_first = FOLD ? p_new[c] : 0.0f;
         q_new[c] = y;
         p_new[c] = pc;
         r_new[c] = rc;
-        if (E2) c32[c] = c32[c] - (a2 * p2 + a * poc);   // 9_grad.f90:458-468, two iterations
+        if (FOLD) {
+            // 9_grad.f90:458-468 for kFold iterations
+            float sum = fd.a[0] * p_first;
+#pragma unroll
+            for (int m = 0; m < kFold - 2; ++m) sum += fd.a[m + 1] * fd.p[m][c];
+            sum += fd.a[kFold - 1] * poc;
+            c32[c] = c32[c] - sum;
+        }
         acc[0] += (double)pc * (double)y;
         acc[1] += (double)y * (double)rc;
         acc[2] += (double)y * (double)y;
@@ -377,12 +394,12 @@ __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int 
 // cg_m_tile with V cells per lane: the same arithmetic per cell (and the
 // same term order), with the lane's inner x neighbours shared in registers
 // and 8- or 16-byte loads and stores.
-template <int V, bool FIRST, bool UPD, bool E2, class CO>
+template <int V, bool FIRST, bool UPD, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
                                            const float *__restrict__ r_old, const float *__restrict__ q_old,
                                            const float *__restrict__ p_old, float *__restrict__ r_new,
                                            float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                           float a, float a2, float b, double acc[4])
+                                           float a, float b, const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -461,16 +478,28 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
             y[m] = pc[m] - sxv[m + 1] * east - sxv[m] * west - syn[m] * pnn[m] - sys[m] * ps[m] - szt[m] * top -
                    szb[m] * pb[m];
         }
-        float p2[V];
-        if (E2) ldv<V>(p_new, c, p2);   // p_{n-2}, before p_n replaces it
+        float sum[V];
+        if (FOLD) {
+            ldv<V>(p_new, c, sum);   // p_{n-kFold}, before p_n replaces it
+#pragma unroll
+            for (int m = 0; m < V; ++m) sum[m] = fd.a[0] * sum[m];
+        }
         stv<V>(q_new, c, y);
         stv<V>(p_new, c, pc);
         stv<V>(r_new, c, rc);
-        if (E2) {
+        if (FOLD) {
+            // 9_grad.f90:458-468 for kFold iterations
+#pragma unroll
+            for (int f = 0; f < kFold - 2; ++f) {
+                float pf[V];
+                ldv<V>(fd.p[f], c, pf);
+#pragma unroll
+                for (int m = 0; m < V; ++m) sum[m] += fd.a[f + 1] * pf[m];
+            }
             float cv[V];
             ldv<V>(c32, c, cv);
 #pragma unroll
-            for (int m = 0; m < V; ++m) cv[m] = cv[m] - (a2 * p2[m] + a * poc[m]);   // 9_grad.f90:458-468
+            for (int m = 0; m < V; ++m) cv[m] = cv[m] - (sum[m] + fd.a[kFold - 1] * poc[m]);
             stv<V>(c32, c, cv);
         }
 #pragma unroll
@@ -495,37 +524,50 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
     }
 }
 
-// One tile of the mixed step with V cells per lane (V = 1: cg_m_tile).
-// e2: fold the updates of the two previous iterations into c this step.
-template <int V, bool FIRST, bool UPD, bool E2, class CO>
-__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, const Tile &tl, float *r_o, float *q_o, float *p_o,
-                                              float *r_w, float *p_w, float *q_w, const CO &co, float a, float a2,
+__device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return parity ? pd.r32b : pd.r32; }
+__device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
+// ring slot of p_k
+__device__ __forceinline__ float *pbuf(const ProbDev &pd, int k) { return pd.p32 + (k & (kFold - 1)) * pd.pslot; }
+__device__ __forceinline__ float alph_of(const CgScalars *S, int k) { return (float)S->alph[k & (kFold - 1)]; }
+
+// One tile of the mixed step n with V cells per lane (V = 1: cg_m_tile).
+// FOLD: c takes the updates of iterations n-kFold .. n-1 this step (their
+// alphas but alpha_{n-1} = a were stored by the previous steps).  The fold's
+// operands are gathered here, inside the FOLD instance only, so that the
+// other instances carry no extra live registers.
+template <int V, bool FIRST, bool UPD, bool FOLD, class CO>
+__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, float *r_o, float *q_o,
+                                              float *p_o, float *r_w, float *p_w, float *q_w, const CO &co, float a,
                                               float b, double acc[4])
 {
+    Fold fd;
+    if constexpr (FOLD) {
+#pragma unroll
+        for (int m = 0; m < kFold - 1; ++m) fd.a[m] = alph_of(pd.S, n - kFold + m);
+        fd.a[kFold - 1] = a;
+#pragma unroll
+        for (int m = 0; m < kFold - 2; ++m) fd.p[m] = pbuf(pd, n - kFold + 1 + m);
+    }
     if constexpr (V == 1)
-        cg_m_tile<FIRST, UPD, E2>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+        cg_m_tile<FIRST, UPD, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, fd, acc);
     else
-        cg_m_tilev<V, FIRST, UPD, E2>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+        cg_m_tilev<V, FIRST, UPD, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, fd, acc);
 }
 
 template <int V, class CO>
-__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool e2, const ProbDev &pd, const Tile &tl,
+__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool fold, const ProbDev &pd, int n, const Tile &tl,
                                           float *r_o, float *q_o, float *p_o, float *r_w, float *p_w, float *q_w,
-                                          const CO &co, float a, float a2, float b, double acc[4])
+                                          const CO &co, float a, float b, double acc[4])
 {
     if (first)
-        cg_m_tile_any<V, true, false, false>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
-    else if (upd && e2)
-        cg_m_tile_any<V, false, true, true>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+        cg_m_tile_any<V, true, false, false>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+    else if (upd && fold)
+        cg_m_tile_any<V, false, true, true>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
     else if (upd)
-        cg_m_tile_any<V, false, true, false>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+        cg_m_tile_any<V, false, true, false>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
     else
-        cg_m_tile_any<V, false, false, false>(pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+        cg_m_tile_any<V, false, false, false>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
 }
-
-__device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return parity ? pd.r32b : pd.r32; }
-__device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
-__device__ __forceinline__ float *pbuf(const ProbDev &pd, int parity) { return parity ? pd.p32_1 : pd.p32_0; }
 
 // The mixed solver's FP32 iteration, one launch per global step (register
 // budget held to four waves per SIMD).  Scalars
@@ -568,47 +610,51 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
             beta = S->beta_after;
         }
         const bool cap = n - 1 >= S->max_iter;
-        // c takes the updates of iterations n-2 and n-1 together at odd n
-        // (replacements fall after even iterations, so a pair never spans
-        // one); alpha_{n-2} was stored by the previous step
-        const bool e2 = upd && (n & 1);
-        const float a2 = e2 ? (float)S->alph[n & 1] : 0.0f;
+        // c takes the updates of iterations n-kFold .. n-1 together at
+        // n = 1 mod kFold (replacements fall after multiples of kFold, so a
+        // group never spans one); their alphas but the last were stored by
+        // the previous steps
+        const bool fold = upd && (n & (kFold - 1)) == 1;
+        const float a = (float)alpha;
         if (tr.tile == 0 && threadIdx.x == 0) {
             S->rr[n & 1] = rho;
             S->iters = n - 1;
             S->alpha = alpha;
-            if (upd) S->alph[(n - 1) & 1] = alpha;
+            if (upd) S->alph[(n - 1) & (kFold - 1)] = alpha;
             if (cap) S->done_at = n;
             if (ran && tr.prob == 0 && !cap) *ran = 1;
         }
         const int o = (n - 1) & 1, w = n & 1;
-        const float a = (float)alpha;
         const Tile tl = tile3(pd.g, tr.tile);
         if (cap) {
-            // finish what c has not taken yet: iteration n-1, and n-2 at odd
-            // n (the replacement that follows recomputes the residual)
+            // finish what c has not taken yet, iterations lo .. n-1 since the
+            // last fold (the replacement that follows recomputes the residual)
+            const int lo = (n - 2) / kFold * kFold + 1;
+            const float *p_old = pbuf(pd, n - 1);
             for (int sub = 0; upd && sub < pd.g.vx; ++sub) {
                 const int ci = tl.i + sub;
                 if (ci >= pd.g.nx || tl.j >= pd.g.ny) break;
-                const float *p_old = pbuf(pd, o), *p_two = pbuf(pd, w);
                 const long long plane = (long long)pd.g.nx * pd.g.ny;
                 long long c = (long long)tl.k0 * plane + (long long)tl.j * pd.g.nx + ci;
-                for (int k = tl.k0; k < tl.k1; ++k, c += plane)
-                    pd.c32[c] = e2 ? pd.c32[c] - (a2 * p_two[c] + a * p_old[c]) : pd.c32[c] - a * p_old[c];
+                for (int k = tl.k0; k < tl.k1; ++k, c += plane) {
+                    float sum = 0.0f;
+                    for (int j = lo; j < n - 1; ++j) sum += alph_of(S, j) * pbuf(pd, j)[c];
+                    pd.c32[c] = pd.c32[c] - (sum + a * p_old[c]);
+                }
             }
             continue;
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const float b = (float)beta;
-        float *r_o = rbuf(pd, o), *q_o = qbuf(pd, o), *p_o = pbuf(pd, o);
-        float *r_w = rbuf(pd, w), *p_w = pbuf(pd, w), *q_w = qbuf(pd, w);
+        float *r_o = rbuf(pd, o), *q_o = qbuf(pd, o), *p_o = pbuf(pd, n - 1);
+        float *r_w = rbuf(pd, w), *p_w = pbuf(pd, n), *q_w = qbuf(pd, w);
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
             if (pd.g.vx == 4)
-                cg_m_step<4>(first, upd, e2, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+                cg_m_step<4>(first, upd, fold, pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
             else if (pd.g.vx == 2)
-                cg_m_step<2>(first, upd, e2, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+                cg_m_step<2>(first, upd, fold, pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
             else
-                cg_m_step<1>(first, upd, e2, pd, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, a2, b, acc);
+                cg_m_step<1>(first, upd, fold, pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
             return 0;
         });
         block_sum4(acc);
@@ -687,9 +733,9 @@ __device__ __forceinline__ bool refresh_due(const ProbDev &pd, int step)
 }
 
 // Mixed precision, residual replacement step 1: fold the FP32 correction
-// into the FP64 iterate, e += c - (alpha_{n-1} p_{n-1} + alpha_n p_n) (the
-// updates of the last two iterations, which c takes in pairs at the step
-// after the pair; alpha_n from step n's partials: r_{n-1}.r_{n-1} /
+// into the FP64 iterate, e += c - sum of alpha_j p_j over j = n-kFold+1 .. n
+// (the updates of the last kFold iterations, which c takes as a group at the
+// step after the group; alpha_n from step n's partials: r_{n-1}.r_{n-1} /
 // p_n.q_n), c = 0.
 __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restrict__ P,
                                                         const TileRef *__restrict__ tab, int ntab, StepRef sref)
@@ -709,11 +755,18 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
             total4_of(pd.P.m4 + (size_t)(n & 1) * 4 * pd.nt3, pd.nt3, T);
             cached = tr.prob;
         }
-        // c has not taken iterations n-1 and n yet (n is even at a
-        // replacement): alpha_n from step n's partials, alpha_{n-1} stored
-        const float a = (float)(T[3] / T[0]);
-        const float a1 = (float)S->alph[(n - 1) & 1];
-        const float *p_last = pbuf(pd, n & 1), *p_prev = pbuf(pd, (n - 1) & 1);
+        // c has not taken iterations n-kFold+1 .. n yet (n is a multiple of
+        // kFold at a replacement): alpha_n from step n's partials, the others
+        // stored (p_n was formed by step n; no later step has run)
+        float fa[kFold];
+        const float *fp[kFold];
+#pragma unroll
+        for (int m = 0; m < kFold - 1; ++m) {
+            fa[m] = alph_of(S, n - kFold + 1 + m);
+            fp[m] = pbuf(pd, n - kFold + 1 + m);
+        }
+        fa[kFold - 1] = (float)(T[3] / T[0]);
+        fp[kFold - 1] = pbuf(pd, n);
         double *e = pd.e;
         float *c = pd.c32;
         const long long len = pd.g.n;
@@ -722,10 +775,15 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
         for (long long v = (long long)tr.tile * kThreads + threadIdx.x; v < nv; v += stride) {
             Vec<float> cv = reinterpret_cast<const Vec<float> *>(c)[v];
             if (pending) {
-                const Vec<float> pv = reinterpret_cast<const Vec<float> *>(p_last)[v];
-                const Vec<float> pp = reinterpret_cast<const Vec<float> *>(p_prev)[v];
+                float sum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - (a1 * pp.v[l] + a * pv.v[l]);
+                for (int m = 0; m < kFold; ++m) {
+                    const Vec<float> pv = reinterpret_cast<const Vec<float> *>(fp[m])[v];
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) sum[l] += fa[m] * pv.v[l];
+                }
+#pragma unroll
+                for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - sum[l];
             }
             Vec<double> e0 = reinterpret_cast<Vec<double> *>(e)[2 * v];
             Vec<double> e1 = reinterpret_cast<Vec<double> *>(e)[2 * v + 1];
@@ -741,7 +799,11 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
             const long long x = nv * 4 + threadIdx.x;
             if (x < len) {
                 float cv = c[x];
-                if (pending) cv = cv - (a1 * p_prev[x] + a * p_last[x]);
+                if (pending) {
+                    float sum = 0.0f;
+                    for (int m = 0; m < kFold; ++m) sum += fa[m] * fp[m][x];
+                    cv = cv - sum;
+                }
                 e[x] += (double)cv;
                 c[x] = 0.0f;
             }

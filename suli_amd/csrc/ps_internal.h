@@ -22,6 +22,17 @@ constexpr int kNotDone = 0x7fffffff;
 constexpr int kRefreshGrain = 8;
 constexpr int kFirstRefresh = 32;
 constexpr int kMaxRefreshGap = 128;
+// Mixed precision: the FP32 correction c takes the updates alpha_j p_j of
+// kFold iterations at once (at iterations n = 1 mod kFold, from a ring of
+// kFold direction vectors), so that a replacement, which falls after an
+// iteration that is a multiple of kFold, never splits a group.  The step
+// then moves 9 + (kFold + 1) / kFold words per cell on average.  Measured
+// per mixed solve at 256x128x96: 26.29 ms folding pairs, 26.05 ms folding
+// fours; folding eights ran the step kernel out of registers (the fold's
+// loads were issued one at a time) and took 33.5 ms.
+constexpr int kFold = 4;
+static_assert(kRefreshGrain % kFold == 0 && kFirstRefresh % kFold == 0, "replacements must end fold groups");
+static_assert((kFold & (kFold - 1)) == 0, "kFold is a power of two");
 
 // Grid geometry as the kernels see it.  Cells are 0-based (i, j, k) with x
 // fastest; a cell vector holds nx*ny*nz values with no halo.  Ghost values
@@ -77,9 +88,7 @@ struct CgScalars {
     double slope;     // d log(r.r) / d iteration between replacements (< 0)
     double rr_after;  // r.r of the replaced residual (mixed), used by the next step
     double beta_after;  // beta of the step after a replacement (mixed)
-    double alph[2];   // mixed: alpha_k in slot k & 1, for the correction c, which
-                      // takes the updates of iterations k-1 and k together at the
-                      // step after k (k even)
+    double alph[kFold];   // mixed: alpha_k in slot k % kFold, for the correction c
     int lag_stop;     // 1: the FP64 stop test reads the r.r that entered the previous
                       // iteration (SuLi-IT's loop, SuLi-IT/9_grad.f90:165-177)
 };
@@ -110,10 +119,13 @@ struct ProbDev {
     // FP64 iterate and, for FP64 solves, the CG vectors (p alternates)
     double *e, *r, *q, *p0, *p1;
     // FP32 inner solve (mixed): operator copy, correction c, vectors.  The
-    // one-launch iteration alternates r, p and q between two buffers each
-    // (r32/r32b, p32_0/p32_1, q32/q32b by iteration parity).
+    // one-launch iteration alternates r and q between two buffers each
+    // (r32/r32b, q32/q32b by iteration parity) and keeps p_k in ring slot
+    // k % kFold (at p32 + (k % kFold) * pslot) until c has taken alpha_k p_k.
     const float *sx32, *sy32, *sz32;
-    float *c32, *r32, *q32, *p32_0, *p32_1, *r32b, *q32b;
+    float *c32, *r32, *q32, *r32b, *q32b;
+    float *p32;
+    long long pslot;
     // constant density (t_press = 2, no solid mask): every face of a plane
     // carries the same scaled coefficient, so the iteration kernels read
     // tx[k], ty[k] (x and y faces of plane k) and tz[f] (z face f) instead
