@@ -139,6 +139,26 @@ def test_iteration_cap_status(solver):
     assert st.status == 1 and st.iters == 7
 
 
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 4, 5, 6, 9, 13, 31, 32, 33, 34, 36, 37, 41])
+def test_mixed_cap_matches_fp64_iterate(solver, max_iter):
+    """An iteration cap ends the mixed solver with every update alpha_j p_j
+    of the iterations run in e, whichever step of a fold group (or a
+    replacement) it falls on: its iterate then matches FP64 CG's after the
+    same number of iterations to FP32 rounding.  A dropped or doubled
+    update would move it by about 1/max_iter relative."""
+    c = cases.random_case(40, 24, 16, seed=12)
+    print(f"mixed cap max_iter={max_iter}", flush=True)
+    p64, st64 = solve(solver, c.grid, c.fields, stop_rule="relative", tol=1e-14, max_iter=max_iter)
+    p32, st32 = solve(solver, c.grid, c.fields, precision="mixed", stop_rule="relative", tol=1e-14,
+                      max_iter=max_iter)
+    assert st64.status == 1 and st64.iters == max_iter, st64.as_dict()
+    assert st32.status == 1 and st32.iters == max_iter, st32.as_dict()
+    mask = helpers.face_mask(p64.shape)
+    diff = helpers.rel_diff(p32, p64, mask)
+    print(f"mixed vs fp64 iterate after {max_iter}: {diff:.3e}", flush=True)
+    assert diff <= 1e-5   # measured <= 7e-7 at 40x24x16
+
+
 def test_zero_rhs_no_iterations(solver):
     c = cases.random_case(16, 16, 8, seed=5)
     f = c.fields.copy()
