@@ -159,6 +159,26 @@ def test_mixed_cap_matches_fp64_iterate(solver, max_iter):
     assert diff <= 1e-5   # measured <= 7e-7 at 40x24x16
 
 
+@pytest.mark.parametrize("max_iter", [5, 13, 34, 37])
+def test_mixed_cap_four_cells_per_lane(solver, max_iter):
+    """The same with tiles of four x cells per lane (the cap folds each of
+    the lane's cells)."""
+    c = cases.random_case(40, 24, 16, seed=12)
+    print(f"mixed cap vx=4 max_iter={max_iter}", flush=True)
+    p64, _ = solve(solver, c.grid, c.fields, stop_rule="relative", tol=1e-14, max_iter=max_iter)
+    solver.set_tiles(c.grid, "mixed", 4, 4)
+    solver.configure(precision="mixed", stop_rule="relative", tol=1e-14, max_iter=max_iter, verbose=False,
+                     autotune=True)
+    assert solver.autotune(c.grid, "mixed")[:2] == (4, 4)
+    f2 = c.fields.copy()
+    st32 = solver.graddin(c.grid, f2)
+    solver.configure(precision="fp64", stop_rule="reference", verbose=False)
+    assert st32.status == 1 and st32.iters == max_iter, st32.as_dict()
+    diff = helpers.rel_diff(f2.prd1, p64, helpers.face_mask(p64.shape))
+    print(f"mixed (vx=4) vs fp64 iterate after {max_iter}: {diff:.3e}", flush=True)
+    assert diff <= 1e-5
+
+
 def test_zero_rhs_no_iterations(solver):
     c = cases.random_case(16, 16, 8, seed=5)
     f = c.fields.copy()
