@@ -323,7 +323,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
-              "(problems shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+              "(independent problems would shard across GPUs, one process per GPU, with no "
+              "communication; this benchmark measures one)", file=sys.stderr)
         sys.exit(2)
 
     from suli_amd import cases
