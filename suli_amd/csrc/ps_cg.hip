@@ -843,14 +843,21 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
             long long c = (long long)t.k0 * plane + (long long)t.j * nx + ci;
             long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + ci;
             long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + ci;
+            // e of the cell and of the cell below carried up the column
+            // (the bottom ghost mirrors the first plane)
+            double ec = e[c];
+            double eb = t.k0 > 0 ? e[c - plane] : ec;
+            double szb = co.z(c, t.k0);
+            const bool dtop = g.dtop != 0;
             for (int k = t.k0; k < t.k1; ++k) {
                 const long long r0 = c - ci;
-                const long long up = (k < nz - 1) ? plane : 0, dn = (k > 0) ? plane : 0;
-                const double ec = e[c];
-                const double et = (g.dtop && k == nz - 1) ? -ec : e[c + up];   // Dirichlet lid
+                const long long up = (k < nz - 1) ? plane : 0;
+                const double eu = e[c + up];
+                const double et = (dtop && k == nz - 1) ? -ec : eu;   // Dirichlet lid
+                const double szt = co.z(c + plane, k + 1);
                 const double r = ec - co.x(fx + 1, k) * e[r0 + ie] - co.x(fx, k) * e[r0 + iw] -
-                                 co.y(fy + nx, k) * e[c + dN] - co.y(fy, k) * e[c + dS] - co.z(c + plane, k + 1) * et -
-                                 co.z(c, k) * e[c - dn] - b[c];
+                                 co.y(fy + nx, k) * e[c + dN] - co.y(fy, k) * e[c + dS] - szt * et -
+                                 szb * eb - b[c];
                 acc += r * r;
                 if (WRITE32) {
                     const float rf = (float)r;
@@ -858,6 +865,9 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
                     acc32 += (double)rf * (double)rf;
                 }
                 if (GAUGE) acc_en += ec * sqrt(pd.d[c]);
+                eb = ec;
+                ec = eu;
+                szb = szt;
                 c += plane;
                 fx += (long long)ny * (nx + 1);
                 fy += (long long)(ny + 1) * nx;
