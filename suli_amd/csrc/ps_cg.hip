@@ -832,10 +832,14 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
         // the replaced residual goes where the next cg_m reads r_{n}
         float *__restrict__ r32 = WRITE32 ? rbuf(pd, (step - pd.n0 + 1) & 1) : nullptr;
         double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
+        // a tile of vx cells per lane is walked as vx one-cell-per-lane
+        // sub-tiles, so that every load is coalesced
+        const int lane = threadIdx.x & (kTileX - 1);
+        const int i0 = t.i - lane * g.vx;
         with_coefs(vecs(pd, 0.0), [&](const auto &co) {
         for (int sub = 0; sub < g.vx; ++sub) {
-            const int ci = t.i + sub;
-            if (ci >= nx || t.j >= ny) break;
+            const int ci = i0 + sub * kTileX + lane;
+            if (ci >= nx || t.j >= ny) continue;
             const long long plane = (long long)nx * ny;
             const int iw = nb_lo(ci, nx, g.px), ie = nb_hi(ci, nx, g.px);
             const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
