@@ -591,25 +591,33 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         const ProbDev &pd = P[tr.prob];
         CgScalars *S = pd.S;
         const int n = step - pd.n0 + 1;
-        if (n < 1 || n > S->done_at) continue;
-        const bool first = n == 1;
-        const bool upd = !first && step != S->skip_e_at;
-        if (!first && tr.prob != cached) {
+        // The problem's scalars and the previous step's partials were both
+        // written by the previous step.  They are requested together, and
+        // the partials are summed before the scalars decide whether the
+        // problem runs at all, so that a step starts after one memory round
+        // trip instead of two (a finished problem's tiles sum them for
+        // nothing).
+        const int done_at = S->done_at, skip_e_at = S->skip_e_at, max_iter = S->max_iter;
+        const double rr1 = S->rr[1], rr_after = S->rr_after, beta_after = S->beta_after;
+        if (n > 1 && tr.prob != cached) {
             total4_of(pd.P.m4 + (size_t)((n - 1) & 1) * 4 * pd.nt3, pd.nt3, T);
             cached = tr.prob;
         }
+        if (n < 1 || n > done_at) continue;
+        const bool first = n == 1;
+        const bool upd = !first && step != skip_e_at;
         double alpha = 0.0, rho, beta = 0.0;
         if (first) {
-            rho = S->rr[1];
+            rho = rr1;
         } else if (upd) {
             alpha = T[3] / T[0];
             rho = T[3] - 2.0 * alpha * T[1] + alpha * alpha * T[2];
             beta = rho / T[3];
         } else {
-            rho = S->rr_after;
-            beta = S->beta_after;
+            rho = rr_after;
+            beta = beta_after;
         }
-        const bool cap = n - 1 >= S->max_iter;
+        const bool cap = n - 1 >= max_iter;
         // c takes the updates of iterations n-kFold .. n-1 together at
         // n = 1 mod kFold (replacements fall after multiples of kFold, so a
         // group never spans one); their alphas but the last were stored by
