@@ -207,24 +207,31 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
         const ProbDev &pd = P[tr.prob];
         CgScalars *S = pd.S;
         const int n = step - pd.n0 + 1;
-        if (n < 1 || n > S->done_at) continue;
+        // the problem's scalars, then the previous kernel's partials, both
+        // requested before the scalars decide whether the problem runs, so
+        // that the step starts after one memory round trip (as in cg_m)
+        const int done_at = S->done_at, skip_e_at = S->skip_e_at, max_iter = S->max_iter;
+        const int lag_stop = S->lag_stop, mixed = S->mixed;
+        const double rr1 = S->rr[1], rr_prev = S->rr[(n - 1) & 1], thresh = S->thresh, alpha_prev = S->alpha;
+        const double rr_sum = n > 1 ? rr_tot.get(tr.prob, pd.P.upd, pd.nt1) : 0.0;
+        if (n < 1 || n > done_at) continue;
         const bool first = n == 1;
-        const bool do_e = !first && step != S->skip_e_at;
+        const bool do_e = !first && step != skip_e_at;
         // r.r entering this iteration: r_0.r_0 from setup, else cg_b's partials
-        const double rr_in = first ? S->rr[1] : rr_tot.get(tr.prob, pd.P.upd, pd.nt1);
-        const double beta = first ? 0.0 : rr_in / S->rr[(n - 1) & 1];   // 9_grad.f90:510-515
+        const double rr_in = first ? rr1 : rr_sum;
+        const double beta = first ? 0.0 : rr_in / rr_prev;   // 9_grad.f90:510-515
         // stop rule (9_grad.f90:398,582): FP64 tests the recurrence's r.r;
         // the mixed solver decides at its FP64 residual replacements
-        const double rr_test = (S->lag_stop && !first) ? S->rr[(n - 1) & 1] : rr_in;
-        const bool conv = !(fabs(rr_test) > S->thresh);
-        const bool cap = n - 1 >= S->max_iter;
-        const bool stop = cap || (conv && !S->mixed);
+        const double rr_test = (lag_stop && !first) ? rr_prev : rr_in;
+        const bool conv = !(fabs(rr_test) > thresh);
+        const bool cap = n - 1 >= max_iter;
+        const bool stop = cap || (conv && !mixed);
         if (tr.tile == 0 && threadIdx.x == 0) {
             S->rr[n & 1] = rr_in;
             S->iters = n - 1;
             if (stop) {
                 S->done_at = n;
-                if (!S->mixed) {
+                if (!mixed) {
                     S->final_ = 1;
                     S->status = conv ? 0 : 1;
                 }
@@ -234,7 +241,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
         const Vecs<T> V = vecs(pd, T());
         T *p_old = (n & 1) ? V.p0 : V.p1;
         T *p_new = (n & 1) ? V.p1 : V.p0;
-        const T a_prev = (T)S->alpha;
+        const T a_prev = (T)alpha_prev;
         const Tile tl = tile3(pd.g, tr.tile);
         if (stop) {
             // finish iteration n-1 only (e -= alpha p, 9_grad.f90:458-468)
@@ -691,9 +698,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P
         const ProbDev &pd = P[tr.prob];
         CgScalars *S = pd.S;
         const int n = step - pd.n0 + 1;
-        if (n < 1 || n >= S->done_at) continue;
-        const double pap = pap_tot.get(tr.prob, pd.P.spmv, pd.nt3);
-        const double alpha = S->rr[n & 1] / pap;
+        // scalars and partials requested together (as in cg_a)
+        const int done_at = S->done_at;
+        const double rr_n = S->rr[n & 1];
+        const double pap = n >= 1 ? pap_tot.get(tr.prob, pd.P.spmv, pd.nt3) : 0.0;
+        if (n < 1 || n >= done_at) continue;
+        const double alpha = rr_n / pap;
         if (tr.tile == 0 && threadIdx.x == 0) {
             S->pap = pap;
             S->alpha = alpha;
