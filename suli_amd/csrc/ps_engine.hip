@@ -762,9 +762,9 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
     TCHK(hipMalloc(&dT, sizeof(TileRef) * max_nt3));
     TCHK(hipEventCreate(&a));
     TCHK(hipEventCreate(&b));
-    // state that keeps the per-step kernel on its working path with finite
-    // values: zero vectors; FP64: entering r.r 0 over a previous 1 (beta 0),
-    // threshold -1; mixed: previous partials p.q = r.r = 1 (alpha = beta = 1)
+    // state that keeps the per-step kernel on its working path: zero
+    // vectors; FP64: entering r.r 0 over a previous 1 (beta 0), threshold
+    // -1; mixed: zero partials (alpha and beta NaN, which no branch reads)
     CgScalars s0{};
     s0.rr[0] = s0.rr[1] = 1.0;
     s0.thresh = -1.0;
@@ -774,9 +774,14 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
     s0.mixed = mixed ? 1 : 0;
     s0.next_refresh = kNotDone;
     std::vector<TileRef> tab(max_nt3);
-    // n = 3 every launch: the same inputs each time (for the mixed step the
-    // heavier of its two kinds, the one that also folds the correction)
-    const int reps = 20, step = 3;
+    // FP64: n = 3 every launch, the same inputs each time.  Mixed: n runs
+    // through 2 .. 9 again and again, so that the timed launches alternate
+    // their buffers and include the fold steps (n = 5, 9) in their real
+    // proportion; repeating one plain step on the same buffers timed
+    // several-cells-per-lane shapes up to 25 % faster than they run in a
+    // solve.  The zero vectors keep every value finite or NaN, and no
+    // branch of the step depends on them.
+    const int reps = 24;
     double best_us = 1e300;
     int best_kz = g_def.kz, best_vx = 1;
     std::vector<std::pair<int, int>> shapes;
@@ -802,24 +807,20 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         pd.S = dS;
         for (int t = 0; t < nt3; ++t) tab[t] = {0, t};
         TCHK(hipMemsetAsync(blk, 0, cv.off, s_));
-        if (mixed) {
-            // step 3 reads slot 0 (and writes slot 1)
-            const double m1[4] = {1.0, 0.0, 0.0, 1.0};
-            TCHK(hipMemcpyAsync(pd.P.m4, m1, sizeof m1, hipMemcpyHostToDevice, s_));
-        }
         TCHK(hipMemcpyAsync(dP, &pd, sizeof pd, hipMemcpyHostToDevice, s_));
         TCHK(hipMemcpyAsync(dS, &s0, sizeof s0, hipMemcpyHostToDevice, s_));
         TCHK(hipMemcpyAsync(dT, tab.data(), sizeof(TileRef) * nt3, hipMemcpyHostToDevice, s_));
         TCHK(hipStreamSynchronize(s_));
         const int grid = std::min(nt3, kMaxGrid);
+        int seq = 0;
         auto launch = [&] {
+            const int step = mixed ? 2 + (seq++ % 8) : 3;
             if (mixed)
                 launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
             else
                 launch_cg_a(false, dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
         };
-        launch();
-        launch();
+        for (int r = 0; r < 8; ++r) launch();
         TCHK(hipEventRecord(a, s_));
         for (int r = 0; r < reps; ++r) launch();
         TCHK(hipEventRecord(b, s_));
