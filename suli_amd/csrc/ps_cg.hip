@@ -212,8 +212,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
         // that the step starts after one memory round trip (as in cg_m)
         const int done_at = S->done_at, skip_e_at = S->skip_e_at, max_iter = S->max_iter;
         const int lag_stop = S->lag_stop, mixed = S->mixed;
-        const double rr1 = S->rr[1], rr_prev = S->rr[(n - 1) & 1], thresh = S->thresh, alpha_prev = S->alpha;
-        const double rr_sum = n > 1 ? rr_tot.get(tr.prob, pd.P.upd, pd.nt1) : 0.0;
+        // (n - 1) & 1 is (sref.off - 1) & 1, known without the replayed step
+        // counter (see cg_m)
+        const double rr1 = S->rr[1], rr_prev = S->rr[(sref.off - 1) & 1], thresh = S->thresh;
+        const double alpha_prev = S->alpha;
+        const double rr_sum = rr_tot.get(tr.prob, pd.P.upd, pd.nt1);   // unused at n = 1
         if (n < 1 || n > done_at) continue;
         const bool first = n == 1;
         const bool do_e = !first && step != skip_e_at;
@@ -604,10 +607,14 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         // problem runs at all, so that a step starts after one memory round
         // trip instead of two (a finished problem's tiles sum them for
         // nothing).
+        // The partials' slot is the parity of n - 1, which is that of
+        // sref.off - 1: n0 - 1 is a multiple of kRefreshGrain (problems are
+        // admitted on the grain) and a replayed group's base step is one too,
+        // so the slot is known without waiting for the replayed step counter.
         const int done_at = S->done_at, skip_e_at = S->skip_e_at, max_iter = S->max_iter;
         const double rr1 = S->rr[1], rr_after = S->rr_after, beta_after = S->beta_after;
-        if (n > 1 && tr.prob != cached) {
-            total4_of(pd.P.m4 + (size_t)((n - 1) & 1) * 4 * pd.nt3, pd.nt3, T);
+        if (tr.prob != cached) {
+            total4_of(pd.P.m4 + (size_t)((sref.off - 1) & 1) * 4 * pd.nt3, pd.nt3, T);
             cached = tr.prob;
         }
         if (n < 1 || n > done_at) continue;
@@ -700,8 +707,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P
         const int n = step - pd.n0 + 1;
         // scalars and partials requested together (as in cg_a)
         const int done_at = S->done_at;
-        const double rr_n = S->rr[n & 1];
-        const double pap = n >= 1 ? pap_tot.get(tr.prob, pd.P.spmv, pd.nt3) : 0.0;
+        const double rr_n = S->rr[sref.off & 1];   // n & 1 (see cg_m)
+        const double pap = pap_tot.get(tr.prob, pd.P.spmv, pd.nt3);
         if (n < 1 || n >= done_at) continue;
         const double alpha = rr_n / pap;
         if (tr.tile == 0 && threadIdx.x == 0) {

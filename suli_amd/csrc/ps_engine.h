@@ -176,6 +176,7 @@ private:
     int *d_step_ = nullptr;            // two counters, alternating by group parity
     bool dstep_valid_ = true;          // the counter of the next group holds step_
     bool graphs_ = true;
+    long long table_cells_ = 0;   // cells of the problems in the launch tables
     bool autotune_ = false;
     struct Tuned {
         int kz, vx;

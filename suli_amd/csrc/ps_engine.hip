@@ -340,9 +340,11 @@ int Engine::rebuild_tables()
     tab_.t3_32.clear();
     tab_.t1_32.clear();
     tab_.mixed_probs.clear();
+    table_cells_ = 0;
     for (int i = 0; i < max_slots_; ++i) {
         const Slot &sl = slots_[i];
         if (!sl.active || sl.finished) continue;
+        table_cells_ += sl.g.n;
         auto &t3 = sl.o.mixed ? tab_.t3_32 : tab_.t3_64;
         auto &t1 = sl.o.mixed ? tab_.t1_32 : tab_.t1_64;
         for (int t = 0; t < sl.pd.nt3; ++t) t3.push_back({i, t});
@@ -567,10 +569,17 @@ int Engine::enqueue_group_graph()
     return 0;
 }
 
+// Replayed graphs save the host its launches, which matters when the
+// steps are short.  From kDirectCells cells in the tables on, the host keeps
+// ahead of the GPU with direct launches, and they were faster: at 256x128x96
+// a mixed solve took 24.6 ms with direct launches against 25.5 ms replayed
+// (FP64: 47.9 against 48.8 ms).
+constexpr long long kDirectCells = 1LL << 20;
+
 int Engine::enqueue_chunk(int K, bool timing)
 {
     int rc;
-    if (graphs_ && !timing) {
+    if (graphs_ && !timing && table_cells_ < kDirectCells) {
         for (int it = 0; it < K; it += kRefreshGrain)
             if ((rc = enqueue_group_graph())) return rc;
     } else if ((rc = enqueue_steps_direct(K, timing))) {
