@@ -13,11 +13,11 @@ the same steps for the comparison.
 Prints one JSON line.  `roofline` is for the dominant kernel, k_cg_m (the
 whole FP32 CG step of the mixed solver in one launch: residual and iterate
 updates of the previous iteration, direction update, SpMV, four dot-product
-partials): algorithmic bytes 10.25 words/cell on average (every step
-reads r, q, p of the previous step and the three face-coefficient arrays
-and writes r, p and q; every fourth step also reads the correction c and
-the three older p of the direction ring and writes c) divided by its mean
-launch time, from HIP
+partials): algorithmic bytes 9 words/cell on average (every step reads
+q and the last two p of the previous steps, from which it recovers r, and
+the three face-coefficient arrays, and writes p and q; every fourth step
+also reads the correction c and the two older p of the direction ring and
+writes c) divided by its mean launch time, from HIP
 events on the solver's stream around windows of 8 consecutive launches (one
 window per 16 steps)
 during two directly launched solves.  The FP64 solver's step (k_cg_a +
@@ -182,7 +182,7 @@ def bench_constant_density(steps):
         st = solver.graddin_dev(g, df)
         w = 4 if precision == "mixed" else 8
         spmv_ms = solver.time_spmv(precision, reps=50)
-        words = 7.25 if precision == "mixed" else 9   # mixed: 6 words, and 11 every fourth step
+        words = 6.0 if precision == "mixed" else 9   # mixed: 5 words, and 9 every fourth step
         out[precision] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iters": st.iters,
                           "rel_true": st.rel_true, "step_ms": st.spmv_ms, "words_per_cell_step": words,
                           "step_achieved_gbs": words * w * n / (st.spmv_ms * 1e-3) / 1e9,
@@ -356,7 +356,7 @@ def main():
         print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy}))
         return
 
-    bytes32 = 41 * n              # k_cg_m: 10.25 FP32 words per cell on average (9, and 14 every 4th step)
+    bytes32 = 36 * n              # k_cg_m: 9 FP32 words per cell on average (8, and 12 every 4th step)
     bytes64 = 12 * 8 * n          # FP64 step (k_cg_a + k_cg_b): 12 FP64 words per cell
     spmv_ms = mixed["spmv_ms"]
     achieved = bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None

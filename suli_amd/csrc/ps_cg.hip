@@ -269,10 +269,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
 // One 3-D tile of the one-launch FP32 iteration n (cg_m).  With UPD the
 // residual of iteration n-1 is formed where needed,
 //   r_{n-1} = r_{n-2} - alpha_{n-1} q_{n-1}            (9_grad.f90:470-480)
-// else (iteration 1, or the step after a residual replacement) r_old is
-// r_{n-1} itself; then as cg_a: p_n = r_{n-1} + beta p_{n-1}, q_n = A p_n,
+// with r_{n-2} = p_{n-1} - beta_{n-1} p_{n-2} (REC: CG's direction update
+// run backwards, so that no step stores r) or read from r_old; without UPD
+// (iteration 1, or the step after a residual replacement) r_old is r_{n-1}
+// itself.  Then as cg_a: p_n = r_{n-1} + beta p_{n-1}, q_n = A p_n, and
 // with FOLD, c -= alpha_j p_j for the kFold iterations j = n-kFold .. n-1
-// on the tile's own cells.  Writes r_{n-1}, p_n, q_n; partials p_n.q_n,
+// on the tile's own cells.  Writes p_n, q_n; partials p_n.q_n,
 // q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}.
 //
 // The fold's operands: p_{n-kFold} is in the ring slot that p_n replaces
@@ -283,12 +285,12 @@ struct Fold {
     float a[kFold];              // alpha_{n-kFold} .. alpha_{n-1}
 };
 
-template <bool FIRST, bool UPD, bool FOLD, class CO>
+template <bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__restrict__ c32,
                                           const float *__restrict__ r_old, const float *__restrict__ q_old,
-                                          const float *__restrict__ p_old, float *__restrict__ r_new,
+                                          const float *__restrict__ p_old, const float *__restrict__ p_prev,
                                           float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                          float a, float b, const Fold &fd, double acc[4])
+                                          float a, float b, float bp, const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -299,10 +301,19 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
     long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
     long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-    auto rn = [&](long long x) -> float { return UPD ? r_old[x] - a * q_old[x] : r_old[x]; };
-    auto pn = [&](long long x) -> float { return FIRST ? rn(x) : rn(x) + b * p_old[x]; };
+    // r_{n-2} at x from p_{n-1} (po) and p_{n-2} (pp) there, or as stored
+    auto rn = [&](long long x, float po, float pp) -> float {
+        const float r = REC ? po - bp * pp : r_old[x];
+        return UPD ? r - a * q_old[x] : r;
+    };
+    auto pn = [&](long long x) -> float {
+        const float po = FIRST ? 0.0f : p_old[x];
+        const float r = rn(x, po, REC ? p_prev[x] : 0.0f);
+        return FIRST ? r : r + b * po;
+    };
     float poc = FIRST ? 0.0f : p_old[c];
-    float rc = rn(c);
+    float ppc = REC ? p_prev[c] : 0.0f;
+    float rc = rn(c, poc, ppc);
     float pc = FIRST ? rc : rc + b * poc;
     float pb = t.k0 > 0 ? pn(c - plane) : pc;
     float szb = co.z(c, t.k0);
@@ -310,7 +321,8 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
         const float pot = FIRST ? 0.0f : p_old[c + up];
-        const float rt = rn(c + up);
+        const float ppt = REC ? p_prev[c + up] : 0.0f;
+        const float rt = rn(c + up, pot, ppt);
         const float pt = FIRST ? rt : rt + b * pot;
         const float szt = co.z(c + plane, k + 1);
         const float pw = pn(r0 + iw), pe = pn(r0 + ie);
@@ -323,12 +335,13 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         const float p_first = FOLD ? p_new[c] : 0.0f;
         q_new[c] = y;
         p_new[c] = pc;
-        r_new[c] = rc;
         if (FOLD) {
-            // 9_grad.f90:458-468 for kFold iterations
+            // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
+            // when r was recovered from it)
             float sum = fd.a[0] * p_first;
 #pragma unroll
-            for (int m = 0; m < kFold - 2; ++m) sum += fd.a[m + 1] * fd.p[m][c];
+            for (int m = 0; m < kFold - 2; ++m)
+                sum += fd.a[m + 1] * ((REC && m == kFold - 3) ? ppc : fd.p[m][c]);
             sum += fd.a[kFold - 1] * poc;
             c32[c] = c32[c] - sum;
         }
@@ -340,6 +353,7 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         pc = pt;
         rc = rt;
         poc = pot;
+        ppc = ppt;
         szb = szt;
         c += plane;
         fx += (long long)ny * (nx + 1);
@@ -404,12 +418,12 @@ __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int 
 // cg_m_tile with V cells per lane: the same arithmetic per cell (and the
 // same term order), with the lane's inner x neighbours shared in registers
 // and 8- or 16-byte loads and stores.
-template <int V, bool FIRST, bool UPD, bool FOLD, class CO>
+template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
                                            const float *__restrict__ r_old, const float *__restrict__ q_old,
-                                           const float *__restrict__ p_old, float *__restrict__ r_new,
+                                           const float *__restrict__ p_old, const float *__restrict__ p_prev,
                                            float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                           float a, float b, const Fold &fd, double acc[4])
+                                           float a, float b, float bp, const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -420,10 +434,24 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
     long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
     long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-    auto rn1 = [&](long long x) -> float { return UPD ? r_old[x] - a * q_old[x] : r_old[x]; };
-    auto pn1 = [&](long long x) -> float { return FIRST ? rn1(x) : rn1(x) + b * p_old[x]; };
-    auto rnv = [&](long long x, float (&r)[V]) {
-        ldv<V>(r_old, x, r);
+    // r_{n-2} from p_{n-1} = r_{n-2} + beta_{n-1} p_{n-2} (REC), or as stored
+    auto pn1 = [&](long long x) -> float {
+        const float po = FIRST ? 0.0f : p_old[x];
+        float r = REC ? po - bp * p_prev[x] : r_old[x];
+        if (UPD) r = r - a * q_old[x];
+        return FIRST ? r : r + b * po;
+    };
+    // r_{n-1} at x given p_{n-1} there; pp receives p_{n-2} (REC)
+    auto rnv = [&](long long x, const float (&po)[V], float (&r)[V], float (&pp)[V]) {
+        if (REC) {
+            ldv<V>(p_prev, x, pp);
+#pragma unroll
+            for (int m = 0; m < V; ++m) r[m] = po[m] - bp * pp[m];
+        } else {
+            ldv<V>(r_old, x, r);
+#pragma unroll
+            for (int m = 0; m < V; ++m) pp[m] = 0.0f;
+        }
         if (UPD) {
             float q[V];
             ldv<V>(q_old, x, q);
@@ -432,22 +460,27 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
         }
     };
     auto pnv = [&](long long x, float (&pv)[V]) {
-        rnv(x, pv);
-        if (!FIRST) {
-            float po[V];
+        float po[V], pp[V];
+        if (FIRST) {
+#pragma unroll
+            for (int m = 0; m < V; ++m) po[m] = 0.0f;
+        } else {
             ldv<V>(p_old, x, po);
+        }
+        rnv(x, po, pv, pp);
+        if (!FIRST) {
 #pragma unroll
             for (int m = 0; m < V; ++m) pv[m] = pv[m] + b * po[m];
         }
     };
-    float poc[V], rc[V], pc[V], pb[V], szb[V];
+    float poc[V], ppc[V], rc[V], pc[V], pb[V], szb[V];
     if (FIRST) {
 #pragma unroll
         for (int m = 0; m < V; ++m) poc[m] = 0.0f;
     } else {
         ldv<V>(p_old, c, poc);
     }
-    rnv(c, rc);
+    rnv(c, poc, rc, ppc);
 #pragma unroll
     for (int m = 0; m < V; ++m) pc[m] = FIRST ? rc[m] : rc[m] + b * poc[m];
     if (t.k0 > 0) {
@@ -460,14 +493,14 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
-        float pot[V], rt[V], pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1];
+        float pot[V], ppt[V], rt[V], pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1];
         if (FIRST) {
 #pragma unroll
             for (int m = 0; m < V; ++m) pot[m] = 0.0f;
         } else {
             ldv<V>(p_old, c + up, pot);
         }
-        rnv(c + up, rt);
+        rnv(c + up, pot, rt, ppt);
 #pragma unroll
         for (int m = 0; m < V; ++m) pt[m] = FIRST ? rt[m] : rt[m] + b * pot[m];
         zv<V>(co, c + plane, k + 1, szt);
@@ -496,13 +529,18 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
         }
         stv<V>(q_new, c, y);
         stv<V>(p_new, c, pc);
-        stv<V>(r_new, c, rc);
         if (FOLD) {
-            // 9_grad.f90:458-468 for kFold iterations
+            // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
+            // when r was recovered from it)
 #pragma unroll
             for (int f = 0; f < kFold - 2; ++f) {
                 float pf[V];
-                ldv<V>(fd.p[f], c, pf);
+                if (REC && f == kFold - 3) {
+#pragma unroll
+                    for (int m = 0; m < V; ++m) pf[m] = ppc[m];
+                } else {
+                    ldv<V>(fd.p[f], c, pf);
+                }
 #pragma unroll
                 for (int m = 0; m < V; ++m) sum[m] += fd.a[f + 1] * pf[m];
             }
@@ -526,6 +564,7 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
             pc[m] = pt[m];
             rc[m] = rt[m];
             poc[m] = pot[m];
+            ppc[m] = ppt[m];
             szb[m] = szt[m];
         }
         c += plane;
@@ -534,21 +573,25 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
     }
 }
 
-__device__ __forceinline__ float *rbuf(const ProbDev &pd, int parity) { return parity ? pd.r32b : pd.r32; }
 __device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
 // ring slot of p_k
 __device__ __forceinline__ float *pbuf(const ProbDev &pd, int k) { return pd.p32 + (k & (kFold - 1)) * pd.pslot; }
 __device__ __forceinline__ float alph_of(const CgScalars *S, int k) { return (float)S->alph[k & (kFold - 1)]; }
 
 // One tile of the mixed step n with V cells per lane (V = 1: cg_m_tile).
+// REC: r_{n-2} is recovered from the directions, r_{n-2} = p_{n-1} -
+// beta_{n-1} p_{n-2} (bp = beta_{n-1}, the FP32 factor that formed p_{n-1}),
+// so that no step stores r; otherwise (the first step, and the two steps
+// after a residual replacement) it is read from r32, where the setup and the
+// replacement leave it.
 // FOLD: c takes the updates of iterations n-kFold .. n-1 this step (their
 // alphas but alpha_{n-1} = a were stored by the previous steps).  The fold's
 // operands are gathered here, inside the FOLD instance only, so that the
 // other instances carry no extra live registers.
-template <int V, bool FIRST, bool UPD, bool FOLD, class CO>
-__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, float *r_o, float *q_o,
-                                              float *p_o, float *r_w, float *p_w, float *q_w, const CO &co, float a,
-                                              float b, double acc[4])
+template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
+__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, const float *r_o,
+                                              const float *q_o, const float *p_o, const float *p_oo, float *p_w,
+                                              float *q_w, const CO &co, float a, float b, float bp, double acc[4])
 {
     Fold fd;
     if constexpr (FOLD) {
@@ -559,24 +602,28 @@ __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Ti
         for (int m = 0; m < kFold - 2; ++m) fd.p[m] = pbuf(pd, n - kFold + 1 + m);
     }
     if constexpr (V == 1)
-        cg_m_tile<FIRST, UPD, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, fd, acc);
+        cg_m_tile<FIRST, UPD, REC, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, fd, acc);
     else
-        cg_m_tilev<V, FIRST, UPD, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, fd, acc);
+        cg_m_tilev<V, FIRST, UPD, REC, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, fd,
+                                             acc);
 }
 
 template <int V, class CO>
-__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool fold, const ProbDev &pd, int n, const Tile &tl,
-                                          float *r_o, float *q_o, float *p_o, float *r_w, float *p_w, float *q_w,
-                                          const CO &co, float a, float b, double acc[4])
+__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, bool fold, const ProbDev &pd, int n,
+                                          const Tile &tl, const float *r_o, const float *q_o, const float *p_o,
+                                          const float *p_oo, float *p_w, float *q_w, const CO &co, float a,
+                                          float b, float bp, double acc[4])
 {
+#define CG_M(F, U, R, D) cg_m_tile_any<V, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc)
     if (first)
-        cg_m_tile_any<V, true, false, false>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-    else if (upd && fold)
-        cg_m_tile_any<V, false, true, true>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
-    else if (upd)
-        cg_m_tile_any<V, false, true, false>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+        CG_M(true, false, false, false);
+    else if (!upd)
+        CG_M(false, false, false, false);
+    else if (rec)
+        fold ? CG_M(false, true, true, true) : CG_M(false, true, true, false);
     else
-        cg_m_tile_any<V, false, false, false>(pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+        fold ? CG_M(false, true, false, true) : CG_M(false, true, false, false);
+#undef CG_M
 }
 
 // The mixed solver's FP32 iteration, one launch per global step (register
@@ -613,6 +660,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         // so the slot is known without waiting for the replayed step counter.
         const int done_at = S->done_at, skip_e_at = S->skip_e_at, max_iter = S->max_iter;
         const double rr1 = S->rr[1], rr_after = S->rr_after, beta_after = S->beta_after;
+        const double beta_prev = S->bet[(n - 1) & 1];
         if (tr.prob != cached) {
             total4_of(pd.P.m4 + (size_t)((sref.off - 1) & 1) * 4 * pd.nt3, pd.nt3, T);
             cached = tr.prob;
@@ -637,9 +685,13 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         // group never spans one); their alphas but the last were stored by
         // the previous steps
         const bool fold = upd && (n & (kFold - 1)) == 1;
+        // p_{n-1} came from r_{n-2} and beta_{n-1} p_{n-2} unless step n-1
+        // was the first or followed a replacement
+        const bool rec = upd && n != 2 && step != skip_e_at + 1;
         const float a = (float)alpha;
         if (tr.tile == 0 && threadIdx.x == 0) {
             S->rr[n & 1] = rho;
+            S->bet[n & 1] = beta;
             S->iters = n - 1;
             S->alpha = alpha;
             if (upd) S->alph[(n - 1) & (kFold - 1)] = alpha;
@@ -668,15 +720,16 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const float b = (float)beta;
-        float *r_o = rbuf(pd, o), *q_o = qbuf(pd, o), *p_o = pbuf(pd, n - 1);
-        float *r_w = rbuf(pd, w), *p_w = pbuf(pd, n), *q_w = qbuf(pd, w);
+        const float bp = (float)beta_prev;
+        const float *q_o = qbuf(pd, o), *p_o = pbuf(pd, n - 1), *p_oo = pbuf(pd, n - 2);
+        float *p_w = pbuf(pd, n), *q_w = qbuf(pd, w);
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
             if (pd.g.vx == 4)
-                cg_m_step<4>(first, upd, fold, pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+                cg_m_step<4>(first, upd, rec, fold, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
             else if (pd.g.vx == 2)
-                cg_m_step<2>(first, upd, fold, pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+                cg_m_step<2>(first, upd, rec, fold, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
             else
-                cg_m_step<1>(first, upd, fold, pd, n, tl, r_o, q_o, p_o, r_w, p_w, q_w, co, a, b, acc);
+                cg_m_step<1>(first, upd, rec, fold, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
             return 0;
         });
         block_sum4(acc);
@@ -855,7 +908,7 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
         const double *__restrict__ e = pd.e;
         const double *__restrict__ b = pd.b;
         // the replaced residual goes where the next cg_m reads r_{n}
-        float *__restrict__ r32 = WRITE32 ? rbuf(pd, (step - pd.n0 + 1) & 1) : nullptr;
+        float *__restrict__ r32 = WRITE32 ? pd.r32 : nullptr;
         double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
         // a tile of vx cells per lane is walked as vx one-cell-per-lane
         // sub-tiles, so that every load is coalesced

@@ -65,7 +65,6 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
         // the p ring: kFold slots kSkew bytes apart
         pd.pslot = (long long)((align_up(n * sizeof(float), 256) + kSkew) / sizeof(float));
         pd.p32 = cv.take<float>((size_t)pd.pslot * kFold);
-        pd.r32b = cv.take<float>(n);
         pd.q32b = cv.take<float>(n);
         pd.P.m4 = cv.take<double>(8 * (size_t)nt3);
     }

@@ -25,11 +25,13 @@ constexpr int kMaxRefreshGap = 128;
 // Mixed precision: the FP32 correction c takes the updates alpha_j p_j of
 // kFold iterations at once (at iterations n = 1 mod kFold, from a ring of
 // kFold direction vectors), so that a replacement, which falls after an
-// iteration that is a multiple of kFold, never splits a group.  The step
-// then moves 9 + (kFold + 1) / kFold words per cell on average.  Measured
-// per mixed solve at 256x128x96: 26.29 ms folding pairs, 26.05 ms folding
-// fours; folding eights ran the step kernel out of registers (the fold's
-// loads were issued one at a time) and took 33.5 ms.
+// iteration that is a multiple of kFold, never splits a group.  A plain
+// step moves 8 words per cell; a folding one also reads c and the kFold - 2
+// oldest directions and writes c, so the step moves 9 words per cell on
+// average.  Measured per mixed solve at 256x128x96 (when the steps still
+// stored r): 26.29 ms folding pairs, 26.05 ms folding fours; folding eights
+// ran the step kernel out of registers (the fold's loads were issued one at
+// a time) and took 33.5 ms.
 constexpr int kFold = 4;
 static_assert(kRefreshGrain % kFold == 0 && kFirstRefresh % kFold == 0, "replacements must end fold groups");
 static_assert((kFold & (kFold - 1)) == 0, "kFold is a power of two");
@@ -89,6 +91,7 @@ struct CgScalars {
     double rr_after;  // r.r of the replaced residual (mixed), used by the next step
     double beta_after;  // beta of the step after a replacement (mixed)
     double alph[kFold];   // mixed: alpha_k in slot k % kFold, for the correction c
+    double bet[2];    // mixed: beta_n in slot n & 1, the factor that formed p_n
     int lag_stop;     // 1: the FP64 stop test reads the r.r that entered the previous
                       // iteration (SuLi-IT's loop, SuLi-IT/9_grad.f90:165-177)
 };
@@ -119,11 +122,13 @@ struct ProbDev {
     // FP64 iterate and, for FP64 solves, the CG vectors (p alternates)
     double *e, *r, *q, *p0, *p1;
     // FP32 inner solve (mixed): operator copy, correction c, vectors.  The
-    // one-launch iteration alternates r and q between two buffers each
-    // (r32/r32b, q32/q32b by iteration parity) and keeps p_k in ring slot
-    // k % kFold (at p32 + (k % kFold) * pslot) until c has taken alpha_k p_k.
+    // one-launch iteration alternates q between two buffers (q32/q32b by
+    // iteration parity) and keeps p_k in ring slot k % kFold (at p32 +
+    // (k % kFold) * pslot) until c has taken alpha_k p_k.  r32 holds the
+    // residual the setup and each replacement leave; the steps recover r
+    // from the directions and store none.
     const float *sx32, *sy32, *sz32;
-    float *c32, *r32, *q32, *r32b, *q32b;
+    float *c32, *r32, *q32, *q32b;
     float *p32;
     long long pslot;
     // constant density (t_press = 2, no solid mask): every face of a plane
