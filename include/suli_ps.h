@@ -139,6 +139,34 @@ int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, doubl
                        double *w, const double *rho, const double *prd1,
                        const double *dz, const double *dzz, double *prd);
 
+/* suli_ps_graddin_dev without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc, code that must not synchronise on an
+ * allocation).  Same arguments, options, arithmetic and results as
+ * suli_ps_graddin_dev, except that the problem's solver state (scaled face
+ * coefficients, diagonal, right-hand side, iterate, Krylov vectors, partial
+ * sums) lives in one fixed device array of SULI_PS_NOALLOC_MAX_BYTES per GPU,
+ * which the library holds from the first context on that GPU until the
+ * process ends.  The call allocates and frees nothing, creates no event and
+ * captures no graph (it launches every kernel directly); the launch tables
+ * and host lists it needs are reserved by suli_ps_create.  A problem whose
+ * state does not fit (problem bytes beyond the maximum: about 80 B per cell
+ * in FP64, 100 B in mixed precision, so up to about 0.8 M / 0.65 M cells) is
+ * rejected with SULI_PS_EINVAL and a message giving both sizes; with
+ * opts.autotune it uses a tile shape already tuned for the size, or the
+ * default rule, and never tunes.  The array keeps the last call's contents
+ * (every value a solve reads is written by its own setup first).  Every
+ * call on one GPU shares that GPU's array: calls that overlap in time on
+ * one GPU, from any contexts or threads, race and must be serialised by
+ * the caller.  After the call the context's last-solve state refers to the
+ * array (suli_ps_time_spmv reads it). */
+#define SULI_PS_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+int suli_ps_graddin_noalloc_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u,
+                                const double *v, const double *w, const double *rho,
+                                const double *obs_lsx, const double *obs_lsy,
+                                const double *obs_lsz, const double *dz, const double *dzz,
+                                double *prd1, suli_ps_stats *stats);
+
 /* graddin / posdin with a solid-cell mask, an extension with no reference
  * counterpart (the reference's IBM only masks the right-hand side,
  * 9_grad.f90:45-88).  solid (nx*ny*nz bytes, x fastest, device; NULL = none)

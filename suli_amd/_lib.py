@@ -32,8 +32,11 @@ EXPORTS = (
     "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
     "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
     "suli_ps_heaviside_dev", "suli_ps_restart_read", "suli_ps_it_graddin", "suli_ps_it_graddin_dev",
-    "suli_ps_it_posdin", "suli_ps_it_posdin_dev",
+    "suli_ps_it_posdin", "suli_ps_it_posdin_dev", "suli_ps_graddin_noalloc_dev",
 )
+
+# SULI_PS_NOALLOC_MAX_BYTES: the fixed device array of the no-allocation solve
+NOALLOC_MAX_BYTES = 64 << 20
 
 
 class Grid(C.Structure):
@@ -116,6 +119,7 @@ def load(path: str = LIB_PATH):
     gargs = [vp, C.POINTER(Grid)] + [vp] * 10 + [C.POINTER(Stats)]
     L.suli_ps_graddin.argtypes = gargs
     L.suli_ps_graddin_dev.argtypes = gargs
+    L.suli_ps_graddin_noalloc_dev.argtypes = gargs
     pargs = [vp, C.POINTER(Grid)] + [vp] * 8
     L.suli_ps_posdin.argtypes = pargs
     L.suli_ps_posdin_dev.argtypes = pargs

@@ -23,9 +23,10 @@ public:
     void put(void *p, size_t bytes);
     void release_all();
     size_t held() const { return held_; }
+    void reserve(size_t blocks) { free_.reserve(blocks); }
 
 private:
-    std::multimap<size_t, void *> free_;
+    std::vector<std::pair<size_t, void *>> free_;   // (bytes, block), unordered
     size_t held_ = 0;   // bytes allocated from the device (free + in use)
 };
 
@@ -64,12 +65,37 @@ size_t problem_bytes(int nx, int ny, int nz, bool mixed);
 
 constexpr int kMinKz = 2;
 
+// Allocation guard.  While a NoAllocScope lives on a thread, every path of
+// the engine that would allocate device or pinned host memory, create an
+// event or instantiate a graph fails instead (SULI_PS_ENOMEM, with a
+// message naming the site), so an entry point that promises not to allocate
+// cannot do so by accident.
+bool no_alloc_active();
+class NoAllocScope {
+public:
+    NoAllocScope();
+    ~NoAllocScope();
+    NoAllocScope(const NoAllocScope &) = delete;
+    NoAllocScope &operator=(const NoAllocScope &) = delete;
+
+private:
+    bool prev_;
+};
+
 class Engine {
 public:
     Engine() = default;
     ~Engine();
     int init(hipStream_t stream, int max_slots);
     const std::string &error() const { return err_; }
+    // Sizes the launch tables, the finish tables, the staging ring and the
+    // host-side lists for `entries` table entries, so that solves whose
+    // tables fit need no allocation (see NoAllocScope).  Never shrinks.
+    int reserve(int entries);
+    int reserved_entries() const { return reserved_; }
+    // Tile shape a size would get without tuning now: the tuned or fixed
+    // shape if one is recorded, else the default rule (kz, vx = 0).
+    void cached_shape(int nx, int ny, int nz, bool mixed, int *kz, int *vx) const;
 
     // Admits a problem: allocates its slot, enqueues the setup kernels on the
     // stream and schedules its iteration 1 at the next global step.
@@ -196,6 +222,10 @@ private:
     int ntimed_ = 0;
     double spmv_ms_ = 0.0;
     int spmv_n_ = 0;
+    // host-side lists of run_all / pump / finish, kept with their capacity
+    std::vector<int> run_slots_, run_finished_, done_, ran_;
+    std::vector<TileRef> fin_t3_;
+    int reserved_ = 0;
 };
 
 }  // namespace sps

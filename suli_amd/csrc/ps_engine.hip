@@ -85,16 +85,30 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
 
 }  // namespace
 
+// ------------------------------------------------------ allocation guard ----
+namespace {
+thread_local bool t_no_alloc = false;
+}
+
+bool no_alloc_active() { return t_no_alloc; }
+NoAllocScope::NoAllocScope() : prev_(t_no_alloc) { t_no_alloc = true; }
+NoAllocScope::~NoAllocScope() { t_no_alloc = prev_; }
+
 // ---------------------------------------------------------------- Pool ----
 Pool::~Pool() { release_all(); }
 
 void *Pool::get(size_t bytes, std::string &err)
 {
-    auto it = free_.find(bytes);
-    if (it != free_.end()) {
-        void *p = it->second;
-        free_.erase(it);
-        return p;
+    for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].first == bytes) {
+            void *p = free_[i].second;
+            free_[i] = free_.back();
+            free_.pop_back();
+            return p;
+        }
+    if (no_alloc_active()) {
+        err = "no-allocation call: the pool holds no free block of this size";
+        return nullptr;
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -114,7 +128,7 @@ void *Pool::get(size_t bytes, std::string &err)
 
 void Pool::put(void *p, size_t bytes)
 {
-    if (p) free_.emplace(bytes, p);
+    if (p) free_.emplace_back(bytes, p);
 }
 
 void Pool::release_all()
@@ -147,6 +161,12 @@ int Engine::fail(int code, const std::string &msg)
     err_ = msg;
     return code;
 }
+
+// Fails an allocating path while a NoAllocScope is active.
+#define NO_ALLOC_HERE(what)                                                                            \
+    do {                                                                                               \
+        if (no_alloc_active()) return fail(SULI_PS_ENOMEM, std::string("no-allocation call would ") + (what)); \
+    } while (0)
 
 Engine::~Engine()
 {
@@ -196,6 +216,63 @@ int Engine::init(hipStream_t stream, int max_slots)
     return 0;
 }
 
+int Engine::reserve(int entries)
+{
+    if (entries <= reserved_) return 0;
+    NO_ALLOC_HERE("reserve tables");
+    ECHK(hipStreamSynchronize(s_));
+    const size_t ntab = (size_t)entries;
+    if (ntab > tab_cap_ || (size_t)max_slots_ > probs_cap_ || !d_tab_) {
+        if (d_tab_) ECHK(hipFree(d_tab_));
+        if (d_probs_list_) ECHK(hipFree(d_probs_list_));
+        d_tab_ = nullptr;
+        d_probs_list_ = nullptr;
+        tab_cap_ = std::max(tab_cap_, ntab);
+        probs_cap_ = std::max(probs_cap_, (size_t)max_slots_);
+        ECHK(hipMalloc(&d_tab_, sizeof(TileRef) * tab_cap_));
+        ECHK(hipMalloc(&d_probs_list_, sizeof(int) * probs_cap_));
+    }
+    // finish tables: every 3-D tile plus the problem list
+    const size_t fin = align_up(sizeof(TileRef) * ntab, 256) + sizeof(int) * max_slots_;
+    if (fin > fin_cap_) {
+        if (d_fin_) ECHK(hipFree(d_fin_));
+        d_fin_ = nullptr;
+        fin_cap_ = fin;
+        ECHK(hipMalloc(&d_fin_, fin_cap_));
+    }
+    const size_t stage = std::max({sizeof(TileRef) * ntab + sizeof(int) * max_slots_ + 64, fin,
+                                   sizeof(CgScalars) + sizeof(ProbDev), (size_t)65536});
+    if (stage > stage_cap_) {
+        for (int i = 0; i < 4; ++i) {
+            if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
+            h_stage_[i] = nullptr;
+            ECHK(hipHostMalloc(&h_stage_[i], stage, hipHostMallocDefault));
+        }
+        stage_cap_ = stage;
+    }
+    tab_.t3_64.reserve(ntab);
+    tab_.t1_64.reserve(ntab);
+    tab_.t3_32.reserve(ntab);
+    tab_.t1_32.reserve(ntab);
+    tab_.mixed_probs.reserve(max_slots_);
+    pool_.reserve(max_slots_);
+    fin_t3_.reserve(ntab);
+    run_slots_.reserve(max_slots_);
+    run_finished_.reserve(max_slots_);
+    done_.reserve(max_slots_);
+    ran_.reserve(2 * kMaxTimed);
+    reserved_ = entries;
+    tables_dirty_ = true;
+    return 0;
+}
+
+void Engine::cached_shape(int nx, int ny, int nz, bool mixed, int *kz, int *vx) const
+{
+    auto hit = tuned_.find(std::make_tuple(nx, ny, nz, mixed));
+    *kz = hit != tuned_.end() ? hit->second.kz : 0;
+    *vx = hit != tuned_.end() ? hit->second.vx : 0;
+}
+
 int Engine::active_count() const
 {
     int n = 0;
@@ -218,7 +295,13 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     sl = Slot{};
     int kz = o.kz > 0 ? std::max(o.kz, kMinKz) : 0;
     int vx = o.vx > 0 ? o.vx : 0;
-    if ((kz == 0 || vx == 0) && autotune_) {
+    if ((kz == 0 || vx == 0) && autotune_ && no_alloc_active()) {
+        // tuning allocates a scratch problem: take what is recorded instead
+        int kz_c = 0, vx_c = 0;
+        cached_shape(gr.nx, gr.ny, gr.nz, o.mixed, &kz_c, &vx_c);
+        if (kz == 0) kz = kz_c;
+        if (vx == 0) vx = vx_c;
+    } else if ((kz == 0 || vx == 0) && autotune_) {
         int kz_t = 0, vx_t = 1;
         double us = 0.0;
         int rc_t = tune(gr.nx, gr.ny, gr.nz, o.mixed, &kz_t, &vx_t, &us);
@@ -286,6 +369,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     {
         // a staging buffer whose previous copy has completed
         if (stage_cap_ < need) {
+            NO_ALLOC_HERE("grow the staging ring");
             ECHK(hipStreamSynchronize(s_));
             for (int i = 0; i < 4; ++i) {
                 if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
@@ -357,6 +441,7 @@ int Engine::rebuild_tables()
     const size_t ntab = off_t1_32 + tab_.t1_32.size();
     const size_t nprobs = tab_.mixed_probs.size();
     if (ntab > tab_cap_ || nprobs > probs_cap_ || !d_tab_) {
+        NO_ALLOC_HERE("grow the launch tables");
         // grow: buffers still read by queued kernels are freed after a sync
         ECHK(hipStreamSynchronize(s_));
         if (d_tab_) ECHK(hipFree(d_tab_));
@@ -368,6 +453,7 @@ int Engine::rebuild_tables()
     }
     const size_t need = sizeof(TileRef) * ntab + sizeof(int) * nprobs + 64;
     if (need > stage_cap_) {
+        NO_ALLOC_HERE("grow the staging ring");
         ECHK(hipStreamSynchronize(s_));
         for (int i = 0; i < 4; ++i) {
             if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
@@ -510,6 +596,7 @@ void Engine::reap_graphs(bool wait)
 // its first launch stores the next group's step in counter 1 - p.
 int Engine::capture_graphs()
 {
+    NO_ALLOC_HERE("capture a graph");
     std::unique_lock<std::mutex> lk;
     if (capture_mutex_) lk = std::unique_lock<std::mutex>(*capture_mutex_);
     retire_graphs();
@@ -578,7 +665,8 @@ constexpr long long kDirectCells = 1LL << 20;
 int Engine::enqueue_chunk(int K, bool timing)
 {
     int rc;
-    if (graphs_ && !timing && table_cells_ < kDirectCells) {
+    // capture and instantiation allocate: a no-allocation call launches directly
+    if (graphs_ && !timing && table_cells_ < kDirectCells && !no_alloc_active()) {
         for (int it = 0; it < K; it += kRefreshGrain)
             if ((rc = enqueue_group_graph())) return rc;
     } else if ((rc = enqueue_steps_direct(K, timing))) {
@@ -600,12 +688,14 @@ int Engine::finish(const std::vector<int> &done)
 {
     if (done.empty()) return 0;
     // tables for the finished problems, staged like the launch tables
-    std::vector<TileRef> t3;
+    std::vector<TileRef> &t3 = fin_t3_;
+    t3.clear();
     for (int si : done)
         for (int t = 0; t < slots_[si].pd.nt3; ++t) t3.push_back({si, t});
     const size_t poff = align_up(sizeof(TileRef) * t3.size(), 256);
     const size_t need = poff + sizeof(int) * done.size();
     if (need > fin_cap_) {
+        NO_ALLOC_HERE("grow the finish tables");
         // grow: the old buffer may still be read by queued kernels
         ECHK(hipStreamSynchronize(s_));
         if (d_fin_) ECHK(hipFree(d_fin_));
@@ -613,6 +703,7 @@ int Engine::finish(const std::vector<int> &done)
         ECHK(hipMalloc(&d_fin_, fin_cap_));
     }
     if (need > stage_cap_) {
+        NO_ALLOC_HERE("grow the staging ring");
         ECHK(hipStreamSynchronize(s_));
         for (int i = 0; i < 4; ++i) {
             if (h_stage_[i]) ECHK(hipHostFree(h_stage_[i]));
@@ -659,7 +750,8 @@ int Engine::pump(bool timing, std::vector<int> &finished)
     ECHK(hipEventSynchronize(ev_status_[which]));
     const CgScalars *h = h_status_ + (size_t)which * max_slots_;
     const int last_step = status_step_[which];
-    std::vector<int> done;
+    std::vector<int> &done = done_;
+    done.clear();
     for (int i = 0; i < max_slots_; ++i) {
         Slot &sl = slots_[i];
         // problems admitted after this chunk was queued took no part in it
@@ -688,10 +780,12 @@ int Engine::run_all(bool timing)
     int rc;
     ntimed_ = 0;
     if (timing) ECHK(hipMemsetAsync(d_ran_, 0, sizeof(int) * 2 * kMaxTimed, s_));
-    std::vector<int> all;
+    std::vector<int> &all = run_slots_;
+    all.clear();
     for (int i = 0; i < max_slots_; ++i)
         if (slots_[i].used && slots_[i].active && !slots_[i].finished) all.push_back(i);
-    std::vector<int> finished;
+    std::vector<int> &finished = run_finished_;
+    finished.clear();
     const int max_rounds = 1 << 20;
     int r = 0;
     for (; r < max_rounds && (active_count() > 0 || npend_ > 0); ++r)
@@ -704,8 +798,10 @@ int Engine::run_all(bool timing)
     spmv_n_ = 0;
     if (timing && ntimed_ > 0) {
         // windows in which problem 0 iterated at both ends
-        std::vector<int> ran(2 * ntimed_);
-        ECHK(hipMemcpy(ran.data(), d_ran_, sizeof(int) * 2 * ntimed_, hipMemcpyDeviceToHost));
+        std::vector<int> &ran = ran_;
+        ran.assign(2 * ntimed_, 0);
+        ECHK(hipMemcpyAsync(ran.data(), d_ran_, sizeof(int) * 2 * ntimed_, hipMemcpyDeviceToHost, s_));
+        ECHK(hipStreamSynchronize(s_));
         double sum = 0.0;
         for (int t = 0; t < ntimed_; ++t) {
             if (!ran[2 * t] || !ran[2 * t + 1]) continue;
@@ -735,6 +831,7 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         *us_out = hit->second.us;
         return 0;
     }
+    NO_ALLOC_HERE("tune a tile shape");
     const Geo g_def = make_geo(nx, ny, nz, 0, 0, 0);
     std::vector<int> cands = {2, 3, 4, 6, 8, 12, 16, 24, 32, 48};
     if (std::find(cands.begin(), cands.end(), g_def.kz) == cands.end()) cands.push_back(g_def.kz);

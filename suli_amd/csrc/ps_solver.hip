@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,9 +18,52 @@
 
 using namespace sps;
 
+// Solver state of suli_ps_graddin_noalloc_dev: one problem's block (the
+// arrays Engine::admit carves), SULI_PS_NOALLOC_MAX_BYTES on every GPU the
+// library runs on, held until the process ends.  In host code `scratch` is
+// only a stand-in for the device array: its device address is looked up
+// once per device (scratch_address) and that address is what the engine
+// and the kernels are given.
+static __device__ __align__(256) unsigned char scratch[SULI_PS_NOALLOC_MAX_BYTES];
+
 namespace {
 
 constexpr int kMaxSlots = 256;   // problems resident at once per context
+// Launch-table entries reserved by every context for the no-allocation entry
+// point (3-D and 1-D tiles of its one problem).
+constexpr int kNoAllocEntries = 32768;
+constexpr int kMaxDevices = 64;
+
+struct ScratchTable {
+    std::mutex mu;
+    bool looked_up[kMaxDevices] = {};
+    hipError_t err[kMaxDevices] = {};
+    void *ptr[kMaxDevices] = {};
+};
+ScratchTable &scratch_table()
+{
+    static ScratchTable t;
+    return t;
+}
+
+// Device address of `scratch` on `device`, which must be current: looked up
+// with hipGetSymbolAddress the first time, then kept (each GPU holds its own
+// copy of the array at its own address).
+hipError_t scratch_address(int device, void **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    ScratchTable &t = scratch_table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (!t.looked_up[device]) {
+        void *p = nullptr;
+        t.err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        t.ptr[device] = t.err[device] == hipSuccess ? p : nullptr;
+        t.looked_up[device] = true;
+    }
+    *out = t.ptr[device];
+    return t.err[device];
+}
 
 // Device copies of the reference-layout arrays for the host-pointer API.
 struct Staging {
@@ -42,6 +86,8 @@ struct suli_ps_ctx {
     Staging st;
     std::vector<int> last_slots;   // slots of the last solve (kept for time_spmv)
     hipEvent_t ev[2] = {nullptr, nullptr};
+    void *scratch = nullptr;       // device address of `scratch` on this device
+    hipError_t scratch_err = hipSuccess;
 };
 
 namespace {
@@ -114,9 +160,12 @@ void warn_capped(const suli_ps_ctx *ctx, const suli_ps_stats &st)
     }
 }
 
+// block: device memory for the problem's solver state (problem_bytes), or
+// NULL for the engine's pool.
 int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, const double *v, const double *w,
                     const double *rho, const double *ox, const double *oy, const double *oz, const double *dz,
-                    const double *dzz, double *prd1, suli_ps_stats *stats, const uint8_t *solid = nullptr)
+                    const double *dzz, double *prd1, suli_ps_stats *stats, const uint8_t *solid = nullptr,
+                    void *block = nullptr)
 {
     int rc = check_problem(ctx, gr, u, v, w, rho, ox, oy, oz, dz, dzz, prd1);
     if (rc) return rc;
@@ -125,7 +174,7 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
     const SetupIn in = make_setup(gr, u, v, w, rho, prd1, ox, oy, oz, dz, dzz, solid);
     int slot = -1;
-    if ((rc = ctx->eng.admit(*gr, in, prd1, solve_opts(ctx->opts), 0, &slot)))
+    if ((rc = ctx->eng.admit(*gr, in, prd1, solve_opts(ctx->opts), 0, &slot, block)))
         return fail(ctx, rc, "%s", ctx->eng.error().c_str());
     ctx->last_slots.push_back(slot);
     if ((rc = ctx->eng.run_all(ctx->opts.timing != 0))) return fail(ctx, rc, "%s", ctx->eng.error().c_str());
@@ -343,10 +392,15 @@ suli_ps_ctx *suli_ps_create(const suli_ps_opts *opts)
               hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&ctx->ev[0]) == hipSuccess && hipEventCreate(&ctx->ev[1]) == hipSuccess;
     ok = ok && ctx->eng.init(ctx->stream, kMaxSlots) == 0;
+    // what suli_ps_graddin_noalloc_dev needs besides the scratch array
+    ok = ok && ctx->eng.reserve(kNoAllocEntries) == 0;
     if (!ok) {
         suli_ps_destroy(ctx);
         return nullptr;
     }
+    ctx->last_slots.reserve(kMaxSlots);
+    // a failed lookup only disables the no-allocation entry point
+    ctx->scratch_err = scratch_address(ctx->device, &ctx->scratch);
     apply_engine_opts(ctx);
     return ctx;
 }
@@ -388,6 +442,43 @@ int suli_ps_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u
 {
     if (!ctx) return SULI_PS_EINVAL;
     return run_graddin_dev(ctx, g, u, v, w, rho, obs_lsx, obs_lsy, obs_lsz, dz, dzz, prd1, stats);
+}
+
+int suli_ps_graddin_noalloc_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, const double *v,
+                                const double *w, const double *rho, const double *obs_lsx, const double *obs_lsy,
+                                const double *obs_lsz, const double *dz, const double *dzz, double *prd1,
+                                suli_ps_stats *stats)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_problem(ctx, g, u, v, w, rho, obs_lsx, obs_lsy, obs_lsz, dz, dzz, prd1);
+    if (rc) return rc;
+    if (ctx->scratch_err != hipSuccess || !ctx->scratch)
+        return fail(ctx, SULI_PS_EHIP, "no-allocation solve: the scratch array of device %d is unavailable (%s)",
+                    ctx->device, hipGetErrorString(ctx->scratch_err));
+    int cur = -1;
+    HIPCHK(ctx, hipGetDevice(&cur));
+    if (cur != ctx->device)
+        return fail(ctx, SULI_PS_EINVAL, "no-allocation solve: device %d is current, the context's is %d", cur,
+                    ctx->device);
+    const bool mixed = ctx->opts.precision == SULI_PS_MIXED;
+    const size_t need = problem_bytes(g->nx, g->ny, g->nz, mixed);
+    if (need > (size_t)SULI_PS_NOALLOC_MAX_BYTES)
+        return fail(ctx, SULI_PS_EINVAL,
+                    "no-allocation solve: a %dx%dx%d %s problem needs %zu bytes of solver state; the fixed array "
+                    "holds %zu (SULI_PS_NOALLOC_MAX_BYTES)",
+                    g->nx, g->ny, g->nz, mixed ? "mixed" : "fp64", need, (size_t)SULI_PS_NOALLOC_MAX_BYTES);
+    // the launch tables this solve will fill (the shape Engine::admit picks)
+    int kz = 0, vx = 0;
+    if (ctx->opts.autotune) ctx->eng.cached_shape(g->nx, g->ny, g->nz, mixed, &kz, &vx);
+    const Geo geo = make_geo(g->nx, g->ny, g->nz, g->ccx0 == 0, g->ccy0 == 0, kz > 0 ? std::max(kz, kMinKz) : 0,
+                             mixed ? vx : 1);
+    const long long entries = (long long)spmv_tiles(geo) + vec_tiles(geo.n);
+    if (entries > ctx->eng.reserved_entries())
+        return fail(ctx, SULI_PS_EINVAL, "no-allocation solve: %lld launch-table entries needed, %d reserved",
+                    entries, ctx->eng.reserved_entries());
+    NoAllocScope guard;
+    return run_graddin_dev(ctx, g, u, v, w, rho, obs_lsx, obs_lsy, obs_lsz, dz, dzz, prd1, stats, nullptr,
+                           ctx->scratch);
 }
 
 int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v, double *w, const double *rho,

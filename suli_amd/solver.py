@@ -180,6 +180,20 @@ class PressureSolver:
                 p["obs_lsz"], p["dz"], p["dzz"], p["prd1"], C.byref(st)))
         return st
 
+    def graddin_noalloc_dev(self, grid: Grid, df: DeviceFields) -> _lib.Stats:
+        """graddin_dev whose solver state lives in the library's fixed device
+        array (suli_ps_graddin_noalloc_dev): no allocation during the call,
+        problems up to NOALLOC_MAX_BYTES of state, no solid mask."""
+        st = _lib.Stats()
+        cg = _c_grid(grid)
+        p = df.ptrs
+        if p.get("solid") is not None:
+            raise ValueError("the no-allocation solve takes no solid-cell mask")
+        self._check(self.lib.suli_ps_graddin_noalloc_dev(
+            self.ctx, C.byref(cg), p["u"], p["v"], p["w"], p["rho"], p["obs_lsx"], p["obs_lsy"],
+            p["obs_lsz"], p["dz"], p["dzz"], p["prd1"], C.byref(st)))
+        return st
+
     def graddin_batch_dev(self, items) -> list:
         """Solve many problems together; ``items`` is a sequence of
         ``(grid, DeviceFields)``.  Returns one Stats per problem."""
