@@ -1,11 +1,11 @@
-"""Compiles the Fortran side of the drop-in.
+"""Compiles the Fortran side of the drop-in that stands alone.
 
-* suli_amd/fortran/suli_ps_iface.f90 (the iso_c_binding interface, which
-  depends on nothing from SuLi) is always compiled, as a check.
-* suli_ps_graddin.F90 uses SuLi's own modules (velpre, disc, cond, obst,
-  ls_param), so it is compiled inside a SuLi build; where the reference
-  sources are present, `make -C oracle dropin` builds the check programs
-  oracle/_ref/dropin_ic and dropin_icdz that link it with SuLi's modules.
+suli_amd/fortran/suli_ps_iface.f90 (the iso_c_binding interface, which
+depends on nothing from SuLi) is compiled here, as a check.  The shims
+suli_ps_graddin*.F90 use SuLi's own modules (velpre, disc, cond, obst,
+ls_param), so they are compiled inside a SuLi build; nothing here compiles
+SuLi's sources.  (The test programs that link the shims with SuLi's modules
+are built with the checker, `make -C oracle dropin`.)
 """
 from __future__ import annotations
 
@@ -33,12 +33,8 @@ def build(verbose: bool = False) -> None:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran interface failed to compile:\n{r.stdout}\n{r.stderr}")
-    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "dropin"],
-                       capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"drop-in check programs failed to build:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print("built Fortran interface" + (r.stdout.strip() and f" ({r.stdout.strip()})"))
+        print("built Fortran interface")
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
 """The Fortran drop-in (suli_amd/fortran/suli_ps_graddin.F90) inside a
 SuLi-style program: graddin()/posdin() through iso_c_binding on the GPU
 versus the reference's own graddin/posdin on the same module arrays.
-The check programs are built by `make -C oracle dropin` where the
+The check programs (tests/fortran/) are built by `make -C oracle dropin` where the
 reference sources are present."""
 import os
 import re
