@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define SULI_PS_ABI_VERSION 1
+#define SULI_PS_ABI_VERSION 2
 
 enum suli_ps_precision {
     SULI_PS_FP64 = 0,   /* FP64 CG, as the reference */
@@ -81,6 +81,15 @@ typedef struct suli_ps_opts {
     int32_t autotune;      /* 1: pick each grid size's tile depth by timing the
                               CG step kernel on its first solve (see
                               suli_ps_autotune); 0: a fixed rule */
+    int32_t reuse_staged;  /* 1: suli_ps_posdin on host pointers, called right
+                              after suli_ps_graddin on the same grid with the
+                              same u, v, w, rho and prd1 pointers, takes those
+                              arrays from the device copies graddin left (the
+                              inputs as uploaded, prd1 as solved) instead of
+                              uploading them again.  The caller promises not
+                              to change them in between, as SuLi's loop does
+                              (SuLi-IC/1_principal.f90:102-103).  0: upload. */
+    int32_t reserved;
 } suli_ps_opts;
 
 typedef struct suli_ps_stats {
@@ -97,6 +106,10 @@ typedef struct suli_ps_stats {
     double spmv_ms;        /* mean time of one CG step (its launches, SpMV
                               included) over windows of 8 consecutive steps,
                               HIP events on the context stream (opts.timing) */
+    int32_t graph_replays; /* captured HIP graphs launched during the call (each
+                              runs 8 CG steps); 0 when every kernel was
+                              launched directly.  0 for served requests. */
+    int32_t reserved;
 } suli_ps_stats;
 
 typedef struct suli_ps_ctx suli_ps_ctx;

@@ -60,7 +60,7 @@ class Opts(C.Structure):
         ("precision", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
         ("max_iter", C.c_int32), ("chunk", C.c_int32), ("verbose", C.c_int32),
         ("timing", C.c_int32), ("keep_null", C.c_int32), ("no_graphs", C.c_int32),
-        ("autotune", C.c_int32),
+        ("autotune", C.c_int32), ("reuse_staged", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -70,6 +70,7 @@ class Stats(C.Structure):
         ("spmv_timed", C.c_int32),
         ("rr0", C.c_double), ("bb", C.c_double), ("rr", C.c_double), ("thresh", C.c_double),
         ("rel_true", C.c_double), ("solve_ms", C.c_double), ("spmv_ms", C.c_double),
+        ("graph_replays", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self):

@@ -151,6 +151,8 @@ public:
     void set_chunk(int steps) { chunk_ = steps; }
     bool graphs() const { return graphs_; }
     int graph_captures() const { return captures_; }
+    // captured graphs launched since the last run_all began
+    int graph_replays() const { return replays_; }
 
 private:
     int fail(int code, const std::string &msg);
@@ -214,6 +216,7 @@ private:
     bool graph_valid_ = false;
     hipGraphExec_t gx_[4] = {nullptr, nullptr, nullptr, nullptr};   // [parity][replacement]
     int captures_ = 0;
+    int replays_ = 0;
     // replaced graphs, destroyed once the stream has passed this event
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired_;
     // timing

@@ -650,6 +650,7 @@ int Engine::enqueue_group_graph()
     if (!dstep_valid_) launch_set_step(d_step_ + par, step_, s_);
     const bool refresh = !tab_.mixed_probs.empty() && refresh_may_be_due(step_ + kRefreshGrain);
     ECHK(hipGraphLaunch(gx_[2 * par + (refresh ? 1 : 0)], s_));
+    ++replays_;
     step_ += kRefreshGrain;
     dstep_valid_ = true;
     return 0;
@@ -779,6 +780,7 @@ int Engine::run_all(bool timing)
 {
     int rc;
     ntimed_ = 0;
+    replays_ = 0;
     if (timing) ECHK(hipMemsetAsync(d_ran_, 0, sizeof(int) * 2 * kMaxTimed, s_));
     std::vector<int> &all = run_slots_;
     all.clear();

@@ -73,6 +73,15 @@ struct Staging {
     double *obsx = nullptr, *obsy = nullptr, *obsz = nullptr, *dz = nullptr, *dzz = nullptr;
     double *hs = nullptr;
     double *dzx = nullptr, *dzy = nullptr, *dz3 = nullptr, *eta1 = nullptr, *deltah = nullptr;   // SuLi-IT
+    // What the last suli_ps_graddin left on the device (opts.reuse_staged):
+    // its host pointers and grid, with u, v, w, rho as uploaded and prd1 as
+    // solved.  Cleared by every other host-pointer call.
+    struct Left {
+        bool valid = false;
+        suli_ps_grid grid{};
+        const double *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *prd1 = nullptr;
+        const double *dz = nullptr, *dzz = nullptr;
+    } left;
 };
 
 }  // namespace
@@ -187,6 +196,7 @@ int run_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *u, c
     st.solve_ms = ms;
     st.spmv_timed = ctx->eng.spmv_samples();
     st.spmv_ms = ctx->eng.spmv_ms_mean();
+    st.graph_replays = ctx->eng.graph_replays();
     if (stats) *stats = st;
     warn_capped(ctx, st);
     return st.status;
@@ -322,6 +332,7 @@ int run_it_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr0, double tetah, 
     st.solve_ms = ms;
     st.spmv_timed = ctx->eng.spmv_samples();
     st.spmv_ms = ctx->eng.spmv_ms_mean();
+    st.graph_replays = ctx->eng.graph_replays();
     if (stats) *stats = st;
     warn_capped(ctx, st);
     return st.status;
@@ -512,6 +523,7 @@ int suli_ps_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, co
     if (!ctx) return SULI_PS_EINVAL;
     int rc = check_problem(ctx, g, u, v, w, rho, obs_lsx, obs_lsy, obs_lsz, dz, dzz, prd1);
     if (rc) return rc;
+    ctx->st.left.valid = false;
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -539,6 +551,18 @@ int suli_ps_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, co
     if (rc < 0) return rc;
     HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
+    if (rho) {
+        Staging::Left &l = st.left;
+        l.grid = *g;
+        l.u = u;
+        l.v = v;
+        l.w = w;
+        l.rho = rho;
+        l.prd1 = prd1;
+        l.dz = dz;
+        l.dzz = dzz;
+        l.valid = true;
+    }
     return rc;
 }
 
@@ -550,6 +574,12 @@ int suli_ps_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v
     if (rc) return rc;
     if (!u || !v || !w || !rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho and prd1 must be non-NULL");
     if ((dz == nullptr) != (dzz == nullptr)) return fail(ctx, SULI_PS_EINVAL, "dz and dzz go together");
+    // the arrays graddin just staged, as SuLi calls posdin right after it
+    // (1_principal.f90:102-103), when the caller allows it
+    const Staging::Left &l = ctx->st.left;
+    const bool reuse = ctx->opts.reuse_staged && l.valid && std::memcmp(&l.grid, g, sizeof *g) == 0 && l.u == u &&
+                       l.v == v && l.w == w && l.rho == rho && l.prd1 == prd1 && l.dz == dz && l.dzz == dzz;
+    ctx->st.left.valid = false;
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -558,14 +588,16 @@ int suli_ps_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
     hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    if (dz) {
-        HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+    if (!reuse) {
+        HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+        if (dz) {
+            HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+            HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+        }
     }
     rc = run_posdin_dev(ctx, g, st.u, st.v, st.w, st.rho, st.prd1, dz ? st.dz : nullptr, dz ? st.dzz : nullptr,
                         nullptr);
@@ -592,6 +624,7 @@ int suli_ps_pressh(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, c
     int rc = check_columns(ctx, g, dz);
     if (rc) return rc;
     if (!rho || !prd1) return fail(ctx, SULI_PS_EINVAL, "rho and prd1 must be non-NULL");
+    ctx->st.left.valid = false;
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const size_t n = (size_t)g->nx * g->ny * g->nz;
@@ -630,6 +663,7 @@ int suli_ps_it_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, do
     if (rc) return rc;
     if (!u || !v || !w || !dzx || !dzy || !dzz || !eta1 || !deltah || !prd1)
         return fail(ctx, SULI_PS_EINVAL, "every SuLi-IT array must be non-NULL");
+    ctx->st.left.valid = false;
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -661,6 +695,7 @@ int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, dou
     int rc = check_it(ctx, g, tetah, gz);
     if (rc) return rc;
     if (!u || !v || !w || !dzz || !prd1) return fail(ctx, SULI_PS_EINVAL, "u, v, w, dzz and prd1 must be non-NULL");
+    ctx->st.left.valid = false;
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -717,6 +752,7 @@ int suli_ps_posdin_hydro(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, dou
     if (rc) return rc;
     if (!u || !v || !w || !rho || !hs || !prd1)
         return fail(ctx, SULI_PS_EINVAL, "u, v, w, rho, hs and prd1 must be non-NULL");
+    ctx->st.left.valid = false;
     if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
     Staging &st = ctx->st;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -779,6 +815,7 @@ int suli_ps_graddin_batch_dev(suli_ps_ctx *ctx, suli_ps_problem *probs, int32_t 
     for (int i = 0; i < count; ++i) {
         fill_stats(ctx->eng.slot(slots[i]), &probs[i].stats);
         probs[i].stats.solve_ms = ms;
+        probs[i].stats.graph_replays = ctx->eng.graph_replays();
         worst = std::max(worst, probs[i].stats.status);
     }
     return worst;

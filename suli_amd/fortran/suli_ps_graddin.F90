@@ -36,6 +36,10 @@ contains
          if (ios /= 0) o%max_iter = 10000
       end if
       o%verbose = 0
+      ! SuLi's loop calls posdin() right after graddin() on the same module
+      ! arrays (1_principal.f90:102-103): posdin takes u, v, w, rho and prd1
+      ! from the device copies graddin left instead of uploading them again
+      o%reuse_staged = 1
       suli_ps_ctx = suli_ps_create(o)
       if (.not. c_associated(suli_ps_ctx)) then
          write (*, *) "suli_ps: no HIP device / context creation failed"

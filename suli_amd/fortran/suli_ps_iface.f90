@@ -33,12 +33,15 @@ module suli_ps_iface
       integer(c_int32_t) :: keep_null = 0
       integer(c_int32_t) :: no_graphs = 0
       integer(c_int32_t) :: autotune = 0
+      integer(c_int32_t) :: reuse_staged = 0
+      integer(c_int32_t) :: reserved = 0
    end type suli_ps_opts
 
    ! struct suli_ps_stats
    type, bind(C), public :: suli_ps_stats
       integer(c_int32_t) :: status, iters, refreshes, spmv_timed
       real(c_double) :: rr0, bb, rr, thresh, rel_true, solve_ms, spmv_ms
+      integer(c_int32_t) :: graph_replays, reserved
    end type suli_ps_stats
 
    public :: suli_ps_create, suli_ps_destroy, suli_ps_graddin, suli_ps_posdin, suli_ps_last_error

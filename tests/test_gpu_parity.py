@@ -167,12 +167,15 @@ def test_mixed_cap_four_cells_per_lane(solver, max_iter):
     print(f"mixed cap vx=4 max_iter={max_iter}", flush=True)
     p64, _ = solve(solver, c.grid, c.fields, stop_rule="relative", tol=1e-14, max_iter=max_iter)
     solver.set_tiles(c.grid, "mixed", 4, 4)
-    solver.configure(precision="mixed", stop_rule="relative", tol=1e-14, max_iter=max_iter, verbose=False,
-                     autotune=True)
-    assert solver.autotune(c.grid, "mixed")[:2] == (4, 4)
-    f2 = c.fields.copy()
-    st32 = solver.graddin(c.grid, f2)
-    solver.configure(precision="fp64", stop_rule="reference", verbose=False)
+    try:
+        solver.configure(precision="mixed", stop_rule="relative", tol=1e-14, max_iter=max_iter,
+                         verbose=False, autotune=True)
+        assert solver.autotune(c.grid, "mixed")[:2] == (4, 4)
+        f2 = c.fields.copy()
+        st32 = solver.graddin(c.grid, f2)
+    finally:
+        # the module's solver goes back to its defaults whatever happened
+        solver.configure(precision="fp64", stop_rule="reference", verbose=False, autotune=False)
     assert st32.status == 1 and st32.iters == max_iter, st32.as_dict()
     diff = helpers.rel_diff(f2.prd1, p64, helpers.face_mask(p64.shape))
     print(f"mixed (vx=4) vs fp64 iterate after {max_iter}: {diff:.3e}", flush=True)
@@ -359,3 +362,33 @@ def test_config1_dirichlet_full_size(solver):
         assert st.status == 0 and st.rel_true <= 1.05e-10, st.as_dict()
         mask = helpers.face_mask(prd1.shape)
         assert helpers.rel_diff(prd1, p_or, mask) <= 1e-7, precision
+
+
+def test_posdin_reuses_what_graddin_staged():
+    """opts.reuse_staged: the host-pointer posdin right after graddin takes
+    u, v, w, rho and prd1 from the device copies graddin left (SuLi calls the
+    two back to back, 1_principal.f90:102-103) with the same bits as
+    uploading them; an array the caller replaced in between (a new pointer)
+    is uploaded."""
+    from suli_amd.solver import PressureSolver
+    c = cases.random_case(40, 24, 16, seed=3)
+    g = c.grid
+    kw = dict(precision="mixed", stop_rule="relative", tol=1e-10, verbose=False)
+    with PressureSolver(reuse_staged=True, **kw) as s1, PressureSolver(reuse_staged=False, **kw) as s0:
+        fa, fb = c.fields.copy(), c.fields.copy()
+        s1.graddin(g, fa)
+        s1.posdin(g, fa)
+        s0.graddin(g, fb)
+        s0.posdin(g, fb)
+        for name in ("u", "v", "w", "prd1"):
+            assert np.array_equal(getattr(fa, name), getattr(fb, name)), name
+        # replaced between the calls: the new array is what posdin corrects
+        fa, fb = c.fields.copy(), c.fields.copy()
+        s1.graddin(g, fa)
+        fa.u = np.asfortranarray(fa.u * 0.5)
+        s1.posdin(g, fa)
+        fb.u = np.asfortranarray(fb.u * 0.5)
+        s0.graddin(g, c.fields.copy())
+        fb.prd1 = fa.prd1.copy(order="F")
+        s0.posdin(g, fb)
+        assert np.array_equal(fa.u, fb.u) and np.array_equal(fa.w, fb.w)
