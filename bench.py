@@ -8,31 +8,36 @@ input already resident in HBM.  The mixed-precision solver (FP32 SpMV/axpy,
 FP64 dots, scalars and true residual) is the headline; the FP64 solver runs
 the same steps for the comparison.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-solves C]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-solves C] [--no-cpu]
 
-Prints one JSON line.  `roofline` is for the dominant kernel, k_cg_m (the
-whole FP32 CG step of the mixed solver in one launch: residual and iterate
-updates of the previous iteration, direction update, SpMV, four dot-product
-partials): algorithmic bytes 9 words/cell on average (every step reads
-q and the last two p of the previous steps, from which it recovers r, and
-the three face-coefficient arrays, and writes p and q; every fourth step
-also reads the correction c and the two older p of the direction ring and
-writes c) divided by its mean launch time, from HIP
-events on the solver's stream around windows of 8 consecutive launches (one
-window per 16 steps)
-during two directly launched solves.  The FP64 solver's step (k_cg_a +
-k_cg_b, 12 FP64 words/cell) is reported beside it.  `traffic` is the HBM
-bytes per launch from rocprofv3's FETCH_SIZE and WRITE_SIZE counters
+Prints one JSON line.
+
+`roofline` is for the dominant kernel, k_cg_m, the whole FP32 CG iteration of
+the mixed solver in one launch.  `achieved` counts the algorithmic bytes of
+one CG iteration, 14 FP32 words per cell (SpMV 5: p and the three face
+coefficients read, Ap written; iterate, residual and direction 3 each), i.e.
+56 B/cell, divided by the kernel's mean launch time, which HIP events on the
+solver's stream measure around windows of 8 consecutive launches (one window
+per 16 steps) during two directly launched solves; a window's time includes
+the gaps between its launches.  Beside it: the 9 FP32 words per cell the
+fused kernel actually moves (`kernel_bytes_per_launch`), and `traffic`, the
+HBM bytes per launch from rocprofv3's FETCH_SIZE and WRITE_SIZE counters
 (profiles/hbm_traffic.json), calibrated on the copy kernel k_copy.
-`cpu_baseline` runs the reference's own Fortran
-graddin (oracle/_ref, compiled from the SuLi sources) with OpenMP on the
-host cores, on one solve of the same case.
+
+The CPU leg (`cpu_baseline`, skipped with --no-cpu) times the reference's own
+Fortran graddin (oracle/_ref, compiled from the SuLi sources) with OpenMP on
+the host cores, median of C solves after one warm-up, and runs the CPU oracle
+(the C restatement, FP64 CG to a 1e-12 relative residual from x0 = 0) on the
+inputs of every GPU result above, to state each result's error against an
+FP64 oracle (`oracle_rel_diff`: max |p - p_oracle| / max |p_oracle| over the
+interior and face ghosts).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -43,6 +48,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "pressure solves/s + p99 latency at 1e-8 residual, mixed vs FP64 CG"
+CG_WORDS = 14          # algorithmic FP32/FP64 words per cell of one CG iteration
+SPMV_WORDS = 5         # algorithmic words per cell of the SpMV
+REF_WORDS = 17         # words per cell per iteration of the reference's sweeps
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def percentile(xs, q):
@@ -50,16 +62,17 @@ def percentile(xs, q):
 
 
 def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
-    """Timed solves (kernels replayed from captured HIP graphs, tile shape
-    from the autotuner, which runs during the first warm-up solve), then two
-    solves launched directly with HIP events around the CG step."""
+    """Timed solves (tile shape from the autotuner, which runs during the
+    first warm-up solve; captured graphs replayed where the engine uses
+    them), then two solves launched directly with HIP events around the CG
+    step.  Returns the figures and the last timed solve's prd1."""
     solver.configure(precision=precision, stop_rule="relative", tol=tol, verbose=False,
                      graphs=graphs, autotune=True)
     for _ in range(warmup):
         df.zero("prd1")
         solver.graddin_dev(grid, df)
     solver.synchronize()
-    lat, iters, rel, refreshes = [], [], [], []
+    lat, iters, rel, refreshes, replays = [], [], [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         ts = time.perf_counter()
@@ -69,10 +82,12 @@ def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
         iters.append(st.iters)
         rel.append(st.rel_true)
         refreshes.append(st.refreshes)
+        replays.append(st.graph_replays)
         if st.status != 0:
             raise RuntimeError(f"{precision} solve did not converge: {st.as_dict()}")
     solver.synchronize()
     total = time.perf_counter() - t0
+    prd1 = df.download("prd1")
     solver.configure(precision=precision, stop_rule="relative", tol=tol, verbose=False,
                      timing=True, autotune=True)
     spmv = []
@@ -84,34 +99,18 @@ def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
     # the plain SpMV q = A p of this solve's operator (k_spmv), 50 launches
     spmv_only = solver.time_spmv(precision, reps=50)
     kz, vx, _ = solver.autotune(grid, precision)
-    return dict(solves_per_s=steps / total, ms_per_step=total * 1e3 / steps,
-                p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
-                iters=int(np.median(iters)), refreshes=int(np.median(refreshes)),
-                rel_true_max=float(max(rel)), graphs=graphs, tiles={"kz": kz, "vx": vx},
-                spmv_ms=float(np.mean(spmv)) if spmv else None, spmv_only_ms=spmv_only)
+    out = dict(solves_per_s=steps / total, ms_per_step=total * 1e3 / steps,
+               p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
+               iters=int(np.median(iters)), refreshes=int(np.median(refreshes)),
+               rel_true_max=float(max(rel)), graphs=bool(max(replays) > 0),
+               graph_replays_per_solve=float(np.mean(replays)), tiles={"kz": kz, "vx": vx},
+               step_ms=float(np.mean(spmv)) if spmv else None, spmv_only_ms=spmv_only)
+    return out, prd1
 
 
-def rel_diff(a, b):
-    """max |a - b| / max |b| over the interior and face ghosts (edge and
-    corner ghosts of the halo box excluded)."""
-    m = np.ones(a.shape, dtype=bool)
-    for ax in range(3):
-        for other in range(3):
-            if other == ax:
-                continue
-            for e in (0, -1):
-                for e2 in (0, -1):
-                    idx = [slice(None)] * 3
-                    idx[ax] = e
-                    idx[other] = e2
-                    m[tuple(idx)] = False
-    return float(np.abs(a - b)[m].max() / np.abs(b[m]).max())
-
-
-def bench_stretched_cylinder(steps):
+def bench_stretched_cylinder(steps, keep):
     """384x192x128 stretched grid with the masked cylinder (not a reference
-    mode): solves/s of both solvers, and the mixed result against the FP64
-    one (the FP64 solver is pinned to the CPU oracle at small sizes)."""
+    mode): solves/s of both solvers."""
     from suli_amd import cases
     from suli_amd.solver import PressureSolver
     c = cases.config3(seed=1)
@@ -139,14 +138,14 @@ def bench_stretched_cylinder(steps):
                           "rel_true": st.rel_true, "tiles": {"kz": kz, "vx": vx}}
     df.free()
     solver.close()
-    out["mixed_vs_fp64_rel_pressure_diff"] = rel_diff(prd["mixed"], prd["fp64"])
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     out["workload"] = (f"stretched dz (tanh, ratio 1.05, finest at 0.6H) {g.nx}x{g.ny}x{g.nz}, masked "
                        "full-height cylinder r = 0.1 Ly, rho 1000/1.2, 1e-8 relative residual")
+    keep["stretched_cylinder"] = [(c, {"mixed": prd["mixed"], "fp64": prd["fp64"]})]
     return out
 
 
-def bench_constant_density(steps):
+def bench_constant_density(steps, keep):
     """The two-phase tank's grid and velocities solved as IPA (t_press = 2:
     every face uses rho_f1 = 1000), whose operator the CG kernels read from
     per-plane tables: per-step time and solves/s of both solvers, and the
@@ -159,7 +158,7 @@ def bench_constant_density(steps):
     g.rho_f1 = 1000.0
     solver = PressureSolver(verbose=False)
     df = solver.upload(g, c.fields)
-    out = {}
+    out, prd = {}, {}
     n = g.ncell
     for precision in ("mixed", "fp64"):
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
@@ -175,6 +174,7 @@ def bench_constant_density(steps):
         dt = (time.perf_counter() - t0) / steps
         if st.status != 0:
             raise RuntimeError(f"IPA {precision} did not converge: {st.as_dict()}")
+        prd[precision] = df.download("prd1")
         # one more solve launched directly, with the CG step timed
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
                          autotune=True, timing=True)
@@ -182,20 +182,19 @@ def bench_constant_density(steps):
         st = solver.graddin_dev(g, df)
         w = 4 if precision == "mixed" else 8
         spmv_ms = solver.time_spmv(precision, reps=50)
-        words = 6.0 if precision == "mixed" else 9   # mixed: 5 words, and 9 every fourth step
         out[precision] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iters": st.iters,
-                          "rel_true": st.rel_true, "step_ms": st.spmv_ms, "words_per_cell_step": words,
-                          "step_achieved_gbs": words * w * n / (st.spmv_ms * 1e-3) / 1e9,
+                          "rel_true": st.rel_true, "step_ms": st.spmv_ms,
                           "spmv_ms": spmv_ms, "spmv_achieved_gbs": 2 * w * n / (spmv_ms * 1e-3) / 1e9}
     df.free()
     solver.close()
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     out["workload"] = (f"IPA (t_press=2, rho_f1=1000) on the tank grid {g.nx}x{g.ny}x{g.nz}, "
                        "1e-8 relative residual; operator read from per-plane tables")
+    keep["constant_density"] = [(c, {"mixed": prd["mixed"], "fp64": prd["fp64"]})]
     return out
 
 
-def bench_batch(seed=1, count=64):
+def bench_batch(keep, seed=1, count=64):
     """64 uneven problems (sizes log-uniform in [32, 128] per axis) solved
     in one batch, against solving them one after another."""
     from suli_amd import cases
@@ -208,6 +207,7 @@ def bench_batch(seed=1, count=64):
     solver = PressureSolver(verbose=False)
     dfs = [solver.upload(c.grid, c.fields) for c in cs]
     items = [(c.grid, df) for c, df in zip(cs, dfs)]
+    results = [dict() for _ in cs]
     for precision in ("mixed", "fp64"):
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
                          autotune=True)
@@ -220,26 +220,34 @@ def bench_batch(seed=1, count=64):
         t0 = time.perf_counter()
         sts = solver.graddin_batch_dev(items)
         t_batch = time.perf_counter() - t0
+        for r, df in zip(results, dfs):
+            r[precision] = df.download("prd1")
         t0 = time.perf_counter()
         for (g, df) in items:
             df.zero("prd1")
             solver.graddin_dev(g, df)
         t_seq = time.perf_counter() - t0
+        its = [s.iters for s in sts]
+        cell_iters = sum(c.grid.ncell * s.iters for c, s in zip(cs, sts))
         out[precision] = {"batch_s": t_batch, "solves_per_s": count / t_batch,
                           "mcells_per_s": cells / t_batch / 1e6,
+                          "cell_iterations_per_s": cell_iters / t_batch,
                           "one_by_one_solves_per_s": count / t_seq,
                           "batching_speedup": t_seq / t_batch,
-                          "max_iters": max(s.iters for s in sts),
+                          "iters": {"min": min(its), "median": float(np.median(its)), "max": max(its),
+                                    "per_problem": its},
+                          "graph_replays": int(sts[0].graph_replays),
                           "not_converged": sum(1 for s in sts if s.status != 0)}
-        print(f"# batch {precision} {out[precision]}", file=sys.stderr, flush=True)
+        log(f"batch {precision} {out[precision]['solves_per_s']:.1f} solves/s")
     for df in dfs:
         df.free()
     solver.close()
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    keep["batch_uneven"] = list(zip(cs, results))
     return out
 
 
-def bench_serving(seed=1, count=48, rates=(60.0, 240.0)):
+def bench_serving(keep, seed=1, count=48, rates=(60.0, 240.0)):
     """Requests of uneven sizes arriving as a Poisson process, served with
     continuous batching; latency from arrival to the result on the host."""
     from suli_amd import cases
@@ -248,9 +256,10 @@ def bench_serving(seed=1, count=48, rates=(60.0, 240.0)):
     out = {"requests": count, "rates_per_s": list(rates),
            "workload": f"{count} requests, nx log-uniform [32,256], ny,nz [32,128] (x8), "
                        "two-phase tank fields, Poisson arrivals, 1e-8 relative"}
+    results = [dict() for _ in base]
     for precision in ("mixed", "fp64"):
         out[precision] = []
-        for rate in rates:
+        for ri, rate in enumerate(rates):
             cs = [cases.Case(c.name, c.grid, c.fields.copy(), c.seed) for c in base]
             with PressureServer(precision=precision, stop_rule="relative", tol=1e-8) as srv:
                 rep = poisson_stream(srv, cs, rate=rate, seed=seed)
@@ -258,8 +267,64 @@ def bench_serving(seed=1, count=48, rates=(60.0, 240.0)):
             info = rep.pop("info")
             rep["peak_resident_mb"] = info["peak_resident_bytes"] / 1e6
             rep["chunks"] = info["chunks"]
+            if ri == 0:
+                for r, c in zip(results, cs):
+                    r[precision] = c.fields.prd1
+            else:
+                # each result is the request's own, whatever else was resident
+                rep["same_bits_as_first_rate"] = all(
+                    np.array_equal(r[precision], c.fields.prd1) for r, c in zip(results, cs))
             out[precision].append(rep)
-            print(f"# serving {precision} {rep}", file=sys.stderr, flush=True)
+            log(f"serving {precision} rate {rate}: p50 {rep['p50_ms']:.1f} p99 {rep['p99_ms']:.1f} ms")
+    keep["serving"] = list(zip(base, results))
+    return out
+
+
+def bench_dropin(stages=6):
+    """The call SuLi makes: host-pointer graddin then posdin (the Fortran
+    shim's path, 1_principal.f90:102-103) on the 256x128x96 tank, once per
+    RK sub-stage, each solve warm-started from the previous prd1.  Between
+    sub-stages the predicted velocity changes by 5 % (u* = (1 + 0.05 s) u0),
+    as successive sub-stages of one time step do.  Pageable host arrays;
+    times include the PCIe copies."""
+    from suli_amd import cases
+    from suli_amd.solver import PressureSolver
+    c = cases.config2(seed=1)
+    g = c.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    n = nx * ny * nz
+    nu, nv, nw = (nx + 3) * (ny + 2) * (nz + 2), (nx + 2) * (ny + 3) * (nz + 2), (nx + 2) * (ny + 2) * (nz + 3)
+    nbox = (nx + 2) * (ny + 2) * (nz + 2)
+    up = 8 * (nu + nv + nw + n + nbox)             # u, v, w, rho, prd1 to the device
+    out = {"workload": f"host-pointer graddin + posdin per RK sub-stage, tank {nx}x{ny}x{nz}, "
+                       f"{stages} sub-stages, warm start, 1e-8 relative",
+           "bytes_per_substage": {"graddin_h2d": up, "graddin_d2h": 8 * nbox,
+                                  "posdin_h2d": up, "posdin_h2d_reused": 0,
+                                  "posdin_d2h": 8 * (nu + nv + nw)}}
+    u0, v0, w0 = c.fields.u.copy(order="F"), c.fields.v.copy(order="F"), c.fields.w.copy(order="F")
+    for precision, reuse in (("mixed", True), ("mixed", False), ("fp64", True)):
+        f = c.fields.copy()
+        f.prd1[...] = 0.0
+        ms, iters = [], []
+        with PressureSolver(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
+                            autotune=True, reuse_staged=reuse) as s:
+            for stage in range(stages + 1):    # the first is a cold start and a warm-up
+                a = 1.0 + 0.05 * stage
+                f.u[...] = a * u0
+                f.v[...] = a * v0
+                f.w[...] = a * w0
+                t0 = time.perf_counter()
+                st = s.graddin(g, f)
+                s.posdin(g, f)
+                dt = (time.perf_counter() - t0) * 1e3
+                if st.status != 0:
+                    raise RuntimeError(f"drop-in {precision} did not converge: {st.as_dict()}")
+                if stage > 0:
+                    ms.append(dt)
+                    iters.append(st.iters)
+        key = f"{precision}_{'reuse' if reuse else 'upload'}"
+        out[key] = {"ms_per_substage": float(np.median(ms)), "iters": iters}
+        log(f"drop-in {key}: {out[key]['ms_per_substage']:.2f} ms per sub-stage, iters {iters}")
     return out
 
 
@@ -272,33 +337,129 @@ def load_traffic(name):
     return d.get(name)
 
 
-def cpu_baseline(case, tol, solves):
+def host_cores():
+    """Physical cores of the host, from lscpu (None if it cannot tell)."""
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        return len({line for line in out.splitlines() if line and not line.startswith("#")})
+    except (OSError, subprocess.CalledProcessError):
+        return None
+
+
+def rel_diff(a, b):
+    """max |a - b| / max |b| over the interior and face ghosts (edge and
+    corner ghosts of the halo box excluded)."""
+    m = np.ones(a.shape, dtype=bool)
+    for ax in range(3):
+        for other in range(3):
+            if other == ax:
+                continue
+            for e in (0, -1):
+                for e2 in (0, -1):
+                    idx = [slice(None)] * 3
+                    idx[ax] = e
+                    idx[other] = e2
+                    m[tuple(idx)] = False
+    return float(np.abs(a - b)[m].max() / np.abs(b[m]).max())
+
+
+def cpu_timing(case, lam, solves, threads, out_npy):
     """The reference's Fortran graddin (or, if it was not built, the C
-    restatement) with OpenMP on the host cores.  Its stop rule is absolute,
-    r.r <= 1e-4/N (SuLi-IC/9_grad.f90:393); inputs are scaled by lam so that
-    it equals the relative 1e-8 rule (the solution scales with lam)."""
+    restatement) with OpenMP on the host cores, run in a process of its own
+    whose OpenMP runtime binds one thread per core (OMP_PROC_BIND=spread,
+    OMP_PLACES=cores): one warm-up solve, then the median of `solves`, all
+    on one thread that starts the runtime.  The reference's stop rule is
+    absolute, r.r <= 1e-4/N (SuLi-IC/9_grad.f90:393); the inputs are scaled
+    by lam so that it equals the relative rule (the solution scales with
+    lam; prd1 / lam is saved to out_npy)."""
     import oracle
     from suli_amd import cases
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    _, st, _ = oracle.graddin(case.grid, case.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-2,
-                              threads=threads)
-    lam = float(np.sqrt((1e-4 / case.grid.ncell) / (tol * tol * st.bb)))
+    g = case.grid
     f = cases.scale_velocity(case.fields, lam)
     kind = "reference" if oracle.ref_available("ic") else "port"
-    times = []
-    for _ in range(solves):
-        t = time.perf_counter()
-        if kind == "reference":
-            oracle.ref_graddin(case.grid, f, omp_t=1)
-        else:
-            oracle.graddin(case.grid, f, omp_t=1, threads=threads)
-        times.append(time.perf_counter() - t)
-    t = float(np.median(times))
+    if kind == "reference":
+        prd_ref, times = oracle.ref_graddin_timed(g, f, solves + 1, omp_t=1)
+    else:
+        times = []
+        for _ in range(solves + 1):
+            t = time.perf_counter()
+            prd_ref, _, _ = oracle.graddin(g, f, omp_t=1, threads=threads)
+            times.append(time.perf_counter() - t)
+    np.save(out_npy, prd_ref / lam)
+    t = float(np.median(times[1:]))
     what = ("SuLi graddin (amdflang -O3 -fopenmp, omp_t=1)" if kind == "reference"
             else "C restatement of SuLi graddin (gcc -O2 -fopenmp)")
     return {"value": 1.0 / t, "unit": "solves/s", "cores": threads, "kind": kind,
-            "sample": f"{solves} FP64 solve(s) of the same 256x128x96 case to 1e-8 relative "
-                      f"(absolute rule with lam={lam:.3e}); {what}; {t:.2f} s per solve"}
+            "sample": f"{solves} FP64 solves (median, after 1 warm-up) of the same {g.nx}x{g.ny}x{g.nz} "
+                      f"case to 1e-8 relative (absolute rule with lam={lam:.3e}); {what}; "
+                      f"{t:.3f} s per solve",
+            "ms_per_solve": t * 1e3, "all_ms": [x * 1e3 for x in times],
+            "threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def cpu_leg(args, case, headline, keep, avail):
+    """The CPU baseline (cpu_timing, in a child process) and the CPU
+    oracle's FP64 solution of every GPU result's problem: FP64 CG to 1e-12
+    relative from x0 = 0, the constant-pressure component of the initial
+    residual removed as the GPU solver does."""
+    import tempfile
+
+    import oracle
+    from suli_amd import cases
+    phys = host_cores()
+    threads = int(os.environ.get("OMP_NUM_THREADS", min(16, avail)))
+    g = case.grid
+    n = g.ncell
+    # lam from b.b of the scaled system; the reference rule's iteration count
+    # on the scaled inputs from the oracle (the reference keeps it local)
+    _, st, _ = oracle.graddin(g, case.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-2, threads=threads)
+    lam = float(np.sqrt((1e-4 / n) / (args.tol * args.tol * st.bb)))
+    _, st_ref, _ = oracle.graddin(g, cases.scale_velocity(case.fields, lam), stop_rule=oracle.STOP_REFERENCE,
+                                  threads=threads, omp_t=1)
+    with tempfile.TemporaryDirectory() as tmp:
+        npy = os.path.join(tmp, "prd_ref.npy")
+        env = dict(os.environ, OMP_PROC_BIND="spread", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-timing-to", npy, "--cpu-lam", repr(lam),
+               "--cpu-solves", str(args.cpu_solves), "--nx", str(args.nx), "--ny", str(args.ny),
+               "--nz", str(args.nz)]
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU timing failed:\n{r.stderr[-2000:]}")
+        cpu = json.loads(r.stdout.strip().splitlines()[-1])
+        prd_ref = np.load(npy)
+    t = cpu["ms_per_solve"] * 1e-3
+    iters = int(st_ref.iters)
+    cpu.update({"iters": iters, "iters_from": "oracle, same input and rule",
+                "gbs_at_17_words": REF_WORDS * 8 * n * iters / t / 1e9,
+                "gbs_at_14_words": CG_WORDS * 8 * n * iters / t / 1e9,
+                "physical_cores_lscpu": phys, "cpus_available": avail})
+    log(f"cpu baseline {cpu['ms_per_solve']:.0f} ms per solve, {iters} iterations")
+
+    def tight(c):
+        p, s, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
+                                  max_iter=50000, threads=threads, omp_t=1, project_null=True)
+        if rc != 0:
+            raise RuntimeError(f"oracle did not converge on {c.name}")
+        return p
+    t0 = time.perf_counter()
+    checks = {}
+    p_or = tight(case)
+    checks["headline"] = {k: rel_diff(v, p_or) for k, v in headline.items()}
+    # the reference itself at the same 1e-8 rule, for scale
+    checks["headline"]["reference_fortran_1e-8"] = rel_diff(prd_ref, p_or)
+    for name, items in keep.items():
+        per = {}
+        for c, res in items:
+            p = tight(c)
+            for k, v in res.items():
+                per.setdefault(k, []).append(rel_diff(v, p))
+        checks[name] = {k: (v[0] if len(v) == 1 else {"max": max(v), "median": float(np.median(v)),
+                                                       "count": len(v)})
+                        for k, v in per.items()}
+        log(f"oracle check {name}: {checks[name]}")
+    cpu["oracle_checks_s"] = time.perf_counter() - t0
+    return cpu, checks
 
 
 def main():
@@ -306,7 +467,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-solves", type=int, default=1)
+    ap.add_argument("--cpu-solves", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--nx", type=int, default=256)
@@ -319,8 +480,18 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the stretched-cylinder, batch and serving workloads")
+                    help="skip the cylinder, IPA, batch, serving and drop-in workloads")
+    ap.add_argument("--cpu-timing-to", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-lam", type=float, default=1.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    # logical CPUs this process may use, read before any OpenMP runtime starts
+    avail = len(os.sched_getaffinity(0))
+    if args.cpu_timing_to:
+        from suli_amd import cases
+        case = cases.config2(seed=1, nx=args.nx, ny=args.ny, nz=args.nz)
+        threads = int(os.environ.get("OMP_NUM_THREADS", min(16, avail)))
+        print(json.dumps(cpu_timing(case, args.cpu_lam, args.cpu_solves, threads, args.cpu_timing_to)))
+        return
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent problems would shard across GPUs, one process per GPU, with no "
@@ -335,64 +506,75 @@ def main():
     n = g.ncell
     solver = PressureSolver(verbose=False)
     df = solver.upload(g, case.fields)
-    print(f"# case {case.name} n={n}", file=sys.stderr, flush=True)
+    log(f"case {case.name} n={n}")
     copy_bytes = args.copy_mib << 20
     copy_ms = solver.copy_bench(copy_bytes, reps=10)
     copy = {"kernel": "k_copy", "bytes_read": copy_bytes, "bytes_written": copy_bytes,
             "ms": copy_ms, "gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
-    print(f"# copy {copy}", file=sys.stderr, flush=True)
+    log(f"copy {copy['gbs']:.0f} GB/s")
     mixed = fp64 = None
+    headline = {}
     if args.only in ("both", "mixed"):
-        mixed = run_solves(solver, g, df, "mixed", args.steps, args.warmup, args.tol,
-                           graphs=not args.no_graphs)
-        print(f"# mixed {mixed}", file=sys.stderr, flush=True)
+        mixed, headline["mixed"] = run_solves(solver, g, df, "mixed", args.steps, args.warmup, args.tol,
+                                              graphs=not args.no_graphs)
+        log(f"mixed {mixed['solves_per_s']:.2f} solves/s, {mixed['iters']} iterations")
     if args.only in ("both", "fp64"):
-        fp64 = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol,
-                          graphs=not args.no_graphs)
-        print(f"# fp64 {fp64}", file=sys.stderr, flush=True)
+        fp64, headline["fp64"] = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol,
+                                            graphs=not args.no_graphs)
+        log(f"fp64 {fp64['solves_per_s']:.2f} solves/s, {fp64['iters']} iterations")
     df.free()
     solver.close()
     if mixed is None or fp64 is None:
         print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy}))
         return
 
-    bytes32 = 36 * n              # k_cg_m: 9 FP32 words per cell on average (8, and 12 every 4th step)
-    bytes64 = 12 * 8 * n          # FP64 step (k_cg_a + k_cg_b): 12 FP64 words per cell
-    spmv_ms = mixed["spmv_ms"]
-    achieved = bytes32 / (spmv_ms * 1e-3) / 1e9 if spmv_ms else None
+    launch_ms = mixed["step_ms"]
+    alg = CG_WORDS * 4 * n                 # one CG iteration in FP32: 56 B/cell
+    kern = 9 * 4 * n                       # what k_cg_m moves: 9 FP32 words/cell on average
+    achieved = alg / (launch_ms * 1e-3) / 1e9
+    spmv_gbs = SPMV_WORDS * 4 * n / (mixed["spmv_only_ms"] * 1e-3) / 1e9
+    fp64_step = fp64["step_ms"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                "frac": achieved / HBM_PEAK_GBS,
                 "traffic": load_traffic(f"k_cg_m {g.nx}x{g.ny}x{g.nz}"),
-                "kernel": "k_cg_m", "bytes_per_launch": bytes32,
-                "launch_ms": spmv_ms,
-                "spmv": {"kernel": "k_spmv<float>", "words_per_cell": 5,
-                         "launch_ms": mixed["spmv_only_ms"],
-                         "achieved": 5 * 4 * n / (mixed["spmv_only_ms"] * 1e-3) / 1e9,
-                         "frac": 5 * 4 * n / (mixed["spmv_only_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "kernel": "k_cg_m", "launch_ms": launch_ms,
+                "algorithmic_bytes_per_launch": alg,
+                "algorithmic_basis": "one FP32 CG iteration, 14 words/cell (SpMV 5 + x, r, p 3 each)",
+                "kernel_bytes_per_launch": kern,
+                "kernel_achieved": kern / (launch_ms * 1e-3) / 1e9,
+                "kernel_frac": kern / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "launch_ms_basis": "HIP events around windows of 8 consecutive launches (gaps included)",
+                "spmv": {"kernel": "k_spmv<float>", "words_per_cell": SPMV_WORDS,
+                         "launch_ms": mixed["spmv_only_ms"], "achieved": spmv_gbs,
+                         "frac": spmv_gbs / HBM_PEAK_GBS,
                          "fp64_launch_ms": fp64["spmv_only_ms"],
-                         "fp64_achieved": 5 * 8 * n / (fp64["spmv_only_ms"] * 1e-3) / 1e9},
-                "fp64_step": {"kernels": "k_cg_a<double> + k_cg_b<double>",
-                              "step_ms": fp64["spmv_ms"], "bytes_per_step": bytes64,
-                              "achieved": (bytes64 / (fp64["spmv_ms"] * 1e-3) / 1e9)
-                              if fp64["spmv_ms"] else None}}
+                         "fp64_achieved": SPMV_WORDS * 8 * n / (fp64["spmv_only_ms"] * 1e-3) / 1e9},
+                "fp64_step": {"kernels": "k_cg_a<double> + k_cg_b<double>", "step_ms": fp64_step,
+                              "achieved": CG_WORDS * 8 * n / (fp64_step * 1e-3) / 1e9 if fp64_step else None,
+                              "kernel_achieved": 12 * 8 * n / (fp64_step * 1e-3) / 1e9 if fp64_step else None}}
 
-    extra = None
+    extra, keep = None, {}
     if not args.no_extra:
         extra = {}
-        print("# stretched cylinder", file=sys.stderr, flush=True)
-        extra["stretched_cylinder"] = bench_stretched_cylinder(2)
-        print(f"# {extra['stretched_cylinder']}", file=sys.stderr, flush=True)
-        extra["constant_density"] = bench_constant_density(3)
-        print(f"# {extra['constant_density']}", file=sys.stderr, flush=True)
-        print("# batch", file=sys.stderr, flush=True)
-        extra["batch_uneven"] = bench_batch()
-        print("# serving", file=sys.stderr, flush=True)
-        extra["serving"] = bench_serving()
+        log("stretched cylinder")
+        extra["stretched_cylinder"] = bench_stretched_cylinder(2, keep)
+        log("constant density")
+        extra["constant_density"] = bench_constant_density(3, keep)
+        log("batch")
+        extra["batch_uneven"] = bench_batch(keep)
+        log("serving")
+        extra["serving"] = bench_serving(keep)
+        log("drop-in call")
+        extra["dropin_call"] = bench_dropin()
 
-    cpu = None
+    cpu = checks = None
     if not args.no_cpu:
-        print("# cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(case, args.tol, args.cpu_solves)
+        log("cpu baseline and oracle checks")
+        cpu, checks = cpu_leg(args, case, headline, keep, avail)
+        mixed["oracle_rel_diff"] = checks["headline"]["mixed"]
+        fp64["oracle_rel_diff"] = checks["headline"]["fp64"]
+        for name in keep:
+            extra[name]["oracle_rel_diff"] = checks[name]
 
     out = {
         "metric": METRIC,
@@ -414,12 +596,31 @@ def main():
         "fp64": fp64,
         "speedup_mixed_vs_fp64": mixed["solves_per_s"] / fp64["solves_per_s"],
         "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
+        "oracle_rel_diff": checks["headline"] if checks else None,
         "roofline": roofline,
         "copy_kernel": copy,
         "workloads": extra,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    reap_children()
+
+
+def reap_children():
+    """End the run with no process of its own left behind: any child still
+    alive is named on stderr, then terminated and waited for."""
+    try:
+        import psutil
+    except ImportError:
+        return
+    kids = psutil.Process().children(recursive=True)
+    for k in kids:
+        try:
+            log(f"child process left at exit: pid {k.pid} {' '.join(k.cmdline())}")
+            k.terminate()
+        except psutil.Error:
+            pass
+    psutil.wait_procs(kids, timeout=10)
 
 
 if __name__ == "__main__":

@@ -315,6 +315,34 @@ def ref_graddin(case_grid, fields, omp_t=0):
     return prd1
 
 
+def ref_graddin_timed(case_grid, fields, reps, omp_t=0):
+    """The reference's compiled graddin run reps times on one thread (the
+    same big-stack thread each time, so that the OpenMP runtime sees one
+    root thread whose affinity nothing pinned before it started).  Returns
+    (prd1 of the last run, wall seconds of each run)."""
+    import time
+    g = case_grid
+    L = ref_lib("icdz" if g.stretched else "ic")
+    _ref_setup(L, g, omp_t)
+    have_obs = 1 if fields.obs_lsx is not None else 0
+    dummy = np.zeros(1)
+    prd1 = fields.prd1.copy(order="F")
+
+    def runs():
+        times = []
+        for _ in range(reps):
+            prd1[...] = fields.prd1
+            t = time.perf_counter()
+            L.suli_ref_graddin(_ptr(fields.u), _ptr(fields.v), _ptr(fields.w), _ptr(fields.rho), have_obs,
+                               _ptr(fields.obs_lsx if have_obs else dummy),
+                               _ptr(fields.obs_lsy if have_obs else dummy),
+                               _ptr(fields.obs_lsz if have_obs else dummy), _ptr(prd1))
+            times.append(time.perf_counter() - t)
+        return times
+
+    return prd1, _big_stack_call(_stack_bytes(g), runs)
+
+
 def ref_posdin(case_grid, fields, prd1):
     g = case_grid
     L = ref_lib("icdz" if g.stretched else "ic")
