@@ -387,9 +387,52 @@ __device__ __forceinline__ void stv(float *p, long long x, const float (&v)[V])
     else
         *reinterpret_cast<float4 *>(p + x) = make_float4(v[0], v[1], v[2], v[3]);
 }
+// V consecutive floats at a 4-byte-aligned address (x faces: a row holds
+// nx + 1 of them) as one vector load.
+template <int V>
+__device__ __forceinline__ void ldvu(const float *p, long long x, float (&o)[V])
+{
+    typedef float vt __attribute__((ext_vector_type(V), aligned(4)));
+    const vt t = *reinterpret_cast<const vt *>(p + x);
+#pragma unroll
+    for (int m = 0; m < V; ++m) o[m] = t[m];
+}
 
-// V consecutive y or z face coefficients of a lane (one vector load, or the
-// plane's value repeated).
+// Wavefront-wide lane shifts by one (DPP wave_shr / wave_shl): lane l gets
+// lane l-1's (from_lower) or lane l+1's (from_upper) value; lane 0, resp.
+// lane 63, which has no such neighbour, keeps `edge`.  Every lane of the
+// wave executes them.
+__device__ __forceinline__ float from_lower(float edge, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_upper(float edge, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_value(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// The V + 1 x faces of a lane's V cells (faces i .. i+V), the V y or z faces
+// at one offset: face arrays, or the plane's value repeated.
+template <int V>
+__device__ __forceinline__ void xv(const FaceArrays<float> &co, long long f, int, float (&o)[V + 1])
+{
+    float t[V];
+    ldvu<V>(co.sx, f, t);
+#pragma unroll
+    for (int m = 0; m < V; ++m) o[m] = t[m];
+    o[V] = co.sx[f + V];
+}
+template <int V>
+__device__ __forceinline__ void xv(const PlaneTables<float> &co, long long, int k, float (&o)[V + 1])
+{
+    const float v = co.tx[k];
+#pragma unroll
+    for (int m = 0; m <= V; ++m) o[m] = v;
+}
 template <int V>
 __device__ __forceinline__ void yv(const FaceArrays<float> &co, long long f, int, float (&o)[V])
 {
@@ -415,9 +458,46 @@ __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int 
     for (int m = 0; m < V; ++m) o[m] = v;
 }
 
-// cg_m_tile with V cells per lane: the same arithmetic per cell (and the
-// same term order), with the lane's inner x neighbours shared in registers
-// and 8- or 16-byte loads and stores.
+// Where a wave of V-cell lanes sits in its row: the wave covers the
+// segment of 64 V cells from x = i0 (one segment when nx <= 64 V).  Lanes
+// past the row's end (nx is a multiple of V) read the row's last V cells
+// instead, store nothing and add nothing.  The x neighbours of a lane's
+// cells come from the adjacent lanes; beyond the segment's ends they are the
+// mirror ghost (the end cell itself), the periodic partner (another lane of
+// a one-segment row), or a cell of the next segment (loaded).
+struct RowWave {
+    int lane, last;      // this lane; the segment's last lane inside the row
+    bool active;         // the lane's cells are inside the row
+    int ci;              // the lane's first cell (clamped for inactive lanes)
+    bool load_w, load_e; // the cell beyond the segment's west / east end is loaded
+    int xw, xe;          // ... at x = xw / xe of the same row
+    bool wrap;           // periodic one-segment row: the partner is lane `last` / lane 0
+};
+
+template <int V>
+__device__ __forceinline__ RowWave row_wave(const Geo &g, const Tile &t)
+{
+    RowWave w;
+    const int nx = g.nx;
+    w.lane = threadIdx.x & (kTileX - 1);
+    const int i0 = t.i - w.lane * V;
+    const int seg = kTileX * V;
+    w.last = min(kTileX - 1, (nx - i0) / V - 1);
+    w.active = t.i < nx;
+    w.ci = w.active ? t.i : nx - V;
+    const bool one = i0 == 0 && nx <= seg;
+    w.load_w = i0 > 0 || (g.px && !one);
+    w.load_e = i0 + seg < nx || (g.px && !one);
+    w.xw = i0 > 0 ? i0 - 1 : nx - 1;
+    w.xe = i0 + seg < nx ? i0 + seg : 0;
+    w.wrap = g.px && one;
+    return w;
+}
+
+// cg_m_tile with V cells per lane in row waves: the same arithmetic per cell
+// (and the same term order).  The x neighbours of p_n are exchanged between
+// lanes (DPP) instead of being formed again from three loads each, and the
+// x faces of a lane come as one vector.
 template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
                                            const float *__restrict__ r_old, const float *__restrict__ q_old,
@@ -427,14 +507,14 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
-    if (t.i >= nx || t.j >= ny) return;
-    long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + V - 1, nx, g.px);
+    if (t.j >= ny) return;   // the whole wave
+    const RowWave w = row_wave<V>(g, t);
+    long long c = (long long)t.k0 * plane + (long long)t.j * nx + w.ci;
     const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
-    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
-    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
-    // r_{n-2} from p_{n-1} = r_{n-2} + beta_{n-1} p_{n-2} (REC), or as stored
+    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + w.ci;
+    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + w.ci;
+    // p_n at one cell (segment ends)
     auto pn1 = [&](long long x) -> float {
         const float po = FIRST ? 0.0f : p_old[x];
         float r = REC ? po - bp * p_prev[x] : r_old[x];
@@ -491,7 +571,7 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
     }
     zv<V>(co, c, t.k0, szb);
     for (int k = t.k0; k < t.k1; ++k) {
-        const long long r0 = c - t.i;
+        const long long r0 = c - w.ci;
         const long long up = (k < nz - 1) ? plane : 0;
         float pot[V], ppt[V], rt[V], pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1];
         if (FIRST) {
@@ -504,13 +584,17 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 #pragma unroll
         for (int m = 0; m < V; ++m) pt[m] = FIRST ? rt[m] : rt[m] + b * pot[m];
         zv<V>(co, c + plane, k + 1, szt);
-        const float pw = pn1(r0 + iw), pe = pn1(r0 + ie);
         pnv(c + dS, ps);
         pnv(c + dN, pnn);
-#pragma unroll
-        for (int m = 0; m <= V; ++m) sxv[m] = co.x(fx + m, k);
+        xv<V>(co, fx, k, sxv);
         yv<V>(co, fy, k, sys);
         yv<V>(co, fy + nx, k, syn);
+        // x neighbours of the lane's end cells: adjacent lanes, or beyond the segment
+        const float ew = w.load_w ? pn1(r0 + w.xw) : (w.wrap ? lane_value(pc[V - 1], w.last) : pc[0]);
+        const float ee = w.load_e ? pn1(r0 + w.xe) : (w.wrap ? lane_value(pc[0], 0) : pc[V - 1]);
+        const float pw = from_lower(ew, pc[V - 1]);
+        const float pe_dpp = from_upper(ee, pc[0]);
+        const float pe = w.lane == w.last ? ee : pe_dpp;
         const bool lid = g.dtop && k == nz - 1;   // Dirichlet lid
         float y[V];
 #pragma unroll
@@ -526,12 +610,6 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
             ldv<V>(p_new, c, sum);   // p_{n-kFold}, before p_n replaces it
 #pragma unroll
             for (int m = 0; m < V; ++m) sum[m] = fd.a[0] * sum[m];
-        }
-        stv<V>(q_new, c, y);
-        stv<V>(p_new, c, pc);
-        if (FOLD) {
-            // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
-            // when r was recovered from it)
 #pragma unroll
             for (int f = 0; f < kFold - 2; ++f) {
                 float pf[V];
@@ -544,20 +622,28 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 #pragma unroll
                 for (int m = 0; m < V; ++m) sum[m] += fd.a[f + 1] * pf[m];
             }
-            float cv[V];
-            ldv<V>(c32, c, cv);
-#pragma unroll
-            for (int m = 0; m < V; ++m) cv[m] = cv[m] - (sum[m] + fd.a[kFold - 1] * poc[m]);
-            stv<V>(c32, c, cv);
         }
+        if (w.active) {
+            stv<V>(q_new, c, y);
+            stv<V>(p_new, c, pc);
+            if (FOLD) {
+                // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
+                // when r was recovered from it)
+                float cv[V];
+                ldv<V>(c32, c, cv);
 #pragma unroll
-        for (int m = 0; m < V; ++m) acc[0] += (double)pc[m] * (double)y[m];
+                for (int m = 0; m < V; ++m) cv[m] = cv[m] - (sum[m] + fd.a[kFold - 1] * poc[m]);
+                stv<V>(c32, c, cv);
+            }
 #pragma unroll
-        for (int m = 0; m < V; ++m) acc[1] += (double)y[m] * (double)rc[m];
+            for (int m = 0; m < V; ++m) acc[0] += (double)pc[m] * (double)y[m];
 #pragma unroll
-        for (int m = 0; m < V; ++m) acc[2] += (double)y[m] * (double)y[m];
+            for (int m = 0; m < V; ++m) acc[1] += (double)y[m] * (double)rc[m];
 #pragma unroll
-        for (int m = 0; m < V; ++m) acc[3] += (double)rc[m] * (double)rc[m];
+            for (int m = 0; m < V; ++m) acc[2] += (double)y[m] * (double)y[m];
+#pragma unroll
+            for (int m = 0; m < V; ++m) acc[3] += (double)rc[m] * (double)rc[m];
+        }
 #pragma unroll
         for (int m = 0; m < V; ++m) {
             pb[m] = pc[m];
@@ -608,11 +694,13 @@ __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Ti
                                              acc);
 }
 
-template <int V, class CO>
-__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, bool fold, const ProbDev &pd, int n,
-                                          const Tile &tl, const float *r_o, const float *q_o, const float *p_o,
-                                          const float *p_oo, float *p_w, float *q_w, const CO &co, float a,
-                                          float b, float bp, double acc[4])
+// FOLD: this launch is one of the steps at which c takes a group (the
+// instances of the other steps are not compiled into it).
+template <int V, bool FOLD, class CO>
+__device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, const ProbDev &pd, int n, const Tile &tl,
+                                          const float *r_o, const float *q_o, const float *p_o, const float *p_oo,
+                                          float *p_w, float *q_w, const CO &co, float a, float b, float bp,
+                                          double acc[4])
 {
 #define CG_M(F, U, R, D) cg_m_tile_any<V, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc)
     if (first)
@@ -620,9 +708,9 @@ __device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, bool f
     else if (!upd)
         CG_M(false, false, false, false);
     else if (rec)
-        fold ? CG_M(false, true, true, true) : CG_M(false, true, true, false);
+        CG_M(false, true, true, FOLD);
     else
-        fold ? CG_M(false, true, false, true) : CG_M(false, true, false, false);
+        CG_M(false, true, false, FOLD);
 #undef CG_M
 }
 
@@ -636,8 +724,20 @@ __device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, bool f
 // spans one iteration.  After a residual replacement rho_{n-1} is the
 // replaced residual's r.r and nothing of iteration n-1 is left to finish.
 // The stop test is the FP64 one at replacements; here only the cap.
-__global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab,
-                                                   int ntab, StepRef sref, int *ran)
+//
+// FOLD: the step folds a group into c (n = 1 mod kFold).  Problems join on
+// the kRefreshGrain grain (n0 - 1 is a multiple of it, and of kFold), so n
+// and the global step agree mod kFold: every problem of a launch folds or
+// none does, and the host launches the instance that does.
+//
+// VX: cells per lane of every problem in the table (the engine launches one
+// instance per width).  Four-cell lanes need more registers than four waves
+// per SIMD leave (about 160 without spilling), so that instance is held to
+// three.
+template <int VX, bool FOLD>
+__global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDev *__restrict__ P,
+                                                                     const TileRef *__restrict__ tab, int ntab,
+                                                                     StepRef sref, int *ran)
 {
     const Run run = my_run(ntab);
     const int step = step_of(sref);
@@ -684,7 +784,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         // n = 1 mod kFold (replacements fall after multiples of kFold, so a
         // group never spans one); their alphas but the last were stored by
         // the previous steps
-        const bool fold = upd && (n & (kFold - 1)) == 1;
+        const bool fold = FOLD && upd;
         // p_{n-1} came from r_{n-2} and beta_{n-1} p_{n-2} unless step n-1
         // was the first or followed a replacement
         const bool rec = upd && n != 2 && step != skip_e_at + 1;
@@ -723,13 +823,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_cg_m(const ProbDev *__restrict_
         const float bp = (float)beta_prev;
         const float *q_o = qbuf(pd, o), *p_o = pbuf(pd, n - 1), *p_oo = pbuf(pd, n - 2);
         float *p_w = pbuf(pd, n), *q_w = qbuf(pd, w);
+        (void)fold;
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
-            if (pd.g.vx == 4)
-                cg_m_step<4>(first, upd, rec, fold, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
-            else if (pd.g.vx == 2)
-                cg_m_step<2>(first, upd, rec, fold, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
-            else
-                cg_m_step<1>(first, upd, rec, fold, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
+            cg_m_step<VX, FOLD>(first, upd, rec, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
             return 0;
         });
         block_sum4(acc);
@@ -1069,21 +1165,22 @@ __global__ void k_set_step(int *base, int value)
     if (threadIdx.x == 0) *base = value;
 }
 
-// Plain FP32 SpMV of one tile with V cells per lane (timing only).
+// Plain FP32 SpMV of one tile with V cells per lane in row waves (timing
+// only): x neighbours from the adjacent lanes, as in cg_m_tilev.
 template <int V, class CO>
 __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const float *__restrict__ p,
                                              float *__restrict__ q, const CO &co)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     double acc = 0.0;
-    if (t.i >= nx || t.j >= ny) return acc;
+    if (t.j >= ny) return acc;
+    const RowWave w = row_wave<V>(g, t);
     const long long plane = (long long)nx * ny;
-    const int iw = nb_lo(t.i, nx, g.px), ie = nb_hi(t.i + V - 1, nx, g.px);
     const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
-    long long c = (long long)t.k0 * plane + (long long)t.j * nx + t.i;
-    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + t.i;
-    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + t.i;
+    long long c = (long long)t.k0 * plane + (long long)t.j * nx + w.ci;
+    long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + w.ci;
+    long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + w.ci;
     float pc[V], pb[V], szb[V];
     ldv<V>(p, c, pc);
     if (t.k0 > 0)
@@ -1093,18 +1190,21 @@ __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const 
         for (int m = 0; m < V; ++m) pb[m] = pc[m];
     zv<V>(co, c, t.k0, szb);
     for (int k = t.k0; k < t.k1; ++k) {
-        const long long r0 = c - t.i;
+        const long long r0 = c - w.ci;
         const long long up = (k < nz - 1) ? plane : 0;
         float pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1], y[V];
         ldv<V>(p, c + up, pt);
         zv<V>(co, c + plane, k + 1, szt);
-        const float pw = p[r0 + iw], pe = p[r0 + ie];
         ldv<V>(p, c + dS, ps);
         ldv<V>(p, c + dN, pnn);
-#pragma unroll
-        for (int m = 0; m <= V; ++m) sxv[m] = co.x(fx + m, k);
+        xv<V>(co, fx, k, sxv);
         yv<V>(co, fy, k, sys);
         yv<V>(co, fy + nx, k, syn);
+        const float ew = w.load_w ? p[r0 + w.xw] : (w.wrap ? lane_value(pc[V - 1], w.last) : pc[0]);
+        const float ee = w.load_e ? p[r0 + w.xe] : (w.wrap ? lane_value(pc[0], 0) : pc[V - 1]);
+        const float pw = from_lower(ew, pc[V - 1]);
+        const float pe_dpp = from_upper(ee, pc[0]);
+        const float pe = w.lane == w.last ? ee : pe_dpp;
         const bool lid = g.dtop && k == nz - 1;
 #pragma unroll
         for (int m = 0; m < V; ++m) {
@@ -1114,9 +1214,11 @@ __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const 
             y[m] = pc[m] - sxv[m + 1] * east - sxv[m] * west - syn[m] * pnn[m] - sys[m] * ps[m] - szt[m] * top -
                    szb[m] * pb[m];
         }
-        stv<V>(q, c, y);
+        if (w.active) {
+            stv<V>(q, c, y);
 #pragma unroll
-        for (int m = 0; m < V; ++m) acc += (double)pc[m] * (double)y[m];
+            for (int m = 0; m < V; ++m) acc += (double)pc[m] * (double)y[m];
+        }
 #pragma unroll
         for (int m = 0; m < V; ++m) {
             pb[m] = pc[m];
@@ -1237,9 +1339,26 @@ void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int gr
         hipLaunchKernelGGL(k_cg_a<double>, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
 }
 
-void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, hipStream_t s)
+void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, int vx,
+                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg_m, dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);
+    // the global step is step.off (direct launch) or base + off with a base
+    // on the refresh grain (replayed group): its phase mod kFold is off's
+    const bool fold = (step.off & (kFold - 1)) == 1;
+#define LAUNCH_CG_M(V)                                                                                          \
+    do {                                                                                                        \
+        if (fold)                                                                                               \
+            hipLaunchKernelGGL((k_cg_m<V, true>), dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);   \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_cg_m<V, false>), dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);  \
+    } while (0)
+    if (vx == 4)
+        LAUNCH_CG_M(4);
+    else if (vx == 2)
+        LAUNCH_CG_M(2);
+    else
+        LAUNCH_CG_M(1);
+#undef LAUNCH_CG_M
 }
 
 void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int grid, StepRef step, hipStream_t s)

@@ -6,13 +6,38 @@
 
 namespace sps {
 
-// Sum over a wave64 with xor butterflies.  Lane 0 holds the sum in a fixed
-// order (the value in other lanes is not used).
+// One double moved between lanes of each row of 16 (DPP row rotate right
+// by N: lane l of a row gets lane (l - N) mod 16 of the same row), both
+// 32-bit halves.
+template <int N>
+__device__ __forceinline__ double row_ror(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x120 + N, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x120 + N, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over a wave64, returned to every lane (a wave-uniform value).  Each
+// row of 16 lanes is summed with DPP rotations (lane 0 of the row adds in a
+// fixed order), then the four rows' sums are added in order.  The bits
+// depend only on the 64 inputs' lane positions, so every wave that sums
+// the same values in the same lanes gets the same total.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += row_ror<8>(v);
+    v += row_ror<4>(v);
+    v += row_ror<2>(v);
+    v += row_ror<1>(v);
+    return ((read_lane(v, 0) + read_lane(v, 16)) + read_lane(v, 32)) + read_lane(v, 48);
 }
 
 // Workgroup sum, fixed order; the result is valid in thread 0 only.
@@ -33,20 +58,15 @@ __device__ __forceinline__ double block_sum(double v)
 }
 
 // Total of n per-workgroup partial sums written by an earlier kernel,
-// returned to every thread.  The summation order depends only on n, so every
-// workgroup (and every kernel) that sums the same partials gets the same
-// bits, and a problem's result does not depend on what else runs.
+// returned to every thread.  Each wave sums all n itself (lane l the
+// partials l, l + 64, ..., then wave_sum), so no wave waits for another and
+// every wave of every workgroup and kernel that sums the same partials gets
+// the same bits: a problem's result does not depend on what else runs.
 __device__ __forceinline__ double total_of(const double *partial, int n)
 {
-    __shared__ double tot;
     double s = 0.0;
-    for (int t = threadIdx.x; t < n; t += kThreads) s += partial[t];
-    s = block_sum(s);
-    if (threadIdx.x == 0) tot = s;
-    __syncthreads();
-    const double r = tot;
-    __syncthreads();
-    return r;
+    for (int t = threadIdx.x & 63; t < n; t += 64) s += partial[t];
+    return wave_sum(s);
 }
 
 // Four workgroup sums at once (fixed order), valid in thread 0 only.
@@ -70,23 +90,16 @@ __device__ __forceinline__ void block_sum4(double v[4])
         }
 }
 
-// Totals of n partial 4-vectors (m[4t + c]), returned to every thread; the
-// order depends only on n.
+// Totals of n partial 4-vectors (m[4t + c]), returned to every thread; as
+// total_of, each wave sums them all in the same order.
 __device__ __forceinline__ void total4_of(const double *m, int n, double out[4])
 {
-    __shared__ double tot4[4];
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < n; t += kThreads)
+    for (int t = threadIdx.x & 63; t < n; t += 64)
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] += m[4 * t + c];
-    block_sum4(v);
-    if (threadIdx.x == 0)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) tot4[c] = v[c];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 4; ++c) out[c] = tot4[c];
-    __syncthreads();
+    for (int c = 0; c < 4; ++c) out[c] = wave_sum(v[c]);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on

@@ -199,6 +199,12 @@ private:
     int stage_next_ = 0;
     // offsets of the four tables and the mixed-problem list in d_tab_/d_probs_list_
     int off_t3_64 = 0, off_t1_64 = 0, off_t3_32 = 0, off_t1_32 = 0;
+    // the FP32 3-D table holds the problems of one lane width after another
+    // (1, 2, 4 cells per lane): where each width's entries start and how many
+    int sub_t3_32[3] = {0, 0, 0}, cnt_t3_32[3] = {0, 0, 0};
+    // launches the mixed step over every lane width's entries; the first
+    // launch carries the step counter update `next`
+    void launch_steps_m(StepRef sr, int *ran);
     // graph replay: step counter on the device, two captured groups of
     // kRefreshGrain steps (without / with the residual replacement)
     int *d_step_ = nullptr;            // two counters, alternating by group parity

@@ -428,11 +428,21 @@ int Engine::rebuild_tables()
         const Slot &sl = slots_[i];
         if (!sl.active || sl.finished) continue;
         table_cells_ += sl.g.n;
-        auto &t3 = sl.o.mixed ? tab_.t3_32 : tab_.t3_64;
         auto &t1 = sl.o.mixed ? tab_.t1_32 : tab_.t1_64;
-        for (int t = 0; t < sl.pd.nt3; ++t) t3.push_back({i, t});
+        if (!sl.o.mixed)
+            for (int t = 0; t < sl.pd.nt3; ++t) tab_.t3_64.push_back({i, t});
         for (int t = 0; t < sl.pd.nt1; ++t) t1.push_back({i, t});
         if (sl.o.mixed) tab_.mixed_probs.push_back(i);
+    }
+    // FP32 3-D entries grouped by lane width (one step kernel per width)
+    for (int w = 0; w < 3; ++w) {
+        sub_t3_32[w] = (int)tab_.t3_32.size();
+        for (int i = 0; i < max_slots_; ++i) {
+            const Slot &sl = slots_[i];
+            if (!sl.active || sl.finished || !sl.o.mixed || sl.g.vx != (1 << w)) continue;
+            for (int t = 0; t < sl.pd.nt3; ++t) tab_.t3_32.push_back({i, t});
+        }
+        cnt_t3_32[w] = (int)tab_.t3_32.size() - sub_t3_32[w];
     }
     off_t3_64 = 0;
     off_t1_64 = off_t3_64 + (int)tab_.t3_64.size();
@@ -521,6 +531,16 @@ bool Engine::refresh_may_be_due(int s) const
     return false;
 }
 
+void Engine::launch_steps_m(StepRef sr, int *ran)
+{
+    for (int w = 0; w < 3; ++w) {
+        const int n = cnt_t3_32[w];
+        if (!n) continue;
+        launch_cg_m(d_probs_, d_tab_ + off_t3_32 + sub_t3_32[w], n, std::min(n, kMaxGrid), sr, ran, 1 << w, s_);
+        sr.next = nullptr;   // one launch updates the counter
+    }
+}
+
 // Kernels of global steps step_+1 .. step_+nsteps launched from the host,
 // each with its step number as an argument.
 int Engine::enqueue_steps_direct(int nsteps, bool timing)
@@ -539,7 +559,7 @@ int Engine::enqueue_steps_direct(int nsteps, bool timing)
         int *ran = t_open ? d_ran_ + 2 * ntimed_ : t_close ? d_ran_ + 2 * ntimed_ + 1 : nullptr;
         if (t_open) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
         if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, ran, s_);
-        if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr, ran, s_);
+        if (n3_32) launch_steps_m(sr, ran);
         if (t_close) {
             ECHK(hipEventRecord(ev_spmv_[2 * ntimed_ + 1], s_));
             ++ntimed_;
@@ -615,7 +635,7 @@ int Engine::capture_graphs()
             const StepRef sr{base, off, off == 1 ? next : nullptr};
             const StepRef sr_m{base, off, (off == 1 && !n3_64) ? next : nullptr};
             if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, nullptr, s_);
-            if (n3_32) launch_cg_m(d_probs_, d_tab_ + off_t3_32, n3_32, std::min(n3_32, kMaxGrid), sr_m, nullptr, s_);
+            if (n3_32) launch_steps_m(sr_m, nullptr);
             if (n1_64)
                 launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid),
                             StepRef{base, off, nullptr}, s_);
@@ -923,7 +943,7 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         auto launch = [&] {
             const int step = mixed ? 2 + (seq++ % 8) : 3;
             if (mixed)
-                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
+                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, g.vx, s_);
             else
                 launch_cg_a(false, dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
         };

@@ -45,7 +45,7 @@ struct Geo {
     int px, py;               // periodic in x / y (z is always mirrored)
     long long n;              // nx*ny*nz
     int tiles_x, tiles_y, tiles_z, kz;   // 3-D tiling: (64 vx) x 4 columns, kz planes
-    int vx;                   // x cells per lane in 3-D tiles: 1, or 2 (FP32, even nx)
+    int vx;                   // x cells per lane in 3-D tiles: 1, or 2 or 4 (FP32, nx a multiple)
     int dtop;                 // 1: p = 0 on the top face (top ghost = -interior)
 };
 
@@ -235,7 +235,9 @@ void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int gr
 // iteration from the exact identity r_{n-1}.r_{n-1} = r_{n-2}.r_{n-2}
 // - 2 alpha q.r_{n-2} + alpha^2 q.q anchored on the directly summed
 // r_{n-2}.r_{n-2}.  ran: as for cg_a.
-void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, hipStream_t s);
+// vx: the cells per lane of every problem in t3
+void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, int vx,
+                 hipStream_t s);
 // Mixed precision residual replacement after global step `step` for the
 // problems of the tables (`probs` lists them, one entry per problem): e += c
 // (with the deferred alpha p term), r = b - A e in FP64 handed to the FP32

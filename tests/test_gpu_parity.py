@@ -392,3 +392,46 @@ def test_posdin_reuses_what_graddin_staged():
         fb.prd1 = fa.prd1.copy(order="F")
         s0.posdin(g, fb)
         assert np.array_equal(fa.u, fb.u) and np.array_equal(fa.w, fb.w)
+
+
+LANE_CASES = [
+    # (vx, nx, periodic_x, extra): rows of one lane, a partial last
+    # segment, whole segments, several segments, periodic rows of one and of
+    # several segments
+    (4, 4, False, {}), (4, 4, True, {}), (4, 72, False, {}), (4, 72, True, {}),
+    (4, 256, False, {}), (4, 256, True, {}), (4, 264, False, {}), (4, 264, True, {}),
+    (4, 520, True, dict(periodic_y=True)), (4, 72, False, dict(t_press=2)),
+    (4, 136, True, dict(lid=True)),
+    (2, 2, True, {}), (2, 66, True, {}), (2, 128, True, {}), (2, 130, False, {}),
+    (2, 130, True, dict(periodic_y=True)), (2, 258, False, dict(t_press=2)),
+]
+
+
+@pytest.mark.parametrize("vx,nx,px,extra", LANE_CASES,
+                         ids=[f"vx{v}-nx{n}-{'px' if p else 'walls'}-{'-'.join(e) or 'plain'}"
+                              for v, n, p, e in LANE_CASES])
+def test_lane_widths_vs_oracle(solver, vx, nx, px, extra):
+    """Two and four cells per lane: the x neighbours of a lane's end cells
+    come from the adjacent lanes, from the lane itself (walls), from the
+    row's other end (periodic rows of one segment) or from a load (segment
+    ends inside a row, periodic rows of several).  Against the oracle."""
+    extra = dict(extra)
+    lid = extra.pop("lid", False)
+    c = cases.random_case(nx, 6, 7, seed=90 + nx + vx, periodic_x=px, **extra)
+    if lid:
+        c.grid.top_bc = 1
+    if c.grid.t_press == 2:
+        c.grid.rho_f1 = 1000.0
+    p_or, _, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
+                                 project_null=not lid)
+    assert rc == 0
+    solver.set_tiles(c.grid, "mixed", 3, vx)
+    try:
+        solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True)
+        assert solver.autotune(c.grid, "mixed")[:2] == (3, vx)
+        f2 = c.fields.copy()
+        st = solver.graddin(c.grid, f2)
+    finally:
+        solver.configure(precision="fp64", stop_rule="reference", verbose=False, autotune=False)
+    assert st.status == 0, st.as_dict()
+    assert helpers.rel_diff(f2.prd1, p_or, helpers.face_mask(p_or.shape)) <= PRESSURE_RTOL
