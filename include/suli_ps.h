@@ -378,9 +378,11 @@ int suli_ps_memcpy_d2h(suli_ps_ctx *ctx, void *dst, const void *src, size_t byte
 int suli_ps_dev_zero(suli_ps_ctx *ctx, void *ptr, size_t bytes); /* async, ctx stream */
 int suli_ps_synchronize(suli_ps_ctx *ctx);
 
-/* Measurement hook: runs the solver's SpMV kernel `reps` times on the
- * operator of the last graddin call (precision as given) and returns the
- * mean launch duration in ms, timed with HIP events on the context stream. */
+/* Measurement hook: runs the solver's SpMV kernel q = A p `reps` times on
+ * the operator of the last graddin call (precision as given) for each of a
+ * few tile shapes of its own (depths of 2..24 planes; 1, 2 or 4 cells per
+ * lane in FP32) and returns the fastest shape's mean launch duration in ms,
+ * timed with HIP events on the context stream. */
 int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double *ms);
 
 /* Tile shape of the CG kernels for a grid size, no reference counterpart:

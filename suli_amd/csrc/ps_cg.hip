@@ -32,11 +32,9 @@
 namespace sps {
 namespace {
 
+// 16 bytes of T as one vector (one 16-byte load or store)
 template <typename T>
-struct alignas(16) Vec {
-    static constexpr int W = 16 / sizeof(T);
-    T v[W];
-};
+using Vec = T __attribute__((ext_vector_type(16 / sizeof(T))));
 
 __device__ __forceinline__ int step_of(StepRef s)
 {
@@ -61,9 +59,11 @@ __device__ __forceinline__ Run my_run(int ntab)
 }
 
 // 3-D tile `tile` of a problem: (64 vx) x 4 columns, planes [k0, k1); a
-// lane owns x cells i .. i + vx - 1.
+// lane owns x cells i .. i + vx - 1.  i0, the tile's first x, is the same
+// in every lane (derived from the tile index alone, so the compiler keeps it
+// and what follows from it in scalar registers).
 struct Tile {
-    int i, j, k0, k1;
+    int i, j, k0, k1, i0;
 };
 
 __device__ __forceinline__ Tile tile3(const Geo &g, int tile)
@@ -73,6 +73,7 @@ __device__ __forceinline__ Tile tile3(const Geo &g, int tile)
     const int ty = rest % g.tiles_y;
     const int tz = rest / g.tiles_y;
     Tile t;
+    t.i0 = tx * kTileX * g.vx;
     t.i = (tx * kTileX + (threadIdx.x & (kTileX - 1))) * g.vx;
     t.j = ty * kTileY + (threadIdx.x / kTileX);
     t.k0 = tz * g.kz;
@@ -84,29 +85,69 @@ __device__ __forceinline__ Tile tile3(const Geo &g, int tile)
 // them: the three face arrays (index f of the face), or, for a constant-
 // density operator, the per-plane tables (every x and y face of plane k
 // holds tx[k], ty[k]; z face kf holds tz[kf]), which store the same values.
+//
+// FP32 face arrays come with row tables: per-plane values (repeated: 8 at
+// rx + 8k, 4 at ry + 4k and rz + 4kf) and a bit per row (j, k), set when
+// every face of the row holds them.  The kernels point the row's
+// coefficient loads at the tables then: the same instructions and values,
+// without the face arrays' traffic.
 template <typename T>
 struct FaceArrays {
-    const T *sx, *sy, *sz;
+    gptr<const T> sx, sy, sz;
+    gptr<const T> rx = nullptr, ry = nullptr, rz = nullptr;
+    cptr<unsigned> rowreg = nullptr;
+    int wpp = 0;   // words of rowreg per plane
     __device__ __forceinline__ T x(long long f, int) const { return sx[f]; }
     __device__ __forceinline__ T y(long long f, int) const { return sy[f]; }
     __device__ __forceinline__ T z(long long f, int) const { return sz[f]; }
+    // Bit k - k0 set when every face of row j of plane k holds the plane's
+    // table values, for the planes k0 .. k1-1 of a tile (up to 64; later
+    // planes read the arrays).  j is the wave's row, the same in all its
+    // lanes: lane l fetches plane k0 + l's bit, the ballot gathers them.
+    __device__ __forceinline__ unsigned long long rows(int j, int k0, int k1) const
+    {
+        if (!rowreg) return 0ull;
+        const int ju = __builtin_amdgcn_readfirstlane(j);
+        const int k = k0 + (int)(threadIdx.x & 63);
+        const bool bit = k < k1 && ((rowreg[(long long)k * wpp + ju / 32] >> (ju % 32)) & 1u);
+        return __ballot(bit);
+    }
+    // where a row's coefficients are read: the tables (reg) or the arrays
+    __device__ __forceinline__ gptr<const T> xp(bool reg, long long f, int k) const { return reg ? rx + 8 * k : sx + f; }
+    __device__ __forceinline__ gptr<const T> yp(bool reg, long long f, int k) const { return reg ? ry + 4 * k : sy + f; }
+    __device__ __forceinline__ gptr<const T> zp(bool reg, long long f, int kf) const { return reg ? rz + 4 * kf : sz + f; }
+    __device__ __forceinline__ T x_at(bool reg, long long f, int k, int m) const { return xp(reg, f, k)[m]; }
+    __device__ __forceinline__ T y_at(bool reg, long long f, int k) const { return *yp(reg, f, k); }
+    __device__ __forceinline__ T z_at(bool reg, long long f, int kf) const { return *zp(reg, f, kf); }
 };
 template <typename T>
 struct PlaneTables {
-    const T *tx, *ty, *tz;
+    cptr<T> tx, ty, tz;
+    __device__ __forceinline__ unsigned long long rows(int, int, int) const { return ~0ull; }
+    __device__ __forceinline__ T x_at(bool, long long, int k, int) const { return tx[k]; }
+    __device__ __forceinline__ T y_at(bool, long long, int k) const { return ty[k]; }
+    __device__ __forceinline__ T z_at(bool, long long, int kf) const { return tz[kf]; }
     __device__ __forceinline__ T x(long long, int k) const { return tx[k]; }
     __device__ __forceinline__ T y(long long, int k) const { return ty[k]; }
     __device__ __forceinline__ T z(long long, int kf) const { return tz[kf]; }
 };
 
-// Per-workgroup cache of a problem's partial-sum total.
+// Plane k's bit in a tile's row mask (planes from k0)
+__device__ __forceinline__ bool row_bit(unsigned long long rows, int k, int k0)
+{
+    const int d = k - k0;
+    return d < 64 && ((rows >> d) & 1ull);
+}
+
+// Per-workgroup cache of a problem's partial-sum total (the FP64 kernels:
+// one wave sums, the others wait at the workgroup barrier).
 struct TotalCache {
     int prob = -1;
     double value = 0.0;
     __device__ __forceinline__ double get(int p, const double *partial, int n)
     {
         if (p != prob) {
-            value = total_of(partial, n);
+            value = block_total_of(partial, n);
             prob = p;
         }
         return value;
@@ -123,9 +164,10 @@ struct TotalCache {
 // deferred e -= alpha p) are template parameters so that the plane loop has
 // no branches: every load of a plane is issued before any is waited for.
 template <typename T, bool FIRST, bool DO_E, class CO>
-__device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, T *__restrict__ e,
-                                            const T *__restrict__ p_old, T *__restrict__ p_new,
-                                            const T *__restrict__ r, const CO &co, T *__restrict__ q, T b, T a_prev)
+__device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, gptr<T> __restrict__ e,
+                                            gptr<const T> __restrict__ p_old, gptr<T> __restrict__ p_new,
+                                            gptr<const T> __restrict__ r, const CO &co, gptr<T> __restrict__ q, T b,
+                                            T a_prev)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -173,18 +215,26 @@ __device__ __forceinline__ double cg_a_tile(const Geo &g, const Tile &t, T *__re
 
 template <typename T>
 struct Vecs {
-    T *e, *r, *q, *p0, *p1;
-    const T *sx, *sy, *sz;
-    const T *tx, *ty, *tz;   // null unless the operator has plane tables
+    gptr<T> e, r, q, p0, p1;
+    gptr<const T> sx, sy, sz;
+    cptr<T> tx, ty, tz;         // null unless the operator has plane tables
+    gptr<const T> rx, ry, rz;   // row tables (FP32), null unless rowreg is set
+    cptr<unsigned> rowreg;
+    int wpp;
 };
 
 __device__ __forceinline__ Vecs<double> vecs(const ProbDev &pd, double)
 {
-    return {pd.e, pd.r, pd.q, pd.p0, pd.p1, pd.sx, pd.sy, pd.sz, pd.tx, pd.ty, pd.tz};
+    return {glob(pd.e),  glob(pd.r),  glob(pd.q),  glob(pd.p0), glob(pd.p1), glob(pd.sx),
+            glob(pd.sy), glob(pd.sz), konst(pd.tx), konst(pd.ty), konst(pd.tz), nullptr,
+            nullptr,     nullptr,     nullptr,     0};
 }
 __device__ __forceinline__ Vecs<float> vecs(const ProbDev &pd, float)
 {
-    return {pd.c32, pd.r32, pd.q32, pd.p32, pd.p32 + pd.pslot, pd.sx32, pd.sy32, pd.sz32, pd.tx32, pd.ty32, pd.tz32};
+    return {glob(pd.c32),  glob(pd.r32),  glob(pd.q32),  glob(pd.p32),  glob(pd.p32 + pd.pslot),
+            glob(pd.sx32), glob(pd.sy32), glob(pd.sz32), konst(pd.tx32), konst(pd.ty32),
+            konst(pd.tz32), glob(pd.rx32), glob(pd.ry32), glob(pd.rz32), konst(pd.rowreg),
+            (pd.g.ny + 31) / 32};
 }
 
 // Calls f with the problem's coefficient source (tables or face arrays).
@@ -192,7 +242,7 @@ template <typename T, class F>
 __device__ __forceinline__ auto with_coefs(const Vecs<T> &V, F &&f)
 {
     if (V.tx) return f(PlaneTables<T>{V.tx, V.ty, V.tz});
-    return f(FaceArrays<T>{V.sx, V.sy, V.sz});
+    return f(FaceArrays<T>{V.sx, V.sy, V.sz, V.rx, V.ry, V.rz, V.rowreg, V.wpp});
 }
 
 template <typename T>
@@ -205,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
         const ProbDev &pd = P[tr.prob];
-        CgScalars *S = pd.S;
+        const gptr<CgScalars> S = glob(pd.S);
         const int n = step - pd.n0 + 1;
         // the problem's scalars, then the previous kernel's partials, both
         // requested before the scalars decide whether the problem runs, so
@@ -242,8 +292,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
             if (ran && tr.prob == 0 && !stop) *ran = 1;
         }
         const Vecs<T> V = vecs(pd, T());
-        T *p_old = (n & 1) ? V.p0 : V.p1;
-        T *p_new = (n & 1) ? V.p1 : V.p0;
+        const gptr<T> p_old = (n & 1) ? V.p0 : V.p1;
+        const gptr<T> p_new = (n & 1) ? V.p1 : V.p0;
         const T a_prev = (T)alpha_prev;
         const Tile tl = tile3(pd.g, tr.tile);
         if (stop) {
@@ -281,16 +331,16 @@ __global__ __launch_bounds__(kThreads) void k_cg_a(const ProbDev *__restrict__ P
 // (read through p_new before the store), p_{n-1} is p_old, and the slots in
 // between are listed in Fold; alphas oldest first.
 struct Fold {
-    const float *p[kFold - 2];   // p_{n-kFold+1} .. p_{n-2}
+    gptr<const float> p[kFold - 2];   // p_{n-kFold+1} .. p_{n-2}
     float a[kFold];              // alpha_{n-kFold} .. alpha_{n-1}
 };
 
 template <bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
-__device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__restrict__ c32,
-                                          const float *__restrict__ r_old, const float *__restrict__ q_old,
-                                          const float *__restrict__ p_old, const float *__restrict__ p_prev,
-                                          float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                          float a, float b, float bp, const Fold &fd, double acc[4])
+__device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, gptr<float> __restrict__ c32,
+                                          gptr<const float> __restrict__ r_old, gptr<const float> __restrict__ q_old,
+                                          gptr<const float> __restrict__ p_old, gptr<const float> __restrict__ p_prev,
+                                          gptr<float> __restrict__ p_new, gptr<float> __restrict__ q_new,
+                                          const CO &co, float a, float b, float bp, const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -316,7 +366,8 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
     float rc = rn(c, poc, ppc);
     float pc = FIRST ? rc : rc + b * poc;
     float pb = t.k0 > 0 ? pn(c - plane) : pc;
-    float szb = co.z(c, t.k0);
+    const unsigned long long rows = co.rows(t.j, t.k0, t.k1);
+    float szb = co.z_at(row_bit(rows, t.k0, t.k0), c, t.k0);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - t.i;
         const long long up = (k < nz - 1) ? plane : 0;
@@ -324,11 +375,12 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
         const float ppt = REC ? p_prev[c + up] : 0.0f;
         const float rt = rn(c + up, pot, ppt);
         const float pt = FIRST ? rt : rt + b * pot;
-        const float szt = co.z(c + plane, k + 1);
         const float pw = pn(r0 + iw), pe = pn(r0 + ie);
         const float ps = pn(c + dS), pnn = pn(c + dN);
-        const float sxw = co.x(fx, k), sxe = co.x(fx + 1, k);
-        const float sys = co.y(fy, k), syn = co.y(fy + nx, k);
+        const bool reg = row_bit(rows, k, t.k0);
+        const float szt = co.z_at(reg, c + plane, k + 1);
+        const float sxw = co.x_at(reg, fx, k, 0), sxe = co.x_at(reg, fx, k, 1);
+        const float sys = co.y_at(reg, fy, k), syn = co.y_at(reg, fy + nx, k);
         const float ptu = (g.dtop && k == nz - 1) ? -pc : pt;   // Dirichlet lid
         const float y = pc - sxe * pe - sxw * pw - syn * pnn - sys * ps - szt * ptu - szb * pb;
         // p_{n-kFold}, in the slot p_n is about to replace (this lane's cell)
@@ -365,35 +417,29 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, float *__
 // cells i .. i+V-1 are one 8- or 16-byte vector in every cell-layout array
 // (all 256-byte aligned, i a multiple of V).
 template <int V>
-__device__ __forceinline__ void ldv(const float *p, long long x, float (&o)[V])
+__device__ __forceinline__ void ldv(gptr<const float> p, long long x, float (&o)[V])
 {
-    if constexpr (V == 2) {
-        const float2 t = *reinterpret_cast<const float2 *>(p + x);
-        o[0] = t.x;
-        o[1] = t.y;
-    } else {
-        const float4 t = *reinterpret_cast<const float4 *>(p + x);
-        o[0] = t.x;
-        o[1] = t.y;
-        o[2] = t.z;
-        o[3] = t.w;
-    }
+    typedef float vt __attribute__((ext_vector_type(V)));
+    const vt t = *reinterpret_cast<gptr<const vt>>(p + x);
+#pragma unroll
+    for (int m = 0; m < V; ++m) o[m] = t[m];
 }
 template <int V>
-__device__ __forceinline__ void stv(float *p, long long x, const float (&v)[V])
+__device__ __forceinline__ void stv(gptr<float> p, long long x, const float (&v)[V])
 {
-    if constexpr (V == 2)
-        *reinterpret_cast<float2 *>(p + x) = make_float2(v[0], v[1]);
-    else
-        *reinterpret_cast<float4 *>(p + x) = make_float4(v[0], v[1], v[2], v[3]);
+    typedef float vt __attribute__((ext_vector_type(V)));
+    vt t;
+#pragma unroll
+    for (int m = 0; m < V; ++m) t[m] = v[m];
+    *reinterpret_cast<gptr<vt>>(p + x) = t;
 }
 // V consecutive floats at a 4-byte-aligned address (x faces: a row holds
 // nx + 1 of them) as one vector load.
 template <int V>
-__device__ __forceinline__ void ldvu(const float *p, long long x, float (&o)[V])
+__device__ __forceinline__ void ldvu(gptr<const float> p, long long x, float (&o)[V])
 {
     typedef float vt __attribute__((ext_vector_type(V), aligned(4)));
-    const vt t = *reinterpret_cast<const vt *>(p + x);
+    const vt t = *reinterpret_cast<gptr<const vt>>(p + x);
 #pragma unroll
     for (int m = 0; m < V; ++m) o[m] = t[m];
 }
@@ -415,28 +461,14 @@ __device__ __forceinline__ float lane_value(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// The V + 1 x faces of a lane's V cells (faces i .. i+V), the V y or z faces
-// at one offset: face arrays, or the plane's value repeated.
-template <int V>
-__device__ __forceinline__ void xv(const FaceArrays<float> &co, long long f, int, float (&o)[V + 1])
-{
-    float t[V];
-    ldvu<V>(co.sx, f, t);
-#pragma unroll
-    for (int m = 0; m < V; ++m) o[m] = t[m];
-    o[V] = co.sx[f + V];
-}
+// Plane tables: the V + 1 x faces of a lane's V cells (faces i .. i+V), the
+// V y or z faces at one offset, all the plane's value.
 template <int V>
 __device__ __forceinline__ void xv(const PlaneTables<float> &co, long long, int k, float (&o)[V + 1])
 {
     const float v = co.tx[k];
 #pragma unroll
     for (int m = 0; m <= V; ++m) o[m] = v;
-}
-template <int V>
-__device__ __forceinline__ void yv(const FaceArrays<float> &co, long long f, int, float (&o)[V])
-{
-    ldv<V>(co.sy, f, o);
 }
 template <int V>
 __device__ __forceinline__ void yv(const PlaneTables<float> &co, long long, int k, float (&o)[V])
@@ -446,11 +478,6 @@ __device__ __forceinline__ void yv(const PlaneTables<float> &co, long long, int 
     for (int m = 0; m < V; ++m) o[m] = v;
 }
 template <int V>
-__device__ __forceinline__ void zv(const FaceArrays<float> &co, long long f, int, float (&o)[V])
-{
-    ldv<V>(co.sz, f, o);
-}
-template <int V>
 __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int kf, float (&o)[V])
 {
     const float v = co.tz[kf];
@@ -458,6 +485,45 @@ __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int 
     for (int m = 0; m < V; ++m) o[m] = v;
 }
 
+// The coefficients of one plane of a row: x faces (V + 1), y faces j and
+// j + 1, z face k + 1: the row tables when the row's bit is set, else the
+// face arrays.
+template <int V>
+__device__ __forceinline__ void coefs_v(const FaceArrays<float> &co, bool reg, long long fx, long long fy,
+                                        long long fz1, int nx, int k, float (&sxv)[V + 1], float (&sys)[V],
+                                        float (&syn)[V], float (&szt)[V])
+{
+    const gptr<const float> px = co.xp(reg, fx, k);
+    float t[V];
+    ldvu<V>(px, 0, t);
+#pragma unroll
+    for (int m = 0; m < V; ++m) sxv[m] = t[m];
+    sxv[V] = px[V];
+    ldv<V>(co.yp(reg, fy, k), 0, sys);
+    ldv<V>(co.yp(reg, fy + nx, k), 0, syn);
+    ldv<V>(co.zp(reg, fz1, k + 1), 0, szt);
+}
+template <int V>
+__device__ __forceinline__ void coefs_v(const PlaneTables<float> &co, bool, long long fx, long long fy,
+                                        long long fz1, int nx, int k, float (&sxv)[V + 1], float (&sys)[V],
+                                        float (&syn)[V], float (&szt)[V])
+{
+    xv<V>(co, fx, k, sxv);
+    yv<V>(co, fy, k, sys);
+    yv<V>(co, fy + nx, k, syn);
+    zv<V>(co, fz1, k + 1, szt);
+}
+// z faces of a plane's row at the tile's first plane
+template <int V>
+__device__ __forceinline__ void zb_v(const FaceArrays<float> &co, bool reg, long long f, int kf, float (&o)[V])
+{
+    ldv<V>(co.zp(reg, f, kf), 0, o);
+}
+template <int V>
+__device__ __forceinline__ void zb_v(const PlaneTables<float> &co, bool, long long f, int kf, float (&o)[V])
+{
+    zv<V>(co, f, kf, o);
+}
 // Where a wave of V-cell lanes sits in its row: the wave covers the
 // segment of 64 V cells from x = i0 (one segment when nx <= 64 V).  Lanes
 // past the row's end (nx is a multiple of V) read the row's last V cells
@@ -480,7 +546,7 @@ __device__ __forceinline__ RowWave row_wave(const Geo &g, const Tile &t)
     RowWave w;
     const int nx = g.nx;
     w.lane = threadIdx.x & (kTileX - 1);
-    const int i0 = t.i - w.lane * V;
+    const int i0 = t.i0;
     const int seg = kTileX * V;
     w.last = min(kTileX - 1, (nx - i0) / V - 1);
     w.active = t.i < nx;
@@ -498,12 +564,16 @@ __device__ __forceinline__ RowWave row_wave(const Geo &g, const Tile &t)
 // (and the same term order).  The x neighbours of p_n are exchanged between
 // lanes (DPP) instead of being formed again from three loads each, and the
 // x faces of a lane come as one vector.
-template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
-__device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *__restrict__ c32,
-                                           const float *__restrict__ r_old, const float *__restrict__ q_old,
-                                           const float *__restrict__ p_old, const float *__restrict__ p_prev,
-                                           float *__restrict__ p_new, float *__restrict__ q_new, const CO &co,
-                                           float a, float b, float bp, const Fold &fd, double acc[4])
+//
+// EDGE: the row has more than one segment, so the cells beyond a segment's
+// ends may have to be loaded; without it (nx <= 64 V) the loop has no
+// branch on them.
+template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, bool EDGE, class CO>
+__device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<float> __restrict__ c32,
+                                           gptr<const float> __restrict__ r_old, gptr<const float> __restrict__ q_old,
+                                           gptr<const float> __restrict__ p_old, gptr<const float> __restrict__ p_prev,
+                                           gptr<float> __restrict__ p_new, gptr<float> __restrict__ q_new,
+                                           const CO &co, float a, float b, float bp, const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -569,7 +639,8 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
 #pragma unroll
         for (int m = 0; m < V; ++m) pb[m] = pc[m];
     }
-    zv<V>(co, c, t.k0, szb);
+    const unsigned long long rows = co.rows(t.j, t.k0, t.k1);
+    zb_v<V>(co, row_bit(rows, t.k0, t.k0), c, t.k0, szb);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - w.ci;
         const long long up = (k < nz - 1) ? plane : 0;
@@ -583,15 +654,14 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
         rnv(c + up, pot, rt, ppt);
 #pragma unroll
         for (int m = 0; m < V; ++m) pt[m] = FIRST ? rt[m] : rt[m] + b * pot[m];
-        zv<V>(co, c + plane, k + 1, szt);
         pnv(c + dS, ps);
         pnv(c + dN, pnn);
-        xv<V>(co, fx, k, sxv);
-        yv<V>(co, fy, k, sys);
-        yv<V>(co, fy + nx, k, syn);
+        coefs_v<V>(co, row_bit(rows, k, t.k0), fx, fy, c + plane, nx, k, sxv, sys, syn, szt);
         // x neighbours of the lane's end cells: adjacent lanes, or beyond the segment
-        const float ew = w.load_w ? pn1(r0 + w.xw) : (w.wrap ? lane_value(pc[V - 1], w.last) : pc[0]);
-        const float ee = w.load_e ? pn1(r0 + w.xe) : (w.wrap ? lane_value(pc[0], 0) : pc[V - 1]);
+        const float ew_row = w.wrap ? lane_value(pc[V - 1], w.last) : pc[0];
+        const float ee_row = w.wrap ? lane_value(pc[0], 0) : pc[V - 1];
+        const float ew = (EDGE && w.load_w) ? pn1(r0 + w.xw) : ew_row;
+        const float ee = (EDGE && w.load_e) ? pn1(r0 + w.xe) : ee_row;
         const float pw = from_lower(ew, pc[V - 1]);
         const float pe_dpp = from_upper(ee, pc[0]);
         const float pe = w.lane == w.last ? ee : pe_dpp;
@@ -659,10 +729,13 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, float *_
     }
 }
 
-__device__ __forceinline__ float *qbuf(const ProbDev &pd, int parity) { return parity ? pd.q32b : pd.q32; }
+__device__ __forceinline__ gptr<float> qbuf(const ProbDev &pd, int parity) { return glob(parity ? pd.q32b : pd.q32); }
 // ring slot of p_k
-__device__ __forceinline__ float *pbuf(const ProbDev &pd, int k) { return pd.p32 + (k & (kFold - 1)) * pd.pslot; }
-__device__ __forceinline__ float alph_of(const CgScalars *S, int k) { return (float)S->alph[k & (kFold - 1)]; }
+__device__ __forceinline__ gptr<float> pbuf(const ProbDev &pd, int k)
+{
+    return glob(pd.p32 + (k & (kFold - 1)) * pd.pslot);
+}
+__device__ __forceinline__ float alph_of(gptr<const CgScalars> S, int k) { return (float)S->alph[k & (kFold - 1)]; }
 
 // One tile of the mixed step n with V cells per lane (V = 1: cg_m_tile).
 // REC: r_{n-2} is recovered from the directions, r_{n-2} = p_{n-1} -
@@ -675,32 +748,36 @@ __device__ __forceinline__ float alph_of(const CgScalars *S, int k) { return (fl
 // operands are gathered here, inside the FOLD instance only, so that the
 // other instances carry no extra live registers.
 template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
-__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, const float *r_o,
-                                              const float *q_o, const float *p_o, const float *p_oo, float *p_w,
-                                              float *q_w, const CO &co, float a, float b, float bp, double acc[4])
+__device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, gptr<const float> r_o,
+                                              gptr<const float> q_o, gptr<const float> p_o, gptr<const float> p_oo,
+                                              gptr<float> p_w, gptr<float> q_w, const CO &co, float a, float b,
+                                              float bp, double acc[4])
 {
     Fold fd;
     if constexpr (FOLD) {
 #pragma unroll
-        for (int m = 0; m < kFold - 1; ++m) fd.a[m] = alph_of(pd.S, n - kFold + m);
+        for (int m = 0; m < kFold - 1; ++m) fd.a[m] = alph_of(glob(pd.S), n - kFold + m);
         fd.a[kFold - 1] = a;
 #pragma unroll
         for (int m = 0; m < kFold - 2; ++m) fd.p[m] = pbuf(pd, n - kFold + 1 + m);
     }
     if constexpr (V == 1)
-        cg_m_tile<FIRST, UPD, REC, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, fd, acc);
+        cg_m_tile<FIRST, UPD, REC, FOLD>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, fd, acc);
+    else if (pd.g.nx > kTileX * V)
+        cg_m_tilev<V, FIRST, UPD, REC, FOLD, true>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp,
+                                                   fd, acc);
     else
-        cg_m_tilev<V, FIRST, UPD, REC, FOLD>(pd.g, tl, pd.c32, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, fd,
-                                             acc);
+        cg_m_tilev<V, FIRST, UPD, REC, FOLD, false>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp,
+                                                    fd, acc);
 }
 
 // FOLD: this launch is one of the steps at which c takes a group (the
 // instances of the other steps are not compiled into it).
 template <int V, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, const ProbDev &pd, int n, const Tile &tl,
-                                          const float *r_o, const float *q_o, const float *p_o, const float *p_oo,
-                                          float *p_w, float *q_w, const CO &co, float a, float b, float bp,
-                                          double acc[4])
+                                          gptr<const float> r_o, gptr<const float> q_o, gptr<const float> p_o,
+                                          gptr<const float> p_oo, gptr<float> p_w, gptr<float> q_w, const CO &co,
+                                          float a, float b, float bp, double acc[4])
 {
 #define CG_M(F, U, R, D) cg_m_tile_any<V, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc)
     if (first)
@@ -746,7 +823,7 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
         const ProbDev &pd = P[tr.prob];
-        CgScalars *S = pd.S;
+        const gptr<CgScalars> S = glob(pd.S);
         const int n = step - pd.n0 + 1;
         // The problem's scalars and the previous step's partials were both
         // written by the previous step.  They are requested together, and
@@ -804,7 +881,8 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
             // finish what c has not taken yet, iterations lo .. n-1 since the
             // last fold (the replacement that follows recomputes the residual)
             const int lo = (n - 2) / kFold * kFold + 1;
-            const float *p_old = pbuf(pd, n - 1);
+            const gptr<const float> p_old = pbuf(pd, n - 1);
+            const gptr<float> c32 = glob(pd.c32);
             for (int sub = 0; upd && sub < pd.g.vx; ++sub) {
                 const int ci = tl.i + sub;
                 if (ci >= pd.g.nx || tl.j >= pd.g.ny) break;
@@ -813,7 +891,7 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
                 for (int k = tl.k0; k < tl.k1; ++k, c += plane) {
                     float sum = 0.0f;
                     for (int j = lo; j < n - 1; ++j) sum += alph_of(S, j) * pbuf(pd, j)[c];
-                    pd.c32[c] = pd.c32[c] - (sum + a * p_old[c]);
+                    c32[c] = c32[c] - (sum + a * p_old[c]);
                 }
             }
             continue;
@@ -821,11 +899,11 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const float b = (float)beta;
         const float bp = (float)beta_prev;
-        const float *q_o = qbuf(pd, o), *p_o = pbuf(pd, n - 1), *p_oo = pbuf(pd, n - 2);
-        float *p_w = pbuf(pd, n), *q_w = qbuf(pd, w);
+        const gptr<const float> q_o = qbuf(pd, o), p_o = pbuf(pd, n - 1), p_oo = pbuf(pd, n - 2);
+        const gptr<float> p_w = pbuf(pd, n), q_w = qbuf(pd, w);
         (void)fold;
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
-            cg_m_step<VX, FOLD>(first, upd, rec, pd, n, tl, pd.r32, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
+            cg_m_step<VX, FOLD>(first, upd, rec, pd, n, tl, glob(pd.r32), q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
             return 0;
         });
         block_sum4(acc);
@@ -852,7 +930,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
         const ProbDev &pd = P[tr.prob];
-        CgScalars *S = pd.S;
+        const gptr<CgScalars> S = glob(pd.S);
         const int n = step - pd.n0 + 1;
         // scalars and partials requested together (as in cg_a)
         const int done_at = S->done_at;
@@ -865,23 +943,23 @@ __global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P
             S->alpha = alpha;
         }
         const Vecs<T> V = vecs(pd, T());
-        T *r = V.r;
-        const T *q = V.q;
+        const gptr<T> r = V.r;
+        const gptr<const T> q = V.q;
         const T a = (T)alpha;
-        constexpr int W = Vec<T>::W;
+        constexpr int W = 16 / sizeof(T);
         const long long len = pd.g.n;
         const long long nv = len / W;
         const long long stride = (long long)pd.nt1 * kThreads;
         double acc = 0.0;
         for (long long v = (long long)tr.tile * kThreads + threadIdx.x; v < nv; v += stride) {
-            Vec<T> rv = reinterpret_cast<Vec<T> *>(r)[v];
-            const Vec<T> qv = reinterpret_cast<const Vec<T> *>(q)[v];
+            Vec<T> rv = reinterpret_cast<gptr<Vec<T>>>(r)[v];
+            const Vec<T> qv = reinterpret_cast<gptr<const Vec<T>>>(q)[v];
 #pragma unroll
             for (int l = 0; l < W; ++l) {
-                rv.v[l] = rv.v[l] - a * qv.v[l];
-                acc += (double)rv.v[l] * (double)rv.v[l];
+                rv[l] = rv[l] - a * qv[l];
+                acc += (double)rv[l] * (double)rv[l];
             }
-            reinterpret_cast<Vec<T> *>(r)[v] = rv;
+            reinterpret_cast<gptr<Vec<T>>>(r)[v] = rv;
         }
         if (tr.tile == 0) {
             const long long c = nv * W + threadIdx.x;
@@ -901,7 +979,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_b(const ProbDev *__restrict__ P
 // last one (the stop path of cg_a left the correction to fold).
 __device__ __forceinline__ bool refresh_due(const ProbDev &pd, int step)
 {
-    const CgScalars *S = pd.S;
+    const gptr<const CgScalars> S = glob((const CgScalars *)pd.S);
     const int n = step - pd.n0 + 1;
     return n >= 1 && !S->final_ && (n == S->next_refresh || S->done_at != kNotDone);
 }
@@ -921,7 +999,7 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
     for (int t = run.begin; t < run.end; ++t) {
         const TileRef tr = tab[t];
         const ProbDev &pd = P[tr.prob];
-        const CgScalars *S = pd.S;
+        const gptr<const CgScalars> S = glob((const CgScalars *)pd.S);
         const int n = step - pd.n0 + 1;
         if (!refresh_due(pd, step)) continue;
         const bool pending = S->done_at == kNotDone;
@@ -933,7 +1011,7 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
         // kFold at a replacement): alpha_n from step n's partials, the others
         // stored (p_n was formed by step n; no later step has run)
         float fa[kFold];
-        const float *fp[kFold];
+        gptr<const float> fp[kFold];
 #pragma unroll
         for (int m = 0; m < kFold - 1; ++m) {
             fa[m] = alph_of(S, n - kFold + 1 + m);
@@ -941,33 +1019,33 @@ __global__ __launch_bounds__(kThreads) void k_refresh_e(const ProbDev *__restric
         }
         fa[kFold - 1] = (float)(T[3] / T[0]);
         fp[kFold - 1] = pbuf(pd, n);
-        double *e = pd.e;
-        float *c = pd.c32;
+        const gptr<double> e = glob(pd.e);
+        const gptr<float> c = glob(pd.c32);
         const long long len = pd.g.n;
         const long long nv = len / 4;
         const long long stride = (long long)pd.nt1 * kThreads;
         for (long long v = (long long)tr.tile * kThreads + threadIdx.x; v < nv; v += stride) {
-            Vec<float> cv = reinterpret_cast<const Vec<float> *>(c)[v];
+            Vec<float> cv = reinterpret_cast<gptr<const Vec<float>>>(c)[v];
             if (pending) {
                 float sum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int m = 0; m < kFold; ++m) {
-                    const Vec<float> pv = reinterpret_cast<const Vec<float> *>(fp[m])[v];
+                    const Vec<float> pv = reinterpret_cast<gptr<const Vec<float>>>(fp[m])[v];
 #pragma unroll
-                    for (int l = 0; l < 4; ++l) sum[l] += fa[m] * pv.v[l];
+                    for (int l = 0; l < 4; ++l) sum[l] += fa[m] * pv[l];
                 }
 #pragma unroll
-                for (int l = 0; l < 4; ++l) cv.v[l] = cv.v[l] - sum[l];
+                for (int l = 0; l < 4; ++l) cv[l] = cv[l] - sum[l];
             }
-            Vec<double> e0 = reinterpret_cast<Vec<double> *>(e)[2 * v];
-            Vec<double> e1 = reinterpret_cast<Vec<double> *>(e)[2 * v + 1];
-            e0.v[0] += (double)cv.v[0];
-            e0.v[1] += (double)cv.v[1];
-            e1.v[0] += (double)cv.v[2];
-            e1.v[1] += (double)cv.v[3];
-            reinterpret_cast<Vec<double> *>(e)[2 * v] = e0;
-            reinterpret_cast<Vec<double> *>(e)[2 * v + 1] = e1;
-            reinterpret_cast<Vec<float> *>(c)[v] = Vec<float>{{0.f, 0.f, 0.f, 0.f}};
+            Vec<double> e0 = reinterpret_cast<gptr<Vec<double>>>(e)[2 * v];
+            Vec<double> e1 = reinterpret_cast<gptr<Vec<double>>>(e)[2 * v + 1];
+            e0[0] += (double)cv[0];
+            e0[1] += (double)cv[1];
+            e1[0] += (double)cv[2];
+            e1[1] += (double)cv[3];
+            reinterpret_cast<gptr<Vec<double>>>(e)[2 * v] = e0;
+            reinterpret_cast<gptr<Vec<double>>>(e)[2 * v + 1] = e1;
+            reinterpret_cast<gptr<Vec<float>>>(c)[v] = Vec<float>{0.f, 0.f, 0.f, 0.f};
         }
         if (tr.tile == 0) {
             const long long x = nv * 4 + threadIdx.x;
@@ -1001,15 +1079,16 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
         const Geo &g = pd.g;
         const Tile t = tile3(g, tr.tile);
         const int nx = g.nx, ny = g.ny, nz = g.nz;
-        const double *__restrict__ e = pd.e;
-        const double *__restrict__ b = pd.b;
+        const gptr<const double> __restrict__ e = glob((const double *)pd.e);
+        const gptr<const double> __restrict__ b = glob(pd.b);
+        const gptr<const double> __restrict__ d = glob(pd.d);
         // the replaced residual goes where the next cg_m reads r_{n}
-        float *__restrict__ r32 = WRITE32 ? pd.r32 : nullptr;
+        const gptr<float> __restrict__ r32 = WRITE32 ? glob(pd.r32) : nullptr;
         double acc = 0.0, acc32 = 0.0, acc_en = 0.0;
         // a tile of vx cells per lane is walked as vx one-cell-per-lane
         // sub-tiles, so that every load is coalesced
         const int lane = threadIdx.x & (kTileX - 1);
-        const int i0 = t.i - lane * g.vx;
+        const int i0 = t.i0;
         with_coefs(vecs(pd, 0.0), [&](const auto &co) {
         for (int sub = 0; sub < g.vx; ++sub) {
             const int ci = i0 + sub * kTileX + lane;
@@ -1042,7 +1121,7 @@ __global__ __launch_bounds__(kThreads) void k_true_residual(const ProbDev *__res
                     r32[c] = rf;
                     acc32 += (double)rf * (double)rf;
                 }
-                if (GAUGE) acc_en += ec * sqrt(pd.d[c]);
+                if (GAUGE) acc_en += ec * sqrt(d[c]);
                 eb = ec;
                 ec = eu;
                 szb = szt;
@@ -1167,9 +1246,9 @@ __global__ void k_set_step(int *base, int value)
 
 // Plain FP32 SpMV of one tile with V cells per lane in row waves (timing
 // only): x neighbours from the adjacent lanes, as in cg_m_tilev.
-template <int V, class CO>
-__device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const float *__restrict__ p,
-                                             float *__restrict__ q, const CO &co)
+template <int V, bool EDGE, class CO>
+__device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, gptr<const float> __restrict__ p,
+                                             gptr<float> __restrict__ q, const CO &co)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     double acc = 0.0;
@@ -1188,20 +1267,20 @@ __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, const 
     else
 #pragma unroll
         for (int m = 0; m < V; ++m) pb[m] = pc[m];
-    zv<V>(co, c, t.k0, szb);
+    const unsigned long long rows = co.rows(t.j, t.k0, t.k1);
+    zb_v<V>(co, row_bit(rows, t.k0, t.k0), c, t.k0, szb);
     for (int k = t.k0; k < t.k1; ++k) {
         const long long r0 = c - w.ci;
         const long long up = (k < nz - 1) ? plane : 0;
         float pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1], y[V];
         ldv<V>(p, c + up, pt);
-        zv<V>(co, c + plane, k + 1, szt);
         ldv<V>(p, c + dS, ps);
         ldv<V>(p, c + dN, pnn);
-        xv<V>(co, fx, k, sxv);
-        yv<V>(co, fy, k, sys);
-        yv<V>(co, fy + nx, k, syn);
-        const float ew = w.load_w ? p[r0 + w.xw] : (w.wrap ? lane_value(pc[V - 1], w.last) : pc[0]);
-        const float ee = w.load_e ? p[r0 + w.xe] : (w.wrap ? lane_value(pc[0], 0) : pc[V - 1]);
+        coefs_v<V>(co, row_bit(rows, k, t.k0), fx, fy, c + plane, nx, k, sxv, sys, syn, szt);
+        const float ew_row = w.wrap ? lane_value(pc[V - 1], w.last) : pc[0];
+        const float ee_row = w.wrap ? lane_value(pc[0], 0) : pc[V - 1];
+        const float ew = (EDGE && w.load_w) ? p[r0 + w.xw] : ew_row;
+        const float ee = (EDGE && w.load_e) ? p[r0 + w.xe] : ee_row;
         const float pw = from_lower(ew, pc[V - 1]);
         const float pe_dpp = from_upper(ee, pc[0]);
         const float pe = w.lane == w.last ? ee : pe_dpp;
@@ -1243,19 +1322,19 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const ProbDev *__restrict__ P
         const ProbDev &pd = P[tr.prob];
         const Geo &g = pd.g;
         const Vecs<T> V = vecs(pd, T());
-        const T *__restrict__ p = V.p0;
-        T *__restrict__ q = V.q;
+        const gptr<const T> __restrict__ p = V.p0;
+        const gptr<T> __restrict__ q = V.q;
         const Tile t = tile3(g, tr.tile);
         const int nx = g.nx, ny = g.ny, nz = g.nz;
         double acc = 0.0;
         with_coefs(V, [&](const auto &co) {
         if constexpr (sizeof(T) == 4) {
             if (g.vx == 4) {
-                acc = spmv_tilev<4>(g, t, p, q, co);
+                acc = g.nx > kTileX * 4 ? spmv_tilev<4, true>(g, t, p, q, co) : spmv_tilev<4, false>(g, t, p, q, co);
                 return 0;
             }
             if (g.vx == 2) {
-                acc = spmv_tilev<2>(g, t, p, q, co);
+                acc = g.nx > kTileX * 2 ? spmv_tilev<2, true>(g, t, p, q, co) : spmv_tilev<2, false>(g, t, p, q, co);
                 return 0;
             }
         }

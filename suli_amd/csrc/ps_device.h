@@ -6,6 +6,33 @@
 
 namespace sps {
 
+// Pointers in global memory.  The arrays the kernels work on are reached
+// through device-side problem descriptors, so to the compiler they are
+// generic pointers, which it accesses with flat instructions: those also
+// count on the LDS/scalar-load counter (every wait for them waits for
+// pending scalar loads too) and take no scalar base register.  Every such
+// array is device memory; accessing it through these pointers gives global
+// instructions.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> glob(T *p)
+{
+    return (gptr<T>)p;
+}
+
+// Read-only device memory: arrays that no kernel writes while the reading
+// kernel runs (tables built by set-up kernels).  A wave-uniform load through
+// such a pointer is a scalar load, which waits on its own counter instead of
+// behind the wave's pending vector loads.
+template <class T>
+using cptr = __attribute__((address_space(4))) const T *;
+template <class T>
+__device__ __forceinline__ cptr<T> konst(const T *p)
+{
+    return (cptr<T>)p;
+}
+
 // One double moved between lanes of each row of 16 (DPP row rotate right
 // by N: lane l of a row gets lane (l - N) mod 16 of the same row), both
 // 32-bit halves.
@@ -64,9 +91,28 @@ __device__ __forceinline__ double block_sum(double v)
 // the same bits: a problem's result does not depend on what else runs.
 __device__ __forceinline__ double total_of(const double *partial, int n)
 {
+    const gptr<const double> p = glob(partial);
     double s = 0.0;
-    for (int t = threadIdx.x & 63; t < n; t += 64) s += partial[t];
+    for (int t = threadIdx.x & 63; t < n; t += 64) s += p[t];
     return wave_sum(s);
+}
+
+// The same total summed once per workgroup (every thread reads some of the
+// partials) and handed to its waves through LDS: for kernels whose waves
+// would otherwise each read many partials.  Every workgroup gets the same bits; a kernel
+// uses one of the two forms for a given set of partials.
+__device__ __forceinline__ double block_total_of(const double *partial, int n)
+{
+    __shared__ double tot;
+    const gptr<const double> p = glob(partial);
+    double s = 0.0;
+    for (int t = threadIdx.x; t < n; t += kThreads) s += p[t];
+    s = block_sum(s);
+    if (threadIdx.x == 0) tot = s;
+    __syncthreads();
+    const double r = tot;
+    __syncthreads();
+    return r;
 }
 
 // Four workgroup sums at once (fixed order), valid in thread 0 only.
@@ -94,10 +140,11 @@ __device__ __forceinline__ void block_sum4(double v[4])
 // total_of, each wave sums them all in the same order.
 __device__ __forceinline__ void total4_of(const double *m, int n, double out[4])
 {
+    const gptr<const double> p = glob(m);
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int t = threadIdx.x & 63; t < n; t += 64)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) v[c] += m[4 * t + c];
+        for (int c = 0; c < 4; ++c) v[c] += p[4 * t + c];
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c] = wave_sum(v[c]);
 }

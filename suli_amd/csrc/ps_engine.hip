@@ -75,6 +75,10 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
         pd.tx32 = cv.take<float>((size_t)g.nz + 1);
         pd.ty32 = cv.take<float>((size_t)g.nz + 1);
         pd.tz32 = cv.take<float>((size_t)g.nz + 1);
+        pd.rx32 = cv.take<float>(8 * (size_t)g.nz);
+        pd.ry32 = cv.take<float>(4 * (size_t)g.nz);
+        pd.rz32 = cv.take<float>(4 * ((size_t)g.nz + 1));
+        pd.rowreg = cv.take<unsigned>((size_t)(g.ny + 31) / 32 * g.nz);
     }
     pd.P.spmv = cv.take<double>(nt3);
     pd.P.upd = cv.take<double>(nt1);
@@ -329,6 +333,9 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     if (gr.t_press != 2 || in.solid) {
         pd.tx = pd.ty = pd.tz = nullptr;
         pd.tx32 = pd.ty32 = pd.tz32 = nullptr;
+    } else {
+        pd.rx32 = pd.ry32 = pd.rz32 = nullptr;
+        pd.rowreg = nullptr;
     }
     pd.nt3 = nt3;
     pd.nt1 = nt1;
@@ -399,6 +406,9 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
         launch_face_tables(sl.g, pd.sx, pd.sy, pd.sz, pd.sx32, pd.sy32, pd.sz32, const_cast<double *>(pd.tx),
                            const_cast<double *>(pd.ty), const_cast<double *>(pd.tz), const_cast<float *>(pd.tx32),
                            const_cast<float *>(pd.ty32), const_cast<float *>(pd.tz32), s_);
+    else if (pd.rowreg)
+        launch_row_tables(sl.g, pd.sx32, pd.sy32, pd.sz32, const_cast<float *>(pd.rx32), const_cast<float *>(pd.ry32),
+                          const_cast<float *>(pd.rz32), const_cast<unsigned *>(pd.rowreg), s_);
     launch_init(sl.g, in, Op<double>{pd.sx, pd.sy, pd.sz}, pd.d, pd.b, pd.e, pd.r, o.mixed ? nullptr : pd.r,
                 pd.r32, pd.c32, pd.P, nt1, pd.S, s_);
     ECHK(hipGetLastError());
@@ -924,8 +934,12 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         Carver cv{static_cast<char *>(blk)};
         const int nt3 = spmv_tiles(g), nt1 = vec_tiles(g.n);
         carve(cv, g, mixed, nt3, nt1, pd);
-        pd.tx = pd.ty = pd.tz = nullptr;   // tuned on the face-array operator
+        // tuned on the face-array operator whose rows all hold their
+        // plane's values (a layered or constant-density field: the FP32
+        // step reads the row tables instead of the arrays)
+        pd.tx = pd.ty = pd.tz = nullptr;
         pd.tx32 = pd.ty32 = pd.tz32 = nullptr;
+        const size_t row_bytes = mixed ? sizeof(unsigned) * (size_t)(ny + 31) / 32 * nz : 0;
         pd.g = g;
         pd.nt3 = nt3;
         pd.nt1 = nt1;
@@ -934,6 +948,7 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         pd.S = dS;
         for (int t = 0; t < nt3; ++t) tab[t] = {0, t};
         TCHK(hipMemsetAsync(blk, 0, cv.off, s_));
+        if (row_bytes) TCHK(hipMemsetAsync(const_cast<unsigned *>(pd.rowreg), 0xff, row_bytes, s_));
         TCHK(hipMemcpyAsync(dP, &pd, sizeof pd, hipMemcpyHostToDevice, s_));
         TCHK(hipMemcpyAsync(dS, &s0, sizeof s0, hipMemcpyHostToDevice, s_));
         TCHK(hipMemcpyAsync(dT, tab.data(), sizeof(TileRef) * nt3, hipMemcpyHostToDevice, s_));
@@ -978,26 +993,63 @@ int Engine::time_spmv(bool fp32, int reps, double *ms)
     if (!sl.used) return fail(SULI_PS_EINVAL, "no operator: call graddin first");
     if (fp32 != (bool)sl.o.mixed)
         return fail(SULI_PS_EINVAL, "the last solve's operator is in the other precision");
-    std::vector<TileRef> t3;
-    for (int t = 0; t < sl.pd.nt3; ++t) t3.push_back({0, t});
+    // the SpMV's own tile shapes: depths of 2 .. 24 planes, and 1, 2 or 4
+    // cells per lane in FP32 (where nx is a multiple); the fastest counts
+    std::vector<std::pair<int, int>> shapes;
+    for (int vx : {4, 2, 1}) {
+        if (vx > 1 && (!fp32 || sl.g.nx % vx)) continue;
+        for (int kz : {2, 3, 4, 6, 8, 12, 16, 24})
+            if (kz <= sl.g.nz) shapes.emplace_back(kz, vx);
+    }
+    // every shape's tiles fit the partial-sum array (sized for the most tiles)
+    const int max_nt3 = spmv_tiles(make_geo(sl.g.nx, sl.g.ny, sl.g.nz, 0, 0, kMinKz));
+    ProbDev *dP = nullptr;
     TileRef *dt = nullptr;
-    ECHK(hipMalloc(&dt, sizeof(TileRef) * t3.size()));
-    ECHK(hipMemcpy(dt, t3.data(), sizeof(TileRef) * t3.size(), hipMemcpyHostToDevice));
-    const int n3 = (int)t3.size();
-    hipEvent_t a, b;
-    ECHK(hipEventCreate(&a));
-    ECHK(hipEventCreate(&b));
-    launch_spmv_only(fp32, d_probs_, dt, n3, std::min(n3, kMaxGrid), s_);
-    ECHK(hipEventRecord(a, s_));
-    for (int i = 0; i < reps; ++i) launch_spmv_only(fp32, d_probs_, dt, n3, std::min(n3, kMaxGrid), s_);
-    ECHK(hipEventRecord(b, s_));
-    ECHK(hipEventSynchronize(b));
-    float t = 0.f;
-    ECHK(hipEventElapsedTime(&t, a, b));
-    *ms = (double)t / reps;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    ECHK(hipFree(dt));
+    hipEvent_t a = nullptr, b = nullptr;
+    auto cleanup = [&] {
+        (void)hipStreamSynchronize(s_);
+        if (dP) (void)hipFree(dP);
+        if (dt) (void)hipFree(dt);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    };
+#define SCHK(call)                                                                                      \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            cleanup();                                                                                  \
+            return fail(SULI_PS_EHIP, std::string("time_spmv: ") + #call + ": " + hipGetErrorString(e_)); \
+        }                                                                                               \
+    } while (0)
+    SCHK(hipMalloc(&dP, sizeof(ProbDev)));
+    SCHK(hipMalloc(&dt, sizeof(TileRef) * max_nt3));
+    SCHK(hipEventCreate(&a));
+    SCHK(hipEventCreate(&b));
+    std::vector<TileRef> t3(max_nt3);
+    for (int t = 0; t < max_nt3; ++t) t3[t] = {0, t};
+    SCHK(hipMemcpy(dt, t3.data(), sizeof(TileRef) * max_nt3, hipMemcpyHostToDevice));
+    double best = 1e300;
+    for (const auto &sh : shapes) {
+        ProbDev pd = sl.pd;
+        pd.g = make_geo(sl.g.nx, sl.g.ny, sl.g.nz, sl.g.px, sl.g.py, sh.first, sh.second);
+        pd.g.dtop = sl.g.dtop;
+        pd.nt3 = spmv_tiles(pd.g);
+        if (pd.nt3 > max_nt3) continue;
+        SCHK(hipMemcpy(dP, &pd, sizeof pd, hipMemcpyHostToDevice));
+        const int n3 = pd.nt3, grid = std::min(n3, kMaxGrid);
+        launch_spmv_only(fp32, dP, dt, n3, grid, s_);
+        SCHK(hipEventRecord(a, s_));
+        for (int i = 0; i < reps; ++i) launch_spmv_only(fp32, dP, dt, n3, grid, s_);
+        SCHK(hipEventRecord(b, s_));
+        SCHK(hipEventSynchronize(b));
+        SCHK(hipGetLastError());
+        float t = 0.f;
+        SCHK(hipEventElapsedTime(&t, a, b));
+        best = std::min(best, (double)t / reps);
+    }
+#undef SCHK
+    cleanup();
+    *ms = best;
     return 0;
 }
 

@@ -137,6 +137,16 @@ struct ProbDev {
     // of the face arrays; nullptr otherwise
     const double *tx, *ty, *tz;
     const float *tx32, *ty32, *tz32;
+    // any other mixed problem: per-plane values of the FP32 face arrays and
+    // one bit per row (j, k) whose faces all hold them (word k * ((ny + 31)
+    // / 32) + j / 32, bit j % 32); such rows read their coefficients from
+    // the tables, which repeat each value (8 times per plane in rx32, 4 in
+    // ry32 and per z face in rz32) so that the same vector loads serve
+    // both.  The FP32 kernels address the tables whenever they read the face
+    // arrays, so these point to valid memory then (rowreg may be null: no
+    // row is read from the tables)
+    const float *rx32, *ry32, *rz32;
+    const unsigned *rowreg;
     Partials P;
     CgScalars *S;
 };
@@ -193,6 +203,10 @@ void launch_scale_faces(const Geo &g, const double *d, double *sx, double *sy, d
                         float *sy32, float *sz32, hipStream_t s);
 // Per-plane coefficient tables of a constant-density operator, copied from
 // its (plane-constant) face arrays; the FP32 ones when sx32 is non-null.
+// The FP32 row tables and the bit per row (j, k) whose faces all hold them
+// (rowreg: (ny + 31) / 32 words per plane).
+void launch_row_tables(const Geo &g, const float *sx32, const float *sy32, const float *sz32, float *rx, float *ry,
+                       float *rz, unsigned *rowreg, hipStream_t s);
 void launch_face_tables(const Geo &g, const double *sx, const double *sy, const double *sz, const float *sx32,
                         const float *sy32, const float *sz32, double *tx, double *ty, double *tz, float *tx32,
                         float *ty32, float *tz32, hipStream_t s);

@@ -670,6 +670,64 @@ __global__ __launch_bounds__(kThreads) void k_face_tables(Geo g, const double *s
     }
 }
 
+// Row tables of the FP32 operator (mixed solver): one value per plane for
+// the x and y faces of plane k (rx[8k .. 8k+7], ry[4k .. 4k+3], the value
+// repeated) and per z face kf (rz[4kf .. 4kf+3]),
+// taken where a centred obstacle or a surface band is least likely (the
+// cell (nx/4, ny/4)), and one bit per row (j, k): set when every x face of
+// the row, its y faces j and j + 1 and its z faces k and k + 1 hold exactly
+// (bit for bit) those values.  Where the density is uniform across a plane
+// (one phase, walls included) the scaled coefficients are the same sums in
+// the same order everywhere, so most rows of a two-phase problem have the
+// bit set.  The step kernels take such a row's coefficients from the tables
+// and read the face arrays only for the others: the same values, the same
+// results.  Bits: word k * row_words(ny) + j / 32, bit j % 32.  One
+// workgroup per word (each wave checks every fourth of its 32 rows); the
+// tables are written by the workgroup of word 0 of each plane.
+__device__ __forceinline__ int row_words(int ny) { return (ny + 31) / 32; }
+
+__global__ __launch_bounds__(kThreads) void k_row_tables(Geo g, const float *sx32, const float *sy32,
+                                                         const float *sz32, float *rx, float *ry, float *rz,
+                                                         unsigned *rowreg)
+{
+    __shared__ unsigned bits[kThreads / 64];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int wpp = row_words(ny);
+    const int k = blockIdx.x / wpp, word = blockIdx.x % wpp;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i0 = nx / 4, j0 = ny / 4;
+    const long long plane = (long long)nx * ny;
+    const long long fxk = (long long)k * ny * (nx + 1), fyk = (long long)k * (ny + 1) * nx;
+    const float vx = sx32[fxk + (long long)j0 * (nx + 1) + i0 + 1];
+    const float vy = sy32[fyk + (long long)(j0 + 1) * nx + i0];
+    const float vz0 = sz32[(long long)k * plane + (long long)j0 * nx + i0];
+    const float vz1 = sz32[(long long)(k + 1) * plane + (long long)j0 * nx + i0];
+    unsigned mine = 0;
+    for (int r = wave; r < 32; r += kThreads / 64) {
+        const int j = word * 32 + r;
+        if (j >= ny) break;   // whole waves
+        bool ok = true;
+        for (int i = lane; i <= nx; i += 64) ok = ok && sx32[fxk + (long long)j * (nx + 1) + i] == vx;
+        for (int i = lane; i < nx; i += 64) {
+            ok = ok && sy32[fyk + (long long)j * nx + i] == vy && sy32[fyk + (long long)(j + 1) * nx + i] == vy;
+            ok = ok && sz32[(long long)k * plane + (long long)j * nx + i] == vz0 &&
+                 sz32[(long long)(k + 1) * plane + (long long)j * nx + i] == vz1;
+        }
+        if (__all(ok)) mine |= 1u << r;
+    }
+    if (lane == 0) bits[wave] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) rowreg[blockIdx.x] = bits[0] | bits[1] | bits[2] | bits[3];
+    if (word == 0 && threadIdx.x < 8) {
+        rx[8 * k + threadIdx.x] = vx;
+        if (threadIdx.x < 4) {
+            ry[4 * k + threadIdx.x] = vy;
+            rz[4 * k + threadIdx.x] = vz0;
+            if (k == nz - 1) rz[4 * nz + threadIdx.x] = vz1;
+        }
+    }
+}
+
 // SuLi-IT posdin (SuLi-IT/10_pos.f90:73-132) without dz_it, which updates
 // the caller's layer thicknesses: eta1 += prd1(:,:,nz)/gz over its whole
 // (0:nx+1,0:ny+1) box (skipped when eta1 is null); u, v -= tetah dt/dx
@@ -756,6 +814,13 @@ void launch_face_tables(const Geo &g, const double *sx, const double *sy, const 
 {
     hipLaunchKernelGGL(k_face_tables, dim3(1), dim3(kThreads), 0, s, g, sx, sy, sz, sx32, sy32, sz32, tx, ty, tz,
                        tx32, ty32, tz32);
+}
+
+void launch_row_tables(const Geo &g, const float *sx32, const float *sy32, const float *sz32, float *rx, float *ry,
+                       float *rz, unsigned *rowreg, hipStream_t s)
+{
+    const int words = (g.ny + 31) / 32 * g.nz;
+    hipLaunchKernelGGL(k_row_tables, dim3(words), dim3(kThreads), 0, s, g, sx32, sy32, sz32, rx, ry, rz, rowreg);
 }
 
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,
