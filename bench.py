@@ -9,6 +9,7 @@ FP64 dots, scalars and true residual) is the headline; the FP64 solver runs
 the same steps for the comparison.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-solves C] [--no-cpu]
+                    [--no-extra] [--no-graphs] [--only mixed|fp64] [--tiles PRECISION=KZxVX]
 
 Prints one JSON line.
 
@@ -19,10 +20,10 @@ coefficients read, Ap written; iterate, residual and direction 3 each), i.e.
 56 B/cell, divided by the kernel's mean launch time, which HIP events on the
 solver's stream measure around windows of 8 consecutive launches (one window
 per 16 steps) during two directly launched solves; a window's time includes
-the gaps between its launches.  Beside it: the 9 FP32 words per cell the
-fused kernel actually moves (`kernel_bytes_per_launch`), and `traffic`, the
-HBM bytes per launch from rocprofv3's FETCH_SIZE and WRITE_SIZE counters
-(profiles/hbm_traffic.json), calibrated on the copy kernel k_copy.
+the gaps between its launches.  The fused kernel moves fewer bytes than
+that: `traffic` is its HBM bytes per launch from rocprofv3's FETCH_SIZE and
+WRITE_SIZE counters (profiles/hbm_traffic.json, calibrated on the copy
+kernel k_copy), and `traffic_achieved` those bytes at the same launch rate.
 
 The CPU leg (`cpu_baseline`, skipped with --no-cpu) times the reference's own
 Fortran graddin (oracle/_ref, compiled from the SuLi sources) with OpenMP on
@@ -47,6 +48,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# captured graphs replayed where the engine uses them (off: --no-graphs)
+GRAPHS = True
+
 METRIC = "pressure solves/s + p99 latency at 1e-8 residual, mixed vs FP64 CG"
 CG_WORDS = 14          # algorithmic FP32/FP64 words per cell of one CG iteration
 SPMV_WORDS = 5         # algorithmic words per cell of the SpMV
@@ -120,7 +124,7 @@ def bench_stretched_cylinder(steps, keep):
     out, prd = {}, {}
     for precision in ("mixed", "fp64"):
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
-                         autotune=True)
+                         autotune=True, graphs=GRAPHS)
         df.zero("prd1")
         solver.graddin_dev(g, df)
         solver.synchronize()
@@ -162,7 +166,7 @@ def bench_constant_density(steps, keep):
     n = g.ncell
     for precision in ("mixed", "fp64"):
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
-                         autotune=True)
+                         autotune=True, graphs=GRAPHS)
         df.zero("prd1")
         solver.graddin_dev(g, df)
         solver.synchronize()
@@ -177,7 +181,7 @@ def bench_constant_density(steps, keep):
         prd[precision] = df.download("prd1")
         # one more solve launched directly, with the CG step timed
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
-                         autotune=True, timing=True)
+                         autotune=True, timing=True, graphs=GRAPHS)
         df.zero("prd1")
         st = solver.graddin_dev(g, df)
         w = 4 if precision == "mixed" else 8
@@ -210,7 +214,7 @@ def bench_batch(keep, seed=1, count=64):
     results = [dict() for _ in cs]
     for precision in ("mixed", "fp64"):
         solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False,
-                         autotune=True)
+                         autotune=True, graphs=GRAPHS)
         for df in dfs:
             df.zero("prd1")
         solver.graddin_batch_dev(items)      # warm-up: tunes each size
@@ -473,17 +477,24 @@ def main():
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=96)
-    ap.add_argument("--only", choices=("both", "mixed", "fp64"), default="both",
-                    help="solver(s) to time (profiling runs)")
+    ap.add_argument("--only", choices=("both", "mixed", "fp64", "none"), default="both",
+                    help="solver(s) to time on the headline workload (profiling runs)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="size of the plain copy kernel that calibrates bandwidth and counters")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the cylinder, IPA, batch, serving and drop-in workloads")
+    ap.add_argument("--extra", default=",".join(EXTRAS),
+                    help="comma-separated subset of the extra workloads to run (profiling runs)")
+    ap.add_argument("--tiles", action="append", default=[], metavar="PRECISION=KZxVX",
+                    help="fix the headline solves' tile shape instead of tuning it (profiling runs, "
+                         "whose serialised launches would tune differently), e.g. mixed=6x4")
     ap.add_argument("--cpu-timing-to", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-lam", type=float, default=1.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     # logical CPUs this process may use, read before any OpenMP runtime starts
     avail = len(os.sched_getaffinity(0))
     if args.cpu_timing_to:
@@ -512,6 +523,11 @@ def main():
     copy = {"kernel": "k_copy", "bytes_read": copy_bytes, "bytes_written": copy_bytes,
             "ms": copy_ms, "gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
     log(f"copy {copy['gbs']:.0f} GB/s")
+    for spec in args.tiles:
+        prec, _, shape = spec.partition("=")
+        kz, _, vx = shape.partition("x")
+        solver.set_tiles(g, prec, int(kz), int(vx))
+        log(f"tiles fixed: {prec} kz {kz} vx {vx}")
     mixed = fp64 = None
     headline = {}
     if args.only in ("both", "mixed"):
@@ -524,25 +540,32 @@ def main():
         log(f"fp64 {fp64['solves_per_s']:.2f} solves/s, {fp64['iters']} iterations")
     df.free()
     solver.close()
+    wanted = [] if args.no_extra else [e for e in args.extra.split(",") if e]
+    unknown = sorted(set(wanted) - set(EXTRAS))
+    if unknown:
+        sys.exit(f"bench.py: unknown --extra workload(s) {unknown}; known: {list(EXTRAS)}")
+    extra, keep = run_extras(wanted)
     if mixed is None or fp64 is None:
-        print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy}))
+        # a profiling run: no headline ratio, roofline or CPU leg
+        print(json.dumps({"profile_run": args.only, "mixed": mixed, "fp64": fp64, "copy": copy,
+                          "workloads": extra}))
         return
 
     launch_ms = mixed["step_ms"]
     alg = CG_WORDS * 4 * n                 # one CG iteration in FP32: 56 B/cell
-    kern = 9 * 4 * n                       # what k_cg_m moves: 9 FP32 words/cell on average
     achieved = alg / (launch_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"k_cg_m {g.nx}x{g.ny}x{g.nz}")
     spmv_gbs = SPMV_WORDS * 4 * n / (mixed["spmv_only_ms"] * 1e-3) / 1e9
     fp64_step = fp64["step_ms"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": load_traffic(f"k_cg_m {g.nx}x{g.ny}x{g.nz}"),
+                "traffic": traffic,
                 "kernel": "k_cg_m", "launch_ms": launch_ms,
                 "algorithmic_bytes_per_launch": alg,
                 "algorithmic_basis": "one FP32 CG iteration, 14 words/cell (SpMV 5 + x, r, p 3 each)",
-                "kernel_bytes_per_launch": kern,
-                "kernel_achieved": kern / (launch_ms * 1e-3) / 1e9,
-                "kernel_frac": kern / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                # the bytes the fused step actually moves (profiled HBM traffic) at its launch rate
+                "traffic_achieved": traffic / (launch_ms * 1e-3) / 1e9 if traffic else None,
+                "traffic_frac": traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                 "launch_ms_basis": "HIP events around windows of 8 consecutive launches (gaps included)",
                 "spmv": {"kernel": "k_spmv<float>", "words_per_cell": SPMV_WORDS,
                          "launch_ms": mixed["spmv_only_ms"], "achieved": spmv_gbs,
@@ -552,20 +575,6 @@ def main():
                 "fp64_step": {"kernels": "k_cg_a<double> + k_cg_b<double>", "step_ms": fp64_step,
                               "achieved": CG_WORDS * 8 * n / (fp64_step * 1e-3) / 1e9 if fp64_step else None,
                               "kernel_achieved": 12 * 8 * n / (fp64_step * 1e-3) / 1e9 if fp64_step else None}}
-
-    extra, keep = None, {}
-    if not args.no_extra:
-        extra = {}
-        log("stretched cylinder")
-        extra["stretched_cylinder"] = bench_stretched_cylinder(2, keep)
-        log("constant density")
-        extra["constant_density"] = bench_constant_density(3, keep)
-        log("batch")
-        extra["batch_uneven"] = bench_batch(keep)
-        log("serving")
-        extra["serving"] = bench_serving(keep)
-        log("drop-in call")
-        extra["dropin_call"] = bench_dropin()
 
     cpu = checks = None
     if not args.no_cpu:
@@ -604,6 +613,29 @@ def main():
     }
     print(json.dumps(out), flush=True)
     reap_children()
+
+
+# the extra workloads, in the order they run: (log line, result key, runner)
+EXTRAS = {
+    "cylinder": ("stretched cylinder", "stretched_cylinder", lambda keep: bench_stretched_cylinder(2, keep)),
+    "ipa": ("constant density", "constant_density", lambda keep: bench_constant_density(3, keep)),
+    "batch": ("batch", "batch_uneven", lambda keep: bench_batch(keep)),
+    "serving": ("serving", "serving", lambda keep: bench_serving(keep)),
+    "dropin": ("drop-in call", "dropin_call", lambda keep: bench_dropin()),
+}
+
+
+def run_extras(wanted):
+    """The extra workloads named in `wanted`; returns their results (None
+    when none ran) and the inputs/outputs kept for the oracle checks."""
+    if not wanted:
+        return None, {}
+    extra, keep = {}, {}
+    for name, (title, key, run) in EXTRAS.items():
+        if name in wanted:
+            log(title)
+            extra[key] = run(keep)
+    return extra, keep
 
 
 def reap_children():
