@@ -229,8 +229,8 @@ int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u,
                              const double *prd1, const double *dz, double *prd);
 
 /* Density from the level set on the device: heaviside with t_tens = 0
- * (SuLi-IC/14_level_set.f90:508-594; the t_tens = 1 surface-tension
- * gradients hsx/hsy/hsz are not computed).  For each of the nx*ny*nz cells
+ * (SuLi-IC/14_level_set.f90:508-594; suli_ps_heaviside_tens_dev adds the
+ * t_tens = 1 surface-tension gradients).  For each of the nx*ny*nz cells
  * (rho's layout): hs = 0 below -eps, 1 above eps, else
  * 0.5*(1 + ls/eps + sin(pi*ls/eps)/pi); rho = rho_f1*(1-hs) + rho_f2*hs
  * with rho_f1 from the grid; mu = mu_f1*(1-hs) + mu_f2*hs (the reference's
@@ -240,6 +240,41 @@ int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u,
 int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *ls,
                           double eps, double rho_f2, double mu_f1, double mu_f2,
                           double *hs, double *rho, double *mu);
+
+/* heaviside with t_tens = 1 (SuLi-IC/14_level_set.f90:508-594): as
+ * suli_ps_heaviside_dev, and the surface-tension gradients hsx, hsy, hsz
+ * (nx,ny,nz): inside the band |ls| <= eps, 0.5*g*(1 + cos(pi*ls/eps))/eps
+ * with g the level set's WENO5 derivative along that axis, of the two
+ * one-sided ones the one of larger magnitude (der_weno, :338-370, :519-544;
+ * SuLi-IC/8_derivadas.f90:675-1095); 0 outside.  The line ends are
+ * periodic in x (y) when ccx0 (ccy0) is 0, else extrapolated with zero
+ * derivative; z always the latter.  Grid: nx..nz (each >= 4), dx, dy, dz
+ * (uniform), ccx0, ccy0, rho_f1.  hs and mu may be NULL.  Device pointers,
+ * context stream; returns once written. */
+int suli_ps_heaviside_tens_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *ls,
+                               double eps, double rho_f2, double mu_f1, double mu_f2,
+                               double *hs, double *rho, double *mu, double *hsx, double *hsy,
+                               double *hsz);
+
+/* exp_press, the IPA scheme's pressure increment at the start of a time step
+ * (SuLi-IC/11_diversos.f90:455-495, called at SuLi-IC/1_principal.f90:82
+ * when t_press = 2; SuLi-ICdz/11_diversos.f90:455-495 when dz is given):
+ * prd0 = prd1 + prd0 over the whole halo box, then with
+ * prdaux = prd1 + prd0 and the face densities of graddin (interp*_cf,
+ * dz-weighted in z when dz is given)
+ *   px = -(prdaux(i) - prdaux(i-1))/dx/rho_x   on the (nx+1,ny,nz) x faces,
+ *   py = -(prdaux(j) - prdaux(j-1))/dy/rho_y   on the (nx,ny+1,nz) y faces,
+ *   pz = -(prdaux(k) - prdaux(k-1))/dzz(k)/rho_z  (dz of the grid when dzz
+ *        is NULL) on the (nx,ny,nz+1) z faces.
+ * rho (nx,ny,nz), prd1 and prd0 the halo box; dz, dzz (0:nz+1) together or
+ * both NULL.  Grid: nx..nz, dx, dy, dz, ccx0.  Host pointers; the _dev form
+ * takes device pointers and runs on the context's stream. */
+int suli_ps_exp_press(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho,
+                      const double *prd1, double *prd0, const double *dz, const double *dzz,
+                      double *px, double *py, double *pz);
+int suli_ps_exp_press_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho,
+                          const double *prd1, double *prd0, const double *dz, const double *dzz,
+                          double *px, double *py, double *pz);
 
 /* SuLi-IT's non-hydrostatic pressure (Casulli's semi-implicit scheme):
  * graddin, SuLi-IT/9_grad.f90:10-263, the same Jacobi-scaled CG with
@@ -280,6 +315,24 @@ int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, dou
 int suli_ps_it_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,
                           double *u, double *v, double *w, const double *dzz, double *eta1,
                           const double *prd1, double *prd);
+
+/* SuLi-IT's posdin whole, as the reference runs it (SuLi-IT/10_pos.f90:73-132
+ * with dz_it, SuLi-IT/11_diversos.f90:80-141): over the (0:nx+1,0:ny+1) box
+ * eta1 += prd1(:,:,nz)/gz and dz(:,:,nz) += eta1 - eta0; then the face
+ * thicknesses of the top planes: dzx and dzy on planes nz and nz+1 (face
+ * means; end faces periodic when ccx0 / ccy0 is 0, else extrapolated from
+ * the two inside), dzz on planes nz, nz+1 (means) and nz+2 (extrapolated);
+ * then the correction of suli_ps_it_posdin with the new dzz; prd = prd1
+ * (prd may be NULL).  dz in prd1's layout (0:nx+1,0:ny+1,0:nz+1), eta0 as
+ * eta1.  Host pointers; the _dev form takes device pointers. */
+int suli_ps_it_posdin_layers(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,
+                             double *u, double *v, double *w, double *dz, double *dzx, double *dzy,
+                             double *dzz, const double *eta0, double *eta1, const double *prd1,
+                             double *prd);
+int suli_ps_it_posdin_layers_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz,
+                                 double *u, double *v, double *w, double *dz, double *dzx,
+                                 double *dzy, double *dzz, const double *eta0, double *eta1,
+                                 const double *prd1, double *prd);
 
 /* SuLi restart file (dados/arquivorestart): restart_salva writes the model
  * state as one list-directed record (SuLi-IC/11_diversos.f90:581-599) that

@@ -83,6 +83,14 @@ def lib():
         _lib.oracle_pressh.argtypes = [C.POINTER(OracleGrid), _dp, C.c_double, _dp]
         _lib.oracle_posdin_hydro.restype = None
         _lib.oracle_posdin_hydro.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        _lib.oracle_exp_press.restype = None
+        _lib.oracle_exp_press.argtypes = [C.POINTER(OracleGrid)] + [_dp] * 6
+        _lib.oracle_ls_gradient.restype = None
+        _lib.oracle_ls_gradient.argtypes = [C.POINTER(OracleGrid)] + [_dp] * 4
+        _lib.oracle_heaviside_tens.restype = None
+        _lib.oracle_heaviside_tens.argtypes = [C.POINTER(OracleGrid), _dp] + [C.c_double] * 5 + [_dp] * 6
+        _lib.oracle_it_posdin_dz.restype = None
+        _lib.oracle_it_posdin_dz.argtypes = [C.POINTER(OracleGrid), C.c_double, C.c_double] + [_dp] * 11
     return _lib
 
 
@@ -210,6 +218,61 @@ def it_posdin(g, f, prd1):
     return u, v, w, eta1, prd
 
 
+def it_posdin_dz(g, f, prd1, dz, eta0):
+    """C restatement of SuLi-IT's posdin with its layer update (eta1 and
+    dz(:,:,nz) from prd1, then dz_it). Returns (u, v, w, dz, dzx, dzy, dzz,
+    eta1, prd)."""
+    og = _it_grid(g, STOP_REFERENCE, 0.0, 0)
+    u, v, w, eta1 = (f.u.copy(order="F"), f.v.copy(order="F"), f.w.copy(order="F"), f.eta1.copy(order="F"))
+    dz, dzx, dzy, dzz = (np.asfortranarray(dz).copy(order="F"), f.dzx.copy(order="F"), f.dzy.copy(order="F"),
+                         f.dzz.copy(order="F"))
+    prd = np.zeros_like(prd1, order="F")
+    lib().oracle_it_posdin_dz(C.byref(og), g.tetah, g.gz, _ptr(u), _ptr(v), _ptr(w), _ptr(dz), _ptr(dzx),
+                              _ptr(dzy), _ptr(dzz), _ptr(np.asfortranarray(eta0)), _ptr(eta1),
+                              _ptr(np.asfortranarray(prd1)), _ptr(prd))
+    return u, v, w, dz, dzx, dzy, dzz, eta1, prd
+
+
+def _unmasked(g):
+    """The grid without its IBM mask, for the functions that never read it."""
+    import dataclasses
+    return dataclasses.replace(g, obst_t=0, ibm_t=0)
+
+
+def exp_press(case_grid, rho, prd1, prd0):
+    """C restatement of exp_press (IPA). Returns (prd0 after the increment,
+    px, py, pz)."""
+    g = case_grid
+    out0 = np.asfortranarray(prd0).copy(order="F")
+    px = np.zeros((g.nx + 1, g.ny, g.nz), order="F")
+    py = np.zeros((g.nx, g.ny + 1, g.nz), order="F")
+    pz = np.zeros((g.nx, g.ny, g.nz + 1), order="F")
+    og = make_grid(_unmasked(g))
+    lib().oracle_exp_press(C.byref(og), _ptr(np.asfortranarray(rho, dtype=np.float64)),
+                           _ptr(np.asfortranarray(prd1)), _ptr(out0), _ptr(px), _ptr(py), _ptr(pz))
+    return out0, px, py, pz
+
+
+def ls_gradient(case_grid, ls):
+    """The level set's WENO5 derivatives as heaviside's t_tens = 1 branch
+    picks them. Returns (gx, gy, gz), each shaped like ls."""
+    ls = np.asfortranarray(ls, dtype=np.float64)
+    gx, gy, gz = (np.empty_like(ls, order="F") for _ in range(3))
+    og = make_grid(_unmasked(case_grid))
+    lib().oracle_ls_gradient(C.byref(og), _ptr(ls), _ptr(gx), _ptr(gy), _ptr(gz))
+    return gx, gy, gz
+
+
+def heaviside_tens(case_grid, ls, eps, rho1, rho2, mu1, mu2):
+    """C restatement of heaviside with t_tens = 1. Returns (hs, rho, mu,
+    hsx, hsy, hsz)."""
+    ls = np.asfortranarray(ls, dtype=np.float64)
+    out = [np.empty_like(ls, order="F") for _ in range(6)]
+    og = make_grid(_unmasked(case_grid))
+    lib().oracle_heaviside_tens(C.byref(og), _ptr(ls), eps, rho1, rho2, mu1, mu2, *[_ptr(a) for a in out])
+    return tuple(out)
+
+
 def heaviside(ls, eps, rho1, rho2, mu1, mu2):
     """C restatement of heaviside (t_tens = 0). Returns (hs, rho, mu), each
     shaped like ls (Fortran order)."""
@@ -244,6 +307,7 @@ def ref_lib(variant: str = "ic"):
         L = C.CDLL(path)
         if variant == "it":
             L.suli_ref_it_graddin.argtypes = [_dp] * 9
+            L.suli_ref_it_posdin.argtypes = [_dp] * 11
             _ref_libs[variant] = L
             return L
         L.suli_ref_setup.restype = None
@@ -253,8 +317,10 @@ def ref_lib(variant: str = "ic"):
         L.suli_ref_posdin.argtypes = [_dp] * 6
         L.suli_ref_pressh.argtypes = [_dp, C.c_double, _dp]
         L.suli_ref_posdin_hydro.argtypes = [_dp] * 7
+        L.suli_ref_exp_press.argtypes = [_dp] * 6
         if variant == "ic":
             L.suli_ref_heaviside.argtypes = [_dp] + [C.c_double] * 6 + [_dp] * 3
+            L.suli_ref_heaviside_tens.argtypes = [_dp] + [C.c_double] * 6 + [_dp] * 6
         _ref_libs[variant] = L
     return _ref_libs[variant]
 
@@ -398,6 +464,47 @@ def ref_heaviside(case_grid, ls, alpha1, dx1, rho1, rho2, mu1, mu2):
     hs, rho, mu = (np.empty_like(ls, order="F") for _ in range(3))
     L.suli_ref_heaviside(_ptr(ls), alpha1, dx1, rho1, rho2, mu1, mu2, _ptr(hs), _ptr(rho), _ptr(mu))
     return hs, rho, mu
+
+
+def ref_it_posdin(f, prd1, dz, eta0):
+    """The reference's SuLi-IT posdin (with dz_it) on fields of its compiled
+    size. Returns (u, v, w, dz, dzx, dzy, dzz, eta1, prd)."""
+    L = ref_lib("it")
+    u, v, w, eta1 = (f.u.copy(order="F"), f.v.copy(order="F"), f.w.copy(order="F"), f.eta1.copy(order="F"))
+    dz, dzx, dzy, dzz = (np.asfortranarray(dz).copy(order="F"), f.dzx.copy(order="F"), f.dzy.copy(order="F"),
+                         f.dzz.copy(order="F"))
+    prd = np.zeros_like(prd1, order="F")
+    L.suli_ref_it_posdin(_ptr(u), _ptr(v), _ptr(w), _ptr(dz), _ptr(dzx), _ptr(dzy), _ptr(dzz),
+                         _ptr(np.asfortranarray(eta0)), _ptr(eta1), _ptr(np.asfortranarray(prd1)), _ptr(prd))
+    return u, v, w, dz, dzx, dzy, dzz, eta1, prd
+
+
+def ref_exp_press(case_grid, rho, prd1, prd0):
+    """The reference's exp_press (SuLi-IC, or SuLi-ICdz on a stretched
+    grid). Returns (prd0 after the increment, px, py, pz)."""
+    g = case_grid
+    L = ref_lib("icdz" if g.stretched else "ic")
+    _ref_setup(L, g, 0)
+    out0 = np.asfortranarray(prd0).copy(order="F")
+    px = np.zeros((g.nx + 1, g.ny, g.nz), order="F")
+    py = np.zeros((g.nx, g.ny + 1, g.nz), order="F")
+    pz = np.zeros((g.nx, g.ny, g.nz + 1), order="F")
+    _big_stack_call(_stack_bytes(g), L.suli_ref_exp_press, _ptr(np.asfortranarray(rho, dtype=np.float64)),
+                    _ptr(np.asfortranarray(prd1)), _ptr(out0), _ptr(px), _ptr(py), _ptr(pz))
+    return out0, px, py, pz
+
+
+def ref_heaviside_tens(case_grid, ls, alpha1, dx1, rho1, rho2, mu1, mu2):
+    """The reference's heaviside with t_tens = 1 (SuLi-IC build): returns
+    (hs, rho, mu, hsx, hsy, hsz)."""
+    g = case_grid
+    L = ref_lib("ic")
+    _ref_setup(L, g, 0)
+    ls = np.asfortranarray(ls, dtype=np.float64)
+    out = [np.empty_like(ls, order="F") for _ in range(6)]
+    _big_stack_call(_stack_bytes(g), L.suli_ref_heaviside_tens, _ptr(ls), alpha1, dx1, rho1, rho2, mu1, mu2,
+                    *[_ptr(a) for a in out])
+    return tuple(out)
 
 
 def ref_posdin_hydro(case_grid, fields, hs, prd1):

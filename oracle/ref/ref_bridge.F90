@@ -170,6 +170,32 @@ contains
       prdout = prd
    end subroutine ref_posdin_hydro
 
+   ! Runs the reference exp_press() (SuLi-IC/11_diversos.f90:455-495, or
+   ! SuLi-ICdz's, compiled on its own by oracle/Makefile) on the grid that
+   ! ref_setup fixed: prd0 is updated in place, px, py, pz are written.
+   subroutine ref_exp_press(rhoin, prd1in, prd0io, pxout, pyout, pzout) &
+      bind(C, name="suli_ref_exp_press")
+      use disc
+      use velpre
+      real(c_double), intent(in) :: rhoin(nx, ny, nz), prd1in(0:nx+1, 0:ny+1, 0:nz+1)
+      real(c_double), intent(inout) :: prd0io(0:nx+1, 0:ny+1, 0:nz+1)
+      real(c_double), intent(out) :: pxout(nx+1, ny, nz), pyout(nx, ny+1, nz), pzout(nx, ny, nz+1)
+
+      if (allocated(prd0)) deallocate(prd0)
+      if (allocated(px)) deallocate(px)
+      if (allocated(py)) deallocate(py)
+      if (allocated(pz)) deallocate(pz)
+      allocate(prd0(0:nx+1, 0:ny+1, 0:nz+1), px(nx1, ny, nz), py(nx, ny1, nz), pz(nx, ny, nz1))
+      rho = rhoin
+      prd1 = prd1in
+      prd0 = prd0io
+      call exp_press()
+      prd0io = prd0
+      pxout = px
+      pyout = py
+      pzout = pz
+   end subroutine ref_exp_press
+
 #ifndef SULI_ICDZ
    ! Runs the reference heaviside() (SuLi-IC/14_level_set.f90:508-594) with
    ! t_tens = 0: hs, rho and ls_mu from the level set ls, alpha1 and dx1.
@@ -198,6 +224,44 @@ contains
       rhoout = rho
       muout = ls_mu
    end subroutine ref_heaviside
+
+   ! heaviside() with t_tens = 1 (:519-550, 563-567): also the surface-tension
+   ! gradients hsx, hsy, hsz, from der_weno on the grid ref_setup fixed
+   ! (dx, dy, dz, ccx0, ccy0).
+   subroutine ref_heaviside_tens(lsin, alpha1_, dx1_, rho1, rho2, mu1, mu2, hsout, rhoout, muout, &
+                                 hsxout, hsyout, hszout) bind(C, name="suli_ref_heaviside_tens")
+      use disc
+      use velpre
+      use ls_param
+      real(c_double), intent(in) :: lsin(nx, ny, nz)
+      real(c_double), value :: alpha1_, dx1_, rho1, rho2, mu1, mu2
+      real(c_double), intent(out) :: hsout(nx, ny, nz), rhoout(nx, ny, nz), muout(nx, ny, nz)
+      real(c_double), intent(out) :: hsxout(nx, ny, nz), hsyout(nx, ny, nz), hszout(nx, ny, nz)
+
+      if (allocated(ls)) deallocate(ls)
+      if (allocated(hs)) deallocate(hs)
+      if (allocated(ls_mu)) deallocate(ls_mu)
+      if (allocated(rho)) deallocate(rho)
+      if (allocated(hsx)) deallocate(hsx)
+      if (allocated(hsy)) deallocate(hsy)
+      if (allocated(hsz)) deallocate(hsz)
+      allocate(ls(nx, ny, nz), hs(nx, ny, nz), ls_mu(nx, ny, nz), rho(nx, ny, nz))
+      allocate(hsx(nx, ny, nz), hsy(nx, ny, nz), hsz(nx, ny, nz))
+      ls = lsin
+      t_tens = 1
+      alpha1 = alpha1_
+      dx1 = dx1_
+      rho_f1 = rho1; rho_f2 = rho2
+      mu_f1 = mu1; mu_f2 = mu2
+      call heaviside()
+      t_tens = 0
+      hsout = hs
+      rhoout = rho
+      muout = ls_mu
+      hsxout = hsx
+      hsyout = hsy
+      hszout = hsz
+   end subroutine ref_heaviside_tens
 #endif
 
 end module suli_ref_bridge

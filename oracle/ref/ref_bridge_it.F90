@@ -35,4 +35,29 @@ contains
       call graddin()
       prd1io = prd1
    end subroutine ref_it_graddin
+
+   ! One posdin() (SuLi-IT/10_pos.f90:73-132 with dz_it, compiled on its own
+   ! by oracle/Makefile): u, v, w, dz, dzx, dzy, dzz and eta1 in and out,
+   ! eta0 and prd1 in, prd out.
+   subroutine ref_it_posdin(uio, vio, wio, dzio, dzxio, dzyio, dzzio, eta0in, eta1io, prd1in, prdout) &
+      bind(C, name="suli_ref_it_posdin")
+      use disc
+      use velpre
+      use dzs
+      real(c_double), intent(inout) :: uio(0:nx1+1, 0:ny+1, 0:nz+1), dzxio(0:nx1+1, 0:ny+1, 0:nz+1)
+      real(c_double), intent(inout) :: vio(0:nx+1, 0:ny1+1, 0:nz+1), dzyio(0:nx+1, 0:ny1+1, 0:nz+1)
+      real(c_double), intent(inout) :: wio(0:nx+1, 0:ny+1, 0:nz1+1), dzzio(0:nx+1, 0:ny+1, 0:nz1+1)
+      real(c_double), intent(inout) :: dzio(0:nx+1, 0:ny+1, 0:nz+1), eta1io(0:nx+1, 0:ny+1)
+      real(c_double), intent(in) :: eta0in(0:nx+1, 0:ny+1), prd1in(0:nx+1, 0:ny+1, 0:nz+1)
+      real(c_double), intent(out) :: prdout(0:nx+1, 0:ny+1, 0:nz+1)
+      u = uio; v = vio; w = wio
+      dz = dzio; dzx = dzxio; dzy = dzyio; dzz = dzzio
+      eta0 = eta0in; eta1 = eta1io
+      prd1 = prd1in
+      call posdin()
+      uio = u; vio = v; wio = w
+      dzio = dz; dzxio = dzx; dzyio = dzy; dzzio = dzz
+      eta1io = eta1
+      prdout = prd
+   end subroutine ref_it_posdin
 end module suli_ref_bridge_it

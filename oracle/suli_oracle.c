@@ -861,14 +861,16 @@ int oracle_it_graddin(const oracle_grid *g, double tetah, double gz, const doubl
 
 /* SuLi-IT posdin (SuLi-IT/10_pos.f90:73-132, without the layer-thickness
  * update dz_it): eta1 += prd1(:,:,nz)/gz on its whole (0:nx+1,0:ny+1)
- * box; u, v, w -= tetah dt grad(prd1) with w over dzz; prd = prd1. */
+ * box (skipped when eta1 is NULL); u, v, w -= tetah dt grad(prd1) with w
+ * over dzz; prd = prd1. */
 void oracle_it_posdin(const oracle_grid *g, double tetah, double gz, double *u, double *v, double *w,
                       const double *dzz, double *eta1, const double *prd1, double *prd)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     int i, j, k;
-    for (j = 0; j <= ny + 1; j++)
-        for (i = 0; i <= nx + 1; i++) eta1[IE(i, j)] = eta1[IE(i, j)] + prd1[IP(i, j, nz)] / gz;
+    if (eta1)
+        for (j = 0; j <= ny + 1; j++)
+            for (i = 0; i <= nx + 1; i++) eta1[IE(i, j)] = eta1[IE(i, j)] + prd1[IP(i, j, nz)] / gz;
     const double aux1 = tetah * g->dt / g->dx;
     const double aux2 = tetah * g->dt / g->dy;
     for (k = 1; k <= nz; k++) {
@@ -884,4 +886,256 @@ void oracle_it_posdin(const oracle_grid *g, double tetah, double gz, double *u, 
             for (i = 1; i <= nx; i++)
                 w[IW(i, j, k)] = w[IW(i, j, k)] - tetah * g->dt / dzz[IW(i, j, k)] * (prd1[IP(i, j, k)] - prd1[IP(i, j, k - 1)]);
     memcpy(prd, prd1, (size_t)(nx + 2) * (ny + 2) * (nz + 2) * sizeof(double));
+}
+
+/* dz_it (SuLi-IT/11_diversos.f90:80-141): the face layer thicknesses of the
+ * two top planes from the cell thicknesses dz (IP layout).  dzx (IU layout)
+ * and dzy (IV) on planes nz and nz+1: face means, the end faces periodic
+ * (ccx0 / ccy0 = 0) or linearly extrapolated; dzz (IW) on planes nz and
+ * nz+1 the means, nz+2 extrapolated. */
+static void it_dz_it(const oracle_grid *g, const double *dz, double *dzx, double *dzy, double *dzz)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    int i, j, k;
+    for (k = nz; k <= nz + 1; k++)             /* :89-106 */
+        for (j = 0; j <= ny + 1; j++) {
+            for (i = 1; i <= nx + 1; i++) dzx[IU(i, j, k)] = (dz[IP(i, j, k)] + dz[IP(i - 1, j, k)]) * 0.5;
+            if (g->ccx0 == 0) {
+                dzx[IU(0, j, k)] = (dz[IP(0, j, k)] + dz[IP(nx, j, k)]) * 0.5;
+                dzx[IU(nx + 2, j, k)] = (dz[IP(1, j, k)] + dz[IP(nx + 1, j, k)]) * 0.5;
+            } else {
+                dzx[IU(0, j, k)] = dzx[IU(1, j, k)] + (dzx[IU(1, j, k)] - dzx[IU(2, j, k)]);
+                dzx[IU(nx + 2, j, k)] = dzx[IU(nx + 1, j, k)] + (dzx[IU(nx + 1, j, k)] - dzx[IU(nx, j, k)]);
+            }
+        }
+    for (k = nz; k <= nz + 1; k++)             /* :108-125 */
+        for (i = 0; i <= nx + 1; i++) {
+            for (j = 1; j <= ny + 1; j++) dzy[IV(i, j, k)] = (dz[IP(i, j, k)] + dz[IP(i, j - 1, k)]) * 0.5;
+            if (g->ccy0 == 0) {
+                dzy[IV(i, 0, k)] = (dz[IP(i, 0, k)] + dz[IP(i, ny, k)]) * 0.5;
+                dzy[IV(i, ny + 2, k)] = (dz[IP(i, 1, k)] + dz[IP(i, ny + 1, k)]) * 0.5;
+            } else {
+                dzy[IV(i, 0, k)] = dzy[IV(i, 1, k)] + (dzy[IV(i, 1, k)] - dzy[IV(i, 2, k)]);
+                dzy[IV(i, ny + 2, k)] = dzy[IV(i, ny + 1, k)] + (dzy[IV(i, ny + 1, k)] - dzy[IV(i, ny, k)]);
+            }
+        }
+    for (j = 0; j <= ny + 1; j++)              /* :127-138 */
+        for (i = 0; i <= nx + 1; i++) {
+            dzz[IW(i, j, nz)] = (dz[IP(i, j, nz)] + dz[IP(i, j, nz - 1)]) * 0.5;
+            dzz[IW(i, j, nz + 1)] = (dz[IP(i, j, nz + 1)] + dz[IP(i, j, nz)]) * 0.5;
+            dzz[IW(i, j, nz + 2)] = dzz[IW(i, j, nz + 1)] + (dzz[IW(i, j, nz + 1)] - dzz[IW(i, j, nz)]);
+        }
+}
+
+/* SuLi-IT posdin as the reference runs it (SuLi-IT/10_pos.f90:73-132):
+ * eta1 += prd1(:,:,nz)/gz and dz(:,:,nz) += eta1 - eta0 over the
+ * (0:nx+1,0:ny+1) box (:84-90), dz_it (:93), then the velocity correction
+ * with the new dzz (:99-121); prd = prd1. */
+void oracle_it_posdin_dz(const oracle_grid *g, double tetah, double gz, double *u, double *v, double *w,
+                         double *dz, double *dzx, double *dzy, double *dzz, const double *eta0, double *eta1,
+                         const double *prd1, double *prd)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    int i, j;
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) {
+            eta1[IE(i, j)] = eta1[IE(i, j)] + prd1[IP(i, j, nz)] / gz;
+            dz[IP(i, j, nz)] = dz[IP(i, j, nz)] + (eta1[IE(i, j)] - eta0[IE(i, j)]);
+        }
+    it_dz_it(g, dz, dzx, dzy, dzz);
+    oracle_it_posdin(g, tetah, gz, u, v, w, dzz, NULL, prd1, prd);
+}
+
+/* ------------------------------------------------------------------------
+ * exp_press, the IPA scheme's pressure increment (SuLi-IC/11_diversos.f90:
+ * 455-495; SuLi-ICdz/11_diversos.f90:455-495 divides the z difference by
+ * dzz(k)): prd0 = prd1 + prd0 over the halo box (:466), prdaux = prd1 + prd0
+ * (:467), the face densities of graddin (:469-471), then
+ *   px = -(prdaux(i) - prdaux(i-1))/dx/rhox   on x faces 1..nx+1 (:473-478)
+ *   py = -(prdaux(j) - prdaux(j-1))/dy/rhoy   on y faces 1..ny+1 (:480-485)
+ *   pz = -(prdaux(k) - prdaux(k-1))/dz/rhoz   on z faces 1..nz+1 (:487-493)
+ * px, py, pz in the 1-based face layouts (nx+1,ny,nz), (nx,ny+1,nz),
+ * (nx,ny,nz+1). */
+void oracle_exp_press(const oracle_grid *g, const double *rho, const double *prd1, double *prd0, double *px,
+                      double *py, double *pz)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    double *aux = malloc(nbox * sizeof(double));
+    double *rhox = malloc((size_t)(nx + 1) * ny * nz * sizeof(double));
+    double *rhoy = malloc((size_t)nx * (ny + 1) * nz * sizeof(double));
+    double *rhoz = malloc((size_t)nx * ny * (nz + 1) * sizeof(double));
+    int i, j, k;
+    for (size_t c = 0; c < nbox; c++) {
+        prd0[c] = prd1[c] + prd0[c];
+        aux[c] = prd1[c] + prd0[c];
+    }
+    oracle_interp_faces(g, rho, rhox, rhoy, rhoz);
+    for (k = 1; k <= nz; k++) {
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx + 1; i++)
+                px[IFX(i, j, k)] = -(aux[IP(i, j, k)] - aux[IP(i - 1, j, k)]) / g->dx / rhox[IFX(i, j, k)];
+        for (j = 1; j <= ny + 1; j++)
+            for (i = 1; i <= nx; i++)
+                py[IFY(i, j, k)] = -(aux[IP(i, j, k)] - aux[IP(i, j - 1, k)]) / g->dy / rhoy[IFY(i, j, k)];
+    }
+    for (k = 1; k <= nz + 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                const double h = g->dzzv ? g->dzzv[k] : g->dz;
+                pz[IFZ(i, j, k)] = -(aux[IP(i, j, k)] - aux[IP(i, j, k - 1)]) / h / rhoz[IFZ(i, j, k)];
+            }
+    free(aux); free(rhox); free(rhoy); free(rhoz);
+}
+
+/* ------------------------------------------------------------------------
+ * WENO5 derivatives of the level set along one grid line (SuLi-IC/
+ * 8_derivadas.f90:675-811 wenox, :815-951 wenoy, :955-1095 wenoz, which
+ * differ only in the direction): ph[m + 2] holds phi1(m), m = -2..n+3, with
+ * phi1(1..n) the line's values; the ends are filled here, periodic (per)
+ * as :702-714 or, otherwise, extrapolated with zero derivative as
+ * :742-756.  dp, dn (1..n at [0..n-1]) get the two one-sided derivatives
+ * (dphidxp, dphidxn). */
+static void weno_line(double *ph, int n, double h, int per, double *dp, double *dn)
+{
+#define PH(m) ph[(m) + 2]
+    const double c11 = 1. / 11.;
+    if (per) {
+        PH(0) = PH(n - 1);
+        PH(-1) = PH(n - 2);
+        PH(-2) = PH(n - 3);
+        PH(n + 1) = PH(2);
+        PH(n + 2) = PH(3);
+        PH(n + 3) = PH(4);
+    } else {
+        PH(0) = c11 * (18. * PH(1) - 9. * PH(2) + 2. * PH(3));
+        PH(-1) = c11 * (18. * PH(0) - 9. * PH(1) + 2. * PH(2));
+        PH(-2) = c11 * (18. * PH(-1) - 9. * PH(0) + 2. * PH(1));
+        PH(n + 1) = c11 * (18. * PH(n) - 9. * PH(n - 1) + 2. * PH(n - 2));
+        PH(n + 2) = c11 * (18. * PH(n + 1) - 9. * PH(n) + 2. * PH(n - 1));
+        PH(n + 3) = c11 * (18. * PH(n + 2) - 9. * PH(n + 1) + 2. * PH(n));
+    }
+    /* up(m) = (phi1(m+3)-phi1(m+2))/h, m = -3..n (:759-765);
+     * un(m) = (phi1(m-2)-phi1(m-3))/h, m = 1..n+4 (:767-773) */
+    double *up = malloc((size_t)(n + 4) * sizeof(double)); /* up[m + 3] */
+    double *un = malloc((size_t)(n + 4) * sizeof(double)); /* un[m - 1] */
+    for (int m = -3; m <= n; m++) up[m + 3] = (PH(m + 3) - PH(m + 2)) / h;
+    for (int m = 1; m <= n + 4; m++) un[m - 1] = (PH(m - 2) - PH(m - 3)) / h;
+#define UP(m) up[(m) + 3]
+#define UN(m) un[(m) - 1]
+    const double a1 = 13. / 12., a2 = 1. / 4., a3 = 1. / 6., eps = 0.00000001;
+    const double wgt[3] = {0.1, 0.6, 0.3};
+    for (int i = 1; i <= n; i++) {   /* :775-809 */
+        double isp[3], isn[3], alp[3], aln[3];
+        double t;
+        t = UP(i) - 2 * UP(i - 1) + UP(i - 2);
+        isp[0] = a1 * t * t;
+        t = UP(i) - 4 * UP(i - 1) + 3 * UP(i - 2);
+        isp[0] = isp[0] + a2 * t * t;
+        t = UN(i) - 2 * UN(i + 1) + UN(i + 2);
+        isn[0] = a1 * t * t;
+        t = UN(i) - 4 * UN(i + 1) + 3 * UN(i + 2);
+        isn[0] = isn[0] + a2 * t * t;
+        t = UP(i - 1) - 2 * UP(i - 2) + UP(i - 3);
+        isp[1] = a1 * t * t;
+        t = UP(i - 1) - UP(i - 3);
+        isp[1] = isp[1] + a2 * t * t;
+        t = UN(i + 1) - 2 * UN(i + 2) + UN(i + 3);
+        isn[1] = a1 * t * t;
+        t = UN(i + 1) - UN(i + 3);
+        isn[1] = isn[1] + a2 * t * t;
+        t = UP(i - 2) - 2 * UP(i - 3) + UP(i - 4);
+        isp[2] = a1 * t * t;
+        t = 3 * UP(i - 2) - 4 * UP(i - 3) + UP(i - 4);
+        isp[2] = isp[2] + a2 * t * t;
+        t = UN(i + 2) - 2 * UN(i + 3) + UN(i + 4);
+        isn[2] = a1 * t * t;
+        t = 3 * UN(i + 2) - 4 * UN(i + 3) + UN(i + 4);
+        isn[2] = isn[2] + a2 * t * t;
+        for (int q = 0; q < 3; q++) {
+            alp[q] = wgt[q] / ((eps + isp[q]) * (eps + isp[q]));
+            aln[q] = wgt[q] / ((eps + isn[q]) * (eps + isn[q]));
+        }
+        const double sp = alp[0] + alp[1] + alp[2], sn = aln[0] + aln[1] + aln[2];
+        const double op0 = alp[0] / sp, op1 = alp[1] / sp, op2 = alp[2] / sp;
+        const double on0 = aln[0] / sn, on1 = aln[1] / sn, on2 = aln[2] / sn;
+        dp[i - 1] = a3 * (op0 * (2 * UP(i) - 7 * UP(i - 1) + 11 * UP(i - 2)) +
+                          op1 * (-UP(i - 1) + 5 * UP(i - 2) + 2 * UP(i - 3)) +
+                          op2 * (2 * UP(i - 2) + 5 * UP(i - 3) - UP(i - 4)));
+        dn[i - 1] = a3 * (on0 * (2 * UN(i) - 7 * UN(i + 1) + 11 * UN(i + 2)) +
+                          on1 * (-UN(i + 1) + 5 * UN(i + 2) + 2 * UN(i + 3)) +
+                          on2 * (2 * UN(i + 2) + 5 * UN(i + 3) - UN(i + 4)));
+    }
+#undef UP
+#undef UN
+#undef PH
+    free(up);
+    free(un);
+}
+
+/* The level set's WENO5 derivative of larger magnitude per direction, as
+ * heaviside's t_tens = 1 branch forms it (SuLi-IC/14_level_set.f90:519-544
+ * over der_weno, :338-370): x and y lines periodic when ccx0 / ccy0 = 0,
+ * else extrapolated; z lines always extrapolated.  gx, gy, gz (nx,ny,nz). */
+void oracle_ls_gradient(const oracle_grid *g, const double *ls, double *gx, double *gy, double *gzo)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+    double *ph = malloc((size_t)(nmax + 6) * sizeof(double));
+    double *dp = malloc((size_t)nmax * sizeof(double));
+    double *dn = malloc((size_t)nmax * sizeof(double));
+    int i, j, k;
+    /* |ta1| > |td1| keeps ta1, else td1 (:523-527) */
+#define PICK(a, b) (fabs(a) > fabs(b) ? (a) : (b))
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++) {
+            for (i = 1; i <= nx; i++) ph[i + 2] = ls[IC(i, j, k)];
+            weno_line(ph, nx, g->dx, g->ccx0 == 0, dp, dn);
+            for (i = 1; i <= nx; i++) gx[IC(i, j, k)] = PICK(dp[i - 1], dn[i - 1]);
+        }
+    for (k = 1; k <= nz; k++)
+        for (i = 1; i <= nx; i++) {
+            for (j = 1; j <= ny; j++) ph[j + 2] = ls[IC(i, j, k)];
+            weno_line(ph, ny, g->dy, g->ccy0 == 0, dp, dn);
+            for (j = 1; j <= ny; j++) gy[IC(i, j, k)] = PICK(dp[j - 1], dn[j - 1]);
+        }
+    for (j = 1; j <= ny; j++)
+        for (i = 1; i <= nx; i++) {
+            for (k = 1; k <= nz; k++) ph[k + 2] = ls[IC(i, j, k)];
+            weno_line(ph, nz, g->dz, 0, dp, dn);
+            for (k = 1; k <= nz; k++) gzo[IC(i, j, k)] = PICK(dp[k - 1], dn[k - 1]);
+        }
+#undef PICK
+    free(ph);
+    free(dp);
+    free(dn);
+}
+
+/* heaviside with t_tens = 1 (SuLi-IC/14_level_set.f90:508-594): hs, rho
+ * and ls_mu as oracle_heaviside, and the surface-tension gradients hsx,
+ * hsy, hsz: 0.5*g*(1 + cos(pi*ls/eps))/eps inside the band (:563-567) with
+ * g from oracle_ls_gradient, 0 outside (:546-548); eps = alpha1*dx1. */
+void oracle_heaviside_tens(const oracle_grid *g, const double *ls, double eps, double rho1, double rho2,
+                           double mu1, double mu2, double *hs, double *rho, double *mu, double *hsx,
+                           double *hsy, double *hsz)
+{
+    const int n = g->nx * g->ny * g->nz;
+    const double pi = acos(-1.0);
+    double *tx = malloc((size_t)n * sizeof(double));
+    double *ty = malloc((size_t)n * sizeof(double));
+    double *tz = malloc((size_t)n * sizeof(double));
+    oracle_ls_gradient(g, ls, tx, ty, tz);
+    oracle_heaviside(n, ls, eps, rho1, rho2, mu1, mu2, hs, rho, mu);
+    for (int c = 0; c < n; c++) {
+        hsx[c] = hsy[c] = hsz[c] = 0.;
+        if (!(ls[c] < -eps) && !(ls[c] > eps)) {
+            const double cs = 1. + cos(pi * ls[c] / eps);
+            hsx[c] = 0.5 * tx[c] * cs / eps;
+            hsy[c] = 0.5 * ty[c] * cs / eps;
+            hsz[c] = 0.5 * tz[c] * cs / eps;
+        }
+    }
+    free(tx);
+    free(ty);
+    free(tz);
 }

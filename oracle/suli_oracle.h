@@ -116,10 +116,37 @@ int oracle_it_graddin(const oracle_grid *g, double tetah, double gz, const doubl
                       const double *w, const double *dzx, const double *dzy, const double *dzz,
                       const double *eta1, const double *deltah, double *prd1, oracle_stats *st);
 
-/* SuLi-IT/10_pos.f90:73-132 without dz_it: eta1 += prd1(:,:,nz)/gz; u, v,
- * w corrected in place; prd = prd1. */
+/* SuLi-IT/10_pos.f90:73-132 without dz_it: eta1 += prd1(:,:,nz)/gz (eta1
+ * may be NULL: skipped); u, v, w corrected in place; prd = prd1. */
 void oracle_it_posdin(const oracle_grid *g, double tetah, double gz, double *u, double *v, double *w,
                       const double *dzz, double *eta1, const double *prd1, double *prd);
+
+/* SuLi-IT/10_pos.f90:73-132 with dz_it (SuLi-IT/11_diversos.f90:80-141):
+ * eta1 and dz(:,:,nz) from prd1 and eta0, the top planes' face thicknesses
+ * (dzx, dzy, dzz), then the correction above with the new dzz.  dz in
+ * prd1's layout, dzx in u's, dzy in v's, dzz in w's, eta0/eta1
+ * (0:nx+1,0:ny+1). */
+void oracle_it_posdin_dz(const oracle_grid *g, double tetah, double gz, double *u, double *v, double *w,
+                         double *dz, double *dzx, double *dzy, double *dzz, const double *eta0, double *eta1,
+                         const double *prd1, double *prd);
+
+/* SuLi-IC/11_diversos.f90:455-495 (SuLi-ICdz with dzv/dzzv): prd0 += prd1
+ * over the halo box; px, py, pz = -grad(prd1 + prd0)/rho_face on the face
+ * layouts (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1). */
+void oracle_exp_press(const oracle_grid *g, const double *rho, const double *prd1, double *prd0, double *px,
+                      double *py, double *pz);
+
+/* The level set's WENO5 derivatives as heaviside's t_tens = 1 branch picks
+ * them (SuLi-IC/14_level_set.f90:338-370,519-544; 8_derivadas.f90:675-1095):
+ * per direction the one-sided derivative of larger magnitude.  Uses nx..dz,
+ * ccx0, ccy0.  All arrays (nx,ny,nz). */
+void oracle_ls_gradient(const oracle_grid *g, const double *ls, double *gx, double *gy, double *gz);
+
+/* SuLi-IC/14_level_set.f90:508-594 with t_tens = 1: oracle_heaviside plus
+ * hsx, hsy, hsz (nx,ny,nz); eps = alpha1*dx1; hs and mu may be NULL. */
+void oracle_heaviside_tens(const oracle_grid *g, const double *ls, double eps, double rho1, double rho2,
+                           double mu1, double mu2, double *hs, double *rho, double *mu, double *hsx,
+                           double *hsy, double *hsz);
 
 #ifdef __cplusplus
 }

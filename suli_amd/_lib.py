@@ -33,6 +33,8 @@ EXPORTS = (
     "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
     "suli_ps_heaviside_dev", "suli_ps_restart_read", "suli_ps_it_graddin", "suli_ps_it_graddin_dev",
     "suli_ps_it_posdin", "suli_ps_it_posdin_dev", "suli_ps_graddin_noalloc_dev",
+    "suli_ps_heaviside_tens_dev", "suli_ps_exp_press", "suli_ps_exp_press_dev",
+    "suli_ps_it_posdin_layers", "suli_ps_it_posdin_layers_dev",
 )
 
 # SULI_PS_NOALLOC_MAX_BYTES: the fixed device array of the no-allocation solve
@@ -139,6 +141,12 @@ def load(path: str = LIB_PATH):
     L.suli_ps_it_posdin.argtypes = it_p
     L.suli_ps_it_posdin_dev.argtypes = it_p
     L.suli_ps_heaviside_dev.argtypes = [vp, C.POINTER(Grid), vp] + [C.c_double] * 4 + [vp] * 3
+    L.suli_ps_heaviside_tens_dev.argtypes = [vp, C.POINTER(Grid), vp] + [C.c_double] * 4 + [vp] * 6
+    L.suli_ps_exp_press.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
+    L.suli_ps_exp_press_dev.argtypes = [vp, C.POINTER(Grid)] + [vp] * 8
+    it_l = [vp, C.POINTER(Grid), C.c_double, C.c_double] + [vp] * 11
+    L.suli_ps_it_posdin_layers.argtypes = it_l
+    L.suli_ps_it_posdin_layers_dev.argtypes = it_l
     L.suli_ps_dev_alloc.restype = vp
     L.suli_ps_dev_alloc.argtypes = [vp, C.c_size_t]
     L.suli_ps_dev_free.argtypes = [vp, vp]

@@ -224,12 +224,26 @@ void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double
 // velocities zeroed where hs < 1e-4 on the face).
 void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu, double gz, double *prd1,
                    hipStream_t s);
-// rho (and hs, mu when non-null) from the level set, heaviside with t_tens = 0.
 // SuLi-IT posdin without dz_it (SuLi-IT/10_pos.f90:73-132).
 void launch_it_posdin(const Geo &g, double dx, double dy, double dt, double tetah, double gz, const double *prd1,
                       const double *dz3, double *eta1, double *u, double *v, double *w, hipStream_t s);
+// SuLi-IT posdin's surface layer before that: eta1 and dz(:,:,nz) from prd1
+// and eta0, then dz_it (dzx, dzy, dzz of the top planes).
+void launch_it_layers(const Geo &g, double gz, const double *prd1, const double *eta0, double *eta1, double *dz,
+                      double *dzx, double *dzy, double *dzz, hipStream_t s);
+// rho (and hs, mu when non-null) from the level set, heaviside with t_tens = 0.
 void launch_heaviside(long long n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
                       double *hs, double *rho, double *mu, hipStream_t s);
+// The same with t_tens = 1: also hsx, hsy, hsz from the level set's WENO5
+// derivatives (x, y periodic per g.px, g.py; uniform spacing dx, dy, dz).
+void launch_heaviside_tens(const Geo &g, const double *ls, double eps, double rho1, double rho2, double mu1,
+                           double mu2, double dx, double dy, double dz, double *hs, double *rho, double *mu,
+                           double *hsx, double *hsy, double *hsz, hipStream_t s);
+// exp_press (IPA): px, py, pz from prd1 + (prd1 + prd0) and the face
+// densities, then prd0 += prd1 over the halo box.
+void launch_exp_press(const Geo &g, const double *rho, const double *dz, const double *dzz, double dx, double dy,
+                      double dzu, const double *prd1, double *prd0, double *px, double *py, double *pz,
+                      hipStream_t s);
 void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, double *u, double *v, double *w,
                          hipStream_t s);
 

@@ -647,6 +647,278 @@ __global__ __launch_bounds__(kThreads) void k_heaviside(long long n, const doubl
     }
 }
 
+// ---- heaviside with t_tens = 1 (SuLi-IC/14_level_set.f90:519-550,563-567)
+
+// phi1(m), m = -2..n+3 (1-based), of the grid line through a cell: the
+// line's values (base[(m-1)*stride]), its periodic wrap, or its ends
+// extrapolated with zero derivative, as the WENO routines fill them
+// (SuLi-IC/8_derivadas.f90:702-714 and :742-756 for x; y and z alike).
+struct Line {
+    const double *base;
+    long long stride;
+    int n;
+    bool per;
+    __device__ __forceinline__ double at(int m) const
+    {
+        if (m >= 1 && m <= n) return base[(long long)(m - 1) * stride];
+        if (per) return base[(long long)((m <= 0 ? m + n - 1 : m - n + 1) - 1) * stride];
+        const double c11 = 1. / 11.;
+        if (m <= 0) {
+            const double p1 = base[0], p2 = base[stride], p3 = base[2 * stride];
+            const double g0 = c11 * (18. * p1 - 9. * p2 + 2. * p3);
+            if (m == 0) return g0;
+            const double g1 = c11 * (18. * g0 - 9. * p1 + 2. * p2);
+            if (m == -1) return g1;
+            return c11 * (18. * g1 - 9. * g0 + 2. * p1);
+        }
+        const double p1 = base[(long long)(n - 1) * stride], p2 = base[(long long)(n - 2) * stride],
+                     p3 = base[(long long)(n - 3) * stride];
+        const double g0 = c11 * (18. * p1 - 9. * p2 + 2. * p3);
+        if (m == n + 1) return g0;
+        const double g1 = c11 * (18. * g0 - 9. * p1 + 2. * p2);
+        if (m == n + 2) return g1;
+        return c11 * (18. * g1 - 9. * g0 + 2. * p1);
+    }
+};
+
+// The WENO5 one-sided derivatives at cell i of a line (wenox's loop body,
+// SuLi-IC/8_derivadas.f90:759-809, in its operation order), returned as the
+// one of larger magnitude, as heaviside picks (14_level_set.f90:523-527).
+__device__ double weno_pick(const Line &L, int i, double h)
+{
+    double ph[7];   // phi1(i-3 .. i+3)
+#pragma unroll
+    for (int m = 0; m < 7; ++m) ph[m] = L.at(i - 3 + m);
+    // up(m) = (phi1(m+3) - phi1(m+2))/h, m = i-4..i -> up[m - i + 4]
+    // un(m) = (phi1(m-2) - phi1(m-3))/h, m = i..i+4 -> un[m - i]
+    double up[5], un[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        up[t] = (ph[t + 2] - ph[t + 1]) / h;
+        un[t] = (ph[t + 1] - ph[t]) / h;
+    }
+    const double a1 = 13. / 12., a2 = 1. / 4., a3 = 1. / 6., eps = 0.00000001;
+    const double wgt[3] = {0.1, 0.6, 0.3};
+    // up(i - d) = up[4 - d]; un(i + d) = un[d]
+#define UP(d) up[4 - (d)]
+#define UN(d) un[(d)]
+    double isp[3], isn[3], t;
+    t = UP(0) - 2 * UP(1) + UP(2);
+    isp[0] = a1 * t * t;
+    t = UP(0) - 4 * UP(1) + 3 * UP(2);
+    isp[0] = isp[0] + a2 * t * t;
+    t = UN(0) - 2 * UN(1) + UN(2);
+    isn[0] = a1 * t * t;
+    t = UN(0) - 4 * UN(1) + 3 * UN(2);
+    isn[0] = isn[0] + a2 * t * t;
+    t = UP(1) - 2 * UP(2) + UP(3);
+    isp[1] = a1 * t * t;
+    t = UP(1) - UP(3);
+    isp[1] = isp[1] + a2 * t * t;
+    t = UN(1) - 2 * UN(2) + UN(3);
+    isn[1] = a1 * t * t;
+    t = UN(1) - UN(3);
+    isn[1] = isn[1] + a2 * t * t;
+    t = UP(2) - 2 * UP(3) + UP(4);
+    isp[2] = a1 * t * t;
+    t = 3 * UP(2) - 4 * UP(3) + UP(4);
+    isp[2] = isp[2] + a2 * t * t;
+    t = UN(2) - 2 * UN(3) + UN(4);
+    isn[2] = a1 * t * t;
+    t = 3 * UN(2) - 4 * UN(3) + UN(4);
+    isn[2] = isn[2] + a2 * t * t;
+    double alp[3], aln[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        alp[q] = wgt[q] / ((eps + isp[q]) * (eps + isp[q]));
+        aln[q] = wgt[q] / ((eps + isn[q]) * (eps + isn[q]));
+    }
+    const double sp = alp[0] + alp[1] + alp[2], sn = aln[0] + aln[1] + aln[2];
+    const double dp = a3 * (alp[0] / sp * (2 * UP(0) - 7 * UP(1) + 11 * UP(2)) +
+                            alp[1] / sp * (-UP(1) + 5 * UP(2) + 2 * UP(3)) +
+                            alp[2] / sp * (2 * UP(2) + 5 * UP(3) - UP(4)));
+    const double dn = a3 * (aln[0] / sn * (2 * UN(0) - 7 * UN(1) + 11 * UN(2)) +
+                            aln[1] / sn * (-UN(1) + 5 * UN(2) + 2 * UN(3)) +
+                            aln[2] / sn * (2 * UN(2) + 5 * UN(3) - UN(4)));
+#undef UP
+#undef UN
+    return fabs(dp) > fabs(dn) ? dp : dn;
+}
+
+// heaviside with the surface-tension gradients: as k_heaviside, and hsx,
+// hsy, hsz = 0.5 g (1 + cos(pi ls/eps))/eps inside the band, 0 outside,
+// with g the level set's WENO5 derivative along each axis (x, y periodic
+// when g.px / g.py, z never).  One thread per cell.
+__global__ __launch_bounds__(kThreads) void k_heaviside_tens(Geo g, const double *__restrict__ ls, double eps,
+                                                             double rho1, double rho2, double mu1, double mu2,
+                                                             double dx, double dy, double dz,
+                                                             double *__restrict__ hs_out, double *__restrict__ rho,
+                                                             double *__restrict__ mu, double *__restrict__ hsx,
+                                                             double *__restrict__ hsy, double *__restrict__ hsz)
+{
+    const double pi = 3.141592653589793;  // acos(-1.) in FP64
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long plane = (long long)nx * ny;
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
+        const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / plane);
+        const double l = ls[c];
+        double hs;
+        double gx = 0., gy = 0., gz = 0.;
+        const bool band = !(l < -eps) && !(l > eps);
+        if (l < -eps)
+            hs = 0.;
+        else if (l > eps)
+            hs = 1.;
+        else
+            hs = 0.5 * (1. + l / eps + 1. / pi * sin(pi * l / eps));
+        if (band) {
+            const double cs = 1. + cos(pi * l / eps);
+            const Line lx{ls + (c - i), 1, nx, g.px != 0};
+            const Line ly{ls + (c - (long long)j * nx), nx, ny, g.py != 0};
+            const Line lz{ls + (c - (long long)k * plane), plane, nz, false};
+            gx = 0.5 * weno_pick(lx, i + 1, dx) * cs / eps;
+            gy = 0.5 * weno_pick(ly, j + 1, dy) * cs / eps;
+            gz = 0.5 * weno_pick(lz, k + 1, dz) * cs / eps;
+        }
+        if (hs_out) hs_out[c] = hs;
+        rho[c] = rho1 * (1. - hs) + rho2 * hs;
+        if (mu) mu[c] = mu1 * (1. - hs) + mu2 * hs;
+        hsx[c] = gx;
+        hsy[c] = gy;
+        hsz[c] = gz;
+    }
+}
+
+// ---- exp_press (SuLi-IC/11_diversos.f90:455-495) ----------------------------
+
+// px, py, pz = -(prdaux - prdaux_behind)/h/rho_face on every face, with
+// prdaux = prd1 + (prd1 + prd0) formed from prd0 before its increment (the
+// reference forms prd0 + prd1 first, :466-467: the same operations).  z
+// differences divide by dzz(k) when dzz is given (SuLi-ICdz), else dzu.
+__global__ __launch_bounds__(kThreads) void k_exp_press(Geo g, const double *__restrict__ rho,
+                                                        const double *__restrict__ dz,
+                                                        const double *__restrict__ dzz, double dx, double dy,
+                                                        double dzu, const double *__restrict__ prd1,
+                                                        const double *__restrict__ prd0, double *__restrict__ px,
+                                                        double *__restrict__ py, double *__restrict__ pz)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long nfx = (long long)(nx + 1) * ny * nz;
+    const long long nfy = (long long)nx * (ny + 1) * nz;
+    const long long nfz = (long long)nx * ny * (nz + 1);
+    const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
+    auto aux = [&](long long x) { return prd1[x] + (prd1[x] + prd0[x]); };
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
+        if (t < nfx) {
+            const int f = (int)(t % (nx + 1));
+            const long long jk = t / (nx + 1);
+            const int j = (int)(jk % ny), k = (int)(jk / ny);
+            const long long ip = (f + 1) + pb * (j + 1) + pbb * (k + 1);
+            px[t] = -(aux(ip) - aux(ip - 1)) / dx / face_x(g, rho, f, j, k);
+        } else if (t < nfx + nfy) {
+            const long long r = t - nfx;
+            const int i = (int)(r % nx);
+            const long long jk = r / nx;
+            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
+            const long long ip = (i + 1) + pb * (f + 1) + pbb * (k + 1);
+            py[r] = -(aux(ip) - aux(ip - pb)) / dy / face_y(g, rho, i, f, k);
+        } else {
+            const long long r = t - nfx - nfy;
+            const int i = (int)(r % nx);
+            const long long jf = r / nx;
+            const int j = (int)(jf % ny), f = (int)(jf / ny);
+            const long long ip = (i + 1) + pb * (j + 1) + pbb * (f + 1);
+            const double h = dzz ? dzz[f + 1] : dzu;
+            pz[r] = -(aux(ip) - aux(ip - pbb)) / h / face_z(g, rho, dz, i, j, f);
+        }
+    }
+}
+
+// prd0 = prd1 + prd0 over the whole halo box (:466).
+__global__ __launch_bounds__(kThreads) void k_add_box(long long n, const double *__restrict__ prd1,
+                                                      double *__restrict__ prd0)
+{
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < n; c += stride) prd0[c] = prd1[c] + prd0[c];
+}
+
+// ---- SuLi-IT posdin's surface layer (SuLi-IT/10_pos.f90:84-93) --------------
+
+// eta1 += prd1(:,:,nz)/gz and dz(:,:,nz) += eta1 - eta0 over the
+// (0:nx+1, 0:ny+1) box; dz in prd1's layout.
+__global__ __launch_bounds__(kThreads) void k_it_layer(Geo g, double gz, const double *__restrict__ prd1,
+                                                       const double *__restrict__ eta0, double *__restrict__ eta1,
+                                                       double *__restrict__ dz)
+{
+    const long long ne = (long long)(g.nx + 2) * (g.ny + 2);
+    const long long top = ne * g.nz;
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < ne; e += stride) {
+        const double et = eta1[e] + prd1[top + e] / gz;
+        eta1[e] = et;
+        dz[top + e] = dz[top + e] + (et - eta0[e]);
+    }
+}
+
+// dz_it (SuLi-IT/11_diversos.f90:80-141): dzx (u layout) and dzy (v layout)
+// on planes nz and nz+1, dzz (w layout) on planes nz, nz+1 and nz+2, from
+// the cell thicknesses dz.  End faces: periodic pairs (x when g.px, y when
+// g.py) or the linear extrapolation of the two faces inside.
+__global__ __launch_bounds__(kThreads) void k_dz_it(Geo g, const double *__restrict__ dz, double *__restrict__ dzx,
+                                                    double *__restrict__ dzy, double *__restrict__ dzz)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
+    const long long nux = (long long)(nx + 3) * (ny + 2), nvy = (long long)(nx + 2) * (ny + 3);
+    const long long nx2 = 2 * nux, ny2 = 2 * nvy, nz3 = 3 * pbb;
+    auto D = [&](int i, int j, int k) { return dz[i + pb * j + pbb * k]; };
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nx2 + ny2 + nz3; t += stride) {
+        if (t < nx2) {
+            const int k = nz + (int)(t / nux);
+            const long long r = t % nux;
+            const int i = (int)(r % (nx + 3)), j = (int)(r / (nx + 3));
+            auto mx = [&](int f) { return (D(f, j, k) + D(f - 1, j, k)) * 0.5; };
+            double v;
+            if (i >= 1 && i <= nx + 1)
+                v = mx(i);
+            else if (g.px)
+                v = i == 0 ? (D(0, j, k) + D(nx, j, k)) * 0.5 : (D(1, j, k) + D(nx + 1, j, k)) * 0.5;
+            else if (i == 0)
+                v = mx(1) + (mx(1) - mx(2));
+            else
+                v = mx(nx + 1) + (mx(nx + 1) - mx(nx));
+            dzx[i + (nx + 3) * (j + (long long)(ny + 2) * k)] = v;
+        } else if (t < nx2 + ny2) {
+            const long long q = t - nx2;
+            const int k = nz + (int)(q / nvy);
+            const long long r = q % nvy;
+            const int i = (int)(r % (nx + 2)), j = (int)(r / (nx + 2));
+            auto my = [&](int f) { return (D(i, f, k) + D(i, f - 1, k)) * 0.5; };
+            double v;
+            if (j >= 1 && j <= ny + 1)
+                v = my(j);
+            else if (g.py)
+                v = j == 0 ? (D(i, 0, k) + D(i, ny, k)) * 0.5 : (D(i, 1, k) + D(i, ny + 1, k)) * 0.5;
+            else if (j == 0)
+                v = my(1) + (my(1) - my(2));
+            else
+                v = my(ny + 1) + (my(ny + 1) - my(ny));
+            dzy[i + (nx + 2) * (j + (long long)(ny + 3) * k)] = v;
+        } else {
+            const long long q = t - nx2 - ny2;
+            const int k = nz + (int)(q / pbb);
+            const long long e = q % pbb;
+            const int i = (int)(e % pb), j = (int)(e / pb);
+            auto mz = [&](int f) { return (D(i, j, f) + D(i, j, f - 1)) * 0.5; };
+            const double v = k <= nz + 1 ? mz(k) : mz(nz + 1) + (mz(nz + 1) - mz(nz));
+            dzz[e + pbb * k] = v;
+        }
+    }
+}
+
 // Tables of a constant-density operator: face 0 of row 0 of each plane
 // (every face of the plane holds the same value).
 __global__ __launch_bounds__(kThreads) void k_face_tables(Geo g, const double *sx, const double *sy,
@@ -852,6 +1124,34 @@ void launch_heaviside(long long n, const double *ls, double eps, double rho1, do
 {
     hipLaunchKernelGGL(k_heaviside, dim3(grid_for(n)), dim3(kThreads), 0, s, n, ls, eps, rho1, rho2, mu1, mu2, hs,
                        rho, mu);
+}
+
+void launch_heaviside_tens(const Geo &g, const double *ls, double eps, double rho1, double rho2, double mu1,
+                           double mu2, double dx, double dy, double dz, double *hs, double *rho, double *mu,
+                           double *hsx, double *hsy, double *hsz, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_heaviside_tens, dim3(grid_for(g.n)), dim3(kThreads), 0, s, g, ls, eps, rho1, rho2, mu1, mu2,
+                       dx, dy, dz, hs, rho, mu, hsx, hsy, hsz);
+}
+
+void launch_exp_press(const Geo &g, const double *rho, const double *dz, const double *dzz, double dx, double dy,
+                      double dzu, const double *prd1, double *prd0, double *px, double *py, double *pz, hipStream_t s)
+{
+    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
+                         (long long)g.nx * g.ny * (g.nz + 1);
+    const long long nbox = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    hipLaunchKernelGGL(k_exp_press, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, rho, dz, dzz, dx, dy, dzu, prd1,
+                       prd0, px, py, pz);
+    hipLaunchKernelGGL(k_add_box, dim3(grid_for(nbox)), dim3(kThreads), 0, s, nbox, prd1, prd0);
+}
+
+void launch_it_layers(const Geo &g, double gz, const double *prd1, const double *eta0, double *eta1, double *dz,
+                      double *dzx, double *dzy, double *dzz, hipStream_t s)
+{
+    const long long ne = (long long)(g.nx + 2) * (g.ny + 2);
+    const long long nt = 2 * (long long)(g.nx + 3) * (g.ny + 2) + 2 * (long long)(g.nx + 2) * (g.ny + 3) + 3 * ne;
+    hipLaunchKernelGGL(k_it_layer, dim3(grid_for(ne)), dim3(kThreads), 0, s, g, gz, prd1, eta0, eta1, dz);
+    hipLaunchKernelGGL(k_dz_it, dim3(grid_for(nt)), dim3(kThreads), 0, s, g, dz, dzx, dzy, dzz);
 }
 
 void launch_it_posdin(const Geo &g, double dx, double dy, double dt, double tetah, double gz, const double *prd1,

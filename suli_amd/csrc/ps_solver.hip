@@ -73,6 +73,7 @@ struct Staging {
     double *obsx = nullptr, *obsy = nullptr, *obsz = nullptr, *dz = nullptr, *dzz = nullptr;
     double *hs = nullptr;
     double *dzx = nullptr, *dzy = nullptr, *dz3 = nullptr, *eta1 = nullptr, *deltah = nullptr;   // SuLi-IT
+    double *box2 = nullptr, *eta0 = nullptr;   // a second halo box (prd0, SuLi-IT's dz) and eta0
     // What the last suli_ps_graddin left on the device (opts.reuse_staged):
     // its host pointers and grid, with u, v, w, rho as uploaded and prd1 as
     // solved.  Cleared by every other host-pointer call.
@@ -355,6 +356,42 @@ int run_it_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double tetah, do
     return 0;
 }
 
+int run_it_posdin_layers_dev(suli_ps_ctx *ctx, const suli_ps_grid *gr, double tetah, double gz, double *u,
+                             double *v, double *w, double *dz, double *dzx, double *dzy, double *dzz,
+                             const double *eta0, double *eta1, const double *prd1, double *prd)
+{
+    int rc = check_it(ctx, gr, tetah, gz);
+    if (rc) return rc;
+    if (!u || !v || !w || !dz || !dzx || !dzy || !dzz || !eta0 || !eta1 || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, dz, dzx, dzy, dzz, eta0, eta1 and prd1 must be non-NULL");
+    const Geo g = make_geo(gr->nx, gr->ny, gr->nz, gr->ccx0 == 0, gr->ccy0 == 0);
+    // the surface and its layer, dz_it, then the correction with the new
+    // dzz: the reference's order (SuLi-IT/10_pos.f90:84-121)
+    launch_it_layers(g, gz, prd1, eta0, eta1, dz, dzx, dzy, dzz, ctx->stream);
+    launch_it_posdin(g, gr->dx, gr->dy, gr->dt, tetah, gz, prd1, dzz, nullptr, u, v, w, ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    if (prd && prd != prd1) {
+        const size_t nbox = (size_t)(gr->nx + 2) * (gr->ny + 2) * (gr->nz + 2);
+        HIPCHK(ctx, hipMemcpyAsync(prd, prd1, nbox * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int check_exp_press(suli_ps_ctx *ctx, const suli_ps_grid *gr, const double *rho, const double *prd1,
+                    const double *prd0, const double *dz, const double *dzz, const double *px, const double *py,
+                    const double *pz)
+{
+    std::string e;
+    const int rc = validate_shape(gr, e);
+    if (rc) return fail(ctx, rc, "%s", e.c_str());
+    if (!rho || !prd1 || !prd0 || !px || !py || !pz)
+        return fail(ctx, SULI_PS_EINVAL, "rho, prd1, prd0, px, py and pz must be non-NULL");
+    if ((dz == nullptr) != (dzz == nullptr)) return fail(ctx, SULI_PS_EINVAL, "dz and dzz go together");
+    if (!dz && !(gr->dz > 0)) return fail(ctx, SULI_PS_EINVAL, "dz must be positive");
+    return 0;
+}
+
 // Staging for the host-pointer API, kept while the grid shape is unchanged.
 int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
 {
@@ -368,20 +405,21 @@ int ensure_staging(suli_ps_ctx *ctx, int nx, int ny, int nz)
     const size_t n = (size_t)nx * ny * nz;
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
-    const size_t sizes[16] = {nu, nv, nw, n, (size_t)(nx + 2) * (ny + 2) * (nz + 2),
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2), ne = (size_t)(nx + 2) * (ny + 2);
+    const size_t sizes[18] = {nu, nv, nw, n, nbox,
                               (size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
-                              (size_t)nz + 2, (size_t)nz + 2, n, nu, nv, nw, (size_t)(nx + 2) * (ny + 2),
-                              (size_t)nx * ny};
-    size_t off[16], total = 0;
-    for (int i = 0; i < 16; ++i) {
+                              (size_t)nz + 2, (size_t)nz + 2, n, nu, nv, nw, ne,
+                              (size_t)nx * ny, nbox, ne};
+    size_t off[18], total = 0;
+    for (int i = 0; i < 18; ++i) {
         off[i] = total;
         total += (sizes[i] * sizeof(double) + 255) / 256 * 256;
     }
     hipError_t e = hipMalloc(&st.block, total);
     if (e != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
-    double **ptrs[16] = {&st.u,   &st.v,   &st.w,   &st.rho, &st.prd1, &st.obsx,   &st.obsy,   &st.obsz,
-                         &st.dz,  &st.dzz, &st.hs,  &st.dzx, &st.dzy,  &st.dz3, &st.eta1, &st.deltah};
-    for (int i = 0; i < 16; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
+    double **ptrs[18] = {&st.u,   &st.v,   &st.w,   &st.rho, &st.prd1, &st.obsx,   &st.obsy, &st.obsz, &st.dz,
+                         &st.dzz, &st.hs,  &st.dzx, &st.dzy, &st.dz3,  &st.eta1, &st.deltah, &st.box2, &st.eta0};
+    for (int i = 0; i < 18; ++i) *ptrs[i] = reinterpret_cast<double *>(static_cast<char *>(st.block) + off[i]);
     st.nx = nx;
     st.ny = ny;
     st.nz = nz;
@@ -734,6 +772,115 @@ int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double 
                      ctx->stream);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int suli_ps_heaviside_tens_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *ls, double eps,
+                               double rho_f2, double mu_f1, double mu_f2, double *hs, double *rho, double *mu,
+                               double *hsx, double *hsy, double *hsz)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    std::string e;
+    const int rc = validate_shape(g, e);
+    if (rc) return fail(ctx, rc, "%s", e.c_str());
+    if (g->nx < 4 || g->ny < 4 || g->nz < 4)
+        return fail(ctx, SULI_PS_EINVAL, "grid %dx%dx%d: the WENO5 stencils need 4 cells in every direction",
+                    g->nx, g->ny, g->nz);
+    if (!(g->dz > 0)) return fail(ctx, SULI_PS_EINVAL, "dz must be positive (a uniform grid)");
+    if (!ls || !rho || !hsx || !hsy || !hsz) return fail(ctx, SULI_PS_EINVAL, "ls, rho, hsx, hsy and hsz must be non-NULL");
+    if (!(eps > 0)) return fail(ctx, SULI_PS_EINVAL, "eps = alpha1*dx1 must be positive");
+    const Geo geo = make_geo(g->nx, g->ny, g->nz, g->ccx0 == 0, g->ccy0 == 0);
+    launch_heaviside_tens(geo, ls, eps, g->rho_f1, rho_f2, mu_f1, mu_f2, g->dx, g->dy, g->dz, hs, rho, mu, hsx, hsy,
+                          hsz, ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int suli_ps_exp_press_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, const double *prd1,
+                          double *prd0, const double *dz, const double *dzz, double *px, double *py, double *pz)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    const int rc = check_exp_press(ctx, g, rho, prd1, prd0, dz, dzz, px, py, pz);
+    if (rc) return rc;
+    const Geo geo = make_geo(g->nx, g->ny, g->nz, g->ccx0 == 0, g->ccy0 == 0);
+    launch_exp_press(geo, rho, dz, dzz, g->dx, g->dy, g->dz, prd1, prd0, px, py, pz, ctx->stream);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int suli_ps_exp_press(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, const double *prd1, double *prd0,
+                      const double *dz, const double *dzz, double *px, double *py, double *pz)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_exp_press(ctx, g, rho, prd1, prd0, dz, dzz, px, py, pz);
+    if (rc) return rc;
+    ctx->st.left.valid = false;
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t n = (size_t)nx * ny * nz, nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.box2, prd0, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    if (dz) {
+        HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+    }
+    if ((rc = suli_ps_exp_press_dev(ctx, g, st.rho, st.prd1, st.box2, dz ? st.dz : nullptr, dz ? st.dzz : nullptr,
+                                    st.obsx, st.obsy, st.obsz)))
+        return rc;
+    HIPCHK(ctx, hipMemcpyAsync(prd0, st.box2, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(px, st.obsx, sizeof(double) * nfx, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(py, st.obsy, sizeof(double) * nfy, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(pz, st.obsz, sizeof(double) * nfz, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return 0;
+}
+
+int suli_ps_it_posdin_layers_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, double *u,
+                                 double *v, double *w, double *dz, double *dzx, double *dzy, double *dzz,
+                                 const double *eta0, double *eta1, const double *prd1, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    return run_it_posdin_layers_dev(ctx, g, tetah, gz, u, v, w, dz, dzx, dzy, dzz, eta0, eta1, prd1, prd);
+}
+
+int suli_ps_it_posdin_layers(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, double *u, double *v,
+                             double *w, double *dz, double *dzx, double *dzy, double *dzz, const double *eta0,
+                             double *eta1, const double *prd1, double *prd)
+{
+    if (!ctx) return SULI_PS_EINVAL;
+    int rc = check_it(ctx, g, tetah, gz);
+    if (rc) return rc;
+    if (!u || !v || !w || !dz || !dzx || !dzy || !dzz || !eta0 || !eta1 || !prd1)
+        return fail(ctx, SULI_PS_EINVAL, "u, v, w, dz, dzx, dzy, dzz, eta0, eta1 and prd1 must be non-NULL");
+    ctx->st.left.valid = false;
+    if ((rc = ensure_staging(ctx, g->nx, g->ny, g->nz))) return rc;
+    Staging &st = ctx->st;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                 nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3), ne = (size_t)(nx + 2) * (ny + 2);
+    hipStream_t s = ctx->stream;
+    struct Io {
+        double *dev;
+        double *host;
+        size_t n;
+    } io[] = {{st.u, u, nu},     {st.v, v, nv},     {st.w, w, nw},     {st.box2, dz, nbox},
+              {st.dzx, dzx, nu}, {st.dzy, dzy, nv}, {st.dz3, dzz, nw}, {st.eta1, eta1, ne}};
+    for (const Io &a : io) HIPCHK(ctx, hipMemcpyAsync(a.dev, a.host, sizeof(double) * a.n, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.eta0, eta0, sizeof(double) * ne, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    if ((rc = run_it_posdin_layers_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.box2, st.dzx, st.dzy, st.dz3,
+                                       st.eta0, st.eta1, st.prd1, nullptr)))
+        return rc;
+    for (const Io &a : io) HIPCHK(ctx, hipMemcpyAsync(a.host, a.dev, sizeof(double) * a.n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);
     return 0;
 }
 

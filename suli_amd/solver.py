@@ -263,6 +263,32 @@ class PressureSolver:
         self._check(self.lib.suli_ps_heaviside_dev(self.ctx, C.byref(cg), ls_ptr, eps, rho_f2, mu_f1, mu_f2,
                                                    hs_ptr, rho_ptr, mu_ptr))
 
+    def heaviside_tens_dev(self, grid: Grid, ls_ptr, eps: float, rho_f2: float, mu_f1: float, mu_f2: float,
+                           rho_ptr, hsx_ptr, hsy_ptr, hsz_ptr, hs_ptr=None, mu_ptr=None) -> None:
+        """heaviside with t_tens = 1: rho (hs, mu) and the surface-tension
+        gradients hsx, hsy, hsz from a device-resident level set."""
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_heaviside_tens_dev(self.ctx, C.byref(cg), ls_ptr, eps, rho_f2, mu_f1, mu_f2,
+                                                        hs_ptr, rho_ptr, mu_ptr, hsx_ptr, hsy_ptr, hsz_ptr))
+
+    def exp_press(self, grid: Grid, rho: np.ndarray, prd1: np.ndarray, prd0: np.ndarray):
+        """exp_press (IPA) on host arrays: prd0 incremented in place; returns
+        (px, py, pz)."""
+        cg = _c_grid(grid)
+        px = np.zeros((grid.nx + 1, grid.ny, grid.nz), order="F")
+        py = np.zeros((grid.nx, grid.ny + 1, grid.nz), order="F")
+        pz = np.zeros((grid.nx, grid.ny, grid.nz + 1), order="F")
+        self._check(self.lib.suli_ps_exp_press(self.ctx, C.byref(cg), _host_ptr(rho), _host_ptr(prd1),
+                                               _host_ptr(prd0), _host_ptr(grid.dzv), _host_ptr(grid.dzzv),
+                                               _host_ptr(px), _host_ptr(py), _host_ptr(pz)))
+        return px, py, pz
+
+    def exp_press_dev(self, grid: Grid, rho_ptr, prd1_ptr, prd0_ptr, px_ptr, py_ptr, pz_ptr, dz_ptr=None,
+                      dzz_ptr=None) -> None:
+        cg = _c_grid(grid)
+        self._check(self.lib.suli_ps_exp_press_dev(self.ctx, C.byref(cg), rho_ptr, prd1_ptr, prd0_ptr, dz_ptr,
+                                                   dzz_ptr, px_ptr, py_ptr, pz_ptr))
+
     def dev_array(self, arr: np.ndarray):
         """Copy a float64 array to a new device allocation; returns the pointer
         (free with dev_free)."""
@@ -312,6 +338,24 @@ class PressureSolver:
             self.ctx, C.byref(cg), g.tetah, g.gz, _host_ptr(f.u), _host_ptr(f.v), _host_ptr(f.w),
             _host_ptr(f.dzz), _host_ptr(f.eta1), _host_ptr(f.prd1), _host_ptr(prd)))
         return prd
+
+    def it_posdin_layers(self, g, f, dz: np.ndarray, eta0: np.ndarray) -> np.ndarray:
+        """SuLi-IT's whole posdin (surface, layer thicknesses, correction) on
+        host arrays: f.u, v, w, dzx, dzy, dzz, eta1 and dz updated in place;
+        returns prd."""
+        cg = self._it_cgrid(g)
+        prd = np.zeros_like(f.prd1, order="F")
+        self._check(self.lib.suli_ps_it_posdin_layers(
+            self.ctx, C.byref(cg), g.tetah, g.gz, _host_ptr(f.u), _host_ptr(f.v), _host_ptr(f.w), _host_ptr(dz),
+            _host_ptr(f.dzx), _host_ptr(f.dzy), _host_ptr(f.dzz), _host_ptr(eta0), _host_ptr(f.eta1),
+            _host_ptr(f.prd1), _host_ptr(prd)))
+        return prd
+
+    def it_posdin_layers_dev(self, g, d: dict, dz_ptr, eta0_ptr) -> None:
+        cg = self._it_cgrid(g)
+        self._check(self.lib.suli_ps_it_posdin_layers_dev(
+            self.ctx, C.byref(cg), g.tetah, g.gz, d["u"], d["v"], d["w"], dz_ptr, d["dzx"], d["dzy"], d["dzz"],
+            eta0_ptr, d["eta1"], d["prd1"], None))
 
     def it_upload(self, f) -> dict:
         """Device copies of ITFields (name -> pointer); free with it_free."""

@@ -59,3 +59,15 @@ def weighted_mean_free(p, d):
     """p minus its d-weighted interior mean (the all-Neumann null space)."""
     pi = p[1:-1, 1:-1, 1:-1]
     return p - (d * pi).sum() / d.sum()
+
+
+def it_layer_state(g, f, seed):
+    """SuLi-IT state for posdin's layer update: cell thicknesses dz
+    (0:nx+1,0:ny+1,0:nz+1) about the layers' own, eta0 (0:nx+1,0:ny+1) and
+    a pressure of the size a solve leaves (the halo box)."""
+    rng = np.random.default_rng(seed)
+    dz0 = float(np.median(f.dzz))
+    dz = np.asfortranarray(dz0 * (1.0 + 0.1 * rng.uniform(-1, 1, (g.nx + 2, g.ny + 2, g.nz + 2))))
+    eta0 = np.asfortranarray(rng.uniform(-0.01, 0.01, (g.nx + 2, g.ny + 2)))
+    prd1 = np.asfortranarray(rng.uniform(-0.05, 0.05, (g.nx + 2, g.ny + 2, g.nz + 2)))
+    return dz, eta0, prd1

@@ -99,3 +99,37 @@ def test_tables_with_wide_lanes_and_spmv(solver):
     p_or, _, _ = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12, project_null=True)
     for p in out:
         assert helpers.rel_diff(p, p_or, helpers.face_mask(p.shape)) <= 1e-8
+
+
+@pytest.mark.parametrize("name", [n for n, _ in CASES if n != "dirichlet_lid"])
+def test_exp_press_vs_oracle(solver, name):
+    """exp_press (SuLi-IC/11_diversos.f90:455-495) on the device against the
+    oracle, itself bit-exact with the compiled reference exp_press
+    (tests/test_oracle_exp_press.py): every operation is IEEE basic
+    arithmetic in the reference's order, so bit for bit; host and device
+    entry points."""
+    c = make(name)
+    g = c.grid
+    rng = np.random.default_rng(90)
+    box = (g.nx + 2, g.ny + 2, g.nz + 2)
+    prd1 = np.asfortranarray(rng.uniform(-50.0, 50.0, box))
+    prd0 = np.asfortranarray(rng.uniform(-1e4, 1e4, box))
+    rho = c.fields.rho
+    e0, epx, epy, epz = oracle.exp_press(g, rho, prd1, prd0)
+    p0 = prd0.copy(order="F")
+    px, py, pz = solver.exp_press(g, rho, prd1, p0)
+    for a, b in ((p0, e0), (px, epx), (py, epy), (pz, epz)):
+        np.testing.assert_array_equal(a, b)
+    ptr = {k: solver.dev_array(a) for k, a in dict(rho=rho, prd1=prd1, prd0=prd0, px=np.zeros_like(px),
+                                                    py=np.zeros_like(py), pz=np.zeros_like(pz)).items()}
+    if g.stretched:
+        ptr["dz"], ptr["dzz"] = solver.dev_array(g.dzv), solver.dev_array(g.dzzv)
+    try:
+        solver.exp_press_dev(g, ptr["rho"], ptr["prd1"], ptr["prd0"], ptr["px"], ptr["py"], ptr["pz"],
+                             ptr.get("dz"), ptr.get("dzz"))
+        np.testing.assert_array_equal(solver.dev_download(ptr["prd0"], box), e0)
+        np.testing.assert_array_equal(solver.dev_download(ptr["px"], px.shape), epx)
+        np.testing.assert_array_equal(solver.dev_download(ptr["pz"], pz.shape), epz)
+    finally:
+        for p in ptr.values():
+            solver.dev_free(p)

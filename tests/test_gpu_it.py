@@ -135,3 +135,35 @@ def test_it_posdin_without_eta(solver):
     np.testing.assert_array_equal(f2.u, u)
     np.testing.assert_array_equal(f2.w, w)
     np.testing.assert_array_equal(f2.eta1, f.eta1)
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_it_posdin_layers_bitwise(solver, name, make):
+    """The whole SuLi-IT posdin on the device (surface, top layer, dz_it,
+    correction) against the oracle, itself bit-exact with the compiled
+    SuLi-IT posdin (tests/test_oracle_it.py): bit for bit, through the
+    device and the host entry points."""
+    g, f = make()
+    dz, eta0, prd1 = helpers.it_layer_state(g, f, 40)
+    f.prd1[...] = prd1
+    exp = oracle.it_posdin_dz(g, f, prd1, dz, eta0)
+    names = ("u", "v", "w", "dz", "dzx", "dzy", "dzz", "eta1", "prd")
+    d = solver.it_upload(f)
+    p_dz, p_e0 = solver.dev_array(dz), solver.dev_array(eta0)
+    try:
+        solver.it_posdin_layers_dev(g, d, p_dz, p_e0)
+        got = {k: solver.dev_download(d[k], getattr(f, k).shape) for k in ("u", "v", "w", "dzx", "dzy", "dzz",
+                                                                            "eta1")}
+        got["dz"] = solver.dev_download(p_dz, dz.shape)
+        for k, e in zip(names, exp):
+            if k in got:
+                np.testing.assert_array_equal(got[k], e, err_msg=k)
+    finally:
+        solver.it_free(d)
+        solver.dev_free(p_dz)
+        solver.dev_free(p_e0)
+    f2, dz2 = f.copy(), dz.copy(order="F")
+    prd = solver.it_posdin_layers(g, f2, dz2, eta0)
+    host = dict(u=f2.u, v=f2.v, w=f2.w, dz=dz2, dzx=f2.dzx, dzy=f2.dzy, dzz=f2.dzz, eta1=f2.eta1, prd=prd)
+    for k, e in zip(names, exp):
+        np.testing.assert_array_equal(host[k], e, err_msg=k)

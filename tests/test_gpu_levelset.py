@@ -102,3 +102,43 @@ def test_graddin_from_level_set_full_size(solver):
     inner = (slice(1, -1),) * 3
     scale = np.abs(p_host[inner]).max()
     assert np.abs(p_dev[inner] - p_host[inner]).max() <= 1e-7 * scale
+
+
+@pytest.mark.parametrize("ccx0,ccy0", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_heaviside_tens_dev_vs_oracle(solver, ccx0, ccy0):
+    """t_tens = 1: the surface-tension gradients hsx, hsy, hsz against the
+    oracle, itself bit-exact with the compiled reference heaviside
+    (tests/test_oracle_levelset.py).  The WENO5 derivatives are IEEE basic
+    arithmetic; hsx = 0.5 g (1 + cos(pi ls/eps))/eps also goes through
+    cos, whose device result may differ in the last place: each gradient
+    is within 0.5 |g| (3e-16)/eps + 4e-16 |hsx| of the oracle, outside the
+    band exactly 0, and most values are bit-exact."""
+    from tests.test_oracle_levelset import wavy_level_set
+    g = cases.Grid(40, 24, 30, 5e-3, 4e-3, 3e-3, ccx0=ccx0, ccy0=ccy0, rho_f1=1.2)
+    eps = 1.5 * max(g.dx, g.dy, g.dz)
+    ls = wavy_level_set(g, 60 + ccx0 + 2 * ccy0)
+    shape = (g.nx, g.ny, g.nz)
+    exp = oracle.heaviside_tens(g, ls, eps, 1.2, 1000.0, 1.8e-5, 1.0e-3)
+    grads = oracle.ls_gradient(g, ls)
+    p_ls = solver.dev_array(ls)
+    outs = [solver.dev_array(np.zeros(shape)) for _ in range(6)]
+    try:
+        solver.heaviside_tens_dev(g, p_ls, eps, 1000.0, 1.8e-5, 1.0e-3, outs[1], outs[3], outs[4], outs[5],
+                                  hs_ptr=outs[0], mu_ptr=outs[2])
+        got = [solver.dev_download(p, shape) for p in outs]
+    finally:
+        for p in [p_ls] + outs:
+            solver.dev_free(p)
+    hs, rho, mu = got[:3]
+    assert np.abs(hs - exp[0]).max() <= 4.5e-16
+    np.testing.assert_allclose(rho, exp[1], rtol=1e-15, atol=(1000.0 - 1.2) * 4.5e-16)
+    band = np.abs(ls) <= eps
+    assert band.mean() > 0.05
+    for k in range(3):
+        a, b, gr = got[3 + k], exp[3 + k], grads[k]
+        assert np.all(a[~band] == 0.0)
+        bound = 0.5 * np.abs(gr) * 3e-16 / eps + 4e-16 * np.abs(b)
+        assert np.all(np.abs(a - b) <= bound), np.max(np.abs(a - b) / np.maximum(bound, 1e-300))
+        exact = np.mean(a[band] == b[band])
+        print(f"hs{'xyz'[k]} bit-exact fraction in the band {exact:.4f}")
+        assert exact > 0.5

@@ -12,6 +12,7 @@ import pytest
 
 import oracle
 from suli_amd import cases
+from tests import helpers
 
 needs_ref = pytest.mark.skipif(not oracle.ref_it_available(), reason="reference not built")
 
@@ -114,3 +115,71 @@ def test_posdin_restores_continuity():
     np.testing.assert_array_equal(prd, p)
     # eta1 moved by p(nz)/gz on its whole box
     np.testing.assert_allclose(eta1 - f.eta1, p[:, :, g.nz] / g.gz, rtol=0, atol=1e-15)
+
+
+# ---- posdin with its layer update (SuLi-IT/10_pos.f90:84-93 and dz_it,
+# SuLi-IT/11_diversos.f90:80-141) ---------------------------------------------
+
+@needs_ref
+def test_posdin_with_layers_matches_reference():
+    g, f = ref_case(8)
+    dz, eta0, prd1 = helpers.it_layer_state(g, f, 8)
+    got = oracle.it_posdin_dz(g, f, prd1, dz, eta0)
+    exp = oracle.ref_it_posdin(f, prd1, dz, eta0)
+    for name, a, b in zip(("u", "v", "w", "dz", "dzx", "dzy", "dzz", "eta1", "prd"), got, exp):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+@pytest.mark.parametrize("px,py", [(False, False), (True, True)])
+def test_layer_update_properties(px, py):
+    """The surface layer takes the surface's rise; the face thicknesses of
+    the top planes are means of their cells; periodic ends wrap (for a dz
+    whose ghosts are the wrapped cells), walls extrapolate linearly."""
+    g, f = cases.it_case(9, 7, 5, seed=13, periodic_x=px, periodic_y=py)
+    dz, eta0, prd1 = helpers.it_layer_state(g, f, 13)
+    nx, ny, nz = g.nx, g.ny, g.nz
+    if px:
+        dz[0], dz[nx + 1] = dz[nx], dz[1]
+        eta0[0], eta0[nx + 1] = eta0[nx], eta0[1]
+        prd1[0], prd1[nx + 1] = prd1[nx], prd1[1]
+        f.eta1[0], f.eta1[nx + 1] = f.eta1[nx], f.eta1[1]
+    if py:
+        dz[:, 0], dz[:, ny + 1] = dz[:, ny], dz[:, 1]
+        eta0[:, 0], eta0[:, ny + 1] = eta0[:, ny], eta0[:, 1]
+        prd1[:, 0], prd1[:, ny + 1] = prd1[:, ny], prd1[:, 1]
+        f.eta1[:, 0], f.eta1[:, ny + 1] = f.eta1[:, ny], f.eta1[:, 1]
+    u, v, w, dz1, dzx, dzy, dzz, eta1, prd = oracle.it_posdin_dz(g, f, prd1, dz, eta0)
+    np.testing.assert_allclose(eta1 - f.eta1, prd1[:, :, nz] / g.gz, atol=1e-15)
+    np.testing.assert_allclose(dz1[:, :, nz] - dz[:, :, nz], eta1 - eta0, atol=1e-15)
+    untouched = [k for k in range(nz + 2) if k != nz]
+    np.testing.assert_array_equal(dz1[:, :, untouched], dz[:, :, untouched])
+    for k in (nz, nz + 1):
+        np.testing.assert_allclose(dzx[1:nx + 2, :, k], (dz1[1:, :, k] + dz1[:-1, :, k]) / 2, rtol=1e-15)
+        np.testing.assert_allclose(dzy[:, 1:ny + 2, k], (dz1[:, 1:, k] + dz1[:, :-1, k]) / 2, rtol=1e-15)
+        if px:
+            # the end faces pair the ghost with the opposite end cell: with
+            # ghosts that wrap, (dz(0) + dz(nx))/2 = dz(nx), and likewise
+            # (dz(1) + dz(nx+1))/2 = dz(1) (the compiled reference fixes
+            # walls, so this branch is pinned by this property alone)
+            np.testing.assert_array_equal(dzx[0, :, k], dz1[nx, :, k])
+            np.testing.assert_array_equal(dzx[nx + 2, :, k], dz1[1, :, k])
+        else:
+            np.testing.assert_allclose(dzx[0, :, k], 2 * dzx[1, :, k] - dzx[2, :, k], rtol=1e-14)
+            np.testing.assert_allclose(dzx[nx + 2, :, k], 2 * dzx[nx + 1, :, k] - dzx[nx, :, k], rtol=1e-14)
+        if py:
+            np.testing.assert_array_equal(dzy[:, 0, k], dz1[:, ny, k])
+            np.testing.assert_array_equal(dzy[:, ny + 2, k], dz1[:, 1, k])
+        else:
+            np.testing.assert_allclose(dzy[:, 0, k], 2 * dzy[:, 1, k] - dzy[:, 2, k], rtol=1e-14)
+            np.testing.assert_allclose(dzy[:, ny + 2, k], 2 * dzy[:, ny + 1, k] - dzy[:, ny, k], rtol=1e-14)
+    for k in (nz, nz + 1):
+        np.testing.assert_allclose(dzz[:, :, k], (dz1[:, :, k] + dz1[:, :, k - 1]) / 2, rtol=1e-15)
+    np.testing.assert_allclose(dzz[:, :, nz + 2], 2 * dzz[:, :, nz + 1] - dzz[:, :, nz], rtol=1e-14)
+    # below the top planes the face thicknesses are the caller's
+    np.testing.assert_array_equal(dzz[:, :, :nz], f.dzz[:, :, :nz])
+    # the correction is oracle_it_posdin's with the new dzz
+    f2 = f.copy()
+    f2.dzz = dzz
+    u2, v2, w2, _, _ = oracle.it_posdin(g, f2, prd1)
+    for a, b in ((u, u2), (v, v2), (w, w2)):
+        np.testing.assert_array_equal(a, b)
