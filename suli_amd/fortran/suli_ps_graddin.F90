@@ -1,7 +1,10 @@
-! Drop-in graddin(), pressh() and posdin() for SuLi on the MI355X library.
+! Drop-in graddin(), pressh(), posdin() and exp_press() for SuLi on the
+! MI355X library.
 !
-! Replaces SuLi-IC/9_grad.f90 (graddin, :10-663; pressh, :667-702) and
-! SuLi-IC/10_pos.f90 (posdin, :1-99) in a SuLi build: same names, no
+! Replaces SuLi-IC/9_grad.f90 (graddin, :10-663; pressh, :667-702),
+! SuLi-IC/10_pos.f90 (posdin, :1-99) and SuLi-IC/11_diversos.f90
+! (exp_press, :455-495: velpre's prd0, px, py, pz) in a SuLi build (the
+! reference's exp_press is then taken out of 11_diversos.f90): same names, no
 ! arguments, same module variables read and written (velpre: u, v, w, rho,
 ! prd1, prd; disc: nx..dt, t_press, obst_t, ibm_t; cond: ccx0, ccy0; obst:
 ! obs_ls*; ls_param: rho_f1, hs; param: gz).  Compile with -DSULI_ICDZ for SuLi-ICdz, whose disc module holds
@@ -160,3 +163,23 @@ SUBROUTINE pressh()
 #endif
    call suli_ps_dropin_check(rc, "pressh")
 END SUBROUTINE pressh
+
+
+SUBROUTINE exp_press()
+   use iso_c_binding
+   use velpre
+   use suli_ps_iface
+   use suli_ps_dropin_state
+   implicit none
+   type(suli_ps_grid) :: g
+   integer(c_int) :: rc
+
+   call suli_ps_dropin_init()
+   call suli_ps_dropin_grid(g)
+#ifdef SULI_ICDZ
+   rc = suli_ps_exp_press(suli_ps_ctx, g, rho, prd1, prd0, dz, dzz, px, py, pz)
+#else
+   rc = suli_ps_exp_press(suli_ps_ctx, g, rho, prd1, prd0, px=px, py=py, pz=pz)
+#endif
+   call suli_ps_dropin_check(rc, "exp_press")
+END SUBROUTINE exp_press

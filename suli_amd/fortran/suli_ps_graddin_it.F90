@@ -1,5 +1,5 @@
 ! Drop-in graddin() for SuLi-IT on the MI355X library (posdin() is in
-! suli_ps_posdin_it.F90, which needs SuLi-IT's dz_it()).
+! suli_ps_posdin_it.F90).
 !
 ! Replaces SuLi-IT/9_grad.f90 (graddin, :10-263) and SuLi-IT/10_pos.f90
 ! (posdin, :73-132) in a SuLi-IT build: same names, no arguments, the same

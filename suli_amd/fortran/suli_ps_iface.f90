@@ -46,6 +46,7 @@ module suli_ps_iface
 
    public :: suli_ps_create, suli_ps_destroy, suli_ps_graddin, suli_ps_posdin, suli_ps_last_error
    public :: suli_ps_pressh, suli_ps_posdin_hydro, suli_ps_it_graddin, suli_ps_it_posdin
+   public :: suli_ps_it_posdin_layers, suli_ps_exp_press
    public :: suli_ps_error_message
 
    interface
@@ -141,6 +142,30 @@ module suli_ps_iface
          real(c_double), intent(inout), optional :: eta1(*), prd(*)
          integer(c_int) :: rc
       end function suli_ps_it_posdin
+
+      function suli_ps_it_posdin_layers(ctx, g, tetah, gz, u, v, w, dz, dzx, dzy, dzz, eta0, eta1, prd1, prd) &
+         bind(C, name="suli_ps_it_posdin_layers") result(rc)
+         import :: c_ptr, c_double, c_int, suli_ps_grid
+         type(c_ptr), value :: ctx
+         type(suli_ps_grid), intent(in) :: g
+         real(c_double), value :: tetah, gz
+         real(c_double), intent(inout) :: u(*), v(*), w(*), dz(*), dzx(*), dzy(*), dzz(*), eta1(*)
+         real(c_double), intent(in) :: eta0(*), prd1(*)
+         real(c_double), intent(inout), optional :: prd(*)
+         integer(c_int) :: rc
+      end function suli_ps_it_posdin_layers
+
+      function suli_ps_exp_press(ctx, g, rho, prd1, prd0, dz, dzz, px, py, pz) &
+         bind(C, name="suli_ps_exp_press") result(rc)
+         import :: c_ptr, c_double, c_int, suli_ps_grid
+         type(c_ptr), value :: ctx
+         type(suli_ps_grid), intent(in) :: g
+         real(c_double), intent(in) :: rho(*), prd1(*)
+         real(c_double), intent(inout) :: prd0(*)
+         real(c_double), intent(in), optional :: dz(*), dzz(*)
+         real(c_double), intent(out) :: px(*), py(*), pz(*)
+         integer(c_int) :: rc
+      end function suli_ps_exp_press
    end interface
 
 contains

@@ -1,9 +1,12 @@
 ! Drop-in posdin() for SuLi-IT (SuLi-IT/10_pos.f90:73-132) on the MI355X
-! library; the context and helpers are suli_ps_graddin_it.F90's.  The free
-! surface and its layer (eta1, dz(:,:,nz), dz_it) stay on the host, as
-! SuLi-IT keeps them; the velocity correction runs on the GPU.
+! library; the context and helpers are suli_ps_graddin_it.F90's.  All of
+! it runs on the GPU in the reference's order: the free surface and its
+! layer (eta1, dz(:,:,nz)), the top planes' face thicknesses (dz_it,
+! SuLi-IT/11_diversos.f90:80-141), then the velocity correction with the
+! new dzz.
 SUBROUTINE posdin()
    use iso_c_binding
+   use disc
    use dzs
    use velpre
    use parametros
@@ -12,19 +15,9 @@ SUBROUTINE posdin()
    implicit none
    type(suli_ps_grid) :: g
    integer(c_int) :: rc
-   integer :: i, j
 
    call suli_ps_it_init()
    call suli_ps_it_grid(g)
-   ! the free surface and its layer first (SuLi-IT/10_pos.f90:84-93), then
-   ! the correction with the updated dzz, as the reference orders them
-   do j = 0, ny + 1
-      do i = 0, nx + 1
-         eta1(i, j) = eta1(i, j) + prd1(i, j, nz) / gz
-         dz(i, j, nz) = dz(i, j, nz) + (eta1(i, j) - eta0(i, j))
-      end do
-   end do
-   call dz_it()
-   rc = suli_ps_it_posdin(suli_ps_it_ctx, g, tetah, gz, u, v, w, dzz, prd1=prd1, prd=prd)
+   rc = suli_ps_it_posdin_layers(suli_ps_it_ctx, g, tetah, gz, u, v, w, dz, dzx, dzy, dzz, eta0, eta1, prd1, prd)
    call suli_ps_it_check(rc, "posdin")
 END SUBROUTINE posdin

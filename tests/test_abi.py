@@ -120,5 +120,6 @@ def test_fortran_types_match_header(tmp_path):
 def test_fortran_shim_declares_same_entry_points():
     shim = open(os.path.join(ROOT, "suli_amd", "fortran", "suli_ps_iface.f90")).read()
     for name in ("suli_ps_create", "suli_ps_graddin", "suli_ps_posdin", "suli_ps_last_error",
-                 "suli_ps_pressh", "suli_ps_posdin_hydro", "suli_ps_it_graddin", "suli_ps_it_posdin"):
+                 "suli_ps_pressh", "suli_ps_posdin_hydro", "suli_ps_it_graddin", "suli_ps_it_posdin",
+                 "suli_ps_it_posdin_layers", "suli_ps_exp_press"):
         assert f'name="{name}"' in shim, name

@@ -53,10 +53,34 @@ def test_fortran_dropin_hydrostatic_matches_reference(variant, args):
     assert re.search(r"ok =\s*T", r.stdout)
 
 
+@pytest.mark.parametrize("variant,args", [
+    ("ic", ["24", "16", "12", "0", "0", "2"]),
+    ("ic", ["33", "20", "15", "1", "0", "2"]),
+    ("icdz", ["24", "16", "13", "0", "0", "2"]),
+])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fortran_dropin_ipa_matches_reference(variant, args, precision):
+    """t_press = 2: the drop-in exp_press() (SuLi-IC/11_diversos.f90:455-495)
+    then graddin() against the reference's: prd0, px, py, pz to 1e-15
+    relative, the pressure to 1e-8."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{variant}")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in check program not built (needs the reference sources)")
+    env = dict(os.environ, SULI_PS_PRECISION=precision)
+    if precision == "mixed":
+        env["SULI_PS_TOL"] = "1e-11"
+    print(f"dropin ipa {variant} {args} {precision}", flush=True)
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"ok =\s*T", r.stdout)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_fortran_dropin_it_matches_reference(precision):
-    """SuLi-IT: the drop-in graddin() (suli_ps_graddin_it.F90) inside a
-    program on SuLi-IT's own modules, against the reference's graddin."""
+    """SuLi-IT: the drop-in graddin() (suli_ps_graddin_it.F90) and posdin()
+    (suli_ps_posdin_it.F90: surface, top layer and dz_it on the GPU) inside
+    a program on SuLi-IT's own modules, against the reference's."""
     exe = os.path.join(ROOT, "oracle", "_ref", "dropin_it")
     if not os.path.exists(exe):
         pytest.skip("drop-in check program not built (needs the reference sources)")
