@@ -15,9 +15,12 @@
 // byxf and takes byxf's count as what is left over.
 //
 // List-directed input rules followed here: values separated by blanks,
-// commas or line ends; "r*c" stands for r copies of c and "r*" for r null
-// values (the target keeps its value); a slash ends the record; reals may
-// use E, D or Q exponents or a bare signed exponent ("1.5-100").
+// line ends or one comma (with blanks around it); a null value (the target
+// keeps its value) is "r*" (r of them), nothing between two commas, or a
+// comma before the record's first value; "r*c" stands for r copies of c; a
+// slash ends the record; reals may use E, D or Q exponents or a bare signed
+// exponent ("1.5-100").  A value longer than the scanner's 127 characters
+// is an error, not cut short.
 #include <cerrno>
 #include <cstdarg>
 #include <cstdio>
@@ -78,17 +81,31 @@ public:
         char tok[128];
         int len = 0;
         int c;
-        // skip separators (one comma between values; blanks and line ends free)
-        while ((c = get()) != EOF && (c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == ',')) {
+        // skip the separator: blanks and line ends, and at most one comma; a
+        // comma that finds one already taken (or the record's start) closes
+        // a null value
+        for (;;) {
+            c = get();
+            if (c == ' ' || c == '\t' || c == '\n' || c == '\r') continue;
+            if (c != ',') break;
+            if (comma_) return 2;
+            comma_ = true;
         }
         if (c == EOF || c == '/') {
             ended_ = true;
             return 0;
         }
         while (c != EOF && !(c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == ',' || c == '/')) {
-            if (len < (int)sizeof(tok) - 1) tok[len++] = (char)c;
+            if (len == (int)sizeof(tok) - 1) {
+                msg = std::string("value longer than ") + std::to_string(sizeof(tok) - 1) + " characters: \"" +
+                      std::string(tok, 24) + "...\"";
+                return -1;
+            }
+            tok[len++] = (char)c;
             c = get();
         }
+        // the value's own separator, when it is a comma
+        comma_ = c == ',';
         if (c == '/') unget();
         tok[len] = 0;
         const char *star = std::strchr(tok, '*');
@@ -168,6 +185,7 @@ private:
     std::vector<char> buf_;
     size_t pos_ = 0, have_ = 0;
     bool ended_ = false;
+    bool comma_ = true;   // a comma closed the last value (the record's start counts as one)
     long long rep_left_ = 0;
     bool rep_null_ = false;
     double rep_val_ = 0.0;

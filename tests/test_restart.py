@@ -132,3 +132,53 @@ def test_errors(tmp_path):
     # a record of another grid size: its byxf gap comes out negative
     with pytest.raises(ValueError, match="fewer than"):
         read_restart(path, nx + 1, ny, nz, ("hs",))
+
+
+def _plain_record(arrays, nx, ny, nz):
+    """The record's values in order, as repr strings (a null-free record)."""
+    from tests.restart_helpers import record_shapes
+    vals = []
+    for name, shape in record_shapes(nx, ny, nz):
+        a = arrays.get(name)
+        a = np.zeros(shape) if a is None else np.asarray(a, dtype=np.float64)
+        vals.extend(repr(float(x)) for x in a.ravel(order="F"))
+    return vals
+
+
+def test_commas_mark_null_values(tmp_path):
+    """List-directed input: nothing between two commas (blanks and line
+    ends aside) is a null value, and so is a comma before the record's
+    first value; a comma at a line end followed by a value on the next
+    line is one separator."""
+    nx, ny, nz = 5, 4, 3
+    arrays = sample_arrays(nx, ny, nz, 7)
+    vals = _plain_record(arrays, nx, ny, nz)
+    # u(0..9): v0,,v2 , ,v4,<newline> v5 ,,, v8 v9 ...: nulls at 1, 3, 6, 7
+    head = f"{vals[0]},,{vals[2]} , ,{vals[4]},\n {vals[5]} ,,, {vals[8]} "
+    path = str(tmp_path / "commas")
+    with open(path, "w") as f:
+        f.write(" " + head + " ".join(vals[9:]) + "\n")
+    u = read_restart(path, nx, ny, nz, ("u", "hs"))
+    got = u["u"].ravel(order="F")
+    exp = arrays["u"].ravel(order="F").copy()
+    exp[[1, 3, 6, 7]] = 0.0               # the reader's buffers start at zero
+    np.testing.assert_array_equal(got[:20], exp[:20])
+    # the nulls count as values: everything after them is in place
+    np.testing.assert_array_equal(got[9:], arrays["u"].ravel(order="F")[9:])
+    np.testing.assert_array_equal(u["hs"], arrays["hs"])
+    # a comma before the first value: u(0) null, the rest shifted by none
+    lead = str(tmp_path / "lead")
+    with open(lead, "w") as f:
+        f.write(" , " + " ".join(vals[1:]) + "\n")
+    got = read_restart(lead, nx, ny, nz, ("u",))["u"].ravel(order="F")
+    assert got[0] == 0.0
+    np.testing.assert_array_equal(got[1:], arrays["u"].ravel(order="F")[1:])
+
+
+def test_overlong_value_is_an_error(tmp_path):
+    nx, ny, nz = 5, 4, 3
+    path = str(tmp_path / "long")
+    with open(path, "w") as f:
+        f.write(" 1.0 2.0 " + "1" * 200 + " 4.0\n")
+    with pytest.raises(ValueError, match="longer than 127"):
+        read_restart(path, nx, ny, nz, ("u",))
