@@ -89,7 +89,14 @@ typedef struct suli_ps_opts {
                               uploading them again.  The caller promises not
                               to change them in between, as SuLi's loop does
                               (SuLi-IC/1_principal.f90:102-103).  0: upload. */
-    int32_t reserved;
+    int32_t stored_residual; /* mixed precision: 1 = each FP32 step stores its
+                              residual r_{n-1} (two FP32 vectors alternate)
+                              and the next step reads it; 0 (default) = the
+                              step recovers r_{n-2} from the directions,
+                              r_{n-2} = p_{n-1} - beta_{n-1} p_{n-2}, and
+                              stores no residual.  Both are CG; they differ
+                              in FP32 rounding (one more word written per
+                              cell and step for stored) */
 } suli_ps_opts;
 
 typedef struct suli_ps_stats {
@@ -164,7 +171,7 @@ int suli_ps_posdin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, doubl
  * captures no graph (it launches every kernel directly); the launch tables
  * and host lists it needs are reserved by suli_ps_create.  A problem whose
  * state does not fit (problem bytes beyond the maximum: about 80 B per cell
- * in FP64, 100 B in mixed precision, so up to about 0.8 M / 0.65 M cells) is
+ * in FP64, 104 B in mixed precision, so up to about 0.8 M / 0.64 M cells) is
  * rejected with SULI_PS_EINVAL and a message giving both sizes; with
  * opts.autotune it uses a tile shape already tuned for the size, or the
  * default rule, and never tunes.  The array keeps the last call's contents

@@ -62,7 +62,7 @@ class Opts(C.Structure):
         ("precision", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
         ("max_iter", C.c_int32), ("chunk", C.c_int32), ("verbose", C.c_int32),
         ("timing", C.c_int32), ("keep_null", C.c_int32), ("no_graphs", C.c_int32),
-        ("autotune", C.c_int32), ("reuse_staged", C.c_int32), ("reserved", C.c_int32),
+        ("autotune", C.c_int32), ("reuse_staged", C.c_int32), ("stored_residual", C.c_int32),
     ]
 
 

@@ -340,7 +340,8 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, gptr<floa
                                           gptr<const float> __restrict__ r_old, gptr<const float> __restrict__ q_old,
                                           gptr<const float> __restrict__ p_old, gptr<const float> __restrict__ p_prev,
                                           gptr<float> __restrict__ p_new, gptr<float> __restrict__ q_new,
-                                          const CO &co, float a, float b, float bp, const Fold &fd, double acc[4])
+                                          gptr<float> __restrict__ r_w, const CO &co, float a, float b, float bp,
+                                          const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -387,6 +388,7 @@ __device__ __forceinline__ void cg_m_tile(const Geo &g, const Tile &t, gptr<floa
         const float p_first = FOLD ? p_new[c] : 0.0f;
         q_new[c] = y;
         p_new[c] = pc;
+        if (r_w) r_w[c] = rc;
         if (FOLD) {
             // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
             // when r was recovered from it)
@@ -573,7 +575,8 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
                                            gptr<const float> __restrict__ r_old, gptr<const float> __restrict__ q_old,
                                            gptr<const float> __restrict__ p_old, gptr<const float> __restrict__ p_prev,
                                            gptr<float> __restrict__ p_new, gptr<float> __restrict__ q_new,
-                                           const CO &co, float a, float b, float bp, const Fold &fd, double acc[4])
+                                           gptr<float> __restrict__ r_w, const CO &co, float a, float b, float bp,
+                                           const Fold &fd, double acc[4])
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long plane = (long long)nx * ny;
@@ -696,6 +699,7 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
         if (w.active) {
             stv<V>(q_new, c, y);
             stv<V>(p_new, c, pc);
+            if (r_w) stv<V>(r_w, c, rc);
             if (FOLD) {
                 // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
                 // when r was recovered from it)
@@ -736,6 +740,9 @@ __device__ __forceinline__ gptr<float> pbuf(const ProbDev &pd, int k)
     return glob(pd.p32 + (k & (kFold - 1)) * pd.pslot);
 }
 __device__ __forceinline__ float alph_of(gptr<const CgScalars> S, int k) { return (float)S->alph[k & (kFold - 1)]; }
+// stored residual r_k (store_r): r32 for even k (where the setup leaves r_0
+// and a replacement, always after an even step, its residual), r32b for odd
+__device__ __forceinline__ gptr<float> rbuf(const ProbDev &pd, int k) { return glob((k & 1) ? pd.r32b : pd.r32); }
 
 // One tile of the mixed step n with V cells per lane (V = 1: cg_m_tile).
 // REC: r_{n-2} is recovered from the directions, r_{n-2} = p_{n-1} -
@@ -750,8 +757,8 @@ __device__ __forceinline__ float alph_of(gptr<const CgScalars> S, int k) { retur
 template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, gptr<const float> r_o,
                                               gptr<const float> q_o, gptr<const float> p_o, gptr<const float> p_oo,
-                                              gptr<float> p_w, gptr<float> q_w, const CO &co, float a, float b,
-                                              float bp, double acc[4])
+                                              gptr<float> p_w, gptr<float> q_w, gptr<float> r_w, const CO &co,
+                                              float a, float b, float bp, double acc[4])
 {
     Fold fd;
     if constexpr (FOLD) {
@@ -762,13 +769,14 @@ __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Ti
         for (int m = 0; m < kFold - 2; ++m) fd.p[m] = pbuf(pd, n - kFold + 1 + m);
     }
     if constexpr (V == 1)
-        cg_m_tile<FIRST, UPD, REC, FOLD>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, fd, acc);
+        cg_m_tile<FIRST, UPD, REC, FOLD>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, fd,
+                                         acc);
     else if (pd.g.nx > kTileX * V)
-        cg_m_tilev<V, FIRST, UPD, REC, FOLD, true>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp,
-                                                   fd, acc);
+        cg_m_tilev<V, FIRST, UPD, REC, FOLD, true>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b,
+                                                   bp, fd, acc);
     else
-        cg_m_tilev<V, FIRST, UPD, REC, FOLD, false>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp,
-                                                    fd, acc);
+        cg_m_tilev<V, FIRST, UPD, REC, FOLD, false>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a,
+                                                    b, bp, fd, acc);
 }
 
 // FOLD: this launch is one of the steps at which c takes a group (the
@@ -776,10 +784,10 @@ __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Ti
 template <int V, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, const ProbDev &pd, int n, const Tile &tl,
                                           gptr<const float> r_o, gptr<const float> q_o, gptr<const float> p_o,
-                                          gptr<const float> p_oo, gptr<float> p_w, gptr<float> q_w, const CO &co,
-                                          float a, float b, float bp, double acc[4])
+                                          gptr<const float> p_oo, gptr<float> p_w, gptr<float> q_w, gptr<float> r_w,
+                                          const CO &co, float a, float b, float bp, double acc[4])
 {
-#define CG_M(F, U, R, D) cg_m_tile_any<V, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc)
+#define CG_M(F, U, R, D) cg_m_tile_any<V, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, acc)
     if (first)
         CG_M(true, false, false, false);
     else if (!upd)
@@ -864,7 +872,9 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
         const bool fold = FOLD && upd;
         // p_{n-1} came from r_{n-2} and beta_{n-1} p_{n-2} unless step n-1
         // was the first or followed a replacement
-        const bool rec = upd && n != 2 && step != skip_e_at + 1;
+        // (with stored residuals r_{n-2} is read instead)
+        const bool store = pd.store_r != 0;
+        const bool rec = !store && upd && n != 2 && step != skip_e_at + 1;
         const float a = (float)alpha;
         if (tr.tile == 0 && threadIdx.x == 0) {
             S->rr[n & 1] = rho;
@@ -901,9 +911,15 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
         const float bp = (float)beta_prev;
         const gptr<const float> q_o = qbuf(pd, o), p_o = pbuf(pd, n - 1), p_oo = pbuf(pd, n - 2);
         const gptr<float> p_w = pbuf(pd, n), q_w = qbuf(pd, w);
+        // the residual read: r32, where the setup and the replacements leave
+        // r_{n-1} for a step without UPD (and r_{n-2} for the step after it);
+        // with stored residuals r_k of buffer k & 1, this step storing
+        // r_{n-1} into the other one
+        const gptr<const float> r_o = store ? rbuf(pd, upd ? n - 2 : n - 1) : glob((const float *)pd.r32);
+        const gptr<float> r_w = store && upd ? rbuf(pd, n - 1) : nullptr;
         (void)fold;
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
-            cg_m_step<VX, FOLD>(first, upd, rec, pd, n, tl, glob(pd.r32), q_o, p_o, p_oo, p_w, q_w, co, a, b, bp, acc);
+            cg_m_step<VX, FOLD>(first, upd, rec, pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, acc);
             return 0;
         });
         block_sum4(acc);

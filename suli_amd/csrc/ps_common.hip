@@ -92,6 +92,7 @@ SolveOpts solve_opts(const suli_ps_opts &o)
     so.tol = o.tol;
     so.max_iter = o.max_iter > 0 ? o.max_iter : 10000;
     so.keep_null = o.keep_null != 0;
+    so.store_r = o.stored_residual != 0;
     return so;
 }
 

@@ -39,6 +39,7 @@ struct SolveOpts {
     bool keep_null = false;
     int kz = 0;          // planes per 3-D tile (0: tuned or default for the size)
     int vx = 0;          // x cells per lane, FP32 only (0: tuned or 1)
+    bool store_r = false;   // mixed: the FP32 steps store the residual
 };
 
 // One problem resident on the device.

@@ -66,6 +66,7 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
         pd.pslot = (long long)((align_up(n * sizeof(float), 256) + kSkew) / sizeof(float));
         pd.p32 = cv.take<float>((size_t)pd.pslot * kFold);
         pd.q32b = cv.take<float>(n);
+        pd.r32b = cv.take<float>(n);
         pd.P.m4 = cv.take<double>(8 * (size_t)nt3);
     }
     pd.tx = cv.take<double>((size_t)g.nz + 1);
@@ -340,6 +341,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     pd.nt3 = nt3;
     pd.nt1 = nt1;
     pd.mixed = o.mixed ? 1 : 0;
+    pd.store_r = o.mixed && o.store_r ? 1 : 0;
     pd.S = d_status_ + si;
     sl.n0 = step_ + 1;   // iteration 1 runs at the next global step
     pd.n0 = sl.n0;

@@ -125,10 +125,13 @@ struct ProbDev {
     // one-launch iteration alternates q between two buffers (q32/q32b by
     // iteration parity) and keeps p_k in ring slot k % kFold (at p32 +
     // (k % kFold) * pslot) until c has taken alpha_k p_k.  r32 holds the
-    // residual the setup and each replacement leave; the steps recover r
-    // from the directions and store none.
+    // residual the setup and each replacement leave (r_0, r_n at a
+    // replacement after an even step n); the steps recover r from the
+    // directions and store none, or with store_r store r_k in r32 (k even)
+    // or r32b (k odd).
     const float *sx32, *sy32, *sz32;
-    float *c32, *r32, *q32, *q32b;
+    float *c32, *r32, *q32, *q32b, *r32b;
+    int store_r;
     float *p32;
     long long pslot;
     // constant density (t_press = 2, no solid mask): every face of a plane

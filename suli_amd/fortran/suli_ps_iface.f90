@@ -34,7 +34,7 @@ module suli_ps_iface
       integer(c_int32_t) :: no_graphs = 0
       integer(c_int32_t) :: autotune = 0
       integer(c_int32_t) :: reuse_staged = 0
-      integer(c_int32_t) :: reserved = 0
+      integer(c_int32_t) :: stored_residual = 0
    end type suli_ps_opts
 
    ! struct suli_ps_stats

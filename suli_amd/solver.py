@@ -104,17 +104,17 @@ class PressureSolver:
     def __init__(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
                  max_iter: int = 10000, chunk: int = 0, verbose: bool = True, timing: bool = False,
                  keep_null: bool = False, graphs: bool = True, autotune: bool = False,
-                 reuse_staged: bool = False):
+                 reuse_staged: bool = False, stored_residual: bool = False):
         self.lib = _lib.load()
         self.opts = _lib.Opts()
         self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs,
-                  autotune, reuse_staged)
+                  autotune, reuse_staged, stored_residual)
         self.ctx = C.c_void_p(self.lib.suli_ps_create(C.byref(self.opts)))
         if not self.ctx:
             raise SuliPsError("suli_ps_create failed (no HIP device?)")
 
     def _set(self, precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null,
-             graphs=True, autotune=False, reuse_staged=False):
+             graphs=True, autotune=False, reuse_staged=False, stored_residual=False):
         self.opts.precision = PRECISIONS[precision]
         self.opts.stop_rule = STOP_RULES[stop_rule]
         self.opts.tol = tol
@@ -126,14 +126,15 @@ class PressureSolver:
         self.opts.no_graphs = 0 if graphs else 1
         self.opts.autotune = 1 if autotune else 0
         self.opts.reuse_staged = 1 if reuse_staged else 0
+        self.opts.stored_residual = 1 if stored_residual else 0
 
     def configure(self, precision: str = "fp64", stop_rule: str = "reference", tol: float = 1e-8,
                   max_iter: int = 10000, chunk: int = 0, verbose: bool = True,
                   timing: bool = False, keep_null: bool = False, graphs: bool = True,
-                  autotune: bool = False, reuse_staged: bool = False) -> None:
+                  autotune: bool = False, reuse_staged: bool = False, stored_residual: bool = False) -> None:
         """Solver options (include/suli_ps.h, suli_ps_opts)."""
         self._set(precision, stop_rule, tol, max_iter, chunk, verbose, timing, keep_null, graphs,
-                  autotune, reuse_staged)
+                  autotune, reuse_staged, stored_residual)
         self._check(self.lib.suli_ps_set_opts(self.ctx, C.byref(self.opts)))
 
     def _check(self, rc: int) -> int:
