@@ -136,17 +136,34 @@ __device__ __forceinline__ void block_sum4(double v[4])
         }
 }
 
-// Totals of n partial 4-vectors (m[4t + c]), returned to every thread; as
-// total_of, each wave sums them all in the same order.
+// Totals of n partial 4-vectors (m[4t + c]), returned to every thread.  The
+// workgroup sums them once: each thread the partials t = tid, tid + 256,
+// ..., each wave its lanes' sums, then every thread adds the four waves'
+// sums in wave order (through LDS).  Every workgroup and kernel that sums
+// the same partials gets the same bits.  Called by all threads of the
+// workgroup together.
 __device__ __forceinline__ void total4_of(const double *m, int n, double out[4])
 {
+    __shared__ double part[kThreads / 64][4];
     const gptr<const double> p = glob(m);
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = threadIdx.x & 63; t < n; t += 64)
+    for (int t = threadIdx.x; t < n; t += kThreads)
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] += p[4 * t + c];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) out[c] = wave_sum(v[c]);
+    for (int c = 0; c < 4; ++c) v[c] = wave_sum(v[c]);
+    __syncthreads();   // the previous call's readers are done with part
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) part[threadIdx.x >> 6][c] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double s = part[0][c];
+#pragma unroll
+        for (int w = 1; w < kThreads / 64; ++w) s += part[w][c];
+        out[c] = s;
+    }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
