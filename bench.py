@@ -102,12 +102,13 @@ def run_solves(solver, grid, df, precision, steps, warmup, tol, graphs=True):
             spmv.append(st.spmv_ms)
     # the plain SpMV q = A p of this solve's operator (k_spmv), 50 launches
     spmv_only = solver.time_spmv(precision, reps=50)
+    from suli_amd.solver import tiles_dict
     kz, vx, _ = solver.autotune(grid, precision)
     out = dict(solves_per_s=steps / total, ms_per_step=total * 1e3 / steps,
                p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
                iters=int(np.median(iters)), refreshes=int(np.median(refreshes)),
                rel_true_max=float(max(rel)), graphs=bool(max(replays) > 0),
-               graph_replays_per_solve=float(np.mean(replays)), tiles={"kz": kz, "vx": vx},
+               graph_replays_per_solve=float(np.mean(replays)), tiles=tiles_dict(kz, vx),
                step_ms=float(np.mean(spmv)) if spmv else None, spmv_only_ms=spmv_only)
     return out, prd1
 
@@ -116,7 +117,7 @@ def bench_stretched_cylinder(steps, keep):
     """384x192x128 stretched grid with the masked cylinder (not a reference
     mode): solves/s of both solvers."""
     from suli_amd import cases
-    from suli_amd.solver import PressureSolver
+    from suli_amd.solver import PressureSolver, tiles_dict
     c = cases.config3(seed=1)
     g = c.grid
     solver = PressureSolver(verbose=False)
@@ -139,7 +140,7 @@ def bench_stretched_cylinder(steps, keep):
         prd[precision] = df.download("prd1")
         kz, vx, _ = solver.autotune(g, precision)
         out[precision] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iters": st.iters,
-                          "rel_true": st.rel_true, "tiles": {"kz": kz, "vx": vx}}
+                          "rel_true": st.rel_true, "tiles": tiles_dict(kz, vx)}
     df.free()
     solver.close()
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
@@ -489,7 +490,8 @@ def main():
                     help="comma-separated subset of the extra workloads to run (profiling runs)")
     ap.add_argument("--tiles", action="append", default=[], metavar="PRECISION=KZxVX",
                     help="fix the headline solves' tile shape instead of tuning it (profiling runs, "
-                         "whose serialised launches would tune differently), e.g. mixed=6x4")
+                         "whose serialised launches would tune differently), e.g. mixed=6x4, or "
+                         "mixed=6x4p for the pipelined FP32 step")
     ap.add_argument("--cpu-timing-to", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-lam", type=float, default=1.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -524,9 +526,11 @@ def main():
             "ms": copy_ms, "gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
     log(f"copy {copy['gbs']:.0f} GB/s")
     for spec in args.tiles:
+        from suli_amd.solver import TILES_PIPELINED
         prec, _, shape = spec.partition("=")
         kz, _, vx = shape.partition("x")
-        solver.set_tiles(g, prec, int(kz), int(vx))
+        pipe = vx.endswith("p")
+        solver.set_tiles(g, prec, int(kz), int(vx.rstrip("p")) | (TILES_PIPELINED if pipe else 0))
         log(f"tiles fixed: {prec} kz {kz} vx {vx}")
     mixed = fp64 = None
     headline = {}

@@ -454,7 +454,14 @@ int suli_ps_time_spmv(suli_ps_ctx *ctx, int32_t precision, int32_t reps, double 
  * that size then use it when opts.autotune is set.  A result depends on
  * the tiling (the order of its sums), so solves of one size agree bit for
  * bit only under the same choice.  suli_ps_set_tiles fixes a shape
- * instead (vx 1, 2 or 4; 1 for FP64 and where nx is not a multiple). */
+ * instead (vx 1, 2 or 4; 1 for FP64 and where nx is not a multiple).
+ *
+ * With two or four cells per lane the FP32 step comes in two kinds: its
+ * plane walk issued plane by plane, or pipelined (each plane's loads issued
+ * while the plane before it is computed: more memory requests in flight per
+ * wave, fewer waves per SIMD).  vx | SULI_PS_TILES_PIPELINED names the
+ * pipelined kind, here and in what suli_ps_autotune returns. */
+#define SULI_PS_TILES_PIPELINED 8
 int suli_ps_autotune(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t *kz,
                      int32_t *vx, double *step_us);
 int suli_ps_set_tiles(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision, int32_t kz,

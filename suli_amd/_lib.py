@@ -17,6 +17,7 @@ SULI_PS_STOP_RELATIVE = 1
 SULI_PS_OK = 0
 SULI_PS_NOT_CONVERGED = 1
 SULI_PS_EAGAIN = -4
+SULI_PS_TILES_PIPELINED = 8
 
 _dp = C.POINTER(C.c_double)
 

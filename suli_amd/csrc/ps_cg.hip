@@ -487,13 +487,13 @@ __device__ __forceinline__ void zv(const PlaneTables<float> &co, long long, int 
     for (int m = 0; m < V; ++m) o[m] = v;
 }
 
-// The coefficients of one plane of a row: x faces (V + 1), y faces j and
-// j + 1, z face k + 1: the row tables when the row's bit is set, else the
-// face arrays.
+// The coefficients of one plane k of a row: x faces (V + 1), y faces j and
+// j + 1, and z face kf (k + 1 or k, at offset fz): the row tables when the
+// row's bit is set, else the face arrays.
 template <int V>
 __device__ __forceinline__ void coefs_v(const FaceArrays<float> &co, bool reg, long long fx, long long fy,
-                                        long long fz1, int nx, int k, float (&sxv)[V + 1], float (&sys)[V],
-                                        float (&syn)[V], float (&szt)[V])
+                                        long long fz, int kf, int nx, int k, float (&sxv)[V + 1], float (&sys)[V],
+                                        float (&syn)[V], float (&sz)[V])
 {
     const gptr<const float> px = co.xp(reg, fx, k);
     float t[V];
@@ -503,17 +503,17 @@ __device__ __forceinline__ void coefs_v(const FaceArrays<float> &co, bool reg, l
     sxv[V] = px[V];
     ldv<V>(co.yp(reg, fy, k), 0, sys);
     ldv<V>(co.yp(reg, fy + nx, k), 0, syn);
-    ldv<V>(co.zp(reg, fz1, k + 1), 0, szt);
+    ldv<V>(co.zp(reg, fz, kf), 0, sz);
 }
 template <int V>
 __device__ __forceinline__ void coefs_v(const PlaneTables<float> &co, bool, long long fx, long long fy,
-                                        long long fz1, int nx, int k, float (&sxv)[V + 1], float (&sys)[V],
-                                        float (&syn)[V], float (&szt)[V])
+                                        long long fz, int kf, int nx, int k, float (&sxv)[V + 1], float (&sys)[V],
+                                        float (&syn)[V], float (&sz)[V])
 {
     xv<V>(co, fx, k, sxv);
     yv<V>(co, fy, k, sys);
     yv<V>(co, fy + nx, k, syn);
-    zv<V>(co, fz1, k + 1, szt);
+    zv<V>(co, fz, kf, sz);
 }
 // z faces of a plane's row at the tile's first plane
 template <int V>
@@ -567,10 +567,42 @@ __device__ __forceinline__ RowWave row_wave(const Geo &g, const Tile &t)
 // lanes (DPP) instead of being formed again from three loads each, and the
 // x faces of a lane come as one vector.
 //
+// PIPE: the walk is software-pipelined: the loads of plane k + 1
+// (PlaneLoads) are issued before plane k is computed, so that a wave has two
+// planes' memory round trips in flight instead of one (the last plane issues
+// its own loads once more, in bounds and unused, so that the loop has no
+// branch on it).  The two load sets take about 60 more registers at four
+// cells per lane, which leaves two waves per SIMD instead of three; without
+// PIPE each plane's loads are issued just before it is computed.
+//
 // EDGE: the row has more than one segment, so the cells beyond a segment's
 // ends may have to be loaded; without it (nx <= 64 V) the loop has no
 // branch on them.
-template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, bool EDGE, class CO>
+static_assert(kFold == 4, "PlaneLoads holds the fold's three older directions");
+template <int V>
+struct PlaneLoads {
+    // p_{n-1}, then p_{n-2} (REC) or r_{n-2}, then q_{n-1} (UPD): the plane
+    // above (t), the south (s) and north (n) rows
+    float to[V], tb[V], tq[V], so[V], sb[V], sq[V], no[V], nb[V], nq[V];
+    float sxv[V + 1], sys[V], syn[V];   // x and y faces
+    float szt[V], szb[V];                // z faces k + 1 and k (szb: PIPE only)
+    float wo, wb, wq, eo, eb, eq;        // cells beyond the segment's ends (EDGE)
+    float f0[V], f1[V], f2[V], cv[V];    // FOLD: p_{n-4}, p_{n-3}, p_{n-2} (unless REC) at the cell; c
+};
+
+// v copied into a register of its own here.  A value that a plane's loads
+// brought and a later plane still needs is copied out before the next loads
+// into the same set are issued: left in place, it would keep those loads
+// from reusing its register, and the loop would then have to copy the set
+// at its back edge, waiting for the loads in flight.
+__device__ __forceinline__ float copy_now(float v)
+{
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v) : "memory");
+    return r;
+}
+
+template <int V, bool PIPE, bool FIRST, bool UPD, bool REC, bool FOLD, bool EDGE, class CO>
 __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<float> __restrict__ c32,
                                            gptr<const float> __restrict__ r_old, gptr<const float> __restrict__ q_old,
                                            gptr<const float> __restrict__ p_old, gptr<const float> __restrict__ p_prev,
@@ -582,89 +614,107 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
     const long long plane = (long long)nx * ny;
     if (t.j >= ny) return;   // the whole wave
     const RowWave w = row_wave<V>(g, t);
+    const long long dfx = (long long)ny * (nx + 1), dfy = (long long)(ny + 1) * nx;   // one plane up
     long long c = (long long)t.k0 * plane + (long long)t.j * nx + w.ci;
     const int js = nb_lo(t.j, ny, g.py), jn = nb_hi(t.j, ny, g.py);
     const long long dS = (long long)(js - t.j) * nx, dN = (long long)(jn - t.j) * nx;
     long long fx = ((long long)t.k0 * ny + t.j) * (nx + 1) + w.ci;
     long long fy = ((long long)t.k0 * (ny + 1) + t.j) * nx + w.ci;
-    // p_n at one cell (segment ends)
-    auto pn1 = [&](long long x) -> float {
-        const float po = FIRST ? 0.0f : p_old[x];
-        float r = REC ? po - bp * p_prev[x] : r_old[x];
-        if (UPD) r = r - a * q_old[x];
-        return FIRST ? r : r + b * po;
+    const gptr<const float> rb = REC ? p_prev : r_old;   // the second operand of r_{n-2}
+    // the operands of p_n at x (V cells, or one)
+    auto raw = [&](long long x, float (&o)[V], float (&rr)[V], float (&q)[V]) {
+        if (!FIRST) ldv<V>(p_old, x, o);
+        ldv<V>(rb, x, rr);
+        if (UPD) ldv<V>(q_old, x, q);
     };
-    // r_{n-1} at x given p_{n-1} there; pp receives p_{n-2} (REC)
-    auto rnv = [&](long long x, const float (&po)[V], float (&r)[V], float (&pp)[V]) {
-        if (REC) {
-            ldv<V>(p_prev, x, pp);
-#pragma unroll
-            for (int m = 0; m < V; ++m) r[m] = po[m] - bp * pp[m];
-        } else {
-            ldv<V>(r_old, x, r);
-#pragma unroll
-            for (int m = 0; m < V; ++m) pp[m] = 0.0f;
-        }
-        if (UPD) {
-            float q[V];
-            ldv<V>(q_old, x, q);
-#pragma unroll
-            for (int m = 0; m < V; ++m) r[m] = r[m] - a * q[m];
-        }
+    auto raw1 = [&](long long x, float &o, float &rr, float &q) {
+        if (!FIRST) o = p_old[x];
+        rr = rb[x];
+        if (UPD) q = q_old[x];
     };
-    auto pnv = [&](long long x, float (&pv)[V]) {
-        float po[V], pp[V];
-        if (FIRST) {
-#pragma unroll
-            for (int m = 0; m < V; ++m) po[m] = 0.0f;
-        } else {
-            ldv<V>(p_old, x, po);
-        }
-        rnv(x, po, pv, pp);
-        if (!FIRST) {
-#pragma unroll
-            for (int m = 0; m < V; ++m) pv[m] = pv[m] + b * po[m];
-        }
+    // r_{n-1} = r_{n-2} - alpha q_{n-1} with r_{n-2} = p_{n-1} - beta_{n-1}
+    // p_{n-2} (REC) or as stored; p_n = r_{n-1} + beta_n p_{n-1}
+    auto r_of = [&](float o, float rr, float q) -> float {
+        float r = REC ? o - bp * rr : rr;
+        if (UPD) r = r - a * q;
+        return r;
     };
-    float poc[V], ppc[V], rc[V], pc[V], pb[V], szb[V];
-    if (FIRST) {
-#pragma unroll
-        for (int m = 0; m < V; ++m) poc[m] = 0.0f;
-    } else {
-        ldv<V>(p_old, c, poc);
-    }
-    rnv(c, poc, rc, ppc);
-#pragma unroll
-    for (int m = 0; m < V; ++m) pc[m] = FIRST ? rc[m] : rc[m] + b * poc[m];
-    if (t.k0 > 0) {
-        pnv(c - plane, pb);
-    } else {
-#pragma unroll
-        for (int m = 0; m < V; ++m) pb[m] = pc[m];
-    }
+    auto p_of = [&](float o, float r) -> float { return FIRST ? r : r + b * o; };
     const unsigned long long rows = co.rows(t.j, t.k0, t.k1);
-    zb_v<V>(co, row_bit(rows, t.k0, t.k0), c, t.k0, szb);
-    for (int k = t.k0; k < t.k1; ++k) {
-        const long long r0 = c - w.ci;
-        const long long up = (k < nz - 1) ? plane : 0;
-        float pot[V], ppt[V], rt[V], pt[V], szt[V], ps[V], pnn[V], sys[V], syn[V], sxv[V + 1];
-        if (FIRST) {
-#pragma unroll
-            for (int m = 0; m < V; ++m) pot[m] = 0.0f;
-        } else {
-            ldv<V>(p_old, c + up, pot);
+    auto issue = [&](int k, long long cc, long long fxx, long long fyy, PlaneLoads<V> &L) {
+        const long long up = k < nz - 1 ? plane : 0;   // the top neighbour (mirror ghost at the top)
+        raw(cc + up, L.to, L.tb, L.tq);
+        raw(cc + dS, L.so, L.sb, L.sq);
+        raw(cc + dN, L.no, L.nb, L.nq);
+        // z faces k + 1 and, pipelined, k (a row's bit covers both).  Plane
+        // by plane, face k stays from the plane below; pipelined, a value
+        // kept from one plane's loads for the next would hold a register
+        // the next loads need, so it is loaded again (an L1 hit)
+        const bool reg = row_bit(rows, k, t.k0);
+        coefs_v<V>(co, reg, fxx, fyy, cc + plane, k + 1, nx, k, L.sxv, L.sys, L.syn, L.szt);
+        if (PIPE) zb_v<V>(co, reg, cc, k, L.szb);
+        if (EDGE) {
+            // loaded whether used or not (the row's first cell then, the
+            // same address in every lane), so that the loop has no branch
+            // around loads in flight
+            const long long r0 = cc - w.ci;
+            raw1(w.load_w ? r0 + w.xw : r0, L.wo, L.wb, L.wq);
+            raw1(w.load_e ? r0 + w.xe : r0, L.eo, L.eb, L.eq);
         }
-        rnv(c + up, pot, rt, ppt);
+        if (FOLD) {
+            ldv<V>(p_new, cc, L.f0);   // p_{n-kFold}, before p_n replaces it
+            ldv<V>(fd.p[0], cc, L.f1);
+            if (!REC) ldv<V>(fd.p[1], cc, L.f2);
+            ldv<V>(c32, cc, L.cv);
+        }
+    };
+    // r_{n-1} and p_n at the plane at hand (c), p_n at the one below (pb),
+    // the z faces below (szb); for the fold, p_{n-1} and p_{n-2} (REC) at c
+    // (copy_now)
+    float rc[V], pc[V], pb[V], szb[V], poc[V], ppc[V];
+    {
+        float o[V], rr[V], q[V];
+        raw(c, o, rr, q);
 #pragma unroll
-        for (int m = 0; m < V; ++m) pt[m] = FIRST ? rt[m] : rt[m] + b * pot[m];
-        pnv(c + dS, ps);
-        pnv(c + dN, pnn);
-        coefs_v<V>(co, row_bit(rows, k, t.k0), fx, fy, c + plane, nx, k, sxv, sys, syn, szt);
+        for (int m = 0; m < V; ++m) {
+            rc[m] = r_of(o[m], rr[m], q[m]);
+            pc[m] = p_of(o[m], rc[m]);
+            if (FOLD) poc[m] = copy_now(o[m]);
+            if (FOLD && REC) ppc[m] = copy_now(rr[m]);
+        }
+    }
+    if (t.k0 > 0) {
+        float o[V], rr[V], q[V];
+        raw(c - plane, o, rr, q);
+#pragma unroll
+        for (int m = 0; m < V; ++m) pb[m] = p_of(o[m], r_of(o[m], rr[m], q[m]));
+    } else {
+#pragma unroll
+        for (int m = 0; m < V; ++m) pb[m] = pc[m];   // mirror ghost
+    }
+    if (!PIPE) zb_v<V>(co, row_bit(rows, t.k0, t.k0), c, t.k0, szb);
+    // one plane: compute it from `cur` (PIPE: while the next plane's loads
+    // go into `nxt`, two sets used in turn, so that no copy waits for loads
+    // in flight)
+    auto plane_step = [&](int k, PlaneLoads<V> &cur, PlaneLoads<V> &nxt) {
+        const bool more = k + 1 < t.k1;
+        if (PIPE)
+            issue(more ? k + 1 : k, more ? c + plane : c, more ? fx + dfx : fx, more ? fy + dfy : fy, nxt);
+        else
+            issue(k, c, fx, fy, cur);
+        float rt[V], pt[V], ps[V], pnn[V];
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            rt[m] = r_of(cur.to[m], cur.tb[m], cur.tq[m]);
+            pt[m] = p_of(cur.to[m], rt[m]);
+            ps[m] = p_of(cur.so[m], r_of(cur.so[m], cur.sb[m], cur.sq[m]));
+            pnn[m] = p_of(cur.no[m], r_of(cur.no[m], cur.nb[m], cur.nq[m]));
+        }
         // x neighbours of the lane's end cells: adjacent lanes, or beyond the segment
         const float ew_row = w.wrap ? lane_value(pc[V - 1], w.last) : pc[0];
         const float ee_row = w.wrap ? lane_value(pc[0], 0) : pc[V - 1];
-        const float ew = (EDGE && w.load_w) ? pn1(r0 + w.xw) : ew_row;
-        const float ee = (EDGE && w.load_e) ? pn1(r0 + w.xe) : ee_row;
+        const float ew = (EDGE && w.load_w) ? p_of(cur.wo, r_of(cur.wo, cur.wb, cur.wq)) : ew_row;
+        const float ee = (EDGE && w.load_e) ? p_of(cur.eo, r_of(cur.eo, cur.eb, cur.eq)) : ee_row;
         const float pw = from_lower(ew, pc[V - 1]);
         const float pe_dpp = from_upper(ee, pc[0]);
         const float pe = w.lane == w.last ? ee : pe_dpp;
@@ -675,38 +725,23 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
             const float east = m == V - 1 ? pe : pc[m + 1];
             const float west = m == 0 ? pw : pc[m - 1];
             const float top = lid ? -pc[m] : pt[m];
-            y[m] = pc[m] - sxv[m + 1] * east - sxv[m] * west - syn[m] * pnn[m] - sys[m] * ps[m] - szt[m] * top -
-                   szb[m] * pb[m];
-        }
-        float sum[V];
-        if (FOLD) {
-            ldv<V>(p_new, c, sum);   // p_{n-kFold}, before p_n replaces it
-#pragma unroll
-            for (int m = 0; m < V; ++m) sum[m] = fd.a[0] * sum[m];
-#pragma unroll
-            for (int f = 0; f < kFold - 2; ++f) {
-                float pf[V];
-                if (REC && f == kFold - 3) {
-#pragma unroll
-                    for (int m = 0; m < V; ++m) pf[m] = ppc[m];
-                } else {
-                    ldv<V>(fd.p[f], c, pf);
-                }
-#pragma unroll
-                for (int m = 0; m < V; ++m) sum[m] += fd.a[f + 1] * pf[m];
-            }
+            y[m] = pc[m] - cur.sxv[m + 1] * east - cur.sxv[m] * west - cur.syn[m] * pnn[m] - cur.sys[m] * ps[m] -
+                   cur.szt[m] * top - (PIPE ? cur.szb[m] : szb[m]) * pb[m];
         }
         if (w.active) {
             stv<V>(q_new, c, y);
             stv<V>(p_new, c, pc);
             if (r_w) stv<V>(r_w, c, rc);
             if (FOLD) {
-                // 9_grad.f90:458-468 for kFold iterations (p_{n-2} is at hand
-                // when r was recovered from it)
+                // 9_grad.f90:458-468 for kFold iterations
                 float cv[V];
-                ldv<V>(c32, c, cv);
 #pragma unroll
-                for (int m = 0; m < V; ++m) cv[m] = cv[m] - (sum[m] + fd.a[kFold - 1] * poc[m]);
+                for (int m = 0; m < V; ++m) {
+                    float sum = fd.a[0] * cur.f0[m];
+                    sum += fd.a[1] * cur.f1[m];
+                    sum += fd.a[2] * (REC ? ppc[m] : cur.f2[m]);
+                    cv[m] = cur.cv[m] - (sum + fd.a[kFold - 1] * poc[m]);
+                }
                 stv<V>(c32, c, cv);
             }
 #pragma unroll
@@ -723,13 +758,28 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
             pb[m] = pc[m];
             pc[m] = pt[m];
             rc[m] = rt[m];
-            poc[m] = pot[m];
-            ppc[m] = ppt[m];
-            szb[m] = szt[m];
+            if (!PIPE) szb[m] = cur.szt[m];
+            if (FOLD) poc[m] = copy_now(cur.to[m]);
+            if (FOLD && REC) ppc[m] = copy_now(cur.tb[m]);
         }
         c += plane;
-        fx += (long long)ny * (nx + 1);
-        fy += (long long)(ny + 1) * nx;
+        fx += dfx;
+        fy += dfy;
+    };
+    // (both exits leave the loop, so that the back edge carries one version
+    // of each set: a merge of two would copy registers whose loads are in
+    // flight, and wait for them)
+    PlaneLoads<V> L0, L1;
+    if (PIPE) {
+        issue(t.k0, c, fx, fy, L0);
+        for (int k = t.k0;; k += 2) {
+            plane_step(k, L0, L1);
+            if (k + 1 >= t.k1) break;
+            plane_step(k + 1, L1, L0);
+            if (k + 2 >= t.k1) break;
+        }
+    } else {
+        for (int k = t.k0; k < t.k1; ++k) plane_step(k, L0, L0);
     }
 }
 
@@ -754,7 +804,7 @@ __device__ __forceinline__ gptr<float> rbuf(const ProbDev &pd, int k) { return g
 // alphas but alpha_{n-1} = a were stored by the previous steps).  The fold's
 // operands are gathered here, inside the FOLD instance only, so that the
 // other instances carry no extra live registers.
-template <int V, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
+template <int V, bool PIPE, bool FIRST, bool UPD, bool REC, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Tile &tl, gptr<const float> r_o,
                                               gptr<const float> q_o, gptr<const float> p_o, gptr<const float> p_oo,
                                               gptr<float> p_w, gptr<float> q_w, gptr<float> r_w, const CO &co,
@@ -768,26 +818,25 @@ __device__ __forceinline__ void cg_m_tile_any(const ProbDev &pd, int n, const Ti
 #pragma unroll
         for (int m = 0; m < kFold - 2; ++m) fd.p[m] = pbuf(pd, n - kFold + 1 + m);
     }
+#define CG_M_ARGS pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, fd, acc
     if constexpr (V == 1)
-        cg_m_tile<FIRST, UPD, REC, FOLD>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, fd,
-                                         acc);
+        cg_m_tile<FIRST, UPD, REC, FOLD>(CG_M_ARGS);
     else if (pd.g.nx > kTileX * V)
-        cg_m_tilev<V, FIRST, UPD, REC, FOLD, true>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b,
-                                                   bp, fd, acc);
+        cg_m_tilev<V, PIPE, FIRST, UPD, REC, FOLD, true>(CG_M_ARGS);
     else
-        cg_m_tilev<V, FIRST, UPD, REC, FOLD, false>(pd.g, tl, glob(pd.c32), r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a,
-                                                    b, bp, fd, acc);
+        cg_m_tilev<V, PIPE, FIRST, UPD, REC, FOLD, false>(CG_M_ARGS);
+#undef CG_M_ARGS
 }
 
 // FOLD: this launch is one of the steps at which c takes a group (the
 // instances of the other steps are not compiled into it).
-template <int V, bool FOLD, class CO>
+template <int V, bool PIPE, bool FOLD, class CO>
 __device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, const ProbDev &pd, int n, const Tile &tl,
                                           gptr<const float> r_o, gptr<const float> q_o, gptr<const float> p_o,
                                           gptr<const float> p_oo, gptr<float> p_w, gptr<float> q_w, gptr<float> r_w,
                                           const CO &co, float a, float b, float bp, double acc[4])
 {
-#define CG_M(F, U, R, D) cg_m_tile_any<V, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, acc)
+#define CG_M(F, U, R, D) cg_m_tile_any<V, PIPE, F, U, R, D>(pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, acc)
     if (first)
         CG_M(true, false, false, false);
     else if (!upd)
@@ -815,14 +864,14 @@ __device__ __forceinline__ void cg_m_step(bool first, bool upd, bool rec, const 
 // and the global step agree mod kFold: every problem of a launch folds or
 // none does, and the host launches the instance that does.
 //
-// VX: cells per lane of every problem in the table (the engine launches one
-// instance per width).  Four-cell lanes need more registers than four waves
-// per SIMD leave (about 160 without spilling), so that instance is held to
-// three.
-template <int VX, bool FOLD>
-__global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDev *__restrict__ P,
-                                                                     const TileRef *__restrict__ tab, int ntab,
-                                                                     StepRef sref, int *ran)
+// VX, PIPE: cells per lane of every problem in the table and whether its
+// walk is pipelined (the engine launches one instance per kind).  Waves per
+// SIMD that the registers allow: four (one cell per lane, or two without
+// PIPE), three (four cells, or two with PIPE), two (four cells with PIPE).
+constexpr int cg_m_waves(int vx, bool pipe) { return vx == 1 ? 4 : vx == 2 ? (pipe ? 3 : 4) : (pipe ? 2 : 3); }
+template <int VX, bool PIPE, bool FOLD>
+__global__ __launch_bounds__(kThreads, cg_m_waves(VX, PIPE))
+void k_cg_m(const ProbDev *__restrict__ P, const TileRef *__restrict__ tab, int ntab, StepRef sref, int *ran)
 {
     const Run run = my_run(ntab);
     const int step = step_of(sref);
@@ -919,7 +968,8 @@ __global__ __launch_bounds__(kThreads, VX == 4 ? 3 : 4) void k_cg_m(const ProbDe
         const gptr<float> r_w = store && upd ? rbuf(pd, n - 1) : nullptr;
         (void)fold;
         with_coefs(vecs(pd, 0.0f), [&](const auto &co) {
-            cg_m_step<VX, FOLD>(first, upd, rec, pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp, acc);
+            cg_m_step<VX, PIPE, FOLD>(first, upd, rec, pd, n, tl, r_o, q_o, p_o, p_oo, p_w, q_w, r_w, co, a, b, bp,
+                                      acc);
             return 0;
         });
         block_sum4(acc);
@@ -1292,7 +1342,7 @@ __device__ __forceinline__ double spmv_tilev(const Geo &g, const Tile &t, gptr<c
         ldv<V>(p, c + up, pt);
         ldv<V>(p, c + dS, ps);
         ldv<V>(p, c + dN, pnn);
-        coefs_v<V>(co, row_bit(rows, k, t.k0), fx, fy, c + plane, nx, k, sxv, sys, syn, szt);
+        coefs_v<V>(co, row_bit(rows, k, t.k0), fx, fy, c + plane, k + 1, nx, k, sxv, sys, syn, szt);
         const float ew_row = w.wrap ? lane_value(pc[V - 1], w.last) : pc[0];
         const float ee_row = w.wrap ? lane_value(pc[0], 0) : pc[V - 1];
         const float ew = (EDGE && w.load_w) ? p[r0 + w.xw] : ew_row;
@@ -1401,7 +1451,9 @@ Geo make_geo(int nx, int ny, int nz, int px, int py, int kz, int vx)
     g.px = px;
     g.py = py;
     g.n = (long long)nx * ny * nz;
-    g.vx = ((vx == 2 || vx == 4) && nx % vx == 0) ? vx : 1;
+    const int w = vx & ~kPipelined;
+    g.vx = ((w == 2 || w == 4) && nx % w == 0) ? w : 1;
+    g.pipe = g.vx > 1 && (vx & kPipelined) ? 1 : 0;
     g.tiles_x = (nx + kTileX * g.vx - 1) / (kTileX * g.vx);
     g.tiles_y = (ny + kTileY - 1) / kTileY;
     if (kz <= 0) {
@@ -1435,24 +1487,28 @@ void launch_cg_a(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, int gr
 }
 
 void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, int vx,
-                 hipStream_t s)
+                 bool pipe, hipStream_t s)
 {
     // the global step is step.off (direct launch) or base + off with a base
     // on the refresh grain (replayed group): its phase mod kFold is off's
     const bool fold = (step.off & (kFold - 1)) == 1;
-#define LAUNCH_CG_M(V)                                                                                          \
-    do {                                                                                                        \
-        if (fold)                                                                                               \
-            hipLaunchKernelGGL((k_cg_m<V, true>), dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);   \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_cg_m<V, false>), dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);  \
+#define LAUNCH_CG_M(V, PIPE)                                                                                     \
+    do {                                                                                                         \
+        if (fold)                                                                                                \
+            hipLaunchKernelGGL((k_cg_m<V, PIPE, true>), dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran);  \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_cg_m<V, PIPE, false>), dim3(grid), dim3(kThreads), 0, s, P, t3, nt3, step, ran); \
     } while (0)
-    if (vx == 4)
-        LAUNCH_CG_M(4);
+    if (vx == 4 && pipe)
+        LAUNCH_CG_M(4, true);
+    else if (vx == 4)
+        LAUNCH_CG_M(4, false);
+    else if (vx == 2 && pipe)
+        LAUNCH_CG_M(2, true);
     else if (vx == 2)
-        LAUNCH_CG_M(2);
+        LAUNCH_CG_M(2, false);
     else
-        LAUNCH_CG_M(1);
+        LAUNCH_CG_M(1, false);
 #undef LAUNCH_CG_M
 }
 

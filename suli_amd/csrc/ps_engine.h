@@ -38,7 +38,7 @@ struct SolveOpts {
     int max_iter = 10000;
     bool keep_null = false;
     int kz = 0;          // planes per 3-D tile (0: tuned or default for the size)
-    int vx = 0;          // x cells per lane, FP32 only (0: tuned or 1)
+    int vx = 0;          // lane code (x cells per lane, kPipelined), FP32 only (0: tuned or 1)
     bool store_r = false;   // mixed: the FP32 steps store the residual
 };
 
@@ -202,7 +202,7 @@ private:
     int off_t3_64 = 0, off_t1_64 = 0, off_t3_32 = 0, off_t1_32 = 0;
     // the FP32 3-D table holds the problems of one lane width after another
     // (1, 2, 4 cells per lane): where each width's entries start and how many
-    int sub_t3_32[3] = {0, 0, 0}, cnt_t3_32[3] = {0, 0, 0};
+    int sub_t3_32[kLaneKinds] = {}, cnt_t3_32[kLaneKinds] = {};
     // launches the mixed step over every lane width's entries; the first
     // launch carries the step counter update `next`
     void launch_steps_m(StepRef sr, int *ran);
@@ -214,7 +214,7 @@ private:
     long long table_cells_ = 0;   // cells of the problems in the launch tables
     bool autotune_ = false;
     struct Tuned {
-        int kz, vx;
+        int kz, vx;   // vx: lane code
         double us;
     };
     std::map<std::tuple<int, int, int, bool>, Tuned> tuned_;

@@ -318,7 +318,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
     sl.g.dtop = gr.top_bc == 1 ? 1 : 0;
     sl.o = o;
     sl.o.kz = sl.g.kz;
-    sl.o.vx = sl.g.vx;
+    sl.o.vx = lane_code(sl.g);
     const int nt3 = spmv_tiles(sl.g), nt1 = vec_tiles(sl.g.n);
     sl.bytes = problem_bytes(gr.nx, gr.ny, gr.nz, o.mixed);
     sl.own_block = block == nullptr;
@@ -446,12 +446,12 @@ int Engine::rebuild_tables()
         for (int t = 0; t < sl.pd.nt1; ++t) t1.push_back({i, t});
         if (sl.o.mixed) tab_.mixed_probs.push_back(i);
     }
-    // FP32 3-D entries grouped by lane width (one step kernel per width)
-    for (int w = 0; w < 3; ++w) {
+    // FP32 3-D entries grouped by lane kind (one step kernel per kind)
+    for (int w = 0; w < kLaneKinds; ++w) {
         sub_t3_32[w] = (int)tab_.t3_32.size();
         for (int i = 0; i < max_slots_; ++i) {
             const Slot &sl = slots_[i];
-            if (!sl.active || sl.finished || !sl.o.mixed || sl.g.vx != (1 << w)) continue;
+            if (!sl.active || sl.finished || !sl.o.mixed || lane_kind(sl.g) != w) continue;
             for (int t = 0; t < sl.pd.nt3; ++t) tab_.t3_32.push_back({i, t});
         }
         cnt_t3_32[w] = (int)tab_.t3_32.size() - sub_t3_32[w];
@@ -545,10 +545,11 @@ bool Engine::refresh_may_be_due(int s) const
 
 void Engine::launch_steps_m(StepRef sr, int *ran)
 {
-    for (int w = 0; w < 3; ++w) {
+    for (int w = 0; w < kLaneKinds; ++w) {
         const int n = cnt_t3_32[w];
         if (!n) continue;
-        launch_cg_m(d_probs_, d_tab_ + off_t3_32 + sub_t3_32[w], n, std::min(n, kMaxGrid), sr, ran, 1 << w, s_);
+        launch_cg_m(d_probs_, d_tab_ + off_t3_32 + sub_t3_32[w], n, std::min(n, kMaxGrid), sr, ran, kind_vx(w),
+                    kind_pipe(w), s_);
         sr.next = nullptr;   // one launch updates the counter
     }
 }
@@ -852,7 +853,7 @@ int Engine::run_all(bool timing)
 void Engine::set_tiles(int nx, int ny, int nz, bool mixed, int kz, int vx)
 {
     const Geo g = make_geo(nx, ny, nz, 0, 0, std::max(kz, kMinKz), mixed ? vx : 1);
-    tuned_[std::make_tuple(nx, ny, nz, mixed)] = Tuned{g.kz, g.vx, 0.0};
+    tuned_[std::make_tuple(nx, ny, nz, mixed)] = Tuned{g.kz, lane_code(g), 0.0};
 }
 
 int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, double *us_out)
@@ -924,8 +925,9 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
     double best_us = 1e300;
     int best_kz = g_def.kz, best_vx = 1;
     std::vector<std::pair<int, int>> shapes;
-    for (int vx : {1, 2, 4}) {
-        if (vx > 1 && (!mixed || nx % vx)) continue;
+    for (int vx : {1, 2, 2 | kPipelined, 4, 4 | kPipelined}) {
+        const int w = vx & ~kPipelined;
+        if (w > 1 && (!mixed || nx % w)) continue;
         for (int kz : cands)
             if (kz <= nz || kz == cands.front()) shapes.emplace_back(kz, vx);
     }
@@ -960,7 +962,7 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         auto launch = [&] {
             const int step = mixed ? 2 + (seq++ % 8) : 3;
             if (mixed)
-                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, g.vx, s_);
+                launch_cg_m(dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, g.vx, g.pipe, s_);
             else
                 launch_cg_a(false, dP, dT, nt3, grid, StepRef{nullptr, step, nullptr}, nullptr, s_);
         };
@@ -976,7 +978,7 @@ int Engine::tune(int nx, int ny, int nz, bool mixed, int *kz_out, int *vx_out, d
         if (us < best_us) {
             best_us = us;
             best_kz = g.kz;
-            best_vx = g.vx;
+            best_vx = lane_code(g);
         }
     }
 #undef TCHK

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/suli_ps.h"
+
 namespace sps {
 
 constexpr int kThreads = 256;        // every kernel: 4 wave64s
@@ -46,8 +48,20 @@ struct Geo {
     long long n;              // nx*ny*nz
     int tiles_x, tiles_y, tiles_z, kz;   // 3-D tiling: (64 vx) x 4 columns, kz planes
     int vx;                   // x cells per lane in 3-D tiles: 1, or 2 or 4 (FP32, nx a multiple)
+    int pipe;                 // the mixed step's plane walk is pipelined (vx 2 or 4)
     int dtop;                 // 1: p = 0 on the top face (top ghost = -interior)
 };
+
+// Lane code of a tile shape: the cells per lane, with kPipelined set for the
+// mixed step's pipelined walk (SULI_PS_TILES_PIPELINED of the C ABI).
+constexpr int kPipelined = SULI_PS_TILES_PIPELINED;
+inline int lane_code(const Geo &g) { return g.vx | (g.pipe ? kPipelined : 0); }
+// The mixed step's kernel kinds, one launch per kind in a step: one cell per
+// lane; two; two, pipelined; four; four, pipelined.
+constexpr int kLaneKinds = 5;
+inline int lane_kind(const Geo &g) { return g.vx == 1 ? 0 : (g.vx == 2 ? 1 : 3) + g.pipe; }
+constexpr int kind_vx(int kind) { return kind == 0 ? 1 : kind < 3 ? 2 : 4; }
+constexpr bool kind_pipe(int kind) { return kind == 2 || kind == 4; }
 
 // Device-resident CG state of one problem.
 //
@@ -267,7 +281,7 @@ void launch_cg_b(bool fp32, const ProbDev *P, const TileRef *t1, int nt1, int gr
 // - 2 alpha q.r_{n-2} + alpha^2 q.q anchored on the directly summed
 // r_{n-2}.r_{n-2}.  ran: as for cg_a.
 // vx: the cells per lane of every problem in t3
-void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, int vx,
+void launch_cg_m(const ProbDev *P, const TileRef *t3, int nt3, int grid, StepRef step, int *ran, int vx, bool pipe,
                  hipStream_t s);
 // Mixed precision residual replacement after global step `step` for the
 // problems of the tables (`probs` lists them, one entry per problem): e += c
@@ -289,6 +303,8 @@ void launch_copy(const void *src, void *dst, long long n16, hipStream_t s);
 
 int spmv_tiles(const Geo &g);
 int vec_tiles(long long n);
+// vx: a lane code (cells per lane, kPipelined for the pipelined walk); a
+// width that does not divide nx falls back to one cell per lane
 Geo make_geo(int nx, int ny, int nz, int px, int py, int kz = 0, int vx = 1);
 
 }  // namespace sps

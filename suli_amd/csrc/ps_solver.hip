@@ -1040,7 +1040,8 @@ int suli_ps_set_tiles(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision
     if (!ctx) return SULI_PS_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
-    if (kz < 1 || (vx != 1 && vx != 2 && vx != 4))
+    const int w = vx & ~SULI_PS_TILES_PIPELINED;
+    if (kz < 1 || (w != 1 && w != 2 && w != 4) || (w == 1 && w != vx))
         return fail(ctx, SULI_PS_EINVAL, "tile shape kz=%d vx=%d", kz, vx);
     ctx->eng.set_tiles(g->nx, g->ny, g->nz, precision == SULI_PS_MIXED, kz, vx);
     return 0;

@@ -42,6 +42,14 @@ def _host_ptr(a: Optional[np.ndarray]):
 
 
 PRECISIONS = {"fp64": _lib.SULI_PS_FP64, "mixed": _lib.SULI_PS_MIXED}
+# flag in a tile shape's vx: the FP32 step's pipelined plane walk (two or
+# four cells per lane; SULI_PS_TILES_PIPELINED)
+TILES_PIPELINED = _lib.SULI_PS_TILES_PIPELINED
+
+
+def tiles_dict(kz: int, vx: int) -> dict:
+    """A tile shape as (kz, vx) codes it, spelt out."""
+    return {"kz": kz, "vx": vx & ~TILES_PIPELINED, "pipelined": bool(vx & TILES_PIPELINED)}
 STOP_RULES = {"reference": _lib.SULI_PS_STOP_REFERENCE, "relative": _lib.SULI_PS_STOP_RELATIVE}
 
 
@@ -386,7 +394,8 @@ class PressureSolver:
         return ms.value
 
     def autotune(self, grid: Grid, precision: str = "mixed"):
-        """(kz, vx, us per CG step) chosen for this grid size (tuned on first use)."""
+        """(kz, vx, us per CG step) chosen for this grid size (tuned on first use);
+        vx carries TILES_PIPELINED for the pipelined FP32 step."""
         kz = C.c_int32()
         vx = C.c_int32()
         us = C.c_double()
@@ -396,7 +405,8 @@ class PressureSolver:
         return kz.value, vx.value, us.value
 
     def set_tiles(self, grid: Grid, precision: str, kz: int, vx: int) -> None:
-        """Fix the tile shape used for this grid size (with autotune on)."""
+        """Fix the tile shape used for this grid size (with autotune on); vx 1,
+        2 or 4, or 2 or 4 | TILES_PIPELINED."""
         cg = _c_grid(grid)
         self._check(self.lib.suli_ps_set_tiles(self.ctx, C.byref(cg), PRECISIONS[precision], kz, vx))
 

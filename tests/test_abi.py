@@ -24,6 +24,17 @@ def test_header_lists_python_exports():
     assert header_functions() == sorted(_lib.EXPORTS)
 
 
+def test_header_constants_match_python():
+    """The integer constants the binding repeats have the header's values."""
+    text = open(HEADER).read()
+    defines = dict(re.findall(r"#define\s+(SULI_PS_[A-Z0-9_]+)\s+(-?\d+)\b", text))
+    enums = dict(re.findall(r"\b(SULI_PS_[A-Z0-9_]+)\s*=\s*(-?\d+)", text))
+    values = {**enums, **defines}
+    for name in ("SULI_PS_FP64", "SULI_PS_MIXED", "SULI_PS_STOP_REFERENCE", "SULI_PS_STOP_RELATIVE",
+                 "SULI_PS_OK", "SULI_PS_NOT_CONVERGED", "SULI_PS_EAGAIN", "SULI_PS_TILES_PIPELINED"):
+        assert int(values[name]) == getattr(_lib, name), name
+
+
 def test_library_exports_every_header_symbol():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.fail("libsuli_ps.so is not built (python -m suli_amd.build)")

@@ -12,6 +12,7 @@ import pytest
 
 import oracle
 from suli_amd import cases
+from suli_amd.solver import TILES_PIPELINED
 from tests import helpers
 
 pytestmark = pytest.mark.gpu
@@ -152,12 +153,14 @@ def test_autotune_picks_a_depth_and_keeps_bit_identity(solver):
     solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
 
 
-@pytest.mark.parametrize("kz,vx", [(3, 2), (8, 2), (3, 4), (8, 4)])
+@pytest.mark.parametrize("kz,vx", [(3, 2), (8, 2), (3, 4), (8, 4), (3, 2 | TILES_PIPELINED),
+                                   (8, 2 | TILES_PIPELINED), (3, 4 | TILES_PIPELINED), (8, 4 | TILES_PIPELINED)])
 def test_wide_lanes_vs_oracle(solver, kz, vx):
     """FP32 tiles with two or four x cells per lane (nx a multiple; walls,
-    x-periodic, y-periodic, stretched, Dirichlet lid): same tolerance
-    against the oracle, and batch results still equal solo ones."""
-    nxp = 70 if vx == 2 else 72
+    x-periodic, y-periodic, stretched, Dirichlet lid), with the step's plane
+    walk issued plane by plane or pipelined: same tolerance against the
+    oracle, and batch results still equal solo ones."""
+    nxp = 70 if vx & ~TILES_PIPELINED == 2 else 72
     cs = [cases.random_case(24, 20, 16, seed=61), cases.random_case(nxp, 33, 19, seed=62, periodic_x=True),
           cases.random_case(64, 4, 8, seed=63, periodic_y=True), cases.random_case(40, 24, 20, seed=64,
                                                                                     stretched=True),
@@ -176,3 +179,31 @@ def test_wide_lanes_vs_oracle(solver, kz, vx):
         alone, st1 = solve_alone(solver, c)
         assert np.array_equal(got[i], alone), f"problem {i}"
     solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
+
+
+def test_every_lane_kind_in_one_batch(solver):
+    """Five problems, each on another kind of the mixed step (one cell per
+    lane; two and four, plane by plane and pipelined), in one batch: every
+    step launches all five kernels.  Each result equals its solo solve bit
+    for bit and agrees with the oracle."""
+    cs = [cases.random_case(40, 20, 14, seed=71), cases.random_case(64, 18, 12, seed=72, periodic_x=True),
+          cases.random_case(48, 22, 16, seed=73, surface=True), cases.random_case(72, 16, 10, seed=74),
+          cases.random_case(264, 12, 9, seed=75, periodic_y=True)]
+    kinds = [1, 2, 2 | TILES_PIPELINED, 4, 4 | TILES_PIPELINED]
+    for c, vx in zip(cs, kinds):
+        solver.set_tiles(c.grid, "mixed", 4, vx)
+        assert solver.autotune(c.grid, "mixed")[:2] == (4, vx)
+    solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True)
+    print("batch of every lane kind", flush=True)
+    try:
+        got, sts = solve_batch(solver, cs)
+        for i, c in enumerate(cs):
+            assert sts[i].status == 0, (i, sts[i].as_dict())
+            p_or, _, rc = oracle.graddin(c.grid, c.fields, stop_rule=oracle.STOP_RELATIVE, tol=1e-12,
+                                         project_null=True)
+            assert rc == 0
+            assert helpers.rel_diff(got[i], p_or, helpers.face_mask(got[i].shape)) <= PRESSURE_RTOL, i
+            alone, _ = solve_alone(solver, c)
+            assert np.array_equal(got[i], alone), f"problem {i}"
+    finally:
+        solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)

@@ -15,6 +15,7 @@ import pytest
 
 import oracle
 from suli_amd import cases
+from suli_amd.solver import TILES_PIPELINED
 from tests import helpers
 
 pytestmark = pytest.mark.gpu
@@ -86,7 +87,7 @@ def test_tables_with_wide_lanes_and_spmv(solver):
     runs on them."""
     c = make("walls")
     out = []
-    for vx in (2, 4):
+    for vx in (2, 4, 2 | TILES_PIPELINED, 4 | TILES_PIPELINED):
         solver.set_tiles(c.grid, "mixed", 6, vx)
         solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True)
         p, st = solve(solver, c)

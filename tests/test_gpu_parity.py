@@ -16,6 +16,7 @@ import pytest
 
 import oracle
 from suli_amd import cases
+from suli_amd.solver import TILES_PIPELINED
 from tests import helpers
 
 pytestmark = pytest.mark.gpu
@@ -159,18 +160,19 @@ def test_mixed_cap_matches_fp64_iterate(solver, max_iter):
     assert diff <= 1e-5   # measured <= 7e-7 at 40x24x16
 
 
+@pytest.mark.parametrize("vx", [4, 4 | TILES_PIPELINED], ids=["vx4", "vx4-pipelined"])
 @pytest.mark.parametrize("max_iter", [5, 13, 34, 37])
-def test_mixed_cap_four_cells_per_lane(solver, max_iter):
+def test_mixed_cap_four_cells_per_lane(solver, max_iter, vx):
     """The same with tiles of four x cells per lane (the cap folds each of
-    the lane's cells)."""
+    the lane's cells), plane by plane and pipelined."""
     c = cases.random_case(40, 24, 16, seed=12)
-    print(f"mixed cap vx=4 max_iter={max_iter}", flush=True)
+    print(f"mixed cap vx={vx} max_iter={max_iter}", flush=True)
     p64, _ = solve(solver, c.grid, c.fields, stop_rule="relative", tol=1e-14, max_iter=max_iter)
-    solver.set_tiles(c.grid, "mixed", 4, 4)
+    solver.set_tiles(c.grid, "mixed", 4, vx)
     try:
         solver.configure(precision="mixed", stop_rule="relative", tol=1e-14, max_iter=max_iter,
                          verbose=False, autotune=True)
-        assert solver.autotune(c.grid, "mixed")[:2] == (4, 4)
+        assert solver.autotune(c.grid, "mixed")[:2] == (4, vx)
         f2 = c.fields.copy()
         st32 = solver.graddin(c.grid, f2)
     finally:
@@ -407,14 +409,18 @@ LANE_CASES = [
 ]
 
 
+@pytest.mark.parametrize("pipe", [False, True], ids=["planewise", "pipelined"])
 @pytest.mark.parametrize("vx,nx,px,extra", LANE_CASES,
                          ids=[f"vx{v}-nx{n}-{'px' if p else 'walls'}-{'-'.join(e) or 'plain'}"
                               for v, n, p, e in LANE_CASES])
-def test_lane_widths_vs_oracle(solver, vx, nx, px, extra):
+def test_lane_widths_vs_oracle(solver, vx, nx, px, extra, pipe):
     """Two and four cells per lane: the x neighbours of a lane's end cells
     come from the adjacent lanes, from the lane itself (walls), from the
     row's other end (periodic rows of one segment) or from a load (segment
-    ends inside a row, periodic rows of several).  Against the oracle."""
+    ends inside a row, periodic rows of several).  Both walks of the step
+    (plane by plane, pipelined), against the oracle."""
+    if pipe:
+        vx |= TILES_PIPELINED
     extra = dict(extra)
     lid = extra.pop("lid", False)
     c = cases.random_case(nx, 6, 7, seed=90 + nx + vx, periodic_x=px, **extra)
