@@ -641,6 +641,13 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
     };
     auto p_of = [&](float o, float r) -> float { return FIRST ? r : r + b * o; };
     const unsigned long long rows = co.rows(t.j, t.k0, t.k1);
+    // the fold's operands at cc (plane by plane: loaded where they are used)
+    auto fold_loads = [&](long long cc, PlaneLoads<V> &L) {
+        ldv<V>(p_new, cc, L.f0);   // p_{n-kFold}, before p_n replaces it
+        ldv<V>(fd.p[0], cc, L.f1);
+        if (!REC) ldv<V>(fd.p[1], cc, L.f2);
+        ldv<V>(c32, cc, L.cv);
+    };
     auto issue = [&](int k, long long cc, long long fxx, long long fyy, PlaneLoads<V> &L) {
         const long long up = k < nz - 1 ? plane : 0;   // the top neighbour (mirror ghost at the top)
         raw(cc + up, L.to, L.tb, L.tq);
@@ -653,20 +660,16 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
         const bool reg = row_bit(rows, k, t.k0);
         coefs_v<V>(co, reg, fxx, fyy, cc + plane, k + 1, nx, k, L.sxv, L.sys, L.syn, L.szt);
         if (PIPE) zb_v<V>(co, reg, cc, k, L.szb);
-        if (EDGE) {
+        if (EDGE && PIPE) {
             // loaded whether used or not (the row's first cell then, the
             // same address in every lane), so that the loop has no branch
-            // around loads in flight
+            // around loads in flight; plane by plane they are loaded where
+            // they are used
             const long long r0 = cc - w.ci;
             raw1(w.load_w ? r0 + w.xw : r0, L.wo, L.wb, L.wq);
             raw1(w.load_e ? r0 + w.xe : r0, L.eo, L.eb, L.eq);
         }
-        if (FOLD) {
-            ldv<V>(p_new, cc, L.f0);   // p_{n-kFold}, before p_n replaces it
-            ldv<V>(fd.p[0], cc, L.f1);
-            if (!REC) ldv<V>(fd.p[1], cc, L.f2);
-            ldv<V>(c32, cc, L.cv);
-        }
+        if (FOLD && PIPE) fold_loads(cc, L);
     };
     // r_{n-1} and p_n at the plane at hand (c), p_n at the one below (pb),
     // the z faces below (szb); for the fold, p_{n-1} and p_{n-2} (REC) at c
@@ -679,8 +682,8 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
         for (int m = 0; m < V; ++m) {
             rc[m] = r_of(o[m], rr[m], q[m]);
             pc[m] = p_of(o[m], rc[m]);
-            if (FOLD) poc[m] = copy_now(o[m]);
-            if (FOLD && REC) ppc[m] = copy_now(rr[m]);
+            if (FOLD) poc[m] = PIPE ? copy_now(o[m]) : o[m];
+            if (FOLD && REC) ppc[m] = PIPE ? copy_now(rr[m]) : rr[m];
         }
     }
     if (t.k0 > 0) {
@@ -713,8 +716,17 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
         // x neighbours of the lane's end cells: adjacent lanes, or beyond the segment
         const float ew_row = w.wrap ? lane_value(pc[V - 1], w.last) : pc[0];
         const float ee_row = w.wrap ? lane_value(pc[0], 0) : pc[V - 1];
-        const float ew = (EDGE && w.load_w) ? p_of(cur.wo, r_of(cur.wo, cur.wb, cur.wq)) : ew_row;
-        const float ee = (EDGE && w.load_e) ? p_of(cur.eo, r_of(cur.eo, cur.eb, cur.eq)) : ee_row;
+        // p_n at a cell beyond the segment's ends
+        auto edge = [&](float o, float rr, float q) { return p_of(o, r_of(o, rr, q)); };
+        auto edge_at = [&](long long x) {
+            float o = 0.0f, rr, q = 0.0f;
+            raw1(x, o, rr, q);
+            return edge(o, rr, q);
+        };
+        const long long r0 = c - w.ci;
+        float ew = ew_row, ee = ee_row;
+        if (EDGE && w.load_w) ew = PIPE ? edge(cur.wo, cur.wb, cur.wq) : edge_at(r0 + w.xw);
+        if (EDGE && w.load_e) ee = PIPE ? edge(cur.eo, cur.eb, cur.eq) : edge_at(r0 + w.xe);
         const float pw = from_lower(ew, pc[V - 1]);
         const float pe_dpp = from_upper(ee, pc[0]);
         const float pe = w.lane == w.last ? ee : pe_dpp;
@@ -728,6 +740,7 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
             y[m] = pc[m] - cur.sxv[m + 1] * east - cur.sxv[m] * west - cur.syn[m] * pnn[m] - cur.sys[m] * ps[m] -
                    cur.szt[m] * top - (PIPE ? cur.szb[m] : szb[m]) * pb[m];
         }
+        if (FOLD && !PIPE) fold_loads(c, cur);   // before p_n replaces p_{n-kFold}
         if (w.active) {
             stv<V>(q_new, c, y);
             stv<V>(p_new, c, pc);
@@ -759,8 +772,8 @@ __device__ __forceinline__ void cg_m_tilev(const Geo &g, const Tile &t, gptr<flo
             pc[m] = pt[m];
             rc[m] = rt[m];
             if (!PIPE) szb[m] = cur.szt[m];
-            if (FOLD) poc[m] = copy_now(cur.to[m]);
-            if (FOLD && REC) ppc[m] = copy_now(cur.tb[m]);
+            if (FOLD) poc[m] = PIPE ? copy_now(cur.to[m]) : cur.to[m];
+            if (FOLD && REC) ppc[m] = PIPE ? copy_now(cur.tb[m]) : cur.tb[m];
         }
         c += plane;
         fx += dfx;
