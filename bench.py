@@ -24,6 +24,9 @@ the gaps between its launches.  The fused kernel moves fewer bytes than
 that: `traffic` is its HBM bytes per launch from rocprofv3's FETCH_SIZE and
 WRITE_SIZE counters (profiles/hbm_traffic.json, calibrated on the copy
 kernel k_copy), and `traffic_achieved` those bytes at the same launch rate.
+`copy_kernel` and `read_kernel` give what a plain 256 MiB copy and a plain
+read reach on the same GPU in the same run: the HBM rates a kernel that
+mixes reads and writes can be held against.
 
 The CPU leg (`cpu_baseline`, skipped with --no-cpu) times the reference's own
 Fortran graddin (oracle/_ref, compiled from the SuLi sources) with OpenMP on
@@ -524,7 +527,10 @@ def main():
     copy_ms = solver.copy_bench(copy_bytes, reps=10)
     copy = {"kernel": "k_copy", "bytes_read": copy_bytes, "bytes_written": copy_bytes,
             "ms": copy_ms, "gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
-    log(f"copy {copy['gbs']:.0f} GB/s")
+    read_ms = solver.read_bench(copy_bytes, reps=10)
+    read = {"kernel": "k_read", "bytes_read": copy_bytes, "ms": read_ms,
+            "gbs": copy_bytes / (read_ms * 1e-3) / 1e9}
+    log(f"copy {copy['gbs']:.0f} GB/s, read {read['gbs']:.0f} GB/s")
     for spec in args.tiles:
         from suli_amd.solver import TILES_PIPELINED
         prec, _, shape = spec.partition("=")
@@ -612,6 +618,7 @@ def main():
         "oracle_rel_diff": checks["headline"] if checks else None,
         "roofline": roofline,
         "copy_kernel": copy,
+        "read_kernel": read,
         "workloads": extra,
         "cpu_baseline": cpu,
     }

@@ -473,6 +473,12 @@ int suli_ps_set_tiles(suli_ps_ctx *ctx, const suli_ps_grid *g, int32_t precision
  * Its known traffic (bytes read, bytes written) calibrates HBM counters. */
 int suli_ps_copy_bench(suli_ps_ctx *ctx, size_t bytes, int32_t reps, double *ms);
 
+/* Measurement hook with no reference counterpart: a plain device read of
+ * `bytes` (16-byte vectors, grid-stride, summed per thread) run `reps`
+ * times; returns the mean launch time in ms.  With the copy it brackets the
+ * HBM bandwidth a kernel mixing reads and writes can reach. */
+int suli_ps_read_bench(suli_ps_ctx *ctx, size_t bytes, int32_t reps, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,7 @@ EXPORTS = (
     "suli_ps_memcpy_d2h", "suli_ps_dev_zero", "suli_ps_synchronize", "suli_ps_time_spmv",
     "suli_ps_graddin_batch_dev", "suli_ps_server_create", "suli_ps_server_destroy",
     "suli_ps_server_last_error", "suli_ps_server_submit", "suli_ps_server_wait",
-    "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_autotune",
+    "suli_ps_server_get_info", "suli_ps_copy_bench", "suli_ps_read_bench", "suli_ps_autotune",
     "suli_ps_graddin_masked_dev", "suli_ps_posdin_masked_dev", "suli_ps_set_tiles",
     "suli_ps_pressh", "suli_ps_pressh_dev", "suli_ps_posdin_hydro", "suli_ps_posdin_hydro_dev",
     "suli_ps_heaviside_dev", "suli_ps_restart_read", "suli_ps_it_graddin", "suli_ps_it_graddin_dev",
@@ -158,6 +158,7 @@ def load(path: str = LIB_PATH):
     L.suli_ps_time_spmv.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.suli_ps_graddin_batch_dev.argtypes = [vp, C.POINTER(Problem), C.c_int32]
     L.suli_ps_copy_bench.argtypes = [vp, C.c_size_t, C.c_int32, C.POINTER(C.c_double)]
+    L.suli_ps_read_bench.argtypes = [vp, C.c_size_t, C.c_int32, C.POINTER(C.c_double)]
     L.suli_ps_autotune.argtypes = [vp, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.suli_ps_set_tiles.argtypes = [vp, C.POINTER(Grid), C.c_int32, C.c_int32, C.c_int32]

@@ -300,6 +300,10 @@ void launch_spmv_only(bool fp32, const ProbDev *P, const TileRef *t3, int nt3, i
 
 // Plain copy of n16 16-byte vectors (measurement only).
 void launch_copy(const void *src, void *dst, long long n16, hipStream_t s);
+// Plain read of n16 16-byte vectors (measurement only); out holds
+// kReadGrid * kThreads floats.
+constexpr int kReadGrid = 2048;
+void launch_read(const void *src, float *out, long long n16, hipStream_t s);
 
 int spmv_tiles(const Geo &g);
 int vec_tiles(long long n);

@@ -1074,4 +1074,32 @@ int suli_ps_copy_bench(suli_ps_ctx *ctx, size_t bytes, int32_t reps, double *ms)
     return rc;
 }
 
+int suli_ps_read_bench(suli_ps_ctx *ctx, size_t bytes, int32_t reps, double *ms)
+{
+    if (!ctx || !ms || reps <= 0 || bytes < 16) return SULI_PS_EINVAL;
+    const size_t n16 = bytes / 16;
+    void *a = nullptr;
+    float *out = nullptr;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess) return fail(ctx, SULI_PS_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    if (hipMalloc(&out, sizeof(float) * kReadGrid * kThreads) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ctx, SULI_PS_ENOMEM, "hipMalloc of the read bench's sums failed");
+    }
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    hipError_t e = hipMemsetAsync(a, 0, n16 * 16, s);
+    launch_read(a, out, (long long)n16, s);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) launch_read(a, out, (long long)n16, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]);
+    if (e != hipSuccess) rc = fail(ctx, SULI_PS_EHIP, "read bench: %s", hipGetErrorString(e));
+    *ms = (double)t / reps;
+    (void)hipFree(a);
+    (void)hipFree(out);
+    return rc;
+}
+
 }  // extern "C"

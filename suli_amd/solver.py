@@ -416,6 +416,12 @@ class PressureSolver:
         self._check(self.lib.suli_ps_copy_bench(self.ctx, nbytes, reps, C.byref(ms)))
         return ms.value
 
+    def read_bench(self, nbytes: int, reps: int = 20) -> float:
+        """Mean time (ms) of the library's plain device read of nbytes."""
+        ms = C.c_double()
+        self._check(self.lib.suli_ps_read_bench(self.ctx, nbytes, reps, C.byref(ms)))
+        return ms.value
+
     def synchronize(self) -> None:
         self._check(self.lib.suli_ps_synchronize(self.ctx))
 
