@@ -203,9 +203,16 @@ private:
     // the FP32 3-D table holds the problems of one lane width after another
     // (1, 2, 4 cells per lane): where each width's entries start and how many
     int sub_t3_32[kLaneKinds] = {}, cnt_t3_32[kLaneKinds] = {};
-    // launches the mixed step over every lane width's entries; the first
-    // launch carries the step counter update `next`
-    void launch_steps_m(StepRef sr, int *ran);
+    // launches the mixed step over every kind's entries; the first launch
+    // carries the step counter update `next`.  concurrent (direct launches
+    // only, never under graph capture): the kinds run on the engine's
+    // stream and up to kSideStreams more, forked from it and joined back by
+    // events, so that the launches of a step overlap; the problems of
+    // different kinds share no memory, and each launch is the same
+    int launch_steps_m(StepRef sr, int *ran, bool concurrent = false);
+    static constexpr int kSideStreams = 3;   // with the engine's own: the box's 4 hardware queues
+    hipStream_t side_[kSideStreams] = {};
+    hipEvent_t fork_ev_ = nullptr, join_ev_[kSideStreams] = {};
     // graph replay: step counter on the device, two captured groups of
     // kRefreshGrain steps (without / with the residual replacement)
     int *d_step_ = nullptr;            // two counters, alternating by group parity

@@ -176,6 +176,14 @@ int Engine::fail(int code, const std::string &msg)
 Engine::~Engine()
 {
     if (s_) (void)hipStreamSynchronize(s_);
+    for (auto &st : side_)
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+    for (auto &e : join_ev_)
+        if (e) (void)hipEventDestroy(e);
     for (auto &sl : slots_)
         if (sl.block && sl.own_block) (void)hipFree(sl.block);
     pool_.release_all();
@@ -218,6 +226,10 @@ int Engine::init(hipStream_t stream, int max_slots)
     ECHK(hipMalloc(&d_step_, 2 * sizeof(int)));
     ECHK(hipMemset(d_step_, 0, 2 * sizeof(int)));
     dstep_valid_ = true;
+    // default priority, as the engine's stream
+    for (auto &st : side_) ECHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    ECHK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    for (auto &e : join_ev_) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return 0;
 }
 
@@ -543,15 +555,28 @@ bool Engine::refresh_may_be_due(int s) const
     return false;
 }
 
-void Engine::launch_steps_m(StepRef sr, int *ran)
+int Engine::launch_steps_m(StepRef sr, int *ran, bool concurrent)
 {
-    for (int w = 0; w < kLaneKinds; ++w) {
+    int kinds[kLaneKinds], nk = 0;
+    for (int w = 0; w < kLaneKinds; ++w)
+        if (cnt_t3_32[w]) kinds[nk++] = w;
+    // kind i on stream i mod (nside + 1), 0 being the engine's own
+    const int nside = concurrent ? std::min(nk - 1, kSideStreams) : 0;
+    if (nside > 0) ECHK(hipEventRecord(fork_ev_, s_));
+    for (int i = 0; i < nk; ++i) {
+        const int w = kinds[i], q = i % (nside + 1);
+        const hipStream_t st = q == 0 ? s_ : side_[q - 1];
+        if (q > 0 && i == q) ECHK(hipStreamWaitEvent(st, fork_ev_, 0));
         const int n = cnt_t3_32[w];
-        if (!n) continue;
         launch_cg_m(d_probs_, d_tab_ + off_t3_32 + sub_t3_32[w], n, std::min(n, kMaxGrid), sr, ran, kind_vx(w),
-                    kind_pipe(w), s_);
+                    kind_pipe(w), st);
         sr.next = nullptr;   // one launch updates the counter
     }
+    for (int q = 1; q <= nside; ++q) {
+        ECHK(hipEventRecord(join_ev_[q - 1], side_[q - 1]));
+        ECHK(hipStreamWaitEvent(s_, join_ev_[q - 1], 0));
+    }
+    return 0;
 }
 
 // Kernels of global steps step_+1 .. step_+nsteps launched from the host,
@@ -572,7 +597,10 @@ int Engine::enqueue_steps_direct(int nsteps, bool timing)
         int *ran = t_open ? d_ran_ + 2 * ntimed_ : t_close ? d_ran_ + 2 * ntimed_ + 1 : nullptr;
         if (t_open) ECHK(hipEventRecord(ev_spmv_[2 * ntimed_], s_));
         if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, ran, s_);
-        if (n3_32) launch_steps_m(sr, ran);
+        if (n3_32) {
+            const int rc = launch_steps_m(sr, ran, true);
+            if (rc) return rc;
+        }
         if (t_close) {
             ECHK(hipEventRecord(ev_spmv_[2 * ntimed_ + 1], s_));
             ++ntimed_;
@@ -648,7 +676,7 @@ int Engine::capture_graphs()
             const StepRef sr{base, off, off == 1 ? next : nullptr};
             const StepRef sr_m{base, off, (off == 1 && !n3_64) ? next : nullptr};
             if (n3_64) launch_cg_a(false, d_probs_, d_tab_ + off_t3_64, n3_64, std::min(n3_64, kMaxGrid), sr, nullptr, s_);
-            if (n3_32) launch_steps_m(sr_m, nullptr);
+            if (n3_32) (void)launch_steps_m(sr_m, nullptr);   // one stream: only launches
             if (n1_64)
                 launch_cg_b(false, d_probs_, d_tab_ + off_t1_64, n1_64, std::min(n1_64, kMaxGrid),
                             StepRef{base, off, nullptr}, s_);

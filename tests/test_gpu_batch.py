@@ -182,11 +182,13 @@ def test_wide_lanes_vs_oracle(solver, kz, vx):
     solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
 
 
-def test_every_lane_kind_in_one_batch(solver):
+@pytest.mark.parametrize("graphs", [True, False], ids=["graphs", "direct"])
+def test_every_lane_kind_in_one_batch(solver, graphs):
     """Five problems, each on another kind of the mixed step (one cell per
     lane; two and four, plane by plane and pipelined), in one batch: every
-    step launches all five kernels.  Each result equals its solo solve bit
-    for bit and agrees with the oracle."""
+    step launches all five kernels (launched directly, on four streams at
+    once).  Each result equals its solo solve bit for bit and agrees with
+    the oracle."""
     cs = [cases.random_case(40, 20, 14, seed=71), cases.random_case(64, 18, 12, seed=72, periodic_x=True),
           cases.random_case(48, 22, 16, seed=73, surface=True), cases.random_case(72, 16, 10, seed=74),
           cases.random_case(264, 12, 9, seed=75, periodic_y=True)]
@@ -194,8 +196,9 @@ def test_every_lane_kind_in_one_batch(solver):
     for c, vx in zip(cs, kinds):
         solver.set_tiles(c.grid, "mixed", 4, vx)
         assert solver.autotune(c.grid, "mixed")[:2] == (4, vx)
-    solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True)
-    print("batch of every lane kind", flush=True)
+    solver.configure(precision="mixed", stop_rule="relative", tol=1e-11, verbose=False, autotune=True,
+                     graphs=graphs)
+    print(f"batch of every lane kind, graphs {graphs}", flush=True)
     try:
         got, sts = solve_batch(solver, cs)
         for i, c in enumerate(cs):
@@ -207,4 +210,4 @@ def test_every_lane_kind_in_one_batch(solver):
             alone, _ = solve_alone(solver, c)
             assert np.array_equal(got[i], alone), f"problem {i}"
     finally:
-        solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False)
+        solver.configure(precision="fp64", stop_rule="relative", tol=1e-12, verbose=False, graphs=True)
