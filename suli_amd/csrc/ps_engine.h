@@ -200,8 +200,8 @@ private:
     int stage_next_ = 0;
     // offsets of the four tables and the mixed-problem list in d_tab_/d_probs_list_
     int off_t3_64 = 0, off_t1_64 = 0, off_t3_32 = 0, off_t1_32 = 0;
-    // the FP32 3-D table holds the problems of one lane width after another
-    // (1, 2, 4 cells per lane): where each width's entries start and how many
+    // the FP32 3-D table holds the problems of one step kind after another
+    // (lane_kind): where each kind's entries start and how many
     int sub_t3_32[kLaneKinds] = {}, cnt_t3_32[kLaneKinds] = {};
     // launches the mixed step over every kind's entries; the first launch
     // carries the step counter update `next`.  concurrent (direct launches
