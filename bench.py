@@ -548,6 +548,10 @@ def main():
         fp64, headline["fp64"] = run_solves(solver, g, df, "fp64", args.steps, args.warmup, args.tol,
                                             graphs=not args.no_graphs)
         log(f"fp64 {fp64['solves_per_s']:.2f} solves/s, {fp64['iters']} iterations")
+    posdin = None
+    if args.only == "both":
+        posdin = time_posdin(solver, g, df, reps=20)
+        log(f"posdin {posdin['ms'] * 1e3:.1f} us per call, {posdin['achieved']:.0f} GB/s")
     df.free()
     solver.close()
     wanted = [] if args.no_extra else [e for e in args.extra.split(",") if e]
@@ -617,6 +621,7 @@ def main():
         "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
         "oracle_rel_diff": checks["headline"] if checks else None,
         "roofline": roofline,
+        "posdin_dev": posdin,
         "copy_kernel": copy,
         "read_kernel": read,
         "workloads": extra,
@@ -624,6 +629,24 @@ def main():
     }
     print(json.dumps(out), flush=True)
     reap_children()
+
+
+def time_posdin(solver, g, df, reps):
+    """The device-resident velocity correction (suli_ps_posdin_dev) on the
+    headline case, after its solves: mean ms per call (host clock around
+    `reps` calls, after one), against its algorithmic bytes in FP64: every
+    face of u, v and w read and written, rho and the prd1 halo box read."""
+    solver.posdin_dev(g, df)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solver.posdin_dev(g, df)
+    solver.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    faces = (g.nx + 1) * g.ny * g.nz + g.nx * (g.ny + 1) * g.nz + g.nx * g.ny * (g.nz + 1)
+    nbytes = 8 * (2 * faces + g.ncell + (g.nx + 2) * (g.ny + 2) * (g.nz + 2))
+    return {"kernel": "k_posdin", "ms": ms, "algorithmic_bytes": nbytes,
+            "achieved": nbytes / (ms * 1e-3) / 1e9, "basis": "host clock around 20 calls"}
 
 
 # the extra workloads, in the order they run: (log line, result key, runner)

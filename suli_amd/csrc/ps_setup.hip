@@ -466,48 +466,55 @@ __global__ __launch_bounds__(kThreads) void k_unscale(Geo g, const double *e, co
 //   v(i,j,k) -= dt/dy (p(j)-p(j-1)) / rhoy,   j = 1..ny+1
 //   w(i,j,k) -= dt/dz (p(k)-p(k-1)) / rhoz,   k = 1..nz+1 (dz -> dzz(k) on stretched grids)
 // With_z false: the u and v faces only, the part both branches share.
-__global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w, bool with_z)
+//
+// One cell (i, j, k) corrects its faces at its low side in x, y and z (face
+// index f = i, j, k), and the cells of the last column, row and plane the
+// face at their high side as well (f = nx, ny, nz): every face once, with no
+// index divisions.  Each face's arithmetic is the reference's, in its order.
+__device__ __forceinline__ void posdin_cell(const Geo &g, const SetupIn &in, double *u, double *v, double *w,
+                                            bool with_z, int i, int j, int k, double aux1, double aux2)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const long long nfx = (long long)(nx + 1) * ny * nz;
-    const long long nfy = (long long)nx * (ny + 1) * nz;
-    const long long nfz = with_z ? (long long)nx * ny * (nz + 1) : 0;
-    const long long stride = (long long)gridDim.x * kThreads;
-    const double aux1 = in.dt / in.dx, aux2 = in.dt / in.dy;
     const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
-    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
-        if (t < nfx) {
-            const int f = (int)(t % (nx + 1));
-            const long long jk = t / (nx + 1);
-            const int j = (int)(jk % ny), k = (int)(jk / ny);
-            const int I = f + 1, J = j + 1, K = k + 1;
-            const long long ip = I + pb * J + pbb * K;
-            const long long iu = I + (long long)(nx + 3) * (J + (long long)(ny + 2) * K);
-            const double rf = rho_face_x(g, in, f, j, k);
-            if (!closed_x(g, in, f, j, k)) u[iu] = u[iu] - aux1 * (in.prd1[ip] - in.prd1[ip - 1]) / rf;
-        } else if (t < nfx + nfy) {
-            const long long s = t - nfx;
-            const int i = (int)(s % nx);
-            const long long jk = s / nx;
-            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
-            const int I = i + 1, J = f + 1, K = k + 1;
-            const long long ip = I + pb * J + pbb * K;
-            const long long iv = I + (long long)(nx + 2) * (J + (long long)(ny + 3) * K);
-            const double rf = rho_face_y(g, in, i, f, k);
-            if (!closed_y(g, in, i, f, k)) v[iv] = v[iv] - aux2 * (in.prd1[ip] - in.prd1[ip - pb]) / rf;
-        } else {
-            const long long s = t - nfx - nfy;
-            const int i = (int)(s % nx);
-            const long long jk = s / nx;
-            const int j = (int)(jk % ny), f = (int)(jk / ny);
-            const int I = i + 1, J = j + 1, K = f + 1;
-            const long long ip = I + pb * J + pbb * K;
-            const long long iw = I + (long long)(nx + 2) * (J + (long long)(ny + 2) * K);
-            const double rf = rho_face_z(g, in, i, j, f);
-            const double dzk = in.dzz ? in.dzz[K] : in.dzu;
-            if (!closed_z(g, in, i, j, f)) w[iw] = w[iw] - in.dt / dzk * (in.prd1[ip] - in.prd1[ip - pbb]) / rf;
-        }
+    const long long ip = (i + 1) + pb * (j + 1) + pbb * (k + 1);   // the cell in the prd1 box
+    auto face_u = [&](int f, long long ipf) {
+        const long long iu = (f + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(ny + 2) * (k + 1));
+        const double rf = rho_face_x(g, in, f, j, k);
+        if (!closed_x(g, in, f, j, k)) u[iu] = u[iu] - aux1 * (in.prd1[ipf] - in.prd1[ipf - 1]) / rf;
+    };
+    auto face_v = [&](int f, long long ipf) {
+        const long long iv = (i + 1) + (long long)(nx + 2) * ((f + 1) + (long long)(ny + 3) * (k + 1));
+        const double rf = rho_face_y(g, in, i, f, k);
+        if (!closed_y(g, in, i, f, k)) v[iv] = v[iv] - aux2 * (in.prd1[ipf] - in.prd1[ipf - pb]) / rf;
+    };
+    auto face_w = [&](int f, long long ipf) {
+        const int K = f + 1;
+        const long long iw = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * K);
+        const double rf = rho_face_z(g, in, i, j, f);
+        const double dzk = in.dzz ? in.dzz[K] : in.dzu;
+        if (!closed_z(g, in, i, j, f)) w[iw] = w[iw] - in.dt / dzk * (in.prd1[ipf] - in.prd1[ipf - pbb]) / rf;
+    };
+    face_u(i, ip);
+    if (i == nx - 1) face_u(nx, ip + 1);
+    face_v(j, ip);
+    if (j == ny - 1) face_v(ny, ip + pb);
+    if (with_z) {
+        face_w(k, ip);
+        if (k == nz - 1) face_w(nz, ip + pbb);
     }
+}
+
+// One thread per cell of 64 x 4 cell tiles of a plane: grid x over the
+// tiles along x, y over those along y, z over the planes, y and z striding
+// over what lies beyond the grid's 65535 limit.
+__global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w, bool with_z)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (i >= g.nx) return;
+    const double aux1 = in.dt / in.dx, aux2 = in.dt / in.dy;
+    for (int k = blockIdx.z; k < g.nz; k += gridDim.z)
+        for (int j = blockIdx.y * 4 + (threadIdx.x >> 6); j < g.ny; j += gridDim.y * 4)
+            posdin_cell(g, in, u, v, w, with_z, i, j, k, aux1, aux2);
 }
 
 // pressh, the hydrostatic pressure of t_press = 0 (SuLi-IC/9_grad.f90:
@@ -1112,11 +1119,15 @@ void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1
     hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1, S, solid);
 }
 
+// k_posdin's grid: 64 x 4 cell tiles of every plane
+static dim3 posdin_grid(const Geo &g)
+{
+    return dim3((unsigned)((g.nx + 63) / 64), (unsigned)std::min((g.ny + 3) / 4, 65535), (unsigned)std::min(g.nz, 65535));
+}
+
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s)
 {
-    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
-                         (long long)g.nx * g.ny * (g.nz + 1);
-    hipLaunchKernelGGL(k_posdin, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, u, v, w, true);
+    hipLaunchKernelGGL(k_posdin, posdin_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, true);
 }
 
 void launch_heaviside(long long n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
@@ -1175,7 +1186,7 @@ void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, doub
                          hipStream_t s)
 {
     const long long nfxy = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz;
-    hipLaunchKernelGGL(k_posdin, dim3(grid_for(nfxy)), dim3(kThreads), 0, s, g, in, u, v, w, false);
+    hipLaunchKernelGGL(k_posdin, posdin_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, false);
     const long long nplane = (long long)(g.nx + 2) * (g.ny + 2);
     hipLaunchKernelGGL(k_posdin_hydro_w, dim3((unsigned)((nplane + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
                        g, u, v, hs, in.dz, in.dzu, in.dx, in.dy, w);
