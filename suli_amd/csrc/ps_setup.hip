@@ -30,12 +30,12 @@ __device__ __forceinline__ double face_x(const Geo &g, const double *field, int 
 {
     const double *r = field + ((long long)k * g.ny + j) * g.nx;
     const int nx = g.nx;
-    if (f >= 1 && f <= nx - 1) return (r[f] + r[f - 1]) * 0.5;
-    if (!g.px) {
-        const int a = f == 0 ? 1 : nx - 1;
-        return (r[a] + r[a - 1]) * 0.5;
-    }
-    return (r[0] + r[nx - 1]) * 0.5;
+    // the two cells as selects, not branches, so the loads issue together
+    // with the caller's others (a sum of two is the same either way round)
+    const bool inner = f >= 1 && f <= nx - 1;
+    const int a = inner ? f : (g.px ? 0 : (f == 0 ? 1 : nx - 1));
+    const int b = inner || !g.px ? a - 1 : nx - 1;
+    return (r[a] + r[b]) * 0.5;
 }
 
 __device__ __forceinline__ double face_y(const Geo &g, const double *field, int i, int f, int k)
@@ -52,12 +52,13 @@ __device__ __forceinline__ double face_z(const Geo &g, const double *field, cons
     const double *r = field + (long long)j * g.nx + i;
     const int nz = g.nz;
     const int a = f == 0 ? 1 : (f == nz ? nz - 1 : f);
+    const double r1 = r[a * plane], r0 = r[(a - 1) * plane];
     if (dz) {
         // Fortran face a+1 between cells a and a+1 (1-based): weights dz(a+1), dz(a)
         const double d1 = dz[a + 1], d0 = dz[a];
-        return (r[a * plane] * d1 + r[(a - 1) * plane] * d0) / (d1 + d0);
+        return (r1 * d1 + r0 * d0) / (d1 + d0);
     }
-    return (r[a * plane] + r[(a - 1) * plane]) * 0.5;
+    return (r1 + r0) * 0.5;
 }
 
 __device__ __forceinline__ double rho_face_x(const Geo &g, const SetupIn &in, int f, int j, int k)
@@ -477,44 +478,81 @@ __device__ __forceinline__ void posdin_cell(const Geo &g, const SetupIn &in, dou
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
     const long long ip = (i + 1) + pb * (j + 1) + pbb * (k + 1);   // the cell in the prd1 box
-    auto face_u = [&](int f, long long ipf) {
-        const long long iu = (f + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(ny + 2) * (k + 1));
-        const double rf = rho_face_x(g, in, f, j, k);
-        if (!closed_x(g, in, f, j, k)) u[iu] = u[iu] - aux1 * (in.prd1[ipf] - in.prd1[ipf - 1]) / rf;
-    };
-    auto face_v = [&](int f, long long ipf) {
-        const long long iv = (i + 1) + (long long)(nx + 2) * ((f + 1) + (long long)(ny + 3) * (k + 1));
-        const double rf = rho_face_y(g, in, i, f, k);
-        if (!closed_y(g, in, i, f, k)) v[iv] = v[iv] - aux2 * (in.prd1[ipf] - in.prd1[ipf - pb]) / rf;
-    };
-    auto face_w = [&](int f, long long ipf) {
-        const int K = f + 1;
-        const long long iw = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * K);
-        const double rf = rho_face_z(g, in, i, j, f);
-        const double dzk = in.dzz ? in.dzz[K] : in.dzu;
-        if (!closed_z(g, in, i, j, f)) w[iw] = w[iw] - in.dt / dzk * (in.prd1[ipf] - in.prd1[ipf - pbb]) / rf;
-    };
-    face_u(i, ip);
-    if (i == nx - 1) face_u(nx, ip + 1);
-    face_v(j, ip);
-    if (j == ny - 1) face_v(ny, ip + pb);
+    const long long iu = (i + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(ny + 2) * (k + 1));
+    const long long iv = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 3) * (k + 1));
+    const long long iw = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1));
+    const long long wplane = (long long)(nx + 2) * (ny + 2);
+    const bool hx = i == nx - 1, hy = j == ny - 1, hz = with_z && k == nz - 1;
+    const double *P = in.prd1;
+
+    // Every load before the first store: the stores could alias the inputs
+    // as far as the compiler knows, so loads after a store would wait for it
+    // and each face would cost the cell another round trip.
+    const double dz0 = with_z ? (in.dzz ? in.dzz[k + 1] : in.dzu) : 1.0;
+    const double pc = P[ip], pxm = P[ip - 1], pym = P[ip - pb];
+    const double u0 = u[iu], v0 = v[iv];
+    const double rx0 = rho_face_x(g, in, i, j, k), ry0 = rho_face_y(g, in, i, j, k);
+    const bool cx0 = closed_x(g, in, i, j, k), cy0 = closed_y(g, in, i, j, k);
+    // face i + 1 in x, stored by the last column only: in bounds for every
+    // lane (the box and u have a column beyond it) and on the lines just
+    // read, so every lane loads it rather than branch
+    const double pxp = P[ip + 1], u1 = u[iu + 1];
+    const double rx1 = rho_face_x(g, in, i + 1, j, k);
+    const bool cx1 = !hx || closed_x(g, in, nx, j, k);
+    double pzm = 0, w0 = 0, rz0 = 1;
+    bool cz0 = true;
     if (with_z) {
-        face_w(k, ip);
-        if (k == nz - 1) face_w(nz, ip + pbb);
+        pzm = P[ip - pbb];
+        w0 = w[iw];
+        rz0 = rho_face_z(g, in, i, j, k);
+        cz0 = closed_z(g, in, i, j, k);
+    }
+
+    // face f between box cells f and f + 1 of its direction (0-based f),
+    // in the reference's operation order
+    if (!cx0) u[iu] = u0 - aux1 * (pc - pxm) / rx0;
+    if (!cx1) u[iu + 1] = u1 - aux1 * (pxp - pc) / rx1;
+    if (!cy0) v[iv] = v0 - aux2 * (pc - pym) / ry0;
+    if (!cz0) w[iw] = w0 - in.dt / dz0 * (pc - pzm) / rz0;
+
+    // The high faces in y and z: j and k are the same on every lane of a
+    // wave, so only the waves of the last row and plane come here, and
+    // their loads after the stores above cost them one more round trip.
+    if (hy) {
+        const double pyp = P[ip + pb], v1 = v[iv + (nx + 2)];
+        const double ry1 = rho_face_y(g, in, i, ny, k);
+        if (!closed_y(g, in, i, ny, k)) v[iv + (nx + 2)] = v1 - aux2 * (pyp - pc) / ry1;
+    }
+    if (hz) {
+        const double pzp = P[ip + pbb], w1 = w[iw + wplane];
+        const double rz1 = rho_face_z(g, in, i, j, nz);
+        const double dz1 = in.dzz ? in.dzz[nz + 1] : in.dzu;
+        if (!closed_z(g, in, i, j, nz)) w[iw + wplane] = w1 - in.dt / dz1 * (pzp - pc) / rz1;
     }
 }
 
-// One thread per cell of 64 x 4 cell tiles of a plane: grid x over the
-// tiles along x, y over those along y, z over the planes, y and z striding
-// over what lies beyond the grid's 65535 limit.
+// One workgroup per tile of 64 x 4 cells of a plane, one thread per cell;
+// tiles are numbered x fastest, then y, then the planes.  xcd_tile gives
+// each XCD one contiguous run of tiles, so the lines that x- and y-adjacent
+// tiles share (rows of u, v, w and the p box do not start on a line) and
+// the plane below are read into its own L2 once; in plain block order
+// adjacent tiles run on different XCDs and both fetch those lines (a third
+// more reads than the arrays hold on the 256x128x96 tank).  Grids with more
+// tiles than kPosdinMaxGrid take the rest in further passes.
+constexpr int kPosdinMaxGrid = 1 << 20;
 __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w, bool with_z)
 {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    if (i >= g.nx) return;
+    const int tx = (g.nx + 63) / 64, ty = (g.ny + 3) / 4;
+    const int ntile = tx * ty * g.nz;   // at most a quarter of the cells
+    const int G = gridDim.x;
     const double aux1 = in.dt / in.dx, aux2 = in.dt / in.dy;
-    for (int k = blockIdx.z; k < g.nz; k += gridDim.z)
-        for (int j = blockIdx.y * 4 + (threadIdx.x >> 6); j < g.ny; j += gridDim.y * 4)
-            posdin_cell(g, in, u, v, w, with_z, i, j, k, aux1, aux2);
+    // a wave is one row of its tile, so j is the same on all its lanes
+    const int row = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int t = xcd_tile(blockIdx.x, G); t < ntile; t += G) {
+        const int bx = t % tx, r = t / tx;
+        const int i = bx * 64 + (threadIdx.x & 63), j = (r % ty) * 4 + row, k = r / ty;
+        if (i < g.nx && j < g.ny) posdin_cell(g, in, u, v, w, with_z, i, j, k, aux1, aux2);
+    }
 }
 
 // pressh, the hydrostatic pressure of t_press = 0 (SuLi-IC/9_grad.f90:
@@ -1119,10 +1157,12 @@ void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1
     hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1, S, solid);
 }
 
-// k_posdin's grid: 64 x 4 cell tiles of every plane
+// k_posdin's grid: a workgroup per 64 x 4 cell tile of every plane, up to
+// kPosdinMaxGrid
 static dim3 posdin_grid(const Geo &g)
 {
-    return dim3((unsigned)((g.nx + 63) / 64), (unsigned)std::min((g.ny + 3) / 4, 65535), (unsigned)std::min(g.nz, 65535));
+    const long long ntile = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
+    return dim3((unsigned)std::min<long long>(ntile, kPosdinMaxGrid));
 }
 
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s)
