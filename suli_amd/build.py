@@ -34,6 +34,7 @@ UNITS = {
     "ps_solver.hip": [],
     "ps_server.hip": [],
     "ps_restart.cpp": [],
+    "ps_transfer.cpp": [],
 }
 
 

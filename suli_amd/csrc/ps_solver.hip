@@ -7,14 +7,17 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/suli_ps.h"
 #include "ps_common.h"
 #include "ps_engine.h"
 #include "ps_internal.h"
+#include "ps_transfer.h"
 
 using namespace sps;
 
@@ -94,6 +97,7 @@ struct suli_ps_ctx {
     std::string err;
     Engine eng;
     Staging st;
+    HostPipe pipe;                 // pinned staging of the host-pointer copies
     std::vector<int> last_slots;   // slots of the last solve (kept for time_spmv)
     hipEvent_t ev[2] = {nullptr, nullptr};
     void *scratch = nullptr;       // device address of `scratch` on this device
@@ -120,6 +124,50 @@ int fail(suli_ps_ctx *ctx, int code, const char *fmt, ...)
             return fail((ctx), SULI_PS_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
                         __LINE__);                                                                           \
     } while (0)
+
+// A host array of n doubles and its device copy, for HostPipe (n = 0 for
+// an array the call does not have).
+HostSeg seg(double *dev, const double *host, size_t n)
+{
+    return HostSeg{dev, const_cast<double *>(host), host ? n * sizeof(double) : 0};
+}
+
+// Host -> device copies of a host-pointer call, ahead of its work on the
+// context's stream.
+int upload(suli_ps_ctx *ctx, std::initializer_list<HostSeg> segs)
+{
+    HIPCHK(ctx, ctx->pipe.upload(ctx->device, ctx->stream, segs.begin(), (int)segs.size()));
+    return 0;
+}
+
+// Device -> host copies after the work on the context's stream; the host
+// arrays hold the results on return.
+int download(suli_ps_ctx *ctx, std::initializer_list<HostSeg> segs)
+{
+    HIPCHK(ctx, ctx->pipe.download(ctx->device, ctx->stream, segs.begin(), (int)segs.size()));
+    return 0;
+}
+
+// download(), and posdin's prd = prd1 (10_pos.f90:95) on the host arrays,
+// done on a thread of its own while the copies run
+int download_copy_prd(suli_ps_ctx *ctx, std::initializer_list<HostSeg> segs, double *prd, const double *prd1,
+                      size_t nbox)
+{
+    const bool copy = prd && prd != prd1;
+    std::thread t;
+    if (copy) {
+        try {
+            t = std::thread([=] { std::memcpy(prd, prd1, sizeof(double) * nbox); });
+        } catch (const std::exception &) {
+        }
+    }
+    const int rc = download(ctx, segs);
+    if (t.joinable())
+        t.join();
+    else if (copy)
+        std::memcpy(prd, prd1, sizeof(double) * nbox);
+    return rc;
+}
 
 suli_ps_opts default_opts()
 {
@@ -569,26 +617,17 @@ int suli_ps_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *u, co
     const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const bool mask = g->obst_t != 0 && g->ibm_t == 2;
-    hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * (nx + 3) * (ny + 2) * (nz + 2), hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * (nx + 2) * (ny + 3) * (nz + 2), hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * (nx + 2) * (ny + 2) * (nz + 3), hipMemcpyHostToDevice, s));
-    if (rho) HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    if (mask) {
-        HIPCHK(ctx, hipMemcpyAsync(st.obsx, obs_lsx, sizeof(double) * nfx, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.obsy, obs_lsy, sizeof(double) * nfy, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.obsz, obs_lsz, sizeof(double) * nfz, hipMemcpyHostToDevice, s));
-    }
-    if (dz) {
-        HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-    }
+    const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
+                 nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
+    int urc = upload(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw), seg(st.rho, rho, n),
+                           seg(st.prd1, prd1, nbox), seg(st.obsx, obs_lsx, mask ? nfx : 0),
+                           seg(st.obsy, obs_lsy, mask ? nfy : 0), seg(st.obsz, obs_lsz, mask ? nfz : 0),
+                           seg(st.dz, dz, nz + 2), seg(st.dzz, dzz, nz + 2)});
+    if (urc) return urc;
     rc = run_graddin_dev(ctx, g, st.u, st.v, st.w, rho ? st.rho : nullptr, st.obsx, st.obsy, st.obsz,
                          dz ? st.dz : nullptr, dz ? st.dzz : nullptr, st.prd1, stats);
     if (rc < 0) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
+    if ((urc = download(ctx, {seg(st.prd1, prd1, nbox)}))) return urc;
     if (rho) {
         Staging::Left &l = st.left;
         l.grid = *g;
@@ -625,27 +664,13 @@ int suli_ps_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
-    hipStream_t s = ctx->stream;
-    if (!reuse) {
-        HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-        if (dz) {
-            HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-            HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-        }
-    }
+    if (!reuse && (rc = upload(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw), seg(st.rho, rho, n),
+                                     seg(st.prd1, prd1, nbox), seg(st.dz, dz, nz + 2), seg(st.dzz, dzz, nz + 2)})))
+        return rc;
     rc = run_posdin_dev(ctx, g, st.u, st.v, st.w, st.rho, st.prd1, dz ? st.dz : nullptr, dz ? st.dzz : nullptr,
                         nullptr);
     if (rc) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);  // prd = prd1 (10_pos.f90:95)
-    return 0;
+    return download_copy_prd(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw)}, prd, prd1, nbox);
 }
 
 int suli_ps_pressh_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, const double *dz, double gz,
@@ -667,14 +692,9 @@ int suli_ps_pressh(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho, c
     Staging &st = ctx->st;
     const size_t n = (size_t)g->nx * g->ny * g->nz;
     const size_t nbox = (size_t)(g->nx + 2) * (g->ny + 2) * (g->nz + 2);
-    hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    if (dz) HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (g->nz + 2), hipMemcpyHostToDevice, s));
+    if ((rc = upload(ctx, {seg(st.rho, rho, n), seg(st.prd1, prd1, nbox), seg(st.dz, dz, g->nz + 2)}))) return rc;
     if ((rc = run_pressh_dev(ctx, g, st.rho, dz ? st.dz : nullptr, gz, st.prd1))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    return 0;
+    return download(ctx, {seg(st.prd1, prd1, nbox)});
 }
 
 int suli_ps_it_graddin_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, const double *u,
@@ -708,21 +728,14 @@ int suli_ps_it_graddin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, do
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
-    hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.dzx, dzx, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.dzy, dzy, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.dz3, dzz, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.eta1, eta1, sizeof(double) * (nx + 2) * (ny + 2), hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.deltah, deltah, sizeof(double) * nx * ny, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    int urc = upload(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw), seg(st.dzx, dzx, nu),
+                           seg(st.dzy, dzy, nv), seg(st.dz3, dzz, nw), seg(st.eta1, eta1, (size_t)(nx + 2) * (ny + 2)),
+                           seg(st.deltah, deltah, (size_t)nx * ny), seg(st.prd1, prd1, nbox)});
+    if (urc) return urc;
     rc = run_it_graddin_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.dzx, st.dzy, st.dz3, st.eta1, st.deltah,
                             st.prd1, stats);
     if (rc < 0) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(prd1, st.prd1, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
+    if ((urc = download(ctx, {seg(st.prd1, prd1, nbox)}))) return urc;
     return rc;
 }
 
@@ -740,23 +753,14 @@ int suli_ps_it_posdin(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, dou
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3), ne = (size_t)(nx + 2) * (ny + 2);
-    hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.dz3, dzz, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    if (eta1) HIPCHK(ctx, hipMemcpyAsync(st.eta1, eta1, sizeof(double) * ne, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    if ((rc = upload(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw), seg(st.dz3, dzz, nw),
+                           seg(st.eta1, eta1, ne), seg(st.prd1, prd1, nbox)})))
+        return rc;
     if ((rc = run_it_posdin_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.dz3, eta1 ? st.eta1 : nullptr, st.prd1,
                                 nullptr)))
         return rc;
-    HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
-    if (eta1) HIPCHK(ctx, hipMemcpyAsync(eta1, st.eta1, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);
-    return 0;
+    return download_copy_prd(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw), seg(st.eta1, eta1, ne)},
+                             prd, prd1, nbox);
 }
 
 int suli_ps_heaviside_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *ls, double eps, double rho_f2,
@@ -822,23 +826,14 @@ int suli_ps_exp_press(suli_ps_ctx *ctx, const suli_ps_grid *g, const double *rho
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t n = (size_t)nx * ny * nz, nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
-    hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.box2, prd0, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    if (dz) {
-        HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemcpyAsync(st.dzz, dzz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
-    }
+    if ((rc = upload(ctx, {seg(st.rho, rho, n), seg(st.prd1, prd1, nbox), seg(st.box2, prd0, nbox),
+                           seg(st.dz, dz, nz + 2), seg(st.dzz, dzz, nz + 2)})))
+        return rc;
     if ((rc = suli_ps_exp_press_dev(ctx, g, st.rho, st.prd1, st.box2, dz ? st.dz : nullptr, dz ? st.dzz : nullptr,
                                     st.obsx, st.obsy, st.obsz)))
         return rc;
-    HIPCHK(ctx, hipMemcpyAsync(prd0, st.box2, sizeof(double) * nbox, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(px, st.obsx, sizeof(double) * nfx, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(py, st.obsy, sizeof(double) * nfy, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(pz, st.obsz, sizeof(double) * nfz, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    return 0;
+    return download(ctx, {seg(st.box2, prd0, nbox), seg(st.obsx, px, nfx), seg(st.obsy, py, nfy),
+                          seg(st.obsz, pz, nfz)});
 }
 
 int suli_ps_it_posdin_layers_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double tetah, double gz, double *u,
@@ -865,23 +860,16 @@ int suli_ps_it_posdin_layers(suli_ps_ctx *ctx, const suli_ps_grid *g, double tet
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3), ne = (size_t)(nx + 2) * (ny + 2);
-    hipStream_t s = ctx->stream;
-    struct Io {
-        double *dev;
-        double *host;
-        size_t n;
-    } io[] = {{st.u, u, nu},     {st.v, v, nv},     {st.w, w, nw},     {st.box2, dz, nbox},
-              {st.dzx, dzx, nu}, {st.dzy, dzy, nv}, {st.dz3, dzz, nw}, {st.eta1, eta1, ne}};
-    for (const Io &a : io) HIPCHK(ctx, hipMemcpyAsync(a.dev, a.host, sizeof(double) * a.n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.eta0, eta0, sizeof(double) * ne, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
+    // what the call updates, both ways, and what it only reads
+    const std::initializer_list<HostSeg> io = {seg(st.u, u, nu),     seg(st.v, v, nv),     seg(st.w, w, nw),
+                                               seg(st.box2, dz, nbox), seg(st.dzx, dzx, nu), seg(st.dzy, dzy, nv),
+                                               seg(st.dz3, dzz, nw),  seg(st.eta1, eta1, ne)};
+    if ((rc = upload(ctx, io)) || (rc = upload(ctx, {seg(st.eta0, eta0, ne), seg(st.prd1, prd1, nbox)})))
+        return rc;
     if ((rc = run_it_posdin_layers_dev(ctx, g, tetah, gz, st.u, st.v, st.w, st.box2, st.dzx, st.dzy, st.dz3,
                                        st.eta0, st.eta1, st.prd1, nullptr)))
         return rc;
-    for (const Io &a : io) HIPCHK(ctx, hipMemcpyAsync(a.host, a.dev, sizeof(double) * a.n, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);
-    return 0;
+    return download_copy_prd(ctx, io, prd, prd1, nbox);
 }
 
 int suli_ps_posdin_hydro_dev(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, double *v, double *w,
@@ -907,22 +895,12 @@ int suli_ps_posdin_hydro(suli_ps_ctx *ctx, const suli_ps_grid *g, double *u, dou
     const size_t nbox = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
     const size_t nu = (size_t)(nx + 3) * (ny + 2) * (nz + 2), nv = (size_t)(nx + 2) * (ny + 3) * (nz + 2),
                  nw = (size_t)(nx + 2) * (ny + 2) * (nz + 3);
-    hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(st.u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.v, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.w, w, sizeof(double) * nw, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.hs, hs, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, hipMemcpyAsync(st.prd1, prd1, sizeof(double) * nbox, hipMemcpyHostToDevice, s));
-    if (dz) HIPCHK(ctx, hipMemcpyAsync(st.dz, dz, sizeof(double) * (nz + 2), hipMemcpyHostToDevice, s));
+    if ((rc = upload(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw), seg(st.rho, rho, n),
+                           seg(st.hs, hs, n), seg(st.prd1, prd1, nbox), seg(st.dz, dz, nz + 2)})))
+        return rc;
     rc = run_posdin_hydro_dev(ctx, g, st.u, st.v, st.w, st.rho, st.hs, st.prd1, dz ? st.dz : nullptr, nullptr);
     if (rc) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(u, st.u, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(v, st.v, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipMemcpyAsync(w, st.w, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    if (prd && prd != prd1) std::memcpy(prd, prd1, sizeof(double) * nbox);  // prd = prd1 (10_pos.f90:95)
-    return 0;
+    return download_copy_prd(ctx, {seg(st.u, u, nu), seg(st.v, v, nv), seg(st.w, w, nw)}, prd, prd1, nbox);
 }
 
 int suli_ps_graddin_batch_dev(suli_ps_ctx *ctx, suli_ps_problem *probs, int32_t count)
