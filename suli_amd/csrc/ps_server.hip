@@ -28,6 +28,7 @@
 
 #include "../../include/suli_ps.h"
 #include "ps_common.h"
+#include "ps_transfer.h"
 #include "ps_engine.h"
 #include "ps_internal.h"
 
@@ -94,6 +95,7 @@ struct suli_ps_server {
     suli_ps_server_opts opts{};
     int device = 0;
     hipStream_t s_dev = nullptr, s_copy = nullptr;
+    HostPipe pipe;   // pinned staging of the copy thread's copies (that thread only)
     Engine eng;
     std::mutex m;
     std::condition_variable cv_copy, cv_dev, cv_done;
@@ -215,7 +217,8 @@ void suli_ps_server::copy_loop()
             Request &r = *down;
             const size_t nbox = (size_t)(r.g.nx + 2) * (r.g.ny + 2) * (r.g.nz + 2);
             CgScalars fin{};
-            hipError_t e = hipMemcpyAsync(r.prd1, r.dprd1, nbox * sizeof(double), hipMemcpyDeviceToHost, s_copy);
+            const HostSeg res{r.dprd1, r.prd1, nbox * sizeof(double)};
+            hipError_t e = pipe.download(device, s_copy, &res, 1);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(&fin, eng.device_status(r.slot), sizeof fin, hipMemcpyDeviceToHost, s_copy);
             if (e == hipSuccess) e = hipStreamSynchronize(s_copy);
@@ -265,9 +268,11 @@ void suli_ps_server::copy_loop()
                 {r.ddz, r.dz, nz + 2},
                 {r.ddzz, r.dzz, nz + 2},
             };
+            HostSeg segs[sizeof cps / sizeof cps[0]];
+            int nseg = 0;
             for (const Cp &c : cps)
-                if (he == hipSuccess && c.dst && c.src)
-                    he = hipMemcpyAsync(c.dst, c.src, c.count * sizeof(double), hipMemcpyHostToDevice, s_copy);
+                if (c.dst && c.src) segs[nseg++] = HostSeg{c.dst, const_cast<double *>(c.src), c.count * sizeof(double)};
+            he = pipe.upload(device, s_copy, segs, nseg);
             if (he == hipSuccess) he = hipEventRecord(r.ev_up, s_copy);
         }
         std::lock_guard<std::mutex> lk(m);
@@ -449,6 +454,7 @@ void suli_ps_server_destroy(suli_ps_server *srv)
     srv->t_copy.join();
     (void)hipStreamSynchronize(srv->s_dev);
     (void)hipStreamSynchronize(srv->s_copy);
+    srv->pipe.release();
     for (auto &kv : srv->reqs) {
         Request *r = kv.second;
         if (r->ev_up) (void)hipEventDestroy(r->ev_up);

@@ -16,8 +16,8 @@ struct Chunk {
     size_t bytes;
 };
 
-// One worker's chunks, host -> device: each chunk is copied into the pinned
-// buffer whose previous DMA has finished, then sent.
+// One worker's chunks, host -> device on stream st: each chunk is copied
+// into the pinned buffer whose previous DMA has finished, then sent.
 hipError_t upload_chunks(hipStream_t st, char *const pin[2], hipEvent_t const ev[2], const std::vector<Chunk> &cs)
 {
     hipError_t e = hipSuccess;
@@ -31,8 +31,9 @@ hipError_t upload_chunks(hipStream_t st, char *const pin[2], hipEvent_t const ev
     return e;
 }
 
-// One worker's chunks, device -> host: the DMA of chunk k + 1 runs while
-// chunk k is copied out of its pinned buffer.
+// One worker's chunks, device -> host on stream st (after the work already
+// on it): the DMA of chunk k + 1 runs while chunk k is copied out of its
+// pinned buffer.
 hipError_t download_chunks(hipStream_t st, char *const pin[2], hipEvent_t const ev[2], const std::vector<Chunk> &cs)
 {
     // the buffers may still be the source of an earlier upload's DMA
@@ -60,24 +61,19 @@ HostPipe::~HostPipe() { release(); }
 void HostPipe::release()
 {
     for (Worker &w : w_) {
-        if (w.st) (void)hipStreamSynchronize(w.st);
+        // no DMA may still use a buffer when it is freed
+        for (hipEvent_t &e : w.ev)
+            if (e) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
         for (char *&p : w.pin)
             if (p) {
                 (void)hipHostFree(p);
                 p = nullptr;
             }
-        for (hipEvent_t &e : w.ev)
-            if (e) {
-                (void)hipEventDestroy(e);
-                e = nullptr;
-            }
-        if (w.join) (void)hipEventDestroy(w.join);
-        if (w.st) (void)hipStreamDestroy(w.st);
-        w.join = nullptr;
-        w.st = nullptr;
     }
-    if (fork_) (void)hipEventDestroy(fork_);
-    fork_ = nullptr;
     ready_ = false;
     device_ = -1;
 }
@@ -86,15 +82,12 @@ hipError_t HostPipe::ensure(int device)
 {
     if (ready_ && device_ == device) return hipSuccess;
     release();
-    hipError_t e = hipEventCreateWithFlags(&fork_, hipEventDisableTiming);
-    for (Worker &w : w_) {
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    for (Worker &w : w_)
         for (int b = 0; b < 2 && e == hipSuccess; ++b) {
             e = hipHostMalloc(reinterpret_cast<void **>(&w.pin[b]), kChunk, hipHostMallocDefault);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev[b], hipEventDisableTiming);
         }
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&w.join, hipEventDisableTiming);
-    }
     if (e != hipSuccess) {
         release();
         return e;
@@ -148,15 +141,9 @@ hipError_t HostPipe::run(int device, hipStream_t s, const HostSeg *segs, int n, 
     }
     const int nw = c < kWorkers ? c : kWorkers;
 
-    // the workers' streams start after the work already on s
-    hipError_t e = hipEventRecord(fork_, s);
-    for (int t = 0; t < nw && e == hipSuccess; ++t) e = hipStreamWaitEvent(w_[t].st, fork_, 0);
-    if (e != hipSuccess) return e;
-
     hipError_t res[kWorkers] = {};
     auto work = [&](int t) {
-        res[t] = up ? upload_chunks(w_[t].st, w_[t].pin, w_[t].ev, lists[t])
-                    : download_chunks(w_[t].st, w_[t].pin, w_[t].ev, lists[t]);
+        res[t] = up ? upload_chunks(s, w_[t].pin, w_[t].ev, lists[t]) : download_chunks(s, w_[t].pin, w_[t].ev, lists[t]);
     };
     // worker 0 on the calling thread (whose device is the context's), the
     // others on threads of their own, made current on the same device
@@ -177,14 +164,7 @@ hipError_t HostPipe::run(int device, hipStream_t s, const HostSeg *segs, int n, 
     for (std::thread &x : th) x.join();
     for (int t = 0; t < nw; ++t)
         if (res[t] != hipSuccess) return res[t];
-
-    // later work on s waits for the uploads; downloads are complete already
-    if (up)
-        for (int t = 0; t < nw && e == hipSuccess; ++t) {
-            e = hipEventRecord(w_[t].join, w_[t].st);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s, w_[t].join, 0);
-        }
-    return e;
+    return hipSuccess;
 }
 
 }  // namespace sps

@@ -4,12 +4,15 @@
 // A pageable hipMemcpyAsync goes through the runtime's own staging at about
 // 28 GB/s to the device and 17 GB/s back on MI355X; pinned memory moves at
 // about 57 GB/s both ways.  HostPipe stages through pinned chunks of its
-// own: kWorkers host threads, each with two kChunk pinned buffers and a
-// stream, copy their share of the chunks between the caller's memory and
-// the pinned buffers with memcpy while the DMA of their previous chunk runs.
-// Transfers below kDirect bytes go straight through hipMemcpyAsync.  Every
-// call returns with the host memory read (upload) or written (download);
-// the calling thread's current device must be the stream's.
+// own: kWorkers host threads, each with two kChunk pinned buffers, copy
+// their share of the chunks between the caller's memory and the pinned
+// buffers with memcpy while the DMA of their previous chunk runs.  The DMAs
+// go on the caller's stream (one stream carries the full PCIe rate; the
+// host copies are what needs the threads), so they are ordered with the
+// work already on it and with what follows.  Transfers below kDirect bytes
+// go straight through hipMemcpyAsync.  Every call returns with the host
+// memory read (upload) or written (download); the calling thread's current
+// device must be the stream's.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -49,15 +52,12 @@ public:
 private:
     struct Worker {
         char *pin[2] = {nullptr, nullptr};
-        hipStream_t st = nullptr;
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        hipEvent_t join = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};   // the last DMA from or into each buffer
     };
     hipError_t ensure(int device);
     hipError_t run(int device, hipStream_t s, const HostSeg *segs, int n, bool up);
 
     Worker w_[kWorkers];
-    hipEvent_t fork_ = nullptr;
     int device_ = -1;
     bool ready_ = false;
 };
