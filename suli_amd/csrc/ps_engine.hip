@@ -84,8 +84,8 @@ void carve(Carver &cv, const Geo &g, bool mixed, int nt3, int nt1, ProbDev &pd)
     pd.P.spmv = cv.take<double>(nt3);
     pd.P.upd = cv.take<double>(nt1);
     pd.P.tres = cv.take<double>(3 * (size_t)nt3);
-    pd.P.init = cv.take<double>(5 * (size_t)nt1);
-    pd.P.proj = cv.take<double>(2 * (size_t)nt1);
+    pd.P.init = cv.take<double>(5 * (size_t)init_blocks(g));
+    pd.P.proj = cv.take<double>(2 * (size_t)init_blocks(g));
 }
 
 }  // namespace
@@ -424,7 +424,7 @@ int Engine::admit(const suli_ps_grid &gr, const SetupIn &in, double *prd1_out, c
         launch_row_tables(sl.g, pd.sx32, pd.sy32, pd.sz32, const_cast<float *>(pd.rx32), const_cast<float *>(pd.ry32),
                           const_cast<float *>(pd.rz32), const_cast<unsigned *>(pd.rowreg), s_);
     launch_init(sl.g, in, Op<double>{pd.sx, pd.sy, pd.sz}, pd.d, pd.b, pd.e, pd.r, o.mixed ? nullptr : pd.r,
-                pd.r32, pd.c32, pd.P, nt1, pd.S, s_);
+                pd.r32, pd.c32, pd.P, init_blocks(sl.g), pd.S, s_);
     ECHK(hipGetLastError());
     tables_dirty_ = true;
     if (slot_out) *slot_out = si;

@@ -15,6 +15,7 @@ constexpr int kTileX = 64;           // 3-D tile: one wave per x row
 constexpr int kTileY = 4;            // four rows per workgroup
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, round-robin dispatch
 constexpr int kVecBlocks = 512;      // 1-D tiles of a large problem
+constexpr int kInitBlocks = 4096;    // workgroups (and partial sums) of the setup reductions
 constexpr int kSpmvBlocks = 1024;    // target 3-D tiles of a large problem
 constexpr int kNotDone = 0x7fffffff;
 // Mixed precision: residual replacements happen at iterations that are
@@ -115,8 +116,8 @@ struct Partials {
     double *spmv;     // [nt3]      p.Ap of cg_a
     double *upd;      // [nt1]      r.r of cg_b (or one total, rest zero, after a replacement)
     double *tres;     // [3*nt3]    true residual r.r (FP64, FP32 copy) and e.sqrt(d)
-    double *init;     // [5*nt1]    setup: r0.r0, b.b, r0.sqrt(d), d, e0.sqrt(d)
-    double *proj;     // [2*nt1]    setup after projection: r0.r0, FP32 r0.r0
+    double *init;     // [5*ni]     setup: r0.r0, b.b, r0.sqrt(d), d, e0.sqrt(d) (ni = init_blocks)
+    double *proj;     // [2*ni]     setup after projection: r0.r0, FP32 r0.r0
     double *m4;       // [2][nt3][4] one-launch FP32 iteration n, slot n & 1:
                       //            p_n.q_n, q_n.r_{n-1}, q_n.q_n, r_{n-1}.r_{n-1}
 };
@@ -227,6 +228,10 @@ void launch_row_tables(const Geo &g, const float *sx32, const float *sy32, const
 void launch_face_tables(const Geo &g, const double *sx, const double *sy, const double *sz, const float *sx32,
                         const float *sy32, const float *sz32, double *tx, double *ty, double *tz, float *tx32,
                         float *ty32, float *tz32, hipStream_t s);
+// Workgroups of launch_init's reductions (its nb0): one per cell tile of
+// 64 x 4 cells, at most kInitBlocks; the setup partials hold 5 and 2 per
+// workgroup.
+int init_blocks(const Geo &g);
 // Initial residual and null-space projection on nb0 workgroups; leaves r
 // (r64, or r32 with c32 = 0 for mixed precision) and S ready for iteration 1.
 void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d, double *b, double *e,

@@ -100,59 +100,105 @@ __device__ __forceinline__ bool closed_z(const Geo &g, const SetupIn &in, int i,
                         solid_cell(g, in, i, j, box_to_cell(f, g.nz, 0)));
 }
 
+// Cell tiles of the setup and correction kernels: 64 x 4 cells of a plane
+// per workgroup pass, one cell per thread, numbered x fastest, then y, then
+// the planes.  Workgroup b starts at tile xcd_tile(b, G) and strides by the
+// grid G, so each XCD walks contiguous runs of tiles: adjacent tiles share
+// cache lines at row ends (rows of the face arrays, u, v, w and the prd1
+// box do not start on a line) and halo rows, and read them into one L2.  A
+// wave is one row of its tile, so j and k are the same on all its lanes.
+__host__ __device__ inline int cell_tiles(const Geo &g)
+{
+    return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;   // at most a quarter of the cells
+}
+
+template <class F>
+__device__ __forceinline__ void for_cells(const Geo &g, F &&f)
+{
+    const int tx = (g.nx + 63) / 64, ty = (g.ny + 3) / 4, ntile = cell_tiles(g);
+    const int G = gridDim.x;
+    const int row = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int t = xcd_tile(blockIdx.x, G); t < ntile; t += G) {
+        const int bx = t % tx, r = t / tx;
+        const int i = bx * 64 + (threadIdx.x & 63), j = (r % ty) * 4 + row, k = r / ty;
+        if (i < g.nx && j < g.ny) f(i, j, k);
+    }
+}
+
 // s = dt/(delta^2 rho_face) (9_grad.f90:118-148; SuLi-ICdz/9_grad.f90:141)
+// of x face f of row (j, k), y face f of column (i, k), z face f of (i, j).
+__device__ __forceinline__ double coef_x(const Geo &g, const SetupIn &in, int f, int j, int k)
+{
+    // SuLi-IT: tetah dt/dx^2 dzx (SuLi-IT/9_grad.f90:44-50)
+    if (in.it) return in.aux1 * in.dzx[(f + 1) + (long long)(g.nx + 3) * ((j + 1) + (long long)(g.ny + 2) * (k + 1))];
+    const double rf = in.ipa ? in.rho_f1 : rho_face_x(g, in, f, j, k);
+    return closed_x(g, in, f, j, k) ? 0.0 : in.aux1 / rf;
+}
+__device__ __forceinline__ double coef_y(const Geo &g, const SetupIn &in, int i, int f, int k)
+{
+    if (in.it) return in.aux2 * in.dzy[(i + 1) + (long long)(g.nx + 2) * ((f + 1) + (long long)(g.ny + 3) * (k + 1))];
+    const double rf = in.ipa ? in.rho_f1 : rho_face_y(g, in, i, f, k);
+    return closed_y(g, in, i, f, k) ? 0.0 : in.aux2 / rf;
+}
+__device__ __forceinline__ double coef_z(const Geo &g, const SetupIn &in, int i, int j, int f)
+{
+    if (in.it) {
+        // tetah dt/dzz; no face above the top plane (:51-60)
+        return f == g.nz ? 0.
+                         : in.tetah * in.dt /
+                               in.dz3[(i + 1) + (long long)(g.nx + 2) * ((j + 1) + (long long)(g.ny + 2) * (f + 1))];
+    }
+    const double rf = in.ipa ? in.rho_f1 : rho_face_z(g, in, i, j, f);
+    if (closed_z(g, in, i, j, f)) return 0.0;
+    if (in.dz) {
+        const double dzk = in.dz[f + 1];
+        return in.dt / (dzk * dzk * rf);
+    }
+    return in.aux3 / rf;
+}
+
+// Cell (i, j, k) writes its faces on the low side in x, y and z, and the
+// cells of the last column, row and plane the face on their high side too:
+// every face once.  In the usual case (not SuLi-IT, no IPA constant) the
+// four face densities a cell needs are loaded together before any branch
+// or store; the values are those of coef_x/y/z.
 __global__ __launch_bounds__(kThreads) void k_face_coefs(Geo g, SetupIn in, double *sx, double *sy, double *sz)
 {
-    const long long nfx = (long long)(g.nx + 1) * g.ny * g.nz;
-    const long long nfy = (long long)g.nx * (g.ny + 1) * g.nz;
-    const long long nfz = (long long)g.nx * g.ny * (g.nz + 1);
-    const long long stride = (long long)gridDim.x * kThreads;
-    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
-        if (t < nfx) {
-            const int f = (int)(t % (g.nx + 1));
-            const long long jk = t / (g.nx + 1);
-            const int j = (int)(jk % g.ny), k = (int)(jk / g.ny);
-            if (in.it) {
-                // SuLi-IT: tetah dt/dx^2 dzx (SuLi-IT/9_grad.f90:44-50)
-                sx[t] = in.aux1 * in.dzx[(f + 1) + (long long)(g.nx + 3) * ((j + 1) + (long long)(g.ny + 2) * (k + 1))];
-                continue;
-            }
-            const double rf = in.ipa ? in.rho_f1 : rho_face_x(g, in, f, j, k);
-            sx[t] = closed_x(g, in, f, j, k) ? 0.0 : in.aux1 / rf;
-        } else if (t < nfx + nfy) {
-            const long long u = t - nfx;
-            const int i = (int)(u % g.nx);
-            const long long jk = u / g.nx;
-            const int f = (int)(jk % (g.ny + 1)), k = (int)(jk / (g.ny + 1));
-            if (in.it) {
-                sy[u] = in.aux2 * in.dzy[(i + 1) + (long long)(g.nx + 2) * ((f + 1) + (long long)(g.ny + 3) * (k + 1))];
-                continue;
-            }
-            const double rf = in.ipa ? in.rho_f1 : rho_face_y(g, in, i, f, k);
-            sy[u] = closed_y(g, in, i, f, k) ? 0.0 : in.aux2 / rf;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    for_cells(g, [&](int i, int j, int k) {
+        const long long fx = ((long long)k * ny + j) * (nx + 1) + i;
+        const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
+        const long long fz = ((long long)k * ny + j) * nx + i;
+        double x0, x1, y0, z0;
+        if (in.it || in.ipa) {
+            x0 = coef_x(g, in, i, j, k);
+            x1 = coef_x(g, in, i + 1, j, k);
+            y0 = coef_y(g, in, i, j, k);
+            z0 = coef_z(g, in, i, j, k);
         } else {
-            const long long u = t - nfx - nfy;
-            const int i = (int)(u % g.nx);
-            const long long jk = u / g.nx;
-            const int j = (int)(jk % g.ny), f = (int)(jk / g.ny);
-            if (in.it) {
-                // tetah dt/dzz; no face above the top plane (:51-60)
-                sz[u] = f == g.nz ? 0.
-                                  : in.tetah * in.dt /
-                                        in.dz3[(i + 1) + (long long)(g.nx + 2) * ((j + 1) + (long long)(g.ny + 2) * (f + 1))];
-                continue;
-            }
-            const double rf = in.ipa ? in.rho_f1 : rho_face_z(g, in, i, j, f);
-            if (closed_z(g, in, i, j, f)) {
-                sz[u] = 0.0;
-            } else if (in.dz) {
-                const double dzk = in.dz[f + 1];
-                sz[u] = in.dt / (dzk * dzk * rf);
-            } else {
-                sz[u] = in.aux3 / rf;
+            // face i + 1 is in range for every lane; only the last column
+            // stores it
+            const double rx0 = rho_face_x(g, in, i, j, k), rx1 = rho_face_x(g, in, i + 1, j, k);
+            const double ry0 = rho_face_y(g, in, i, j, k), rz0 = rho_face_z(g, in, i, j, k);
+            const double dzk = in.dz ? in.dz[k + 1] : 0.0;
+            x0 = in.aux1 / rx0;
+            x1 = in.aux1 / rx1;
+            y0 = in.aux2 / ry0;
+            z0 = in.dz ? in.dt / (dzk * dzk * rz0) : in.aux3 / rz0;
+            if (in.solid) {
+                if (closed_x(g, in, i, j, k)) x0 = 0.0;
+                if (closed_x(g, in, i + 1, j, k)) x1 = 0.0;
+                if (closed_y(g, in, i, j, k)) y0 = 0.0;
+                if (closed_z(g, in, i, j, k)) z0 = 0.0;
             }
         }
-    }
+        sx[fx] = x0;
+        if (i == nx - 1) sx[fx + 1] = x1;
+        sy[fy] = y0;
+        sz[fz] = z0;
+        if (j == ny - 1) sy[fy + nx] = coef_y(g, in, i, ny, k);
+        if (k == nz - 1) sz[fz + (long long)nx * ny] = coef_z(g, in, i, j, nz);
+    });
 }
 
 // d = sum of the six face coefficients (9_grad.f90:152-160) and the scaled
@@ -304,51 +350,49 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
                                                             double *b, double *e, double *r0, Partials P)
 {
     const int nx = g.nx, ny = g.ny;
-    const long long stride = (long long)gridDim.x * kThreads;
+    const long long pl = (long long)nx * ny;
     double acc_rr = 0.0, acc_bb = 0.0, acc_rn = 0.0, acc_dd = 0.0, acc_en = 0.0;
-    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
-        const int i = (int)(c % nx);
-        const long long jk = c / nx;
-        const int j = (int)(jk % ny), k = (int)(jk / ny);
+    for_cells(g, [&](int i, int j, int k) {
+        const long long c = cidx(g, i, j, k);
         const int I = i + 1, J = j + 1, K = k + 1;
         const long long fx = ((long long)k * ny + j) * (nx + 1) + i;
         const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
-        const long long fz = ((long long)k * ny + j) * nx + i;
-        const long long pl = (long long)nx * ny;
+        const long long fz = c;
+        const bool xe = i + 1 < nx, xw = i > 0, yn = j + 1 < ny, ys = j > 0, zt = k + 1 < g.nz, zb = k > 0;
         // e of the cells (written with d by k_diag_rhs); ghost positions from
         // the caller's prd1 ghosts, as the reference's e over the whole box
         const double ec = e[c];
-        const double eE = i + 1 < nx ? e[c + 1] : e_box(g, in, d, I + 1, J, K);
-        const double eW = i > 0 ? e[c - 1] : e_box(g, in, d, I - 1, J, K);
-        const double eN = j + 1 < ny ? e[c + nx] : e_box(g, in, d, I, J + 1, K);
-        const double eS = j > 0 ? e[c - nx] : e_box(g, in, d, I, J - 1, K);
-        const double eT = k + 1 < g.nz ? e[c + pl] : e_box(g, in, d, I, J, K + 1);
-        const double eB = k > 0 ? e[c - pl] : e_box(g, in, d, I, J, K - 1);
-        const double a_in = ec - op.sx[fx + 1] * eE - op.sx[fx] * eW - op.sy[fy + nx] * eN - op.sy[fy] * eS -
-                            op.sz[fz + pl] * eT - op.sz[fz] * eB;
-        // neighbours with mirrored/periodic ghosts
-        const int iE = box_to_cell(I + 1, nx, g.px), iW = box_to_cell(I - 1, nx, g.px);
-        const int jN = box_to_cell(J + 1, ny, g.py), jS = box_to_cell(J - 1, ny, g.py);
-        const int kT = box_to_cell(K + 1, g.nz, 0), kB = box_to_cell(K - 1, g.nz, 0);
-        const double mE = e[cidx(g, iE, j, k)], mW = e[cidx(g, iW, j, k)];
-        const double mN = e[cidx(g, i, jN, k)], mS = e[cidx(g, i, jS, k)];
-        const double mT = (g.dtop && K == g.nz) ? -ec : e[cidx(g, i, j, kT)];
-        const double mB = e[cidx(g, i, j, kB)];
-        const double a_mir = ec - op.sx[fx + 1] * mE - op.sx[fx] * mW - op.sy[fy + nx] * mN - op.sy[fy] * mS -
-                             op.sz[fz + pl] * mT - op.sz[fz] * mB;
+        const double eE = xe ? e[c + 1] : e_box(g, in, d, I + 1, J, K);
+        const double eW = xw ? e[c - 1] : e_box(g, in, d, I - 1, J, K);
+        const double eN = yn ? e[c + nx] : e_box(g, in, d, I, J + 1, K);
+        const double eS = ys ? e[c - nx] : e_box(g, in, d, I, J - 1, K);
+        const double eT = zt ? e[c + pl] : e_box(g, in, d, I, J, K + 1);
+        const double eB = zb ? e[c - pl] : e_box(g, in, d, I, J, K - 1);
+        // the same neighbours with mirrored/periodic ghosts: the cells above
+        // wherever the neighbour is a cell
+        const double mE = xe ? eE : e[cidx(g, box_to_cell(I + 1, nx, g.px), j, k)];
+        const double mW = xw ? eW : e[cidx(g, box_to_cell(I - 1, nx, g.px), j, k)];
+        const double mN = yn ? eN : e[cidx(g, i, box_to_cell(J + 1, ny, g.py), k)];
+        const double mS = ys ? eS : e[cidx(g, i, box_to_cell(J - 1, ny, g.py), k)];
+        const double mT = (g.dtop && K == g.nz) ? -ec : (zt ? eT : e[cidx(g, i, j, box_to_cell(K + 1, g.nz, 0))]);
+        const double mB = zb ? eB : e[cidx(g, i, j, box_to_cell(K - 1, g.nz, 0))];
+        const double sE = op.sx[fx + 1], sW = op.sx[fx], sN = op.sy[fy + nx], sS = op.sy[fy];
+        const double sT = op.sz[fz + pl], sB = op.sz[fz];
         const double bc = b[c];
+        // null vector: sqrt(d) on fluid cells, 0 on solid ones
+        const double dc = solid_cell(g, in, i, j, k) ? 0.0 : d[c];
+        const double a_in = ec - sE * eE - sW * eW - sN * eN - sS * eS - sT * eT - sB * eB;
+        const double a_mir = ec - sE * mE - sW * mW - sN * mN - sS * mS - sT * mT - sB * mB;
         const double r = a_in - bc;
         b[c] = bc + (a_mir - a_in);
         r0[c] = r;
-        // null vector: sqrt(d) on fluid cells, 0 on solid ones
-        const double dc = solid_cell(g, in, i, j, k) ? 0.0 : d[c];
         const double sdc = sqrt(dc);
         acc_rr += r * r;
         acc_bb += bc * bc;
         acc_rn += r * sdc;
         acc_dd += dc;
         acc_en += ec * sdc;
-    }
+    });
     const double s0 = block_sum(acc_rr), s1 = block_sum(acc_bb), s2 = block_sum(acc_rn), s3 = block_sum(acc_dd);
     const double s4 = block_sum(acc_en);
     if (threadIdx.x == 0) {
@@ -369,16 +413,38 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
 // residual is removed (gamma = r.n / n.n) and b_eff adjusted to match; this
 // changes the solution only by a constant (the pressure gauge).  The first
 // search direction p = r (9_grad.f90:372-380) is formed by the first cg_a.
+// One workgroup: the totals of k_init_residual's nb0 partials of each sum,
+// taken once; the null-space component gamma and the stop threshold.
+__global__ __launch_bounds__(kThreads) void k_init_totals(Partials P, int nb0, CgScalars *S)
+{
+    // each thread its partials t, t + kThreads, ... of all five sums in one
+    // loop (unrolled, so many loads are in flight), then the block sums:
+    // the additions of block_total_of, in its order
+    const gptr<const double> p = glob(static_cast<const double *>(P.init));
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int t = threadIdx.x; t < nb0; t += kThreads)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a[q] += p[(long long)q * nb0 + t];
+    const double rr = block_sum(a[0]), bb = block_sum(a[1]), rn = block_sum(a[2]), dd = block_sum(a[3]);
+    const double en = block_sum(a[4]);
+    if (threadIdx.x == 0) {
+        S->gamma = S->project ? rn / dd : 0.0;
+        S->bb = bb;
+        S->rr0 = rr;
+        S->en0 = en;
+        S->dd = dd;
+        S->thresh = S->rel_rule ? S->tol * S->tol * bb : S->thresh_ref;
+    }
+}
+
+// r -= gamma sqrt(d) (and b += gamma sqrt(d)), the FP64 or FP32 residual
+// the CG starts from, and its r.r partials (one per workgroup).
 __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const uint8_t *solid, const double *d, double *b,
                                                            const double *r0, double *r64, float *r32, float *c32,
-                                                           Partials P, int nb0, CgScalars *S)
+                                                           Partials P, const CgScalars *S)
 {
-    const double rr = total_of(P.init, nb0);
-    const double bb = total_of(P.init + nb0, nb0);
-    const double rn = total_of(P.init + 2 * nb0, nb0);
-    const double dd = total_of(P.init + 3 * nb0, nb0);
-    const double en = total_of(P.init + 4 * nb0, nb0);
-    const double gamma = S->project ? rn / dd : 0.0;
+    const double gamma = S->gamma;
     const long long stride = (long long)gridDim.x * kThreads;
     double acc = 0.0, acc32 = 0.0;
     for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
@@ -399,24 +465,16 @@ __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const uint8_t 
     if (threadIdx.x == 0) {
         P.proj[blockIdx.x] = s0;
         P.proj[gridDim.x + blockIdx.x] = s1;
-        if (blockIdx.x == 0) {
-            S->gamma = gamma;
-            S->bb = bb;
-            S->rr0 = rr;
-            S->en0 = en;
-            S->dd = dd;
-            S->thresh = S->rel_rule ? S->tol * S->tol * bb : S->thresh_ref;
-        }
     }
 }
 
 // One workgroup: totals after the projection; S ready for iteration 1.
 // The reference tests the stop rule before the first iteration
 // (do while (|r.r| > thresh .and. cont < 10000), 9_grad.f90:398).
-__global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int nb0, CgScalars *S)
+__global__ __launch_bounds__(kThreads) void k_init_finalize(Partials P, int np, CgScalars *S)
 {
-    const double rr = total_of(P.proj, nb0);
-    const double rs = total_of(P.proj + nb0, nb0);
+    const double rr = block_total_of(P.proj, np);
+    const double rs = block_total_of(P.proj + np, np);
     if (threadIdx.x == 0) {
         const double rrc = S->mixed ? rs : rr;
         S->rr[0] = rrc;
@@ -531,28 +589,11 @@ __device__ __forceinline__ void posdin_cell(const Geo &g, const SetupIn &in, dou
     }
 }
 
-// One workgroup per tile of 64 x 4 cells of a plane, one thread per cell;
-// tiles are numbered x fastest, then y, then the planes.  xcd_tile gives
-// each XCD one contiguous run of tiles, so the lines that x- and y-adjacent
-// tiles share (rows of u, v, w and the p box do not start on a line) and
-// the plane below are read into its own L2 once; in plain block order
-// adjacent tiles run on different XCDs and both fetch those lines (a third
-// more reads than the arrays hold on the 256x128x96 tank).  Grids with more
-// tiles than kPosdinMaxGrid take the rest in further passes.
-constexpr int kPosdinMaxGrid = 1 << 20;
+// One thread per cell of the cell tiles (for_cells).
 __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *u, double *v, double *w, bool with_z)
 {
-    const int tx = (g.nx + 63) / 64, ty = (g.ny + 3) / 4;
-    const int ntile = tx * ty * g.nz;   // at most a quarter of the cells
-    const int G = gridDim.x;
     const double aux1 = in.dt / in.dx, aux2 = in.dt / in.dy;
-    // a wave is one row of its tile, so j is the same on all its lanes
-    const int row = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int t = xcd_tile(blockIdx.x, G); t < ntile; t += G) {
-        const int bx = t % tx, r = t / tx;
-        const int i = bx * 64 + (threadIdx.x & 63), j = (r % ty) * 4 + row, k = r / ty;
-        if (i < g.nx && j < g.ny) posdin_cell(g, in, u, v, w, with_z, i, j, k, aux1, aux2);
-    }
+    for_cells(g, [&](int i, int j, int k) { posdin_cell(g, in, u, v, w, with_z, i, j, k, aux1, aux2); });
 }
 
 // pressh, the hydrostatic pressure of t_press = 0 (SuLi-IC/9_grad.f90:
@@ -1023,12 +1064,14 @@ __global__ __launch_bounds__(kThreads) void k_row_tables(Geo g, const float *sx3
     for (int r = wave; r < 32; r += kThreads / 64) {
         const int j = word * 32 + r;
         if (j >= ny) break;   // whole waves
+        // & rather than &&: every load is issued without waiting for the
+        // comparisons before it
         bool ok = true;
-        for (int i = lane; i <= nx; i += 64) ok = ok && sx32[fxk + (long long)j * (nx + 1) + i] == vx;
+        for (int i = lane; i <= nx; i += 64) ok &= sx32[fxk + (long long)j * (nx + 1) + i] == vx;
         for (int i = lane; i < nx; i += 64) {
-            ok = ok && sy32[fyk + (long long)j * nx + i] == vy && sy32[fyk + (long long)(j + 1) * nx + i] == vy;
-            ok = ok && sz32[(long long)k * plane + (long long)j * nx + i] == vz0 &&
-                 sz32[(long long)(k + 1) * plane + (long long)j * nx + i] == vz1;
+            ok &= (sy32[fyk + (long long)j * nx + i] == vy) & (sy32[fyk + (long long)(j + 1) * nx + i] == vy);
+            ok &= (sz32[(long long)k * plane + (long long)j * nx + i] == vz0) &
+                  (sz32[(long long)(k + 1) * plane + (long long)j * nx + i] == vz1);
         }
         if (__all(ok)) mine |= 1u << r;
     }
@@ -1101,13 +1144,18 @@ int grid_for(long long n)
     return (int)b;
 }
 
+// Grid of a for_cells kernel without partial sums: a workgroup per cell
+// tile, up to kCellMaxGrid (further tiles take further passes).
+constexpr int kCellMaxGrid = 1 << 20;
+dim3 cell_grid(const Geo &g) { return dim3((unsigned)std::min(cell_tiles(g), kCellMaxGrid)); }
+
 }  // namespace
+
+int init_blocks(const Geo &g) { return std::max(1, std::min(cell_tiles(g), kInitBlocks)); }
 
 void launch_face_coefs(const Geo &g, const SetupIn &in, double *sx, double *sy, double *sz, hipStream_t s)
 {
-    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
-                         (long long)g.nx * g.ny * (g.nz + 1);
-    hipLaunchKernelGGL(k_face_coefs, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, in, sx, sy, sz);
+    hipLaunchKernelGGL(k_face_coefs, cell_grid(g), dim3(kThreads), 0, s, g, in, sx, sy, sz);
 }
 
 void launch_diag_rhs(const Geo &g, const SetupIn &in, const double *sx, const double *sy, const double *sz,
@@ -1144,10 +1192,12 @@ void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d
                  double *r0, double *r64, float *r32, float *c32, const Partials &P, int nb0, CgScalars *S,
                  hipStream_t s)
 {
+    // the projection streams its cells on at most kVecBlocks workgroups
+    const int np = std::min(nb0, kVecBlocks);
     hipLaunchKernelGGL(k_init_residual, dim3(nb0), dim3(kThreads), 0, s, g, in, op, d, b, e, r0, P);
-    hipLaunchKernelGGL(k_init_project, dim3(nb0), dim3(kThreads), 0, s, g, in.solid, d, b, r0, r64, r32, c32, P, nb0,
-                       S);
-    hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kThreads), 0, s, P, nb0, S);
+    hipLaunchKernelGGL(k_init_totals, dim3(1), dim3(kThreads), 0, s, P, nb0, S);
+    hipLaunchKernelGGL(k_init_project, dim3(np), dim3(kThreads), 0, s, g, in.solid, d, b, r0, r64, r32, c32, P, S);
+    hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kThreads), 0, s, P, np, S);
 }
 
 void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1, const CgScalars *S,
@@ -1157,17 +1207,9 @@ void launch_unscale(const Geo &g, const double *e, const double *d, double *prd1
     hipLaunchKernelGGL(k_unscale, dim3(grid_for(nbox)), dim3(kThreads), 0, s, g, e, d, prd1, S, solid);
 }
 
-// k_posdin's grid: a workgroup per 64 x 4 cell tile of every plane, up to
-// kPosdinMaxGrid
-static dim3 posdin_grid(const Geo &g)
-{
-    const long long ntile = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
-    return dim3((unsigned)std::min<long long>(ntile, kPosdinMaxGrid));
-}
-
 void launch_posdin(const Geo &g, const SetupIn &in, double *u, double *v, double *w, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_posdin, posdin_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, true);
+    hipLaunchKernelGGL(k_posdin, cell_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, true);
 }
 
 void launch_heaviside(long long n, const double *ls, double eps, double rho1, double rho2, double mu1, double mu2,
@@ -1226,7 +1268,7 @@ void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, doub
                          hipStream_t s)
 {
     const long long nfxy = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz;
-    hipLaunchKernelGGL(k_posdin, posdin_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, false);
+    hipLaunchKernelGGL(k_posdin, cell_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, false);
     const long long nplane = (long long)(g.nx + 2) * (g.ny + 2);
     hipLaunchKernelGGL(k_posdin_hydro_w, dim3((unsigned)((nplane + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
                        g, u, v, hs, in.dz, in.dzu, in.dx, in.dy, w);
