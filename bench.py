@@ -649,6 +649,108 @@ def time_posdin(solver, g, df, reps):
             "achieved": nbytes / (ms * 1e-3) / 1e9, "basis": "host clock around 20 calls"}
 
 
+def time_calls(fn, reps):
+    """Mean ms per call of fn (which returns once its work is done): the host
+    clock around `reps` calls, after one untimed call."""
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def bench_rows(reps=20, it_steps=3):
+    """The rows widened beside the projection step, device-resident:
+    rho from the level set (heaviside, t_tens 0 and 1), the hydrostatic
+    step (pressh, then posdin's other branch) and IPA's exp_press on the
+    256x128x96 tank grid, and SuLi-IT's solve and its whole posdin on a
+    256x128x48 SuLi-IT state.  Each entry point: mean ms per call (host
+    clock, launch and synchronisation included) and the bytes of the arrays
+    it reads and writes, each counted once, at that rate."""
+    from suli_amd import cases
+    from suli_amd.solver import PressureSolver
+    c = cases.config2(seed=1)
+    g = c.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    n = nx * ny * nz
+    nbox = (nx + 2) * (ny + 2) * (nz + 2)
+    nu, nv, nw = (nx + 3) * (ny + 2) * (nz + 2), (nx + 2) * (ny + 3) * (nz + 2), (nx + 2) * (ny + 2) * (nz + 3)
+    faces = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+    rate = lambda nbytes, ms: {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+    H = nz * g.dz
+    ls = cases.sloshing_level_set(g, 0.25 * H / 0.48, 0.02 * H / 0.48)
+    eps, rho_w, mu_a, mu_w = 1.5 * g.dx, 1000.0, 1.8e-5, 1.0e-3
+    out = {}
+    solver = PressureSolver(verbose=False)
+    f = c.fields.copy()
+    f.hs = np.zeros((nx, ny, nz), order="F")
+    df = solver.upload(g, f)
+    p = df.ptrs
+    p_ls = solver.dev_array(ls)
+    p_mu = solver.dev_array(np.zeros((nx, ny, nz)))
+    grads = [solver.dev_array(np.zeros((nx, ny, nz))) for _ in range(3)]
+    out["heaviside"] = rate(8 * 4 * n, time_calls(
+        lambda: solver.heaviside_dev(g, p_ls, eps, rho_w, mu_a, mu_w, p["rho"], p["hs"], p_mu), reps))
+    out["heaviside_tens"] = rate(8 * 7 * n, time_calls(
+        lambda: solver.heaviside_tens_dev(g, p_ls, eps, rho_w, mu_a, mu_w, p["rho"], *grads, p["hs"], p_mu),
+        reps))
+    # the hydrostatic step: prd1 from rho, then u, v corrected, w from
+    # continuity and the faces where hs < 1e-4 closed
+    out["pressh"] = rate(8 * (n + nbox), time_calls(lambda: solver.pressh_dev(g, df, 9.81), reps))
+    out["posdin_hydro"] = rate(8 * (2 * (nu + nv) + nw + 2 * n + nbox),
+                               time_calls(lambda: solver.posdin_hydro_dev(g, df), reps))
+    prd0 = solver.dev_array(np.zeros((nx + 2, ny + 2, nz + 2)))
+    pxyz = [solver.dev_array(np.zeros(shape)) for shape in
+            ((nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1))]
+    out["exp_press"] = rate(8 * (n + 3 * nbox + faces), time_calls(
+        lambda: solver.exp_press_dev(g, p["rho"], p["prd1"], prd0, *pxyz), reps))
+    for ptr in (p_ls, p_mu, prd0, *grads, *pxyz):
+        solver.dev_free(ptr)
+    df.free()
+    out["tank"] = f"{nx}x{ny}x{nz}"
+
+    # SuLi-IT: the solve from prd1 = 0 to a 1e-8 relative residual, both
+    # precisions, then posdin with its surface and layer update
+    gi, fi = cases.it_case(256, 128, 48, seed=1)
+    d = solver.it_upload(fi)
+    box_bytes = 8 * (gi.nx + 2) * (gi.ny + 2) * (gi.nz + 2)
+    it = {"state": f"{gi.nx}x{gi.ny}x{gi.nz}"}
+    for precision in ("mixed", "fp64"):
+        solver.configure(precision=precision, stop_rule="relative", tol=1e-8, verbose=False, autotune=True,
+                         graphs=GRAPHS)
+        ms = []
+        for k in range(it_steps + 1):
+            solver._check(solver.lib.suli_ps_dev_zero(solver.ctx, d["prd1"], box_bytes))
+            solver.synchronize()
+            t0 = time.perf_counter()
+            st = solver.it_graddin_dev(gi, d)
+            solver.synchronize()
+            if k:
+                ms.append((time.perf_counter() - t0) * 1e3)
+            if st.status != 0:
+                raise RuntimeError(f"SuLi-IT {precision} did not converge: {st.as_dict()}")
+        it[precision] = {"solves_per_s": 1e3 / float(np.mean(ms)), "ms_per_solve": float(np.mean(ms)),
+                         "iters": st.iters, "rel_true": st.rel_true}
+    it["speedup_mixed_vs_fp64"] = it["mixed"]["solves_per_s"] / it["fp64"]["solves_per_s"]
+    rng = np.random.default_rng(2)
+    dz = np.asfortranarray(float(np.median(fi.dzz)) *
+                           (1.0 + 0.1 * rng.uniform(-1, 1, (gi.nx + 2, gi.ny + 2, gi.nz + 2))))
+    eta0 = np.asfortranarray(rng.uniform(-0.01, 0.01, (gi.nx + 2, gi.ny + 2)))
+    p_dz, p_e0 = solver.dev_array(dz), solver.dev_array(eta0)
+    inu, inv, inw = ((gi.nx + 3) * (gi.ny + 2) * (gi.nz + 2), (gi.nx + 2) * (gi.ny + 3) * (gi.nz + 2),
+                     (gi.nx + 2) * (gi.ny + 2) * (gi.nz + 3))
+    # u, v, w and dzx, dzy, dzz read and written, dz read and written, prd1 read
+    it["posdin_layers"] = rate(8 * 4 * (inu + inv + inw) + 3 * box_bytes, time_calls(
+        lambda: solver.it_posdin_layers_dev(gi, d, p_dz, p_e0), reps))
+    solver.dev_free(p_dz)
+    solver.dev_free(p_e0)
+    solver.it_free(d)
+    solver.close()
+    out["suli_it"] = it
+    out["basis"] = "host clock around 20 calls after one (launch and synchronisation included)"
+    return out
+
+
 # the extra workloads, in the order they run: (log line, result key, runner)
 EXTRAS = {
     "cylinder": ("stretched cylinder", "stretched_cylinder", lambda keep: bench_stretched_cylinder(2, keep)),
@@ -656,6 +758,7 @@ EXTRAS = {
     "batch": ("batch", "batch_uneven", lambda keep: bench_batch(keep)),
     "serving": ("serving", "serving", lambda keep: bench_serving(keep)),
     "dropin": ("drop-in call", "dropin_call", lambda keep: bench_dropin()),
+    "rows": ("widened rows", "rows", lambda keep: bench_rows()),
 }
 
 

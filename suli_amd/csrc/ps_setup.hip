@@ -46,19 +46,27 @@ __device__ __forceinline__ double face_y(const Geo &g, const double *field, int 
     return (r[(long long)a * g.nx] + r[(long long)(a - 1) * g.nx]) * 0.5;
 }
 
+// z face f takes cells a - 1 and a (0-based), a = f clamped to 1 .. nz - 1,
+// r0 and r1 their values
+__device__ __forceinline__ int face_z_upper(const Geo &g, int f) { return f == 0 ? 1 : (f == g.nz ? g.nz - 1 : f); }
+// with weights d1 = dz(a+1), d0 = dz(a) (Fortran face a+1 between cells a
+// and a+1, 1-based)
+__device__ __forceinline__ double face_z_weighted(double r1, double r0, double d1, double d0)
+{
+    return (r1 * d1 + r0 * d0) / (d1 + d0);
+}
+__device__ __forceinline__ double face_z_of(double r1, double r0, const double *dz, int a)
+{
+    if (dz) return face_z_weighted(r1, r0, dz[a + 1], dz[a]);
+    return (r1 + r0) * 0.5;
+}
 __device__ __forceinline__ double face_z(const Geo &g, const double *field, const double *dz, int i, int j, int f)
 {
     const long long plane = (long long)g.nx * g.ny;
     const double *r = field + (long long)j * g.nx + i;
-    const int nz = g.nz;
-    const int a = f == 0 ? 1 : (f == nz ? nz - 1 : f);
+    const int a = face_z_upper(g, f);
     const double r1 = r[a * plane], r0 = r[(a - 1) * plane];
-    if (dz) {
-        // Fortran face a+1 between cells a and a+1 (1-based): weights dz(a+1), dz(a)
-        const double d1 = dz[a + 1], d0 = dz[a];
-        return (r1 * d1 + r0 * d0) / (d1 + d0);
-    }
-    return (r1 + r0) * 0.5;
+    return face_z_of(r1, r0, dz, a);
 }
 
 __device__ __forceinline__ double rho_face_x(const Geo &g, const SetupIn &in, int f, int j, int k)
@@ -596,6 +604,10 @@ __global__ __launch_bounds__(kThreads) void k_posdin(Geo g, SetupIn in, double *
     for_cells(g, [&](int i, int j, int k) { posdin_cell(g, in, u, v, w, with_z, i, j, k, aux1, aux2); });
 }
 
+// Planes whose loads a column sweep (k_pressh, k_posdin_hydro_w) issues
+// together before it runs its recurrence over them.
+constexpr int kColumnAhead = 8;
+
 // pressh, the hydrostatic pressure of t_press = 0 (SuLi-IC/9_grad.f90:
 // 667-702; dz(k) weights: SuLi-ICdz/9_grad.f90:665-700).  One lane per
 // (i, j) column walks k from the top: the top cell takes dz g rho, the one
@@ -616,33 +628,43 @@ __global__ __launch_bounds__(kThreads) void k_pressh(Geo g, const double *__rest
     double *pc = prd1 + (i + 1) + (long long)(nx + 2) * (j + 1);
     double p1 = 0.0, p2 = 0.0;  // prd1 at k+1, k+2
     double r1 = 0.0, r2 = 0.0;  // rho at k+1, k+2
-    for (int K = nz; K >= 1; --K) {
-        const double r0 = rc[(long long)(K - 1) * plane];
-        double p;
-        if (dz) {
-            if (K == nz) {
-                p = dz[K] * gz * r0;
-            } else if (K == nz - 1) {
-                p = gz * (r0 * dz[K] + r1 * dz[K + 1]) * 0.5 + p1;
+    // rho of kColumnAhead planes loaded together, then the recurrence over
+    // them: one round trip per block of planes instead of one per plane
+    for (int K0 = nz; K0 >= 1; K0 -= kColumnAhead) {
+        double rb[kColumnAhead];
+#pragma unroll
+        for (int m = 0; m < kColumnAhead; ++m) rb[m] = K0 - m >= 1 ? rc[(long long)(K0 - m - 1) * plane] : 0.0;
+#pragma unroll
+        for (int m = 0; m < kColumnAhead; ++m) {
+            const int K = K0 - m;
+            if (K < 1) break;
+            const double r0 = rb[m];
+            double p;
+            if (dz) {
+                if (K == nz) {
+                    p = dz[K] * gz * r0;
+                } else if (K == nz - 1) {
+                    p = gz * (r0 * dz[K] + r1 * dz[K + 1]) * 0.5 + p1;
+                } else {
+                    const double intz = (r0 * dz[K] + 4. * r1 * dz[K + 1] + r2 * dz[K + 2]) / 3.;
+                    p = gz * intz + p2;
+                }
             } else {
-                const double intz = (r0 * dz[K] + 4. * r1 * dz[K + 1] + r2 * dz[K + 2]) / 3.;
-                p = gz * intz + p2;
+                if (K == nz) {
+                    p = dzu * gz * r0;
+                } else if (K == nz - 1) {
+                    p = dzu * gz * (r0 + r1) * 0.5 + p1;
+                } else {
+                    const double intz = dzu * (r0 + 4. * r1 + r2) / 3.;
+                    p = gz * intz + p2;
+                }
             }
-        } else {
-            if (K == nz) {
-                p = dzu * gz * r0;
-            } else if (K == nz - 1) {
-                p = dzu * gz * (r0 + r1) * 0.5 + p1;
-            } else {
-                const double intz = dzu * (r0 + 4. * r1 + r2) / 3.;
-                p = gz * intz + p2;
-            }
+            pc[(long long)K * pbb] = p;
+            p2 = p1;
+            p1 = p;
+            r2 = r1;
+            r1 = r0;
         }
-        pc[(long long)K * pbb] = p;
-        p2 = p1;
-        p1 = p;
-        r2 = r1;
-        r1 = r0;
     }
 }
 
@@ -654,6 +676,7 @@ __global__ __launch_bounds__(kThreads) void k_pressh(Geo g, const double *__rest
 // the chain itself runs on the values before zeroing, as in the reference,
 // where the zeroing is a later sweep.  Lanes over the (nx+2)(ny+2) plane;
 // the ring only clears its bottom face.
+template <bool DZ>
 __global__ __launch_bounds__(kThreads) void k_posdin_hydro_w(Geo g, const double *u, const double *v,
                                                              const double *hs, const double *dz, double dzu,
                                                              double dx, double dy, double *w)
@@ -671,12 +694,38 @@ __global__ __launch_bounds__(kThreads) void k_posdin_hydro_w(Geo g, const double
     const double *vc = v + I + (long long)(nx + 2) * J;
     double wk = 0.;
     if (face_z(g, hs, dz, I - 1, J - 1, 0) < 0.0001) wc[wbb] = 0.;
-    for (int K = 1; K <= nz; ++K) {
-        const double dzk = dz ? dz[K] : dzu;
-        const double du = uc[(long long)K * ubb + 1] - uc[(long long)K * ubb];
-        const double dv = vc[(long long)K * vbb + (nx + 2)] - vc[(long long)K * vbb];
-        wk = wk - du * dzk / dx - dv * dzk / dy;
-        wc[(long long)(K + 1) * wbb] = face_z(g, hs, dz, I - 1, J - 1, K) < 0.0001 ? 0. : wk;
+    // u, v and the two hs cells of the z face of kColumnAhead planes loaded
+    // together (no branch between the loads), then the chain over them
+    // (DZ: the stretched grid's dz(K) and the face weights too)
+    const long long plane = (long long)nx * ny;
+    const double *hc = hs + (long long)(J - 1) * nx + (I - 1);
+    for (int K0 = 1; K0 <= nz; K0 += kColumnAhead) {
+        double u0[kColumnAhead], u1[kColumnAhead], v0[kColumnAhead], v1[kColumnAhead];
+        double h1[kColumnAhead], h0[kColumnAhead], dk[kColumnAhead], d1[kColumnAhead], d0[kColumnAhead];
+#pragma unroll
+        for (int m = 0; m < kColumnAhead; ++m) {
+            const int K = K0 + m <= nz ? K0 + m : nz;
+            const int a = face_z_upper(g, K);
+            u1[m] = uc[(long long)K * ubb + 1];
+            u0[m] = uc[(long long)K * ubb];
+            v1[m] = vc[(long long)K * vbb + (nx + 2)];
+            v0[m] = vc[(long long)K * vbb];
+            h1[m] = hc[a * plane];
+            h0[m] = hc[(a - 1) * plane];
+            dk[m] = DZ ? dz[K] : dzu;
+            d1[m] = DZ ? dz[a + 1] : 0.0;
+            d0[m] = DZ ? dz[a] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < kColumnAhead; ++m) {
+            const int K = K0 + m;
+            if (K > nz) break;
+            const double dzk = dk[m];
+            const double du = u1[m] - u0[m], dv = v1[m] - v0[m];
+            wk = wk - du * dzk / dx - dv * dzk / dy;
+            const double hf = DZ ? face_z_weighted(h1[m], h0[m], d1[m], d0[m]) : (h1[m] + h0[m]) * 0.5;
+            wc[(long long)(K + 1) * wbb] = hf < 0.0001 ? 0. : wk;
+        }
     }
 }
 
@@ -1270,8 +1319,13 @@ void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, doub
     const long long nfxy = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz;
     hipLaunchKernelGGL(k_posdin, cell_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, false);
     const long long nplane = (long long)(g.nx + 2) * (g.ny + 2);
-    hipLaunchKernelGGL(k_posdin_hydro_w, dim3((unsigned)((nplane + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                       g, u, v, hs, in.dz, in.dzu, in.dx, in.dy, w);
+    const dim3 grid((unsigned)((nplane + kThreads - 1) / kThreads));
+    if (in.dz)
+        hipLaunchKernelGGL(k_posdin_hydro_w<true>, grid, dim3(kThreads), 0, s, g, u, v, hs, in.dz, in.dzu, in.dx, in.dy,
+                           w);
+    else
+        hipLaunchKernelGGL(k_posdin_hydro_w<false>, grid, dim3(kThreads), 0, s, g, u, v, hs, in.dz, in.dzu, in.dx,
+                           in.dy, w);
     hipLaunchKernelGGL(k_posdin_hydro_uv, dim3(grid_for(nfxy)), dim3(kThreads), 0, s, g, hs, u, v);
 }
 
