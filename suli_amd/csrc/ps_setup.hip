@@ -734,26 +734,20 @@ __global__ __launch_bounds__(kThreads) void k_posdin_hydro_w(Geo g, const double
 // reads the values before zeroing.
 __global__ __launch_bounds__(kThreads) void k_posdin_hydro_uv(Geo g, const double *hs, double *u, double *v)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const long long nfx = (long long)(nx + 1) * ny * nz;
-    const long long nfy = (long long)nx * (ny + 1) * nz;
-    const long long stride = (long long)gridDim.x * kThreads;
-    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy; t += stride) {
-        if (t < nfx) {
-            const int f = (int)(t % (nx + 1));
-            const long long jk = t / (nx + 1);
-            const int j = (int)(jk % ny), k = (int)(jk / ny);
-            if (face_x(g, hs, f, j, k) < 0.0001)
-                u[(f + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(ny + 2) * (k + 1))] = 0.;
-        } else {
-            const long long s = t - nfx;
-            const int i = (int)(s % nx);
-            const long long jk = s / nx;
-            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
-            if (face_y(g, hs, i, f, k) < 0.0001)
-                v[(i + 1) + (long long)(nx + 2) * ((f + 1) + (long long)(ny + 3) * (k + 1))] = 0.;
-        }
-    }
+    const int nx = g.nx, ny = g.ny;
+    // one thread per cell of the cell tiles: its low x and y faces, and in
+    // the last column and row the high one too
+    for_cells(g, [&](int i, int j, int k) {
+        const long long iu = (i + 1) + (long long)(nx + 3) * ((j + 1) + (long long)(g.ny + 2) * (k + 1));
+        const long long iv = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(g.ny + 3) * (k + 1));
+        const bool cx0 = face_x(g, hs, i, j, k) < 0.0001, cy0 = face_y(g, hs, i, j, k) < 0.0001;
+        // face i + 1 is in range for every lane; the last column acts on it
+        const bool cx1 = face_x(g, hs, i + 1, j, k) < 0.0001;
+        if (cx0) u[iu] = 0.;
+        if (i == nx - 1 && cx1) u[iu + 1] = 0.;
+        if (cy0) v[iv] = 0.;
+        if (j == ny - 1 && face_y(g, hs, i, ny, k) < 0.0001) v[iv + (nx + 2)] = 0.;
+    });
 }
 
 // heaviside with t_tens = 0 (SuLi-IC/14_level_set.f90:555-590): the
@@ -931,6 +925,9 @@ __global__ __launch_bounds__(kThreads) void k_heaviside_tens(Geo g, const double
 // prdaux = prd1 + (prd1 + prd0) formed from prd0 before its increment (the
 // reference forms prd0 + prd1 first, :466-467: the same operations).  z
 // differences divide by dzz(k) when dzz is given (SuLi-ICdz), else dzu.
+// One thread per cell of the cell tiles (for_cells): its faces on the low
+// side in x, y and z, and in the last column, row and plane the face on the
+// high side too; prdaux of the cell is formed once for its faces.
 __global__ __launch_bounds__(kThreads) void k_exp_press(Geo g, const double *__restrict__ rho,
                                                         const double *__restrict__ dz,
                                                         const double *__restrict__ dzz, double dx, double dy,
@@ -939,36 +936,27 @@ __global__ __launch_bounds__(kThreads) void k_exp_press(Geo g, const double *__r
                                                         double *__restrict__ py, double *__restrict__ pz)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const long long nfx = (long long)(nx + 1) * ny * nz;
-    const long long nfy = (long long)nx * (ny + 1) * nz;
-    const long long nfz = (long long)nx * ny * (nz + 1);
     const long long pb = nx + 2, pbb = (long long)(nx + 2) * (ny + 2);
     auto aux = [&](long long x) { return prd1[x] + (prd1[x] + prd0[x]); };
-    const long long stride = (long long)gridDim.x * kThreads;
-    for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < nfx + nfy + nfz; t += stride) {
-        if (t < nfx) {
-            const int f = (int)(t % (nx + 1));
-            const long long jk = t / (nx + 1);
-            const int j = (int)(jk % ny), k = (int)(jk / ny);
-            const long long ip = (f + 1) + pb * (j + 1) + pbb * (k + 1);
-            px[t] = -(aux(ip) - aux(ip - 1)) / dx / face_x(g, rho, f, j, k);
-        } else if (t < nfx + nfy) {
-            const long long r = t - nfx;
-            const int i = (int)(r % nx);
-            const long long jk = r / nx;
-            const int f = (int)(jk % (ny + 1)), k = (int)(jk / (ny + 1));
-            const long long ip = (i + 1) + pb * (f + 1) + pbb * (k + 1);
-            py[r] = -(aux(ip) - aux(ip - pb)) / dy / face_y(g, rho, i, f, k);
-        } else {
-            const long long r = t - nfx - nfy;
-            const int i = (int)(r % nx);
-            const long long jf = r / nx;
-            const int j = (int)(jf % ny), f = (int)(jf / ny);
-            const long long ip = (i + 1) + pb * (j + 1) + pbb * (f + 1);
-            const double h = dzz ? dzz[f + 1] : dzu;
-            pz[r] = -(aux(ip) - aux(ip - pbb)) / h / face_z(g, rho, dz, i, j, f);
+    for_cells(g, [&](int i, int j, int k) {
+        const long long ip = (i + 1) + pb * (j + 1) + pbb * (k + 1);
+        const long long fx = ((long long)k * ny + j) * (nx + 1) + i;
+        const long long fy = ((long long)k * (ny + 1) + j) * nx + i;
+        const long long fz = ((long long)k * ny + j) * nx + i;
+        const double a0 = aux(ip);
+        const double hk = dzz ? dzz[k + 1] : dzu;
+        px[fx] = -(a0 - aux(ip - 1)) / dx / face_x(g, rho, i, j, k);
+        // face i + 1 is in the box for every lane; the last column stores it
+        const double x1 = -(aux(ip + 1) - a0) / dx / face_x(g, rho, i + 1, j, k);
+        if (i == nx - 1) px[fx + 1] = x1;
+        py[fy] = -(a0 - aux(ip - pb)) / dy / face_y(g, rho, i, j, k);
+        pz[fz] = -(a0 - aux(ip - pbb)) / hk / face_z(g, rho, dz, i, j, k);
+        if (j == ny - 1) py[fy + nx] = -(aux(ip + pb) - a0) / dy / face_y(g, rho, i, ny, k);
+        if (k == nz - 1) {
+            const double ht = dzz ? dzz[nz + 1] : dzu;
+            pz[fz + (long long)nx * ny] = -(aux(ip + pbb) - a0) / ht / face_z(g, rho, dz, i, j, nz);
         }
-    }
+    });
 }
 
 // prd0 = prd1 + prd0 over the whole halo box (:466).
@@ -1279,11 +1267,9 @@ void launch_heaviside_tens(const Geo &g, const double *ls, double eps, double rh
 void launch_exp_press(const Geo &g, const double *rho, const double *dz, const double *dzz, double dx, double dy,
                       double dzu, const double *prd1, double *prd0, double *px, double *py, double *pz, hipStream_t s)
 {
-    const long long nf = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
-                         (long long)g.nx * g.ny * (g.nz + 1);
     const long long nbox = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
-    hipLaunchKernelGGL(k_exp_press, dim3(grid_for(nf)), dim3(kThreads), 0, s, g, rho, dz, dzz, dx, dy, dzu, prd1,
-                       prd0, px, py, pz);
+    hipLaunchKernelGGL(k_exp_press, cell_grid(g), dim3(kThreads), 0, s, g, rho, dz, dzz, dx, dy, dzu, prd1, prd0, px,
+                       py, pz);
     hipLaunchKernelGGL(k_add_box, dim3(grid_for(nbox)), dim3(kThreads), 0, s, nbox, prd1, prd0);
 }
 
@@ -1316,7 +1302,6 @@ void launch_pressh(const Geo &g, const double *rho, const double *dz, double dzu
 void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, double *u, double *v, double *w,
                          hipStream_t s)
 {
-    const long long nfxy = (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz;
     hipLaunchKernelGGL(k_posdin, cell_grid(g), dim3(kThreads), 0, s, g, in, u, v, w, false);
     const long long nplane = (long long)(g.nx + 2) * (g.ny + 2);
     const dim3 grid((unsigned)((nplane + kThreads - 1) / kThreads));
@@ -1326,7 +1311,7 @@ void launch_posdin_hydro(const Geo &g, const SetupIn &in, const double *hs, doub
     else
         hipLaunchKernelGGL(k_posdin_hydro_w<false>, grid, dim3(kThreads), 0, s, g, u, v, hs, in.dz, in.dzu, in.dx,
                            in.dy, w);
-    hipLaunchKernelGGL(k_posdin_hydro_uv, dim3(grid_for(nfxy)), dim3(kThreads), 0, s, g, hs, u, v);
+    hipLaunchKernelGGL(k_posdin_hydro_uv, cell_grid(g), dim3(kThreads), 0, s, g, hs, u, v);
 }
 
 }  // namespace sps
