@@ -9,9 +9,17 @@ FP64 dots, scalars and true residual) is the headline; the FP64 solver runs
 the same steps for the comparison.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-solves C] [--no-cpu]
-                    [--no-extra] [--no-graphs] [--only mixed|fp64] [--tiles PRECISION=KZxVX]
+                    [--no-extra] [--extra NAMES] [--no-graphs] [--only mixed|fp64|both|none]
+                    [--tiles PRECISION=KZxVX]
 
-Prints one JSON line.
+Prints one JSON line.  Beside the headline it reports `workloads` (--extra,
+all by default): the stretched masked cylinder, the tank as IPA, the
+64-problem batch, the serving stream, the host-pointer drop-in call per RK
+sub-stage, and `rows`, the device-resident entry points of the rows beside
+the projection step (density from the level set, the hydrostatic step,
+exp_press, SuLi-IT's solve and posdin), each with its time per call and the
+bytes it moves.  With --only both it also times the device-resident posdin
+(`posdin_dev`).
 
 `roofline` is for the dominant kernel, k_cg_m, the whole FP32 CG iteration of
 the mixed solver in one launch.  `achieved` counts the algorithmic bytes of
