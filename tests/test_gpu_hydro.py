@@ -44,6 +44,10 @@ SIZES = [
     ("nz3_odd", dict(nx=13, ny=3, nz=3, seed=23)),
     ("stretched", dict(nx=20, ny=12, nz=17, seed=24, stretched=True)),
     ("periodic_x", dict(nx=33, ny=10, nz=9, seed=25, periodic_x=True)),
+    # two 64-cell tiles along x (the second partial), a partial row of
+    # tiles along y, columns of 19 planes (two blocks of eight and three)
+    ("ragged_tiles", dict(nx=70, ny=33, nz=19, seed=26)),
+    ("ragged_tiles_stretched", dict(nx=70, ny=33, nz=21, seed=27, stretched=True)),
 ]
 
 

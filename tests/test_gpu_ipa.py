@@ -47,6 +47,8 @@ CASES = [
     ("stretched", stretched_ipa),
     ("ibm_rhs", lambda: ipa(cases.random_case(40, 20, 16, seed=85, ibm=True))),
     ("dirichlet_lid", lambda: ipa(cases.config1(seed=86, nx=32, ny=24, nz=16, top_dirichlet=True))),
+    # two 64-cell tiles along x (the second partial), a partial row of tiles along y
+    ("ragged_tiles", lambda: ipa(cases.random_case(70, 33, 19, seed=88, periodic_x=True))),
 ]
 
 

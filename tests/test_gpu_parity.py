@@ -123,6 +123,35 @@ def test_stretched_vs_oracle(solver, precision):
     np.testing.assert_array_equal(f2.w, w1)
 
 
+@pytest.mark.parametrize("vname,kw", [("walls", dict()), ("px", dict(periodic_x=True)),
+                                      ("py", dict(periodic_y=True)), ("ibm", dict(ibm=True)),
+                                      ("stretched", dict(stretched=True))])
+def test_posdin_ragged_tiles_exact(solver, vname, kw):
+    """posdin on a grid whose cell tiles (64 x 4 cells of a plane) are
+    partial in x and y: two tiles along x (70 cells), 33 rows, 19 planes;
+    host and device entry points bit for bit against the oracle's posdin
+    with the same pressure."""
+    c = cases.random_case(70, 33, 19, seed=91, **kw)
+    print(f"posdin ragged {vname}", flush=True)
+    rng = np.random.default_rng(92)
+    p = np.asfortranarray(rng.uniform(-1.0, 1.0, c.fields.prd1.shape))
+    u1, v1, w1, _ = oracle.posdin(c.grid, c.fields, p)
+    f2 = c.fields.copy()
+    f2.prd1 = p.copy(order="F")
+    prd = solver.posdin(c.grid, f2)
+    for got, exp in ((f2.u, u1), (f2.v, v1), (f2.w, w1), (prd, p)):
+        np.testing.assert_array_equal(got, exp)
+    f3 = c.fields.copy()
+    f3.prd1 = p.copy(order="F")
+    df = solver.upload(c.grid, f3)
+    try:
+        solver.posdin_dev(c.grid, df)
+        for name, exp in (("u", u1), ("v", v1), ("w", w1)):
+            np.testing.assert_array_equal(df.download(name), exp)
+    finally:
+        df.free()
+
+
 def test_reference_rule_iteration_count(solver):
     """Reference stop rule: same iteration count as the oracle (+-1: the
     dot products are summed in another order)."""
