@@ -506,6 +506,7 @@ void suli_ps_destroy(suli_ps_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->pipe.release();   // pinned staging buffers, before their stream goes
     release_last(ctx);
     if (ctx->st.block) (void)hipFree(ctx->st.block);
     for (auto &e : ctx->ev)
