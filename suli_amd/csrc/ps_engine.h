@@ -210,7 +210,7 @@ private:
     // events, so that the launches of a step overlap; the problems of
     // different kinds share no memory, and each launch is the same
     int launch_steps_m(StepRef sr, int *ran, bool concurrent = false);
-    static constexpr int kSideStreams = 3;   // with the engine's own: the box's 4 hardware queues
+    static constexpr int kSideStreams = 3;   // with the engine's own: HIP's default 4 hardware queues per process
     hipStream_t side_[kSideStreams] = {};
     hipEvent_t fork_ev_ = nullptr, join_ev_[kSideStreams] = {};
     // graph replay: step counter on the device, two captured groups of
