@@ -728,7 +728,7 @@ def bench_rows(reps=20, it_steps=3):
                          graphs=GRAPHS)
         ms = []
         for k in range(it_steps + 1):
-            solver._check(solver.lib.suli_ps_dev_zero(solver.ctx, d["prd1"], box_bytes))
+            solver.dev_zero(d["prd1"], box_bytes)
             solver.synchronize()
             t0 = time.perf_counter()
             st = solver.it_graddin_dev(gi, d)

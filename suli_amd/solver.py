@@ -317,6 +317,10 @@ class PressureSolver:
     def dev_free(self, ptr) -> None:
         self.lib.suli_ps_dev_free(self.ctx, ptr)
 
+    def dev_zero(self, ptr, nbytes: int) -> None:
+        """Zero nbytes of a device allocation (on the context's stream)."""
+        self._check(self.lib.suli_ps_dev_zero(self.ctx, ptr, nbytes))
+
     # -- SuLi-IT -------------------------------------------------------------
     @staticmethod
     def _it_cgrid(g) -> _lib.Grid:
