@@ -92,20 +92,24 @@ __device__ __forceinline__ bool solid_cell(const Geo &g, const SetupIn &in, int 
 {
     return in.solid && in.solid[cidx(g, i, j, k)] != 0;
 }
+// (both cells' bytes are loaded and combined with |, not ||, so the two
+// loads are issued together instead of the second waiting on the first)
 __device__ __forceinline__ bool closed_x(const Geo &g, const SetupIn &in, int f, int j, int k)
 {
-    return in.solid && (solid_cell(g, in, box_to_cell(f + 1, g.nx, g.px), j, k) ||
-                        solid_cell(g, in, box_to_cell(f, g.nx, g.px), j, k));
+    if (!in.solid) return false;
+    return (in.solid[cidx(g, box_to_cell(f + 1, g.nx, g.px), j, k)] | in.solid[cidx(g, box_to_cell(f, g.nx, g.px), j, k)]) !=
+           0;
 }
 __device__ __forceinline__ bool closed_y(const Geo &g, const SetupIn &in, int i, int f, int k)
 {
-    return in.solid && (solid_cell(g, in, i, box_to_cell(f + 1, g.ny, g.py), k) ||
-                        solid_cell(g, in, i, box_to_cell(f, g.ny, g.py), k));
+    if (!in.solid) return false;
+    return (in.solid[cidx(g, i, box_to_cell(f + 1, g.ny, g.py), k)] | in.solid[cidx(g, i, box_to_cell(f, g.ny, g.py), k)]) !=
+           0;
 }
 __device__ __forceinline__ bool closed_z(const Geo &g, const SetupIn &in, int i, int j, int f)
 {
-    return in.solid && (solid_cell(g, in, i, j, box_to_cell(f + 1, g.nz, 0)) ||
-                        solid_cell(g, in, i, j, box_to_cell(f, g.nz, 0)));
+    if (!in.solid) return false;
+    return (in.solid[cidx(g, i, j, box_to_cell(f + 1, g.nz, 0))] | in.solid[cidx(g, i, j, box_to_cell(f, g.nz, 0))]) != 0;
 }
 
 // Cell tiles of the setup and correction kernels: 64 x 4 cells of a plane
@@ -1229,8 +1233,7 @@ void launch_init(const Geo &g, const SetupIn &in, Op<double> op, const double *d
                  double *r0, double *r64, float *r32, float *c32, const Partials &P, int nb0, CgScalars *S,
                  hipStream_t s)
 {
-    // the projection streams its cells on at most kVecBlocks workgroups
-    const int np = std::min(nb0, kVecBlocks);
+    const int np = nb0;   // the projection's workgroups (and partials)
     hipLaunchKernelGGL(k_init_residual, dim3(nb0), dim3(kThreads), 0, s, g, in, op, d, b, e, r0, P);
     hipLaunchKernelGGL(k_init_totals, dim3(1), dim3(kThreads), 0, s, P, nb0, S);
     hipLaunchKernelGGL(k_init_project, dim3(np), dim3(kThreads), 0, s, g, in.solid, d, b, r0, r64, r32, c32, P, S);
