@@ -391,8 +391,10 @@ __global__ __launch_bounds__(kThreads) void k_init_residual(Geo g, SetupIn in, O
         const double sE = op.sx[fx + 1], sW = op.sx[fx], sN = op.sy[fy + nx], sS = op.sy[fy];
         const double sT = op.sz[fz + pl], sB = op.sz[fz];
         const double bc = b[c];
-        // null vector: sqrt(d) on fluid cells, 0 on solid ones
-        const double dc = solid_cell(g, in, i, j, k) ? 0.0 : d[c];
+        // null vector: sqrt(d) on fluid cells, 0 on solid ones (d and the
+        // mask byte loaded together, then selected)
+        const double dl = d[c];
+        const double dc = solid_cell(g, in, i, j, k) ? 0.0 : dl;
         const double a_in = ec - sE * eE - sW * eW - sN * eN - sS * eS - sT * eT - sB * eB;
         const double a_mir = ec - sE * mE - sW * mW - sN * mN - sS * mS - sT * mT - sB * mB;
         const double r = a_in - bc;
@@ -460,7 +462,8 @@ __global__ __launch_bounds__(kThreads) void k_init_project(Geo g, const uint8_t 
     const long long stride = (long long)gridDim.x * kThreads;
     double acc = 0.0, acc32 = 0.0;
     for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < g.n; c += stride) {
-        const double sq = (solid && solid[c]) ? 0.0 : sqrt(d[c]);
+        const double dl = d[c];
+        const double sq = (solid && solid[c]) ? 0.0 : sqrt(dl);
         const double r = gamma != 0.0 ? r0[c] - gamma * sq : r0[c];
         if (gamma != 0.0) b[c] = b[c] + gamma * sq;
         acc += r * r;
