@@ -433,6 +433,9 @@ int suli_ps_server_get_info(suli_ps_server *srv, suli_ps_server_info *info);
 /* Device-memory helpers for callers without their own HIP code. */
 void *suli_ps_dev_alloc(suli_ps_ctx *ctx, size_t bytes);
 int suli_ps_dev_free(suli_ps_ctx *ctx, void *ptr);
+/* Host -> device and device -> host copies on the context's stream, through
+ * the context's pinned staging above a few MiB; each returns once the copy is
+ * complete.  NULL pointers with a nonzero size are rejected. */
 int suli_ps_memcpy_h2d(suli_ps_ctx *ctx, void *dst, const void *src, size_t bytes);
 int suli_ps_memcpy_d2h(suli_ps_ctx *ctx, void *dst, const void *src, size_t bytes);
 int suli_ps_dev_zero(suli_ps_ctx *ctx, void *ptr, size_t bytes); /* async, ctx stream */

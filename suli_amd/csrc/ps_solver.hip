@@ -963,17 +963,22 @@ int suli_ps_dev_free(suli_ps_ctx *ctx, void *ptr)
     return 0;
 }
 
+// Staged through the context's pinned chunks above a few MiB, as the
+// host-pointer entry points are; both return once the copy is complete.
 int suli_ps_memcpy_h2d(suli_ps_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
-    HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx || (bytes && (!dst || !src))) return SULI_PS_EINVAL;
+    const HostSeg sg{dst, const_cast<void *>(src), bytes};
+    HIPCHK(ctx, ctx->pipe.upload(ctx->device, ctx->stream, &sg, 1));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
 int suli_ps_memcpy_d2h(suli_ps_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
-    HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx || (bytes && (!dst || !src))) return SULI_PS_EINVAL;
+    const HostSeg sg{const_cast<void *>(src), dst, bytes};
+    HIPCHK(ctx, ctx->pipe.download(ctx->device, ctx->stream, &sg, 1));
     return 0;
 }
 
